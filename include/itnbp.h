@@ -1,0 +1,191 @@
+/*
+ * itnbp.h - C ABI of libitnbp.so, the MI355X belief-propagation library.
+ *
+ * Drop-in boundary for the BP hot path of ITensorNetworks.jl: the Julia shim
+ * (ccall) or the Python host package (ctypes) marshals an ITensorNetwork's
+ * site tensors into flat column-major buffers and calls these entry points;
+ * every array crossing the boundary is a plain pointer plus sizes, owned by
+ * the caller, and copied by the library (no caller pointer is kept after a
+ * call returns).
+ *
+ * Conventions
+ *   - Network: nv vertices, ne undirected edges edges[2e], edges[2e+1].
+ *     Directed message ids: 2e is edges[2e] -> edges[2e+1], 2e+1 the reverse.
+ *   - Bond (axis) order per vertex: CSR inc_ptr[nv+1] / inc_edge[2ne]; axis
+ *     a+1 of psi_v is the bond inc_edge[inc_ptr[v]+a]; axis 0 is physical.
+ *     (test/utils.jl:32-36: inds_v = [siteinds[v]; link_v])
+ *   - Site tensors: column-major (first index fastest), [s, b_1, ..., b_k],
+ *     concatenated over vertices in vertex order.
+ *   - Messages: chi x chi column-major M[b_ket, b_bra]
+ *     (formnetworks/quadraticformnetwork.jl:115-119), concatenated by id.
+ *   - One-site operators: d x d column-major O[s', s] (ITensors op(name, s));
+ *     two-site: (d_u d_v) x (d_u d_v) O[(s_u', s_v'), (s_u, s_v)], s_u fastest.
+ *   - Every call returns ITNBP_OK or an error code; itnbp_last_error() gives
+ *     the text. No C++ exception crosses the ABI. Not thread-safe per context.
+ */
+#ifndef ITNBP_H
+#define ITNBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ITNBP_OK = 0,
+  ITNBP_ERR_ARG = 1,         /* invalid argument / shape */
+  ITNBP_ERR_HIP = 2,         /* HIP runtime error */
+  ITNBP_ERR_NOMEM = 3,       /* device allocation failed */
+  ITNBP_ERR_NONFINITE = 4,   /* NaN/Inf detected in a result */
+  ITNBP_ERR_UNSUPPORTED = 5, /* shape or mode this build does not handle */
+  ITNBP_ERR_STATE = 6        /* e.g. message read before it was set */
+};
+
+/* storage element types of site tensors */
+enum { ITNBP_F64 = 0, ITNBP_F32 = 1, ITNBP_BF16 = 2 };
+
+/* BP schedules (edge_sequences.jl:9-51) */
+enum { ITNBP_SCHED_SEQUENTIAL = 0, ITNBP_SCHED_PARALLEL = 1 };
+
+/* arithmetic of the message contractions */
+enum {
+  ITNBP_PREC_F64 = 0,         /* FP64 everywhere (the plain path) */
+  ITNBP_PREC_F32 = 1,         /* FP32 messages and arithmetic */
+  ITNBP_PREC_BF16_F64 = 2     /* BF16 MFMA with FP32 accumulation until the
+                                 residual < switch_tol, then FP64 refinement */
+};
+
+typedef struct itnbp_ctx itnbp_ctx;
+typedef struct itnbp_net itnbp_net;
+
+typedef struct {
+  int schedule;          /* ITNBP_SCHED_* */
+  int maxiter;           /* >= 1; required (abstractbeliefpropagationcache.jl:315-317) */
+  double tol;            /* stop when mean message_diff <= tol; < 0 disables */
+  double res_tol;        /* stop when max ||new-old||/||new|| <= res_tol; < 0 disables */
+  int precision;         /* ITNBP_PREC_* */
+  double switch_tol;     /* BF16 phase stops when max residual < switch_tol */
+  int normalize;         /* divide each message by its Frobenius norm (default 1) */
+  int nseq;              /* length of seq (0: default schedule of the network) */
+  const int* seq;        /* directed message ids in update order */
+  int verbose;
+} itnbp_update_opts;
+
+typedef struct {
+  int iterations;        /* sweeps run (all phases) */
+  int iterations_low;    /* sweeps of the reduced-precision phase */
+  double mean_diff;      /* mean message_diff of the last sweep */
+  double max_residual;   /* max ||new-old||_F/||new||_F of the last sweep */
+  int converged;
+  double device_ms;      /* device time of the sweeps (HIP events) */
+} itnbp_update_result;
+
+/* ---- context ---------------------------------------------------------- */
+int itnbp_create(int device, itnbp_ctx** out);
+int itnbp_destroy(itnbp_ctx* ctx);
+const char* itnbp_last_error(itnbp_ctx* ctx);
+int itnbp_synchronize(itnbp_ctx* ctx);
+
+/* ---- network ---------------------------------------------------------- */
+/* Replaces BeliefPropagationCache(QuadraticFormNetwork(psi)) construction
+ * (caches/beliefpropagationcache.jl:20-35, formnetworks/quadraticformnetwork.jl:75-89):
+ * the bra layer is conj(psi) on the fly and the identity operator is elided. */
+int itnbp_net_create(itnbp_ctx* ctx, int nv, int ne, const int* edges,
+                     const int* inc_ptr, const int* inc_edge, const int* bond_dims,
+                     const int* phys_dims, int dtype, itnbp_net** out);
+int itnbp_net_destroy(itnbp_net* net);
+/* deep copy (functional API: update/set_message return new caches,
+ * caches/beliefpropagationcache.jl:43-47) */
+int itnbp_net_copy(itnbp_net* src, itnbp_net** out);
+int64_t itnbp_net_psi_numel(itnbp_net* net);
+int64_t itnbp_net_msg_numel(itnbp_net* net);
+
+/* site tensors, all vertices, column-major, in the net's dtype
+ * (bpc[v] = ... of caches/abstractbeliefpropagationcache.jl:158-171) */
+int itnbp_set_sites(itnbp_net* net, const void* data);
+int itnbp_get_sites(itnbp_net* net, void* data);
+int itnbp_set_site(itnbp_net* net, int v, const void* data);
+
+/* messages (FP64 host buffers), all directed ids
+ * (messages / set_messages!, abstractbeliefpropagationcache.jl:173-200) */
+int itnbp_set_messages(itnbp_net* net, const double* data);
+int itnbp_get_messages(itnbp_net* net, double* data);
+int itnbp_set_message(itnbp_net* net, int msg_id, const double* data);
+int itnbp_get_message(itnbp_net* net, int msg_id, double* data);
+/* identity_messages (formnetworks/quadraticformnetwork.jl:96-124) */
+int itnbp_init_identity_messages(itnbp_net* net);
+/* 1 if message id has been set (message() has no default,
+ * abstractbeliefpropagationcache.jl:173-175) */
+int itnbp_message_is_set(itnbp_net* net, int msg_id);
+
+/* default sequential schedule (edge_sequences.jl:32-47); out has 2*ne ids */
+int itnbp_forest_cover_sequence(itnbp_net* net, int* out);
+
+/* ---- BP --------------------------------------------------------------- */
+/* update(bpc; alg="bp", maxiter, tol, edge_sequence, message_update_alg)
+ * (abstractbeliefpropagationcache.jl:313-337) */
+int itnbp_update(itnbp_net* net, const itnbp_update_opts* opts, itnbp_update_result* res);
+/* updated_message(Algorithm"contract", bpc, e) for a list of directed ids
+ * (abstractbeliefpropagationcache.jl:225-239); out: chi^2 per id, FP64 */
+int itnbp_updated_messages(itnbp_net* net, int n, const int* msg_ids, int normalize,
+                           double* out);
+
+/* ---- environments / expectation values / scalars ---------------------- */
+/* one-site RDMs rho_v[s, s'] (d x d, unnormalised) of the BP environment
+ * (environment(bpc, [operator vertex]), caches/beliefpropagationcache.jl:100-105;
+ * expect.jl:5-19 contracts it with op and with delta) */
+int itnbp_rdm1(itnbp_net* net, int n, const int* verts, double* out);
+/* <O_v> = tr(O rho_v)/tr(rho_v); ops: n d x d matrices (expect.jl:5-41) */
+int itnbp_expect1(itnbp_net* net, int n, const int* verts, const double* ops, double* out);
+/* two-site RDMs of neighbouring pairs given by edge ids, (d_u d_v)^2 each
+ * (test/test_belief_propagation.jl:64-91 pattern) */
+int itnbp_rdm2(itnbp_net* net, int n, const int* edge_ids, double* out);
+int itnbp_expect2(itnbp_net* net, int n, const int* edge_ids, const double* ops, double* out);
+/* region_scalar for every vertex and every edge (caches/beliefpropagationcache.jl:107-119) */
+int itnbp_region_scalars(itnbp_net* net, double* vertex_scalars, double* edge_scalars);
+/* logscalar(bpc) (abstractbeliefpropagationcache.jl:397-408): re/im parts;
+ * *is_neg_inf set when an edge scalar is 0 */
+int itnbp_logscalar(itnbp_net* net, double* re, double* im, int* is_neg_inf);
+/* rescale_messages + rescale_partitions as used by normalize(alg="bp")
+ * (caches/beliefpropagationcache.jl:121-139, abstractbeliefpropagationcache.jl:349-395,
+ * normalize.jl:125-142) */
+int itnbp_normalize(itnbp_net* net);
+
+/* ---- BP-gauged two-site gates ----------------------------------------- */
+/* apply(o, psi; envs, maxdim, cutoff, normalize, callback) for n gates on
+ * disjoint bonds, envs = the net's current messages (apply.jl:33-146).
+ * gates: n (d_u d_v)^2 FP64 matrices. maxdim <= 0: no cap; cutoff < 0: none.
+ * truncerr[n]; svals[n*max_keep] (zero padded) may be NULL; kept[n] may be NULL.
+ * The bond keeps its dimension: if fewer values survive, the tensors are
+ * zero padded (the network is unchanged). */
+int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
+                      int maxdim, double cutoff, int normalize, double* truncerr,
+                      double* svals, int max_keep, int* kept);
+
+/* ---- adjoint ---------------------------------------------------------- */
+/* BP energy E = sum_e <O2_e> + sum_v <O1_v> after nsweeps Jacobi sweeps from
+ * identity messages, and dE/dpsi for every site tensor (FP64), by hand-written
+ * adjoint kernels; sweeps are checkpointed every `checkpoint` sweeps (<= 0:
+ * every sweep) and recomputed in the backward pass. No counterpart in the
+ * reference (it has no autodiff); the Julia shim wraps it as a ChainRulesCore rrule.
+ * ops1: nv d x d (may be NULL), ops2: ne (d_u d_v)^2 (may be NULL). */
+int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, const double* ops1,
+                      const double* ops2, double* energy, double* grad);
+
+/* ---- batched heterogeneous BP ----------------------------------------- */
+/* n networks (any lattices, bond dims, same dtype) updated together: every
+ * message of every network in one launch per sweep; converged networks drop
+ * out (Jacobi schedule). res may hold n results. */
+int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_opts* opts,
+                         itnbp_update_result* res);
+
+/* ---- diagnostics ------------------------------------------------------ */
+/* name of the kernel path the last itnbp_update used */
+const char* itnbp_last_path(itnbp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

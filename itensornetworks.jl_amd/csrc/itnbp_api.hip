@@ -1,0 +1,848 @@
+// C ABI of libitnbp: contexts, networks, messages, the BP sweep driver,
+// environments / expectation values / scalars, normalisation.
+//
+// Reference semantics (paths relative to the ITensorNetworks.jl repository):
+//   update(bpc; alg="bp", ...)       src/caches/abstractbeliefpropagationcache.jl:272-337
+//   updated_message("contract")      src/caches/abstractbeliefpropagationcache.jl:225-239
+//   identity_messages                src/formnetworks/quadraticformnetwork.jl:96-124
+//   default edge sequence            src/edge_sequences.jl:9-51
+//   environment / region_scalar      src/caches/beliefpropagationcache.jl:100-119
+//   expect                           src/expect.jl:5-41
+//   logscalar / rescale              src/caches/abstractbeliefpropagationcache.jl:339-408,
+//                                    src/caches/beliefpropagationcache.jl:121-139
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "itnbp_internal.hpp"
+#include "itnbp_ops.hpp"
+
+namespace {
+
+int set_err(itnbp_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                               \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess)                                                               \
+      return set_err((ctx), ITNBP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define RETCHK(call)          \
+  do {                        \
+    int _r = (call);          \
+    if (_r != ITNBP_OK) return _r; \
+  } while (0)
+
+template <class F>
+int guarded(itnbp_ctx* ctx, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return set_err(ctx, ITNBP_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return set_err(ctx, ITNBP_ERR_ARG, e.what());
+  } catch (...) {
+    return set_err(ctx, ITNBP_ERR_ARG, "unknown error");
+  }
+}
+
+}  // namespace
+
+// ---- shared helpers (declared in itnbp_ops.hpp) ------------------------------
+size_t dtype_size(int dtype) { return dtype == ITNBP_F64 ? 8 : dtype == ITNBP_F32 ? 4 : 2; }
+
+int itnbp_fail(itnbp_ctx* ctx, int code, const std::string& msg) { return set_err(ctx, code, msg); }
+
+int ensure_buf(itnbp_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.bytes >= bytes) return ITNBP_OK;
+  if (b.p) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipFree(b.p));
+  }
+  b.p = nullptr;
+  b.bytes = 0;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_err(ctx, ITNBP_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  b.bytes = bytes;
+  return ITNBP_OK;
+}
+
+void free_buf(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+int upload(itnbp_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
+  RETCHK(ensure_buf(ctx, b, bytes));
+  HIPCHK(ctx, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return ITNBP_OK;
+}
+
+int msg_id_into(const itnbp_net* net, int e, int to) {
+  // id 2e runs edges[2e] -> edges[2e+1]
+  return net->edges[2 * e + 1] == to ? 2 * e : 2 * e + 1;
+}
+int msg_src(const itnbp_net* net, int m) { return net->edges[2 * (m / 2) + (m & 1)]; }
+int msg_dst(const itnbp_net* net, int m) { return net->edges[2 * (m / 2) + 1 - (m & 1)]; }
+
+int axis_of(const itnbp_net* net, int v, int e) {
+  for (int a = net->inc_ptr[v]; a < net->inc_ptr[v + 1]; ++a)
+    if (net->inc_edge[a] == e) return a - net->inc_ptr[v] + 1;
+  return -1;
+}
+
+int all_bonds_mask(const itnbp_net* net, int v) {
+  int k = net->inc_ptr[v + 1] - net->inc_ptr[v];
+  return ((1 << (k + 1)) - 1) & ~1;
+}
+
+OcTask message_task(const itnbp_net* net, int m, int slot, bool with_old, int normalize) {
+  OcTask t;
+  const int u = msg_src(net, m);
+  t.v = u;
+  t.j = axis_of(net, u, m / 2);
+  t.s_open = 0;
+  t.absorb_mask = all_bonds_mask(net, u) & ~(1 << t.j);
+  t.out_off = net->msg_off[m];
+  t.old_off = with_old ? net->msg_off[m] : -1;
+  t.normalize = normalize;
+  t.slot = slot;
+  return t;
+}
+
+// Plan a set of open-contraction tasks: order them by path (small LDS classes,
+// then large batches) and size the scratch buffers.  Appends to plan.
+int plan_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, TaskPlan& plan) {
+  itnbp_ctx* ctx = net->ctx;
+  const int eb = acc_f64 ? 8 : 4;
+  std::vector<OcTask> small[3], large;
+  const size_t lds_cls[3] = {32 * 1024, 64 * 1024, 160 * 1024};
+  for (const auto& t : tasks) {
+    const size_t b = oc_small_lds_bytes(net->vdesc[t.v], t, eb);
+    if (b == 0) {
+      large.push_back(t);
+      continue;
+    }
+    const int c = b <= lds_cls[0] ? 0 : b <= lds_cls[1] ? 1 : 2;
+    small[c].push_back(t);
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (small[c].empty()) continue;
+    LaunchGroup lg{};
+    lg.kind = 0;
+    lg.begin = plan.tasks.size();
+    lg.count = small[c].size();
+    for (auto& t : small[c]) lg.lds = std::max(lg.lds, oc_small_lds_bytes(net->vdesc[t.v], t, eb));
+    plan.tasks.insert(plan.tasks.end(), small[c].begin(), small[c].end());
+    plan.groups.push_back(lg);
+  }
+  if (!large.empty()) {
+    size_t budget = (size_t)6 << 30;
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) == hipSuccess) budget = std::min(budget, freeb / 3);
+    size_t i0 = 0;
+    while (i0 < large.size()) {
+      long long maxD = 0;
+      int maxP2 = 1, maxR = 1;
+      size_t i1 = i0;
+      for (; i1 < large.size(); ++i1) {
+        const VDesc& V = net->vdesc[large[i1].v];
+        const long long nD = std::max<long long>(maxD, V.D);
+        const size_t nneed = (i1 - i0 + 1) * (size_t)(2 * nD * eb);
+        if (i1 > i0 && nneed > budget) break;
+        maxD = nD;
+      }
+      for (size_t i = i0; i < i1; ++i) {
+        const VDesc& V = net->vdesc[large[i].v];
+        const OcTask& t = large[i];
+        const int ds = t.s_open ? V.dims[0] : 1;
+        const int nj = t.j > 0 ? V.dims[t.j] : 1;
+        const int P = ds * nj;
+        if (P > 64) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "open dimension > 64 is not supported");
+        maxP2 = std::max(maxP2, P * P);
+        maxR = std::max(maxR, V.D / P);
+      }
+      LaunchGroup lg{};
+      lg.kind = 1;
+      lg.begin = plan.tasks.size();
+      lg.count = i1 - i0;
+      lg.maxD = maxD;
+      lg.nsplit = std::max(1, std::min(64, maxR / 1024));
+      lg.part_stride = (long long)lg.nsplit * maxP2;
+      plan.scratch_bytes = std::max(plan.scratch_bytes, lg.count * (size_t)(2 * maxD * eb));
+      plan.partial_bytes = std::max(plan.partial_bytes, lg.count * (size_t)lg.part_stride * eb);
+      plan.tasks.insert(plan.tasks.end(), large.begin() + i0, large.begin() + i1);
+      plan.groups.push_back(lg);
+      i0 = i1;
+    }
+  }
+  plan.acc_f64 = acc_f64;
+  return ITNBP_OK;
+}
+
+// upload the task table (synchronously) and size the scratch buffers
+int upload_plan(itnbp_net* net, const TaskPlan& plan, DevBuf& taskbuf) {
+  itnbp_ctx* ctx = net->ctx;
+  if (plan.tasks.empty()) return ITNBP_OK;
+  RETCHK(ensure_buf(ctx, taskbuf, plan.tasks.size() * sizeof(OcTask)));
+  HIPCHK(ctx, hipMemcpyAsync(taskbuf.p, plan.tasks.data(), plan.tasks.size() * sizeof(OcTask),
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (plan.scratch_bytes) RETCHK(ensure_buf(ctx, net->scratch, plan.scratch_bytes));
+  if (plan.partial_bytes) RETCHK(ensure_buf(ctx, net->partials, plan.partial_bytes));
+  return ITNBP_OK;
+}
+
+// launch groups [g0, g1) of an uploaded plan
+int launch_plan(itnbp_net* net, const TaskPlan& plan, size_t g0, size_t g1, const OcTask* d_tasks,
+                const void* psi, int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats) {
+  itnbp_ctx* ctx = net->ctx;
+  for (size_t gi = g0; gi < g1; ++gi) {
+    const LaunchGroup& lg = plan.groups[gi];
+    if (lg.kind == 0) {
+      HIPCHK(ctx, launch_oc_small(psi_dtype, plan.acc_f64, net->d_vdesc, d_tasks + lg.begin, (int)lg.count,
+                                  psi, msg_in, out, d_stats, lg.lds, ctx->stream));
+    } else {
+      HIPCHK(ctx, launch_oc_large(psi_dtype, plan.acc_f64, net->d_vdesc, d_tasks + lg.begin,
+                                  plan.tasks.data() + lg.begin, net->vdesc.data(), (int)lg.count, psi,
+                                  msg_in, out, d_stats, net->scratch.p, lg.maxD, net->partials.p,
+                                  lg.nsplit, lg.part_stride, ctx->stream));
+    }
+  }
+  return ITNBP_OK;
+}
+
+// plan + upload + launch in one go (environments, scalars, single messages)
+int run_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, const void* psi,
+              int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats, DevBuf& taskbuf) {
+  if (tasks.empty()) return ITNBP_OK;
+  TaskPlan plan;
+  RETCHK(plan_tasks(net, tasks, acc_f64, plan));
+  RETCHK(upload_plan(net, plan, taskbuf));
+  RETCHK(launch_plan(net, plan, 0, plan.groups.size(), (const OcTask*)taskbuf.p, psi, psi_dtype, msg_in,
+                     out, d_stats));
+  return ITNBP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sweep driver
+// ---------------------------------------------------------------------------
+namespace {
+
+bool reads(const itnbp_net* net, int reader, int m) {
+  // message reader = (a -> b) reads m = (u -> v) iff v == a and u != b
+  return msg_dst(net, m) == msg_src(net, reader) && msg_src(net, m) != msg_dst(net, reader);
+}
+
+// consecutive runs of the sequence in which no message reads another one of the run
+std::vector<std::pair<int, int>> sequential_runs(const itnbp_net* net, const std::vector<int>& seq) {
+  std::vector<std::pair<int, int>> runs;
+  int start = 0;
+  for (int i = 0; i < (int)seq.size(); ++i) {
+    bool conflict = false;
+    for (int j = start; j < i && !conflict; ++j)
+      conflict = seq[j] == seq[i] || reads(net, seq[i], seq[j]) || reads(net, seq[j], seq[i]);
+    if (conflict) {
+      runs.push_back({start, i});
+      start = i;
+    }
+  }
+  if (start < (int)seq.size()) runs.push_back({start, (int)seq.size()});
+  return runs;
+}
+
+struct Phase {
+  int acc_f64;
+  const void* psi;
+  int psi_dtype;
+  void* msg[2];  // double buffers in the accumulator precision
+};
+
+}  // namespace
+
+int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_opts* o,
+                const Phase& ph, int maxiter, double stop_res2, bool stop_on_tol, int* iters,
+                double* mean_diff, double* max_res, int* converged, bool use_fast) {
+  itnbp_ctx* ctx = net->ctx;
+  const int n = (int)seq.size();
+  RETCHK(ensure_buf(ctx, net->stats, sizeof(MsgStat) * (size_t)std::max(1, n)));
+  RETCHK(ensure_buf(ctx, net->red, 8 * sizeof(double)));
+  MsgStat* d_stats = (MsgStat*)net->stats.p;
+  double* d_red = (double*)net->red.p;
+  const size_t eb = ph.acc_f64 ? 8 : 4;
+  const bool partial = (o->schedule == ITNBP_SCHED_PARALLEL) && n != 2 * net->ne;
+  std::vector<std::pair<int, int>> runs;
+  if (o->schedule == ITNBP_SCHED_PARALLEL) {
+    runs.push_back({0, n});
+  } else {
+    runs = sequential_runs(net, seq);
+  }
+  // one plan for the whole sweep; run r launches groups [run_groups[r], run_groups[r+1])
+  TaskPlan plan;
+  std::vector<size_t> run_groups{0};
+  for (auto& r : runs) {
+    std::vector<OcTask> rt;
+    rt.reserve(r.second - r.first);
+    for (int i = r.first; i < r.second; ++i) rt.push_back(message_task(net, seq[i], i, true, o->normalize));
+    RETCHK(plan_tasks(net, rt, ph.acc_f64, plan));
+    run_groups.push_back(plan.groups.size());
+  }
+  RETCHK(upload_plan(net, plan, net->tasks));
+  const OcTask* d_tasks = (const OcTask*)net->tasks.p;
+  int it = 0;
+  *converged = 0;
+  double red[4] = {0, 0, 0, 0};
+  for (it = 1; it <= maxiter; ++it) {
+    if (o->schedule == ITNBP_SCHED_PARALLEL) {
+      void* cur = ph.msg[net->cur];
+      void* nxt = ph.msg[1 - net->cur];
+      if (partial)
+        HIPCHK(ctx, hipMemcpyAsync(nxt, cur, net->msg_total * eb, hipMemcpyDeviceToDevice, ctx->stream));
+      bool done = false;
+      if (use_fast) {
+        int r = fast_parallel_sweep(net, seq, ph.acc_f64, ph.psi, ph.psi_dtype, cur, nxt, d_stats,
+                                    o->normalize, &done);
+        if (r != ITNBP_OK) return r;
+      }
+      if (!done) RETCHK(launch_plan(net, plan, 0, plan.groups.size(), d_tasks, ph.psi, ph.psi_dtype, cur, nxt, d_stats));
+      net->cur = 1 - net->cur;
+    } else {
+      void* cur = ph.msg[net->cur];
+      for (size_t r = 0; r + 1 < run_groups.size(); ++r)
+        RETCHK(launch_plan(net, plan, run_groups[r], run_groups[r + 1], d_tasks, ph.psi, ph.psi_dtype, cur, cur, d_stats));
+    }
+    HIPCHK(ctx, launch_reduce_stats(d_stats, n, d_red, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(red, d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (red[2] >= 1.0)
+      return set_err(ctx, ITNBP_ERR_NONFINITE, "non-finite message after sweep " + std::to_string(it));
+    *mean_diff = red[0] / n;
+    *max_res = std::sqrt(red[1]);
+    bool stop = false;
+    if (stop_on_tol && o->tol >= 0 && *mean_diff <= o->tol) stop = true;
+    if (stop_on_tol && o->res_tol >= 0 && red[1] <= o->res_tol * o->res_tol) stop = true;
+    if (stop_res2 >= 0 && red[1] <= stop_res2) stop = true;
+    if (stop) {
+      *converged = 1;
+      if (o->verbose) printf("BP converged to desired precision after %d iterations.\n", it);
+      break;
+    }
+  }
+  *iters = std::min(it, maxiter);
+  return ITNBP_OK;
+}
+
+namespace {
+
+int default_sequence(const itnbp_net* net, int schedule, std::vector<int>& seq);
+
+int check_sequence(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_opts* o) {
+  std::vector<char> set(net->msg_set.begin(), net->msg_set.end());
+  const bool need_old = o->tol >= 0 || o->res_tol >= 0 || o->precision == ITNBP_PREC_BF16_F64;
+  for (int m : seq) {
+    if (m < 0 || m >= 2 * net->ne) return set_err(net->ctx, ITNBP_ERR_ARG, "message id out of range in edge sequence");
+    const int u = msg_src(net, m), v = msg_dst(net, m);
+    for (int a = net->inc_ptr[u]; a < net->inc_ptr[u + 1]; ++a) {
+      const int e = net->inc_edge[a];
+      const int in = msg_id_into(net, e, u);
+      if (msg_src(net, in) == v) continue;
+      if (!set[in])
+        return set_err(net->ctx, ITNBP_ERR_STATE,
+                       "message " + std::to_string(in) + " is read before it is set (messages have no default)");
+    }
+    if (need_old && !set[m])
+      return set_err(net->ctx, ITNBP_ERR_STATE,
+                     "message " + std::to_string(m) + " has no previous value to compare with (tol needs one)");
+    if (o->schedule == ITNBP_SCHED_SEQUENTIAL) set[m] = 1;
+  }
+  if (o->schedule == ITNBP_SCHED_PARALLEL)
+    for (int m : seq) set[m] = 1;
+  return ITNBP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int itnbp_create(int device, itnbp_ctx** out) {
+  if (!out) return ITNBP_ERR_ARG;
+  *out = nullptr;
+  auto* ctx = new (std::nothrow) itnbp_ctx();
+  if (!ctx) return ITNBP_ERR_NOMEM;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    delete ctx;
+    return ITNBP_ERR_HIP;
+  }
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    delete ctx;
+    return ITNBP_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+  *out = ctx;
+  return ITNBP_OK;
+}
+
+int itnbp_destroy(itnbp_ctx* ctx) {
+  if (!ctx) return ITNBP_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ITNBP_OK;
+}
+
+const char* itnbp_last_error(itnbp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* itnbp_last_path(itnbp_ctx* ctx) { return ctx ? ctx->path.c_str() : ""; }
+
+int itnbp_synchronize(itnbp_ctx* ctx) {
+  if (!ctx) return ITNBP_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ITNBP_OK;
+}
+
+int itnbp_net_create(itnbp_ctx* ctx, int nv, int ne, const int* edges, const int* inc_ptr,
+                     const int* inc_edge, const int* bond_dims, const int* phys_dims, int dtype,
+                     itnbp_net** out) {
+  if (!ctx || !out) return ITNBP_ERR_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&]() -> int {
+    if (nv <= 0 || ne < 0 || (ne > 0 && (!edges || !bond_dims)) || !inc_ptr || (ne > 0 && !inc_edge) || !phys_dims)
+      return set_err(ctx, ITNBP_ERR_ARG, "null or empty network description");
+    if (dtype != ITNBP_F64 && dtype != ITNBP_F32 && dtype != ITNBP_BF16)
+      return set_err(ctx, ITNBP_ERR_ARG, "unknown dtype");
+    auto* net = new itnbp_net();
+    std::unique_ptr<itnbp_net> guard(net);
+    net->ctx = ctx;
+    net->nv = nv;
+    net->ne = ne;
+    net->dtype = dtype;
+    net->edges.assign(edges, edges + 2 * ne);
+    net->inc_ptr.assign(inc_ptr, inc_ptr + nv + 1);
+    net->inc_edge.assign(inc_edge, inc_edge + (ne > 0 ? inc_ptr[nv] : 0));
+    net->chi.assign(bond_dims, bond_dims + ne);
+    net->phys.assign(phys_dims, phys_dims + nv);
+    if (net->inc_ptr[0] != 0 || net->inc_ptr[nv] != 2 * ne)
+      return set_err(ctx, ITNBP_ERR_ARG, "inc_ptr must start at 0 and end at 2*ne");
+    std::vector<int> seen(ne, 0);
+    for (int e = 0; e < ne; ++e) {
+      const int a = net->edges[2 * e], b = net->edges[2 * e + 1];
+      if (a < 0 || a >= nv || b < 0 || b >= nv || a == b) return set_err(ctx, ITNBP_ERR_ARG, "bad edge");
+      if (net->chi[e] < 1 || net->chi[e] > 64) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "bond dimension must be in 1..64");
+    }
+    for (int v = 0; v < nv; ++v) {
+      const int k = net->inc_ptr[v + 1] - net->inc_ptr[v];
+      if (k < 0 || k > ITNBP_MAXK) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "vertex degree > 6 is not supported");
+      if (net->phys[v] < 1 || net->phys[v] > 8) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "physical dimension must be in 1..8");
+      for (int a = net->inc_ptr[v]; a < net->inc_ptr[v + 1]; ++a) {
+        const int e = net->inc_edge[a];
+        if (e < 0 || e >= ne) return set_err(ctx, ITNBP_ERR_ARG, "incident edge out of range");
+        if (net->edges[2 * e] != v && net->edges[2 * e + 1] != v)
+          return set_err(ctx, ITNBP_ERR_ARG, "incident edge does not touch its vertex");
+        seen[e]++;
+      }
+    }
+    for (int e = 0; e < ne; ++e)
+      if (seen[e] != 2) return set_err(ctx, ITNBP_ERR_ARG, "every edge must be incident to both endpoints once");
+    // messages
+    net->msg_off.resize(2 * ne);
+    long long mo = 0;
+    for (int m = 0; m < 2 * ne; ++m) {
+      net->msg_off[m] = mo;
+      mo += (long long)net->chi[m / 2] * net->chi[m / 2];
+      net->max_chi = std::max(net->max_chi, net->chi[m / 2]);
+    }
+    net->msg_total = mo;
+    net->msg_set.assign(2 * ne, 0);
+    // vertices
+    net->psi_off.resize(nv);
+    net->vdesc.resize(nv);
+    long long po = 0;
+    for (int v = 0; v < nv; ++v) {
+      VDesc& V = net->vdesc[v];
+      memset(&V, 0, sizeof(V));
+      V.k = net->inc_ptr[v + 1] - net->inc_ptr[v];
+      net->max_k = std::max(net->max_k, V.k);
+      V.dims[0] = net->phys[v];
+      long long D = V.dims[0];
+      for (int a = 1; a <= V.k; ++a) {
+        const int e = net->inc_edge[net->inc_ptr[v] + a - 1];
+        V.dims[a] = net->chi[e];
+        D *= V.dims[a];
+        V.in_msg[a] = net->msg_off[msg_id_into(net, e, v)];
+      }
+      if (D > (1LL << 30)) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "site tensor larger than 2^30 elements");
+      V.D = (int)D;
+      V.psi_off = po;
+      net->psi_off[v] = po;
+      po += D;
+    }
+    net->psi_total = po;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (hipMalloc(&net->d_psi, std::max<size_t>(16, po * dtype_size(dtype))) != hipSuccess ||
+        hipMalloc((void**)&net->d_msg[0], std::max<size_t>(16, mo * 8)) != hipSuccess ||
+        hipMalloc((void**)&net->d_msg[1], std::max<size_t>(16, mo * 8)) != hipSuccess ||
+        hipMalloc((void**)&net->d_vdesc, sizeof(VDesc) * nv) != hipSuccess) {
+      (void)hipGetLastError();
+      itnbp_net_destroy(guard.release());
+      return set_err(ctx, ITNBP_ERR_NOMEM, "device allocation for the network failed");
+    }
+    HIPCHK(ctx, hipMemcpyAsync(net->d_vdesc, net->vdesc.data(), sizeof(VDesc) * nv, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(net->d_psi, 0, po * dtype_size(dtype), ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(net->d_msg[0], 0, mo * 8, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *out = guard.release();
+    return ITNBP_OK;
+  });
+}
+
+int itnbp_net_destroy(itnbp_net* net) {
+  if (!net) return ITNBP_OK;
+  if (net->ctx) {
+    (void)hipSetDevice(net->ctx->device);
+    (void)hipStreamSynchronize(net->ctx->stream);
+  }
+  if (net->d_psi) (void)hipFree(net->d_psi);
+  if (net->d_psi64) (void)hipFree(net->d_psi64);
+  for (int i = 0; i < 2; ++i) {
+    if (net->d_msg[i]) (void)hipFree(net->d_msg[i]);
+    if (net->d_msg32[i]) (void)hipFree(net->d_msg32[i]);
+  }
+  if (net->d_vdesc) (void)hipFree(net->d_vdesc);
+  free_buf(net->scratch);
+  free_buf(net->tasks);
+  free_buf(net->stats);
+  free_buf(net->partials);
+  free_buf(net->red);
+  free_buf(net->aux);
+  release_fast_state(net);
+  delete net;
+  return ITNBP_OK;
+}
+
+int itnbp_net_copy(itnbp_net* src, itnbp_net** out) {
+  if (!src || !out) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = src->ctx;
+  RETCHK(itnbp_net_create(ctx, src->nv, src->ne, src->edges.data(), src->inc_ptr.data(),
+                          src->inc_edge.data(), src->chi.data(), src->phys.data(), src->dtype, out));
+  itnbp_net* dst = *out;
+  HIPCHK(ctx, hipMemcpyAsync(dst->d_psi, src->d_psi, src->psi_total * dtype_size(src->dtype),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dst->d_msg[0], src->d_msg[src->cur], src->msg_total * 8,
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  dst->cur = 0;
+  dst->msg_set = src->msg_set;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ITNBP_OK;
+}
+
+int64_t itnbp_net_psi_numel(itnbp_net* net) { return net ? net->psi_total : -1; }
+int64_t itnbp_net_msg_numel(itnbp_net* net) { return net ? net->msg_total : -1; }
+
+int itnbp_set_sites(itnbp_net* net, const void* data) {
+  if (!net || !data) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  HIPCHK(ctx, hipMemcpyAsync(net->d_psi, data, net->psi_total * dtype_size(net->dtype),
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  invalidate_fast_state(net);
+  return ITNBP_OK;
+}
+
+int itnbp_get_sites(itnbp_net* net, void* data) {
+  if (!net || !data) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  HIPCHK(ctx, hipMemcpyAsync(data, net->d_psi, net->psi_total * dtype_size(net->dtype),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ITNBP_OK;
+}
+
+int itnbp_set_site(itnbp_net* net, int v, const void* data) {
+  if (!net || !data || v < 0 || v >= net->nv) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  const size_t eb = dtype_size(net->dtype);
+  HIPCHK(ctx, hipMemcpyAsync((char*)net->d_psi + net->psi_off[v] * eb, data, net->vdesc[v].D * eb,
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  invalidate_fast_state(net);
+  return ITNBP_OK;
+}
+
+int itnbp_set_messages(itnbp_net* net, const double* data) {
+  if (!net || !data) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  HIPCHK(ctx, hipMemcpyAsync(net->d_msg[net->cur], data, net->msg_total * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::fill(net->msg_set.begin(), net->msg_set.end(), 1);
+  return ITNBP_OK;
+}
+
+int itnbp_get_messages(itnbp_net* net, double* data) {
+  if (!net || !data) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  HIPCHK(ctx, hipMemcpyAsync(data, net->d_msg[net->cur], net->msg_total * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ITNBP_OK;
+}
+
+int itnbp_set_message(itnbp_net* net, int m, const double* data) {
+  if (!net || !data || m < 0 || m >= 2 * net->ne) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  const long long n = (long long)net->chi[m / 2] * net->chi[m / 2];
+  HIPCHK(ctx, hipMemcpyAsync(net->d_msg[net->cur] + net->msg_off[m], data, n * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  net->msg_set[m] = 1;
+  return ITNBP_OK;
+}
+
+int itnbp_get_message(itnbp_net* net, int m, double* data) {
+  if (!net || !data || m < 0 || m >= 2 * net->ne) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  if (!net->msg_set[m]) return set_err(ctx, ITNBP_ERR_STATE, "message " + std::to_string(m) + " is not set");
+  const long long n = (long long)net->chi[m / 2] * net->chi[m / 2];
+  HIPCHK(ctx, hipMemcpyAsync(data, net->d_msg[net->cur] + net->msg_off[m], n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ITNBP_OK;
+}
+
+int itnbp_message_is_set(itnbp_net* net, int m) {
+  if (!net || m < 0 || m >= 2 * net->ne) return 0;
+  return net->msg_set[m] ? 1 : 0;
+}
+
+int itnbp_init_identity_messages(itnbp_net* net) {
+  if (!net) return ITNBP_ERR_ARG;
+  return guarded(net->ctx, [&]() -> int {
+    std::vector<double> h(net->msg_total, 0.0);
+    for (int m = 0; m < 2 * net->ne; ++m) {
+      const int c = net->chi[m / 2];
+      for (int i = 0; i < c; ++i) h[net->msg_off[m] + i + (long long)c * i] = 1.0;
+    }
+    return itnbp_set_messages(net, h.data());
+  });
+}
+
+int itnbp_forest_cover_sequence(itnbp_net* net, int* out) {
+  if (!net || !out) return ITNBP_ERR_ARG;
+  return guarded(net->ctx, [&]() -> int {
+    std::vector<int> seq;
+    RETCHK(default_sequence(net, ITNBP_SCHED_SEQUENTIAL, seq));
+    std::copy(seq.begin(), seq.end(), out);
+    return ITNBP_OK;
+  });
+}
+
+int itnbp_update(itnbp_net* net, const itnbp_update_opts* o, itnbp_update_result* res) {
+  if (!net || !o) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  return guarded(ctx, [&]() -> int {
+    if (o->maxiter < 1) return set_err(ctx, ITNBP_ERR_ARG, "You need to specify a number of iterations for BP!");
+    if (o->schedule != ITNBP_SCHED_SEQUENTIAL && o->schedule != ITNBP_SCHED_PARALLEL)
+      return set_err(ctx, ITNBP_ERR_ARG, "unknown schedule");
+    std::vector<int> seq;
+    if (o->nseq > 0) {
+      if (!o->seq) return set_err(ctx, ITNBP_ERR_ARG, "nseq > 0 with a null seq");
+      seq.assign(o->seq, o->seq + o->nseq);
+    } else {
+      RETCHK(default_sequence(net, o->schedule, seq));
+    }
+    if (seq.empty()) {
+      if (res) *res = itnbp_update_result{0, 0, 0.0, 0.0, 1, 0.0};
+      return ITNBP_OK;
+    }
+    RETCHK(check_sequence(net, seq, o));
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    itnbp_update_result r{};
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    int it = 0, conv = 0;
+    double md = 0, mr = 0;
+    if (o->precision == ITNBP_PREC_F64) {
+      Phase ph{1, net->d_psi, net->dtype, {net->d_msg[0], net->d_msg[1]}};
+      const bool fast = fast_path_available(net, 1, o->schedule);
+      ctx->path = fast ? fast_path_name(net, 1) : "generic-f64";
+      RETCHK(sweep_phase(net, seq, o, ph, o->maxiter, -1.0, true, &it, &md, &mr, &conv, fast));
+      r.iterations = it;
+    } else if (o->precision == ITNBP_PREC_F32 || o->precision == ITNBP_PREC_BF16_F64) {
+      for (int i = 0; i < 2; ++i)
+        if (!net->d_msg32[i]) HIPCHK(ctx, hipMalloc((void**)&net->d_msg32[i], std::max<size_t>(16, net->msg_total * 4)));
+      HIPCHK(ctx, launch_convert_msgs(net->d_msg[net->cur], net->d_msg32[net->cur], net->msg_total, ctx->stream));
+      Phase ph{0, net->d_psi, net->dtype, {net->d_msg32[0], net->d_msg32[1]}};
+      const bool low_only = o->precision == ITNBP_PREC_F32;
+      const double sw2 = low_only ? -1.0 : o->switch_tol * o->switch_tol;
+      const bool fast = fast_path_available(net, 0, o->schedule);
+      ctx->path = fast ? fast_path_name(net, 0) : "generic-f32";
+      RETCHK(sweep_phase(net, seq, o, ph, o->maxiter, sw2, low_only, &it, &md, &mr, &conv, fast));
+      HIPCHK(ctx, launch_convert_msgs_back(net->d_msg32[net->cur], net->d_msg[net->cur], net->msg_total, ctx->stream));
+      r.iterations_low = it;
+      r.iterations = it;
+      if (!low_only) {
+        conv = 0;
+        const int left = o->maxiter - it;
+        if (left > 0) {
+          Phase ph2{1, net->d_psi, net->dtype, {net->d_msg[0], net->d_msg[1]}};
+          const bool fast2 = fast_path_available(net, 1, o->schedule);
+          ctx->path += std::string("+") + (fast2 ? fast_path_name(net, 1) : "generic-f64");
+          int it2 = 0;
+          RETCHK(sweep_phase(net, seq, o, ph2, left, -1.0, true, &it2, &md, &mr, &conv, fast2));
+          r.iterations += it2;
+        }
+      }
+    } else {
+      return set_err(ctx, ITNBP_ERR_ARG, "unknown precision mode");
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    r.device_ms = ms;
+    r.mean_diff = md;
+    r.max_residual = mr;
+    r.converged = conv;
+    for (int m : seq) net->msg_set[m] = 1;
+    if (res) *res = r;
+    return ITNBP_OK;
+  });
+}
+
+int itnbp_updated_messages(itnbp_net* net, int n, const int* ids, int normalize, double* out) {
+  if (!net || n < 0 || (n > 0 && (!ids || !out))) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  return guarded(ctx, [&]() -> int {
+    if (n == 0) return ITNBP_OK;
+    std::vector<OcTask> tasks;
+    long long tot = 0;
+    std::vector<long long> offs;
+    for (int i = 0; i < n; ++i) {
+      if (ids[i] < 0 || ids[i] >= 2 * net->ne) return set_err(ctx, ITNBP_ERR_ARG, "message id out of range");
+      OcTask t = message_task(net, ids[i], i, false, normalize);
+      t.out_off = tot;
+      offs.push_back(tot);
+      tot += (long long)net->chi[ids[i] / 2] * net->chi[ids[i] / 2];
+      tasks.push_back(t);
+    }
+    itnbp_update_opts o{};
+    o.tol = -1;
+    o.res_tol = -1;
+    o.schedule = ITNBP_SCHED_PARALLEL;
+    RETCHK(check_sequence(net, std::vector<int>(ids, ids + n), &o));
+    RETCHK(ensure_buf(ctx, net->aux, tot * 8));
+    RETCHK(ensure_buf(ctx, net->stats, sizeof(MsgStat) * n));
+    RETCHK(run_tasks(net, tasks, 1, net->d_psi, net->dtype, net->d_msg[net->cur], net->aux.p,
+                     (MsgStat*)net->stats.p, net->tasks));
+    HIPCHK(ctx, hipMemcpyAsync(out, net->aux.p, tot * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return ITNBP_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// default schedules
+// ---------------------------------------------------------------------------
+namespace {
+
+int default_sequence(const itnbp_net* net, int schedule, std::vector<int>& seq) {
+  seq.clear();
+  const int ne = net->ne, nv = net->nv;
+  if (schedule == ITNBP_SCHED_PARALLEL) {
+    // vcat(edges(g), reverse.(edges(g)))  (edge_sequences.jl:49-51)
+    for (int e = 0; e < ne; ++e) seq.push_back(2 * e);
+    for (int e = 0; e < ne; ++e) seq.push_back(2 * e + 1);
+    return ITNBP_OK;
+  }
+  // forest cover: BFS spanning forests of the edges left over, each tree
+  // rooted at its lowest vertex; post-order DFS edges then their reverses
+  // (edge_sequences.jl:32-47)
+  std::vector<char> remaining(ne, 1);
+  int left = ne;
+  while (left > 0) {
+    std::vector<std::vector<std::pair<int, int>>> adj(nv);  // (neighbour, edge)
+    for (int e = 0; e < ne; ++e)
+      if (remaining[e]) {
+        const int a = net->edges[2 * e], b = net->edges[2 * e + 1];
+        adj[a].push_back({b, e});
+        adj[b].push_back({a, e});
+      }
+    for (auto& l : adj) std::sort(l.begin(), l.end());
+    std::vector<char> seen(nv, 0);
+    std::vector<std::vector<std::pair<int, int>>> children(nv);  // (child, edge)
+    std::vector<int> roots;
+    std::vector<char> in_forest(ne, 0);
+    for (int r = 0; r < nv; ++r) {
+      if (seen[r] || adj[r].empty()) continue;
+      seen[r] = 1;
+      roots.push_back(r);
+      std::deque<int> q{r};
+      while (!q.empty()) {
+        const int x = q.front();
+        q.pop_front();
+        for (auto& ye : adj[x]) {
+          if (!seen[ye.first]) {
+            seen[ye.first] = 1;
+            children[x].push_back(ye);
+            in_forest[ye.second] = 1;
+            q.push_back(ye.first);
+          }
+        }
+      }
+    }
+    for (int r : roots) {
+      std::vector<int> post;  // directed ids child -> parent
+      // iterative post-order DFS
+      std::vector<std::pair<int, size_t>> st{{r, 0}};
+      std::vector<int> parent_edge(nv, -1);
+      while (!st.empty()) {
+        auto& top = st.back();
+        const int x = top.first;
+        if (top.second < children[x].size()) {
+          auto ce = children[x][top.second++];
+          parent_edge[ce.first] = ce.second;
+          st.push_back({ce.first, 0});
+        } else {
+          st.pop_back();
+          if (parent_edge[x] >= 0) {
+            const int e = parent_edge[x];
+            const int parent = net->edges[2 * e] == x ? net->edges[2 * e + 1] : net->edges[2 * e];
+            post.push_back(msg_id_into(net, e, parent));
+          }
+        }
+      }
+      for (int m : post) seq.push_back(m);
+      for (int i = (int)post.size() - 1; i >= 0; --i) seq.push_back(post[i] ^ 1);
+    }
+    for (int e = 0; e < ne; ++e)
+      if (in_forest[e]) {
+        remaining[e] = 0;
+        --left;
+      }
+  }
+  return ITNBP_OK;
+}
+
+}  // namespace

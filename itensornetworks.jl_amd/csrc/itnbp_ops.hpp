@@ -1,0 +1,51 @@
+// Host-side helpers shared by the libitnbp translation units.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "itnbp_internal.hpp"
+
+int itnbp_fail(itnbp_ctx* ctx, int code, const std::string& msg);
+int ensure_buf(itnbp_ctx* ctx, DevBuf& b, size_t bytes);
+void free_buf(DevBuf& b);
+int upload(itnbp_ctx* ctx, DevBuf& b, const void* src, size_t bytes);
+
+int msg_id_into(const itnbp_net* net, int e, int to);
+int msg_src(const itnbp_net* net, int m);
+int msg_dst(const itnbp_net* net, int m);
+int axis_of(const itnbp_net* net, int v, int e);
+int all_bonds_mask(const itnbp_net* net, int v);
+OcTask message_task(const itnbp_net* net, int m, int slot, bool with_old, int normalize);
+
+int plan_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, TaskPlan& plan);
+int upload_plan(itnbp_net* net, const TaskPlan& plan, DevBuf& taskbuf);
+int launch_plan(itnbp_net* net, const TaskPlan& plan, size_t g0, size_t g1, const OcTask* d_tasks,
+                const void* psi, int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats);
+// run open-contraction tasks; msg_in/out in the accumulator precision
+int run_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, const void* psi,
+              int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats, DevBuf& taskbuf);
+
+// ---- specialised sweep kernels (k_fast*.hip) -------------------------------
+// true when the specialised parallel-sweep pipeline handles this network
+bool fast_path_available(itnbp_net* net, int acc_f64, int schedule);
+const char* fast_path_name(itnbp_net* net, int acc_f64);
+// one Jacobi sweep over seq from msg_in into msg_out; *done=false if it declined
+int fast_parallel_sweep(itnbp_net* net, const std::vector<int>& seq, int acc_f64, const void* psi,
+                        int psi_dtype, void* msg_in, void* msg_out, MsgStat* d_stats, int normalize,
+                        bool* done);
+void release_fast_state(itnbp_net* net);
+void invalidate_fast_state(itnbp_net* net);
+
+#define ITNBP_RETCHK(call)          \
+  do {                              \
+    int _r = (call);                \
+    if (_r != ITNBP_OK) return _r;  \
+  } while (0)
+
+#define ITNBP_HIPCHK(ctx, call)                                                                   \
+  do {                                                                                            \
+    hipError_t _e = (call);                                                                       \
+    if (_e != hipSuccess)                                                                         \
+      return itnbp_fail((ctx), ITNBP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)

@@ -1,0 +1,15 @@
+"""itnbp: MI355X-native belief propagation on the norm network of a tensor
+network state, with the BeliefPropagationCache / update / environment /
+expect / apply API of ITensorNetworks.jl.
+
+Every compute call goes through libitnbp.so (hand-written HIP kernels for
+gfx950); there is no CPU fallback.
+"""
+from ._lib import ITNBPError, ITNBPLibraryMissing, Context, default_context, load  # noqa: F401
+from .network import (ITensorNetwork, NamedGraph, comb_tree, heavy_hex, named_grid,  # noqa: F401
+                      path_graph, productstate, random_tensornetwork)
+from .cache import (BeliefPropagationCache, QuadraticFormNetwork, OPS, default_edge_sequence,  # noqa: F401
+                    edge_scalars, environment, expect, expect2_on_cache, expect_on_cache,
+                    identity_messages, initialize_cache, logscalar, message_diff, norm_sqr_network,
+                    normalize, rdm, region_scalars, rescale, rescale_, scalar, update, update_,
+                    updated_message, vertex_scalars)

@@ -1,0 +1,540 @@
+"""BeliefPropagationCache and the BP API of ITensorNetworks.jl, on the MI355X
+library.
+
+Mirrors (paths relative to the ITensorNetworks.jl repository):
+  BeliefPropagationCache / copy / messages     src/caches/beliefpropagationcache.jl:13-47
+  initialize_cache                             src/initialize_cache.jl:10-29
+  identity_messages                            src/formnetworks/quadraticformnetwork.jl:96-124
+  update / updated_message / message_diff      src/caches/abstractbeliefpropagationcache.jl:32-36, 225-337
+  environment                                  src/caches/beliefpropagationcache.jl:100-105, src/environment.jl:6-49
+  region_scalar / logscalar / scalar           src/caches/beliefpropagationcache.jl:107-119,
+                                               src/caches/abstractbeliefpropagationcache.jl:397-412
+  rescale / normalize                          src/caches/beliefpropagationcache.jl:121-139,
+                                               src/caches/abstractbeliefpropagationcache.jl:339-395,
+                                               src/normalize.jl:125-142
+  expect                                       src/expect.jl:5-109
+
+The functional API returns new caches (a device-side copy); ``update_`` and the
+``*_`` setters mutate in place, like the reference's ``!`` variants.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .network import ITensorNetwork
+
+# ITensors op(name, s) for S=1/2 sites, O[s', s]
+OPS = {
+    "Sz": np.array([[0.5, 0.0], [0.0, -0.5]]),
+    "Sx": np.array([[0.0, 0.5], [0.5, 0.0]]),
+    "X": np.array([[0.0, 1.0], [1.0, 0.0]]),
+    "Z": np.array([[1.0, 0.0], [0.0, -1.0]]),
+    "N": np.array([[0.0, 0.0], [0.0, 1.0]]),
+    "Id": np.eye(2),
+    "I": np.eye(2),
+}
+
+PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16_f64": L.PREC_BF16_F64}
+
+
+def op_matrix(op, d=2):
+    if isinstance(op, str):
+        if op not in OPS:
+            raise ValueError(f"operator {op!r} is not available (real S=1/2 operators only)")
+        m = OPS[op]
+    else:
+        m = np.asarray(op, dtype=np.float64)
+    if m.shape != (d, d):
+        raise ValueError(f"operator shape {m.shape} does not match the physical dimension {d}")
+    return m
+
+
+def bf16_round(a):
+    """Round to the nearest bfloat16 (ties to even); returns (float32 values, uint16 bits)."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    bits = r.astype(np.uint16)
+    back = (bits.astype(np.uint32) << 16).view(np.float32)
+    return back, bits
+
+
+class QuadraticFormNetwork:
+    """<psi|I|psi> (src/formnetworks/quadraticformnetwork.jl:75-89): the ket is
+    ``psi``, the bra is conj(psi) with every index primed, the operator layer
+    is the identity.  Only the ket is stored; the device builds the bra on the
+    fly and elides the identity."""
+
+    def __init__(self, ket: ITensorNetwork):
+        self.ket = ket
+
+    def ket_network(self):
+        return self.ket
+
+    @property
+    def graph(self):
+        return self.ket.graph
+
+    def is_tree(self):
+        return self.ket.graph.is_tree()
+
+
+def norm_sqr_network(psi):
+    return QuadraticFormNetwork(psi)
+
+
+def _flat_sites(psi: ITensorNetwork, storage):
+    parts = [np.asarray(psi[v]).reshape(-1, order="F") for v in psi.graph.vertices]
+    flat = np.concatenate(parts) if parts else np.zeros(0)
+    if storage == L.F64:
+        return np.ascontiguousarray(flat, dtype=np.float64)
+    if storage == L.F32:
+        return np.ascontiguousarray(flat, dtype=np.float32)
+    return bf16_round(flat)[1]
+
+
+def _unflat_sites(psi_like: ITensorNetwork, flat, storage):
+    if storage == L.BF16:
+        flat = (flat.astype(np.uint32) << 16).view(np.float32)
+    out = {}
+    off = 0
+    for v in psi_like.graph.vertices:
+        shape = psi_like[v].shape
+        n = int(np.prod(shape))
+        out[v] = np.array(flat[off:off + n]).reshape(shape, order="F")
+        off += n
+    return ITensorNetwork(psi_like.graph, out)
+
+
+class BeliefPropagationCache:
+    """BP cache of the norm network of ``psi`` held on one GPU.
+
+    ``storage``: "f64", "f32" or "bf16" (site tensors rounded to bfloat16);
+    defaults to the tensors' dtype.
+    """
+
+    def __init__(self, tn, messages=None, *, storage=None, ctx=None, _handle=None):
+        qf = tn if isinstance(tn, QuadraticFormNetwork) else QuadraticFormNetwork(tn)
+        self.qf = qf
+        self.ctx = ctx or L.default_context()
+        self.lib = self.ctx.lib
+        psi = qf.ket
+        g = psi.graph
+        if storage is None:
+            storage = "f32" if psi.dtype == np.float32 else "f64"
+        if np.iscomplexobj(np.zeros(0, dtype=psi.dtype)):
+            raise ValueError("complex site tensors are not supported by this build")
+        self.storage = {"f64": L.F64, "f32": L.F32, "bf16": L.BF16}[storage]
+        self.storage_name = storage
+        self._host_psi = psi
+        self._psi_dirty = False
+        self.last_update = None
+        if _handle is not None:
+            self.h = _handle
+            return
+        edges, inc_ptr, inc_edge = g.csr()
+        chis = np.array([psi.bond_dim(a, b) for (a, b) in g.edges], dtype=np.int32)
+        phys = np.array([psi.phys_dim(v) for v in g.vertices], dtype=np.int32)
+        h = C.c_void_p()
+        self.ctx.check(self.lib.itnbp_net_create(
+            self.ctx.h, g.nv, g.ne, L.iptr(edges) if g.ne else None, L.iptr(inc_ptr),
+            L.iptr(inc_edge) if g.ne else None, L.iptr(chis) if g.ne else None, L.iptr(phys),
+            self.storage, C.byref(h)))
+        self.h = h
+        flat = _flat_sites(psi, self.storage)
+        self.ctx.check(self.lib.itnbp_set_sites(self.h, L.vptr(flat)))
+        if self.storage == L.BF16:
+            self._host_psi = _unflat_sites(psi, flat, L.BF16)
+        if messages:
+            self.set_messages_(messages)
+
+    # -- lifetime ------------------------------------------------------------
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.itnbp_net_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def copy(self):
+        """Deep copy (src/caches/beliefpropagationcache.jl:43-47)."""
+        h = C.c_void_p()
+        self.ctx.check(self.lib.itnbp_net_copy(self.h, C.byref(h)))
+        c = BeliefPropagationCache.__new__(BeliefPropagationCache)
+        c.qf = QuadraticFormNetwork(self.tensornetwork())
+        c.ctx, c.lib, c.h = self.ctx, self.lib, h
+        c.storage, c.storage_name = self.storage, self.storage_name
+        c._host_psi = c.qf.ket
+        c._psi_dirty = False
+        c.last_update = self.last_update
+        return c
+
+    # -- structure -------------------------------------------------------------
+    @property
+    def graph(self):
+        return self.qf.ket.graph
+
+    def tensornetwork(self) -> ITensorNetwork:
+        """The ket network (site tensors as held on the device)."""
+        if self._psi_dirty:
+            n = int(self.lib.itnbp_net_psi_numel(self.h))
+            dt = {L.F64: np.float64, L.F32: np.float32, L.BF16: np.uint16}[self.storage]
+            flat = np.empty(n, dtype=dt)
+            self.ctx.check(self.lib.itnbp_get_sites(self.h, L.vptr(flat)))
+            self._host_psi = _unflat_sites(self._host_psi, flat, self.storage)
+            self.qf = QuadraticFormNetwork(self._host_psi)
+            self._psi_dirty = False
+        return self._host_psi
+
+    def __getitem__(self, v):
+        return self.tensornetwork()[v]
+
+    def __setitem__(self, v, t):
+        """bpc[v] = t: replaces the ket tensor of v (the bra follows as its conjugate)."""
+        psi = self.tensornetwork()
+        psi[v] = t
+        i = self.graph.index[v]
+        arr = np.asarray(t).reshape(-1, order="F")
+        if self.storage == L.F64:
+            buf = np.ascontiguousarray(arr, dtype=np.float64)
+        elif self.storage == L.F32:
+            buf = np.ascontiguousarray(arr, dtype=np.float32)
+        else:
+            back, buf = bf16_round(arr)
+            psi[v] = back.reshape(psi[v].shape, order="F")
+        self.ctx.check(self.lib.itnbp_set_site(self.h, i, L.vptr(buf)))
+
+    def scalartype(self):
+        return {L.F64: np.float64, L.F32: np.float32, L.BF16: np.float32}[self.storage]
+
+    # -- messages --------------------------------------------------------------
+    def _mid(self, e):
+        a, b = e
+        return self.graph.message_id(a, b)
+
+    def has_message(self, e):
+        return bool(self.lib.itnbp_message_is_set(self.h, self._mid(e)))
+
+    def message(self, e):
+        """message(bpc, e) (abstractbeliefpropagationcache.jl:173-175): M[b_ket, b_bra]."""
+        a, b = e
+        chi = self.qf.ket.bond_dim(a, b)
+        out = np.empty(chi * chi, dtype=np.float64)
+        self.ctx.check(self.lib.itnbp_get_message(self.h, self._mid(e), L.dptr(out)))
+        return out.reshape(chi, chi, order="F").astype(self.scalartype())
+
+    def messages(self):
+        """Dictionary of every set message."""
+        n = int(self.lib.itnbp_net_msg_numel(self.h))
+        flat = np.empty(max(n, 1), dtype=np.float64)
+        self.ctx.check(self.lib.itnbp_get_messages(self.h, L.dptr(flat)))
+        out = {}
+        off = 0
+        g = self.graph
+        for m in range(2 * g.ne):
+            a, b = g.message_edge(m)
+            chi = self.qf.ket.bond_dim(a, b)
+            if self.lib.itnbp_message_is_set(self.h, m):
+                out[(a, b)] = flat[off:off + chi * chi].reshape(chi, chi, order="F").astype(self.scalartype())
+            off += chi * chi
+        return out
+
+    def set_message_(self, e, m):
+        a, b = e
+        chi = self.qf.ket.bond_dim(a, b)
+        m = np.asarray(m, dtype=np.float64)
+        if m.shape != (chi, chi):
+            raise ValueError(f"message shape {m.shape} != {(chi, chi)}")
+        buf = np.ascontiguousarray(m.reshape(-1, order="F"))
+        self.ctx.check(self.lib.itnbp_set_message(self.h, self._mid(e), L.dptr(buf)))
+        return self
+
+    def set_messages_(self, msgs):
+        for e, m in msgs.items():
+            self.set_message_(e, m)
+        return self
+
+    def set_message(self, e, m):
+        return self.copy().set_message_(e, m)
+
+    def set_messages(self, msgs):
+        return self.copy().set_messages_(msgs)
+
+    def update_factor(self, v, t):
+        c = self.copy()
+        c[v] = t
+        return c
+
+    def default_bp_maxiter(self):
+        """1 on trees, nothing otherwise (abstractbeliefpropagationcache.jl:38-41)."""
+        return 1 if self.graph.is_tree() else None
+
+
+# ----------------------------------------------------------------------------
+def identity_messages(qf):
+    """delta(b_bra, b_ket) on every directed edge (quadraticformnetwork.jl:96-124)."""
+    psi = qf.ket if isinstance(qf, QuadraticFormNetwork) else qf
+    g = psi.graph
+    out = {}
+    for (a, b) in g.edges:
+        chi = psi.bond_dim(a, b)
+        out[(a, b)] = np.eye(chi)
+        out[(b, a)] = np.eye(chi)
+    return out
+
+
+def initialize_cache(tn, alg="bp", *, messages=None, storage=None, ctx=None):
+    """initialize_cache(Algorithm"bp", ::QuadraticFormNetwork) (initialize_cache.jl:14-29):
+    empty messages on trees, identity messages on loopy graphs."""
+    if alg != "bp":
+        raise ValueError(f"unknown cache algorithm {alg!r}")
+    qf = tn if isinstance(tn, QuadraticFormNetwork) else QuadraticFormNetwork(tn)
+    bpc = BeliefPropagationCache(qf, storage=storage, ctx=ctx)
+    if messages is None:
+        if not qf.is_tree():
+            bpc.ctx.check(bpc.lib.itnbp_init_identity_messages(bpc.h))
+    else:
+        bpc.set_messages_(messages)
+    return bpc
+
+
+def default_edge_sequence(bpc):
+    """Forest-cover sequential schedule (edge_sequences.jl:9-47) as directed edges."""
+    g = bpc.graph
+    out = np.empty(2 * g.ne, dtype=np.int32)
+    bpc.ctx.check(bpc.lib.itnbp_forest_cover_sequence(bpc.h, L.iptr(out)))
+    return [g.message_edge(int(m)) for m in out]
+
+
+def _schedule(bpc, edge_sequence):
+    g = bpc.graph
+    if edge_sequence is None or edge_sequence == "forest_cover":
+        return L.SCHED_SEQUENTIAL, None
+    if edge_sequence == "parallel":
+        return L.SCHED_PARALLEL, None
+    seq = list(edge_sequence)
+    if seq and isinstance(seq[0], (list,)):
+        ids = [g.message_id(*e) for grp in seq for e in grp]
+        return L.SCHED_PARALLEL, np.array(ids, dtype=np.int32)
+    ids = [g.message_id(*e) for e in seq]
+    return L.SCHED_SEQUENTIAL, np.array(ids, dtype=np.int32)
+
+
+def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, verbose=False,
+            message_update_alg=None, precision="f64", res_tol=None, switch_tol=1e-4):
+    """In-place update(bpc; alg="bp", maxiter, tol, edge_sequence, verbose,
+    message_update_alg) (abstractbeliefpropagationcache.jl:313-337).
+
+    ``edge_sequence``: None / "forest_cover" (sequential, the default), "parallel",
+    a list of directed edges (sequential) or a list of edge groups (parallel).
+    Extensions: ``res_tol`` stops on max ||new-old||/||new||; ``precision`` is
+    "f64", "f32" or "bf16_f64" (BF16 MFMA sweeps until the residual drops below
+    ``switch_tol``, then FP64)."""
+    if alg != "bp":
+        raise ValueError(f"unknown update algorithm {alg!r}")
+    if isinstance(maxiter, str):
+        maxiter = bpc.default_bp_maxiter()
+    if maxiter is None:
+        raise ValueError("You need to specify a number of iterations for BP!")
+    normalize = True
+    if message_update_alg is not None:
+        mua = dict(message_update_alg)
+        if mua.get("alg", "contract") != "contract":
+            raise ValueError("only the 'contract' message update is available")
+        normalize = bool(mua.get("normalize", True))
+    sched, seq = _schedule(bpc, edge_sequence)
+    o = L.UpdateOpts()
+    o.schedule = sched
+    o.maxiter = int(maxiter)
+    o.tol = -1.0 if tol is None else float(tol)
+    o.res_tol = -1.0 if res_tol is None else float(res_tol)
+    o.precision = PRECISIONS[precision]
+    o.switch_tol = float(switch_tol)
+    o.normalize = int(normalize)
+    o.verbose = int(bool(verbose))
+    if seq is not None:
+        o.nseq = len(seq)
+        o.seq = L.iptr(seq)
+    else:
+        o.nseq = 0
+        o.seq = None
+    r = L.UpdateResult()
+    bpc.ctx.check(bpc.lib.itnbp_update(bpc.h, C.byref(o), C.byref(r)))
+    bpc.last_update = dict(iterations=r.iterations, iterations_low=r.iterations_low,
+                           mean_diff=r.mean_diff, max_residual=r.max_residual,
+                           converged=bool(r.converged), device_ms=r.device_ms,
+                           path=bpc.ctx.last_path())
+    return bpc
+
+
+def update(bpc, alg="bp", **kwargs):
+    """Functional update: returns a new cache (abstractbeliefpropagationcache.jl:331-337)."""
+    return update_(bpc.copy(), alg, **kwargs)
+
+
+def updated_message(bpc, e, *, normalize=True):
+    """updated_message(bpc, e) (abstractbeliefpropagationcache.jl:225-239, 254-261)."""
+    a, b = e
+    chi = bpc.qf.ket.bond_dim(a, b)
+    ids = np.array([bpc.graph.message_id(a, b)], dtype=np.int32)
+    out = np.empty(chi * chi, dtype=np.float64)
+    bpc.ctx.check(bpc.lib.itnbp_updated_messages(bpc.h, 1, L.iptr(ids), int(normalize), L.dptr(out)))
+    return out.reshape(chi, chi, order="F").astype(bpc.scalartype())
+
+
+def message_diff(a, b):
+    """1 - |<a/|a|, b/|b|>|^2 (abstractbeliefpropagationcache.jl:32-36)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    f = abs(np.vdot(a / np.linalg.norm(a), b / np.linalg.norm(b))) ** 2
+    return 1.0 - f
+
+
+# ----------------------------------------------------------------------------
+def environment(bpc, verts):
+    """environment(bpc, verts) for whole partitions (beliefpropagationcache.jl:100-105):
+    the incoming messages into the set ``verts`` from outside it, as a dict
+    {(w, v): M_{w->v}}.  For one or two neighbouring vertices this is the
+    product environment ``apply`` needs."""
+    g = bpc.graph
+    vs = set(verts)
+    out = {}
+    for v in verts:
+        for w in g.incident(v):
+            if w not in vs:
+                out[(w, v)] = bpc.message((w, v))
+    return out
+
+
+def rdm(bpc, verts):
+    """BP reduced density matrix of one vertex (d x d, rho[s, s']) or of two
+    neighbouring vertices ((d_u d_v)^2, first site fastest), unnormalised."""
+    g = bpc.graph
+    if len(verts) == 1:
+        v = verts[0]
+        d = bpc.qf.ket.phys_dim(v)
+        out = np.empty(d * d, dtype=np.float64)
+        ids = np.array([g.index[v]], dtype=np.int32)
+        bpc.ctx.check(bpc.lib.itnbp_rdm1(bpc.h, 1, L.iptr(ids), L.dptr(out)))
+        return out.reshape(d, d, order="F")
+    if len(verts) == 2:
+        u, v = verts
+        e = g.edge_id(u, v)
+        du, dv = bpc.qf.ket.phys_dim(g.edges[e][0]), bpc.qf.ket.phys_dim(g.edges[e][1])
+        out = np.empty((du * dv) ** 2, dtype=np.float64)
+        ids = np.array([e], dtype=np.int32)
+        bpc.ctx.check(bpc.lib.itnbp_rdm2(bpc.h, 1, L.iptr(ids), L.dptr(out)))
+        r = out.reshape(du * dv, du * dv, order="F")
+        if g.edges[e] != (u, v):  # reorder to (u, v) with u fastest
+            r = r.reshape(du, dv, du, dv, order="F").transpose(1, 0, 3, 2).reshape(du * dv, du * dv, order="F")
+        return r
+    raise ValueError("rdm supports one vertex or one neighbouring pair")
+
+
+def expect_on_cache(bpc, op, vertices=None):
+    """<O_v> = tr(O rho_v)/tr(rho_v) for every vertex (expect.jl:5-19)."""
+    g = bpc.graph
+    vertices = list(g.vertices) if vertices is None else list(vertices)
+    ids = np.array([g.index[v] for v in vertices], dtype=np.int32)
+    mats = np.concatenate([op_matrix(op, bpc.qf.ket.phys_dim(v)).reshape(-1, order="F") for v in vertices])
+    out = np.empty(len(vertices), dtype=np.float64)
+    bpc.ctx.check(bpc.lib.itnbp_expect1(bpc.h, len(vertices), L.iptr(ids), L.dptr(mats), L.dptr(out)))
+    return dict(zip(vertices, out))
+
+
+def expect2_on_cache(bpc, op2, edges=None):
+    """<O_uv> for neighbouring pairs from the BP pair environment (the two-site RDM
+    pattern of test/test_belief_propagation.jl:64-91); ``op2`` is a (d_u d_v)^2
+    matrix O[(s_u', s_v'), (s_u, s_v)] with s_u fastest, or a callable (u, v) -> matrix."""
+    g = bpc.graph
+    edges = list(g.edges) if edges is None else list(edges)
+    ids, mats = [], []
+    for (u, v) in edges:
+        e = g.edge_id(u, v)
+        m = op2(u, v) if callable(op2) else np.asarray(op2, dtype=np.float64)
+        if g.edges[e] != (u, v):
+            du, dv = bpc.qf.ket.phys_dim(u), bpc.qf.ket.phys_dim(v)
+            m = m.reshape(du, dv, du, dv, order="F").transpose(1, 0, 3, 2).reshape(du * dv, du * dv, order="F")
+        ids.append(e)
+        mats.append(np.asarray(m, dtype=np.float64).reshape(-1, order="F"))
+    ids = np.array(ids, dtype=np.int32)
+    mats = np.concatenate(mats)
+    out = np.empty(len(edges), dtype=np.float64)
+    bpc.ctx.check(bpc.lib.itnbp_expect2(bpc.h, len(edges), L.iptr(ids), L.dptr(mats), L.dptr(out)))
+    return dict(zip(edges, out))
+
+
+def expect(psi, op, vertices=None, *, alg="bp", cache=None, update_cache=None,
+           cache_update_kwargs=None, cache_construction_kwargs=None):
+    """expect(psi, op; alg="bp", cache!, update_cache, cache_update_kwargs,
+    cache_construction_kwargs) (expect.jl:21-41, 102-109).  ``cache`` is a
+    one-element list used like the reference's ``cache!::Ref``."""
+    if alg != "bp":
+        raise ValueError("only alg='bp' runs on the device; exact contraction is not part of this library")
+    if update_cache is None:
+        update_cache = cache is None
+    if cache is None:
+        cache = [initialize_cache(psi, **(cache_construction_kwargs or {}))]
+    if update_cache:
+        cache[0] = update_(cache[0], **(cache_update_kwargs or {}))
+    return expect_on_cache(cache[0], op, vertices)
+
+
+def region_scalars(bpc):
+    g = bpc.graph
+    vs = np.empty(g.nv, dtype=np.float64)
+    es = np.empty(max(g.ne, 1), dtype=np.float64)
+    bpc.ctx.check(bpc.lib.itnbp_region_scalars(bpc.h, L.dptr(vs), L.dptr(es)))
+    return vs, es[:g.ne]
+
+
+def vertex_scalars(bpc):
+    return region_scalars(bpc)[0]
+
+
+def edge_scalars(bpc):
+    return region_scalars(bpc)[1]
+
+
+def logscalar(bpc):
+    """logscalar(bpc) (abstractbeliefpropagationcache.jl:397-408)."""
+    re, im, ninf = C.c_double(), C.c_double(), C.c_int()
+    bpc.ctx.check(bpc.lib.itnbp_logscalar(bpc.h, C.byref(re), C.byref(im), C.byref(ninf)))
+    if ninf.value:
+        return -np.inf
+    return complex(re.value, im.value) if im.value != 0 else re.value
+
+
+def scalar(bpc):
+    return np.exp(logscalar(bpc))
+
+
+def rescale_(bpc):
+    """rescale(bpc) = rescale_messages then rescale_partitions of the ket and bra
+    vertices (abstractbeliefpropagationcache.jl:391-395)."""
+    bpc.ctx.check(bpc.lib.itnbp_normalize(bpc.h))
+    bpc._psi_dirty = True
+    return bpc
+
+
+def rescale(bpc):
+    return rescale_(bpc.copy())
+
+
+def normalize(psi, alg="bp", *, cache=None, update_cache=None, cache_update_kwargs=None,
+              cache_construction_kwargs=None):
+    """normalize(psi; alg="bp") (normalize.jl:125-142): returns the rescaled ket network."""
+    if alg != "bp":
+        raise ValueError("only alg='bp' runs on the device")
+    if update_cache is None:
+        update_cache = cache is None
+    if cache is None:
+        cache = [initialize_cache(psi, **(cache_construction_kwargs or {}))]
+    if update_cache:
+        cache[0] = update_(cache[0], **(cache_update_kwargs or {}))
+    cache[0] = rescale_(cache[0])
+    return cache[0].tensornetwork()
