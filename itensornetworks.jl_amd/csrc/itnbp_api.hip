@@ -277,7 +277,8 @@ struct Phase {
 
 int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_opts* o,
                 const Phase& ph, int maxiter, double stop_res2, bool stop_on_tol, int* iters,
-                double* mean_diff, double* max_res, int* converged, bool use_fast) {
+                double* mean_diff, double* max_res, int* converged, bool use_fast_in) {
+  bool use_fast = use_fast_in;
   itnbp_ctx* ctx = net->ctx;
   const int n = (int)seq.size();
   RETCHK(ensure_buf(ctx, net->stats, sizeof(MsgStat) * (size_t)std::max(1, n)));
@@ -292,13 +293,26 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
   } else {
     runs = sequential_runs(net, seq);
   }
+  // the specialised pipeline produces the messages of its bulk vertices; the
+  // generic plan everything else
+  std::vector<char> handled(n, 0);
+  if (use_fast) {
+    const int fr = fast_prepare(net, seq, ph.acc_f64, handled);
+    if (fr == ITNBP_ERR_UNSUPPORTED) {
+      use_fast = false;
+      std::fill(handled.begin(), handled.end(), 0);
+    } else if (fr != ITNBP_OK) {
+      return fr;
+    }
+  }
   // one plan for the whole sweep; run r launches groups [run_groups[r], run_groups[r+1])
   TaskPlan plan;
   std::vector<size_t> run_groups{0};
   for (auto& r : runs) {
     std::vector<OcTask> rt;
     rt.reserve(r.second - r.first);
-    for (int i = r.first; i < r.second; ++i) rt.push_back(message_task(net, seq[i], i, true, o->normalize));
+    for (int i = r.first; i < r.second; ++i)
+      if (!handled[i]) rt.push_back(message_task(net, seq[i], i, true, o->normalize));
     RETCHK(plan_tasks(net, rt, ph.acc_f64, plan));
     run_groups.push_back(plan.groups.size());
   }
@@ -313,13 +327,8 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
       void* nxt = ph.msg[1 - net->cur];
       if (partial)
         HIPCHK(ctx, hipMemcpyAsync(nxt, cur, net->msg_total * eb, hipMemcpyDeviceToDevice, ctx->stream));
-      bool done = false;
-      if (use_fast) {
-        int r = fast_parallel_sweep(net, seq, ph.acc_f64, ph.psi, ph.psi_dtype, cur, nxt, d_stats,
-                                    o->normalize, &done);
-        if (r != ITNBP_OK) return r;
-      }
-      if (!done) RETCHK(launch_plan(net, plan, 0, plan.groups.size(), d_tasks, ph.psi, ph.psi_dtype, cur, nxt, d_stats));
+      if (use_fast) RETCHK(fast_sweep(net, ph.acc_f64, cur, nxt, d_stats, o->normalize));
+      RETCHK(launch_plan(net, plan, 0, plan.groups.size(), d_tasks, ph.psi, ph.psi_dtype, cur, nxt, d_stats));
       net->cur = 1 - net->cur;
     } else {
       void* cur = ph.msg[net->cur];

@@ -127,6 +127,7 @@ struct itnbp_net {
   int cur = 0;
   VDesc* d_vdesc = nullptr;
   DevBuf scratch, tasks, stats, partials, red, aux;
+  void* fast = nullptr;  // state of the specialised pipeline (k_fast.hip)
 };
 
 size_t dtype_size(int dtype);

@@ -30,10 +30,11 @@ int run_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, con
 // true when the specialised parallel-sweep pipeline handles this network
 bool fast_path_available(itnbp_net* net, int acc_f64, int schedule);
 const char* fast_path_name(itnbp_net* net, int acc_f64);
-// one Jacobi sweep over seq from msg_in into msg_out; *done=false if it declined
-int fast_parallel_sweep(itnbp_net* net, const std::vector<int>& seq, int acc_f64, const void* psi,
-                        int psi_dtype, void* msg_in, void* msg_out, MsgStat* d_stats, int normalize,
-                        bool* done);
+// mark the entries of seq the pipeline produces and build its buffers
+int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::vector<char>& handled);
+// launch the pipeline for one Jacobi sweep from msg_in into msg_out
+int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, MsgStat* stats,
+               int normalize);
 void release_fast_state(itnbp_net* net);
 void invalidate_fast_state(itnbp_net* net);
 

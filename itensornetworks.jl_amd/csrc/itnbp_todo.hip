@@ -1,8 +1,5 @@
 #include "itnbp_ops.hpp"
 extern "C" {
-int itnbp_apply_gates(itnbp_net* net, int, const int*, const double*, int, double, int, double*, double*, int, int*) {
-  return itnbp_fail(net ? net->ctx : nullptr, ITNBP_ERR_UNSUPPORTED, "not built yet");
-}
 int itnbp_energy_grad(itnbp_net* net, int, int, const double*, const double*, double*, double*) {
   return itnbp_fail(net ? net->ctx : nullptr, ITNBP_ERR_UNSUPPORTED, "not built yet");
 }

@@ -1,0 +1,733 @@
+// BP-gauged two-site gate application (simple update) on the device.
+//
+// Semantics: ITensors.apply(o, psi; envs, maxdim, cutoff, normalize) with a
+// product environment (src/apply.jl:97-146) -> simple_update_bp (src/apply.jl:33-95):
+//   1. every environment message M (chi x chi, Hermitian) -> sqrt(M) and M^{-1/2}
+//      by a Hermitian eigendecomposition with the eigenvalue truncation
+//      cutoff = 10 eps (map_eigvals, src/apply.jl:9-25);
+//   2. psi_u~ = psi_u with sqrt(M) on every external bond, reshaped to
+//      X_u (external bonds) x (s_u, b_uv), QR X_u = Q_u R_u (src/apply.jl:74-75);
+//   3. Theta = gate * (R_u R_v) over b_uv (src/apply.jl:78), SVD, truncation to
+//      maxdim / cutoff with truncerr = discarded weight (src/apply.jl:81-89);
+//   4. psi_u' = (Q_u with M^{-1/2} on the external bonds) U sqrt(S), likewise
+//      psi_v' with sqrt(S) V^T (src/apply.jl:90-93); optional normalisation
+//      (src/apply.jl:134-137).
+// One workgroup per bond; the bonds of one launch are disjoint.  Small dense
+// linear algebra runs in LDS: parallel (round-robin) two-sided Jacobi for the
+// symmetric eigenproblems, Householder QR, one-sided (Hestenes) Jacobi SVD.
+#include <cmath>
+#include <vector>
+
+#include "itnbp_ops.hpp"
+
+namespace {
+
+constexpr int AP_XY = 8192;   // max E * n (external size x d chi_b) per side
+constexpr int AP_TH = 32;     // max rows/cols of Theta (r d)
+constexpr int AP_ENV = 8;     // max external bond dimension
+constexpr int AP_MAXEXT = 3;  // max external bonds per side (degree 4)
+
+struct ApplyItem {
+  int u, v;      // vertices
+  int ju, jv;    // bond axis at u, v
+  long long gate;  // offset of the (d_u d_v)^2 gate
+  int slot;
+  int pad;
+};
+
+struct ApplyParams {
+  int maxdim;         // <= 0: none (still capped by the bond dimension)
+  double cutoff;      // < 0: none
+  int normalize;
+  double eig_cutoff;  // map_eigvals cutoff (10 eps)
+  int max_keep;       // stride of the singular value output
+};
+
+__device__ inline double wave_sum_d(double v) {
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ double block_sum_d(double v, double* red) {
+  v = wave_sum_d(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+// Round-robin pairing: step st of a cyclic sweep over n (even) indices.
+__device__ inline void rr_pair(int n, int st, int i, int& p, int& q) {
+  // positions 0..n-1; position 0 fixed, others rotate
+  auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + st) % (n - 1); };
+  p = at(i);
+  q = at(n - 1 - i);
+  if (p > q) {
+    int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+// Symmetric eigendecomposition A = V diag(w) V^T, A n x n (row-major, ld n) in
+// LDS, overwritten; parallel two-sided Jacobi.  n <= 64.
+__device__ void jacobi_eigh(double* A, double* V, double* w, int n, double* rot) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n * n; i += blockDim.x) V[i] = (i / n == i % n) ? 1.0 : 0.0;
+  __syncthreads();
+  if (n == 1) {
+    if (tid == 0) w[0] = A[0];
+    __syncthreads();
+    return;
+  }
+  const int ne = n + (n & 1);  // pad to even with a dummy index n
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    // off-diagonal norm
+    double off = 0, diag = 0;
+    for (int i = tid; i < n * n; i += blockDim.x) {
+      const double a = A[i];
+      if (i / n != i % n) off += a * a;
+      else diag += a * a;
+    }
+    __shared__ double redj[8];
+    off = block_sum_d(off, redj);
+    diag = block_sum_d(diag, redj);
+    if (off <= 1e-30 * diag || off == 0.0) break;
+    for (int st = 0; st < ne - 1; ++st) {
+      // rotations
+      for (int i = tid; i < ne / 2; i += blockDim.x) {
+        int p, q;
+        rr_pair(ne, st, i, p, q);
+        double c = 1, s = 0;
+        if (q < n) {
+          const double apq = A[p * n + q];
+          if (apq != 0.0) {
+            const double zeta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+            const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+            c = 1 / sqrt(1 + t * t);
+            s = c * t;
+          }
+        }
+        rot[4 * i] = p;
+        rot[4 * i + 1] = q;
+        rot[4 * i + 2] = c;
+        rot[4 * i + 3] = s;
+      }
+      __syncthreads();
+      // rows: A' = J^T A
+      for (int idx = tid; idx < (ne / 2) * n; idx += blockDim.x) {
+        const int i = idx / n, col = idx % n;
+        const int p = (int)rot[4 * i], q = (int)rot[4 * i + 1];
+        if (q >= n) continue;
+        const double c = rot[4 * i + 2], s = rot[4 * i + 3];
+        const double ap = A[p * n + col], aq = A[q * n + col];
+        A[p * n + col] = c * ap - s * aq;
+        A[q * n + col] = s * ap + c * aq;
+      }
+      __syncthreads();
+      // columns: A'' = A' J, V = V J
+      for (int idx = tid; idx < (ne / 2) * n; idx += blockDim.x) {
+        const int i = idx / n, row = idx % n;
+        const int p = (int)rot[4 * i], q = (int)rot[4 * i + 1];
+        if (q >= n) continue;
+        const double c = rot[4 * i + 2], s = rot[4 * i + 3];
+        const double ap = A[row * n + p], aq = A[row * n + q];
+        A[row * n + p] = c * ap - s * aq;
+        A[row * n + q] = s * ap + c * aq;
+        const double vp = V[row * n + p], vq = V[row * n + q];
+        V[row * n + p] = c * vp - s * vq;
+        V[row * n + q] = s * vp + c * vq;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < n; i += blockDim.x) w[i] = A[i * n + i];
+  __syncthreads();
+}
+
+// sqrt(M) and M^{-1/2} of a symmetrised chi x chi message (column-major in global)
+__device__ void env_roots(const double* M, int chi, double cutoff, double* S, double* Si, double* A,
+                          double* V, double* w, double* rot) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < chi * chi; i += blockDim.x) {
+    const int r = i / chi, c = i % chi;
+    A[i] = 0.5 * (M[r + chi * c] + M[c + chi * r]);
+  }
+  __syncthreads();
+  jacobi_eigh(A, V, w, chi, rot);
+  // truncation: sort |lambda| descending; drop from the small end while the
+  // dropped weight / total <= cutoff (keep >= 1)
+  __shared__ int keepmask[AP_ENV * 8];
+  if (tid == 0) {
+    int order[64];
+    for (int i = 0; i < chi; ++i) order[i] = i;
+    for (int i = 0; i < chi; ++i)
+      for (int j = i + 1; j < chi; ++j)
+        if (fabs(w[order[j]]) > fabs(w[order[i]])) {
+          int t = order[i];
+          order[i] = order[j];
+          order[j] = t;
+        }
+    double tot = 0;
+    for (int i = 0; i < chi; ++i) tot += fabs(w[i]);
+    int keep = chi;
+    double disc = 0;
+    while (keep > 1 && tot > 0 && (disc + fabs(w[order[keep - 1]])) / tot <= cutoff) {
+      disc += fabs(w[order[keep - 1]]);
+      --keep;
+    }
+    for (int i = 0; i < chi; ++i) keepmask[i] = 0;
+    for (int i = 0; i < keep; ++i) keepmask[order[i]] = 1;
+  }
+  __syncthreads();
+  for (int i = tid; i < chi * chi; i += blockDim.x) {
+    const int r = i / chi, c = i % chi;
+    double s = 0, si = 0;
+    for (int m = 0; m < chi; ++m) {
+      if (!keepmask[m]) continue;
+      const double lam = w[m];
+      const double vv = V[r * chi + m] * V[c * chi + m];
+      const double sq = sqrt(lam);  // NaN for a kept negative eigenvalue
+      s += vv * sq;
+      si += vv / sq;
+    }
+    S[i] = s;   // row-major [b][b']
+    Si[i] = si;
+  }
+  __syncthreads();
+}
+
+struct SideGeom {
+  int d, k, j, chib, E, n, r;
+  int dims[ITNBP_MAXR];
+  int ext[AP_MAXEXT];  // external axes in order
+  int next;
+  long long psi_off;
+};
+
+__device__ SideGeom side_geom(const VDesc& V, int j) {
+  SideGeom g;
+  g.d = V.dims[0];
+  g.k = V.k;
+  g.j = j;
+  g.chib = V.dims[j];
+  g.E = 1;
+  g.next = 0;
+  for (int a = 0; a <= V.k; ++a) g.dims[a] = V.dims[a];
+  for (int a = 1; a <= V.k; ++a)
+    if (a != j) {
+      g.ext[g.next++] = a;
+      g.E *= V.dims[a];
+    }
+  g.n = g.d * g.chib;
+  g.r = g.E < g.n ? g.E : g.n;
+  g.psi_off = V.psi_off;
+  return g;
+}
+
+// natural offset of psi element (s, external multi-index e, bond b)
+__device__ inline int nat_off(const SideGeom& g, int s, int e, int b) {
+  int off = 0, stride = g.d, rem = e;
+  for (int a = 1; a <= g.k; ++a) {
+    int x;
+    if (a == g.j) {
+      x = b;
+    } else {
+      x = rem % g.dims[a];
+      rem /= g.dims[a];
+    }
+    off += stride * x;
+    stride *= g.dims[a];
+  }
+  return s + off;
+}
+
+// mode product on external axis index pa of X (E x cols, column-major rows e):
+// Y[e with b'] = sum_b X[e with b] * Mt[b][b'] (Mt row-major, ld chi)
+__device__ void ext_mode_product(const SideGeom& g, int pa, const double* X, double* Y, int cols,
+                                 const double* Mt, bool transpose) {
+  int stride = 1;
+  for (int i = 0; i < pa; ++i) stride *= g.dims[g.ext[i]];
+  const int chi = g.dims[g.ext[pa]];
+  for (int idx = threadIdx.x; idx < g.E * cols; idx += blockDim.x) {
+    const int e = idx % g.E, c = idx / g.E;
+    const int lo = e % stride, b2 = (e / stride) % chi, hi = e / (stride * chi);
+    double acc = 0;
+    for (int b = 0; b < chi; ++b) {
+      const double m = transpose ? Mt[b2 * chi + b] : Mt[b * chi + b2];
+      acc += X[(lo + stride * (b + chi * hi)) + (long long)g.E * c] * m;
+    }
+    Y[idx] = acc;
+  }
+  __syncthreads();
+}
+
+// Householder QR in place on X (E x n, column-major); R in the upper triangle,
+// reflector tails below, heads in vh[], betas in beta[]
+__device__ void householder_qr(double* X, int E, int n, double* vh, double* beta, double* red) {
+  const int tid = threadIdx.x;
+  const int r = E < n ? E : n;
+  for (int j = 0; j < r; ++j) {
+    double s = 0;
+    for (int i = j + tid; i < E; i += blockDim.x) s += X[i + (long long)E * j] * X[i + (long long)E * j];
+    s = block_sum_d(s, red);
+    const double xjj = X[j + (long long)E * j];
+    const double nrm = sqrt(s);
+    const double alpha = xjj >= 0 ? -nrm : nrm;
+    const double v0 = xjj - alpha;
+    const double vv = s - xjj * xjj + v0 * v0;
+    const double b = (nrm == 0.0 || vv == 0.0) ? 0.0 : 2.0 / vv;
+    if (tid == 0) {
+      vh[j] = v0;
+      beta[j] = b;
+    }
+    // apply to columns c > j: w = v . X[:, c]; X[:, c] -= b w v
+    const int w = tid >> 6, l = tid & 63;
+    for (int c = j + 1 + w; c < n; c += (int)(blockDim.x >> 6)) {
+      double dot = 0;
+      for (int i = j + l; i < E; i += 64) {
+        const double vi = (i == j) ? v0 : X[i + (long long)E * j];
+        dot += vi * X[i + (long long)E * c];
+      }
+      dot = wave_sum_d(dot);
+      for (int i = j + l; i < E; i += 64) {
+        const double vi = (i == j) ? v0 : X[i + (long long)E * j];
+        X[i + (long long)E * c] -= b * dot * vi;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) X[j + (long long)E * j] = (b == 0.0) ? xjj : alpha;
+    __syncthreads();
+  }
+}
+
+// Y (E x cols) <- H_0 ... H_{r-1} Y using the reflectors stored in X
+__device__ void apply_q(const double* X, int E, int r, const double* vh, const double* beta, double* Y,
+                        int cols) {
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  for (int j = r - 1; j >= 0; --j) {
+    const double b = beta[j];
+    if (b != 0.0) {
+      for (int c = w; c < cols; c += (int)(blockDim.x >> 6)) {
+        double dot = 0;
+        for (int i = j + l; i < E; i += 64) {
+          const double vi = (i == j) ? vh[j] : X[i + (long long)E * j];
+          dot += vi * Y[i + (long long)E * c];
+        }
+        dot = wave_sum_d(dot);
+        for (int i = j + l; i < E; i += 64) {
+          const double vi = (i == j) ? vh[j] : X[i + (long long)E * j];
+          Y[i + (long long)E * c] -= b * dot * vi;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One-sided Jacobi SVD of A (M x N, column-major, M >= N) in place:
+// A <- U diag(s), V (N x N column-major) accumulated.
+__device__ void jacobi_svd(double* A, int M, int N, double* V, double* rot, double* red) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N * N; i += blockDim.x) V[i] = (i % N == i / N) ? 1.0 : 0.0;
+  __syncthreads();
+  if (N == 1) return;
+  const int ne = N + (N & 1);
+  __shared__ int changed;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (tid == 0) changed = 0;
+    __syncthreads();
+    for (int st = 0; st < ne - 1; ++st) {
+      // each pair handled by a group of G threads
+      const int npair = ne / 2;
+      const int G = (int)blockDim.x / npair >= 64 ? 64 : ((int)blockDim.x / npair >= 32 ? 32 : ((int)blockDim.x / npair >= 16 ? 16 : 8));
+      const int pi = tid / G, gl = tid % G;
+      double alpha = 0, beta = 0, gamma = 0;
+      int p = 0, q = 0;
+      const bool active = pi < npair;
+      if (active) {
+        rr_pair(ne, st, pi, p, q);
+        if (q < N) {
+          for (int i = gl; i < M; i += G) {
+            const double ap = A[i + (long long)M * p], aq = A[i + (long long)M * q];
+            alpha += ap * ap;
+            beta += aq * aq;
+            gamma += ap * aq;
+          }
+        }
+      }
+      for (int m = G / 2; m > 0; m >>= 1) {
+        alpha += __shfl_xor(alpha, m);
+        beta += __shfl_xor(beta, m);
+        gamma += __shfl_xor(gamma, m);
+      }
+      double c = 1, s = 0;
+      if (active && q < N && fabs(gamma) > 1e-15 * sqrt(alpha * beta) && gamma != 0.0) {
+        const double zeta = (beta - alpha) / (2 * gamma);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+        c = 1 / sqrt(1 + t * t);
+        s = c * t;
+        if (gl == 0) changed = 1;
+      }
+      __syncthreads();
+      if (active && q < N && s != 0.0) {
+        for (int i = gl; i < M; i += G) {
+          const double ap = A[i + (long long)M * p], aq = A[i + (long long)M * q];
+          A[i + (long long)M * p] = c * ap - s * aq;
+          A[i + (long long)M * q] = s * ap + c * aq;
+        }
+        for (int i = gl; i < N; i += G) {
+          const double vp = V[i + N * p], vq = V[i + N * q];
+          V[i + N * p] = c * vp - s * vq;
+          V[i + N * q] = s * vp + c * vq;
+        }
+      }
+      __syncthreads();
+    }
+    if (!changed) break;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict__ items,
+                                                     const VDesc* __restrict__ vds,
+                                                     double* __restrict__ psi,
+                                                     const double* __restrict__ msg,
+                                                     const double* __restrict__ gates, ApplyParams prm,
+                                                     double* __restrict__ scratch,
+                                                     double* __restrict__ truncerr,
+                                                     double* __restrict__ svals, int* __restrict__ kept,
+                                                     int* __restrict__ status) {
+  extern __shared__ __align__(16) double sm[];
+  double* X = sm;                       // AP_XY: QR, then SVD workspace
+  double* Y = X + AP_XY;                // AP_XY: mode-product partner, rebuild
+  double* Ru = Y + AP_XY;               // 1024: R_u (r x n), then the left factor
+  double* Rv = Ru + 1024;               // 1024: R_v, then the right factor
+  double* roots = Rv + 1024;            // 2 sides x AP_MAXEXT x (sqrt, inv sqrt) x AP_ENV^2
+  double* small = roots + 2 * AP_MAXEXT * 2 * AP_ENV * AP_ENV;
+  double* ea = small;
+  double* ev = ea + 64;
+  double* ew = ev + 64;
+  double* rot = ew + 64;
+  double* vh = rot + 4 * 32;
+  double* beta = vh + 64;
+  double* vhv = beta + 64;
+  double* betav = vhv + 64;
+  double* svec = betav + 64;
+  double* red = svec + 64;
+  __shared__ int sh_keep;
+  __shared__ int order[AP_TH];
+
+  const ApplyItem it = items[blockIdx.x];
+  const int tid = threadIdx.x;
+  const SideGeom gu = side_geom(vds[it.u], it.ju);
+  const SideGeom gv = side_geom(vds[it.v], it.jv);
+  double* scr_u = scratch + (long long)blockIdx.x * 2 * AP_XY;
+  double* scr_v = scr_u + AP_XY;
+
+  // 1. environment roots (sqrt and inverse sqrt), both sides
+  for (int side = 0; side < 2; ++side) {
+    const SideGeom& g = side == 0 ? gu : gv;
+    const VDesc& V = vds[side == 0 ? it.u : it.v];
+    for (int pa = 0; pa < g.next; ++pa) {
+      const int a = g.ext[pa];
+      const int chi = g.dims[a];
+      double* S = roots + ((side * AP_MAXEXT + pa) * 2) * AP_ENV * AP_ENV;
+      double* Si = S + AP_ENV * AP_ENV;
+      env_roots(msg + V.in_msg[a], chi, prm.eig_cutoff, S, Si, ea, ev, ew, rot);
+    }
+  }
+  // 2. X = psi with sqrt(env) on the external bonds, QR; reflectors to scratch
+  for (int side = 0; side < 2; ++side) {
+    const SideGeom& g = side == 0 ? gu : gv;
+    for (int idx = tid; idx < g.E * g.n; idx += blockDim.x) {
+      const int e = idx % g.E, c = idx / g.E;
+      const int s = c % g.d, b = c / g.d;
+      X[idx] = psi[g.psi_off + nat_off(g, s, e, b)];
+    }
+    __syncthreads();
+    double* cur = X;
+    double* oth = Y;
+    for (int pa = 0; pa < g.next; ++pa) {
+      const double* S = roots + ((side * AP_MAXEXT + pa) * 2) * AP_ENV * AP_ENV;
+      ext_mode_product(g, pa, cur, oth, g.n, S, false);
+      double* t = cur;
+      cur = oth;
+      oth = t;
+    }
+    if (cur != X) {
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = cur[i];
+      __syncthreads();
+    }
+    double* vhs = side == 0 ? vh : vhv;
+    double* bts = side == 0 ? beta : betav;
+    householder_qr(X, g.E, g.n, vhs, bts, red);
+    double* R = side == 0 ? Ru : Rv;
+    for (int idx = tid; idx < g.r * g.n; idx += blockDim.x) {
+      const int i = idx % g.r, c = idx / g.r;
+      R[i + g.r * c] = (i <= c) ? X[i + (long long)g.E * c] : 0.0;  // R[i][c], column-major (ld r)
+    }
+    double* scr = side == 0 ? scr_u : scr_v;
+    for (int i = tid; i < g.E * g.n; i += blockDim.x) scr[i] = X[i];
+    __syncthreads();
+  }
+  // 3. Theta[(ru, su'), (rv, sv')] = sum G[(su',sv'),(su,sv)] Ru[ru, su + du b] Rv[rv, sv + dv b],
+  //    stored directly in the taller orientation (X: M x N column-major), V in X + 1024
+  const int du = gu.d, dv = gv.d, chib = gu.chib;
+  const int Mr = gu.r * du, Nc = gv.r * dv;
+  const int dd = du * dv;
+  const double* G = gates + it.gate;
+  const bool tr = Mr < Nc;
+  const int M = tr ? Nc : Mr, N = tr ? Mr : Nc;
+  double* A = X;
+  double* VV = X + 1024;
+  for (int idx = tid; idx < Mr * Nc; idx += blockDim.x) {
+    const int row = idx % Mr, col = idx / Mr;
+    const int ru = row % gu.r, sup = row / gu.r;
+    const int rv = col % gv.r, svp = col / gv.r;
+    double acc = 0;
+    for (int su = 0; su < du; ++su)
+      for (int sv = 0; sv < dv; ++sv) {
+        const double gval = G[(sup + du * svp) + dd * (su + du * sv)];
+        if (gval == 0.0) continue;
+        double rr = 0;
+        for (int b = 0; b < chib; ++b) rr += Ru[ru + gu.r * (su + du * b)] * Rv[rv + gv.r * (sv + dv * b)];
+        acc += gval * rr;
+      }
+    if (tr) A[col + Nc * row] = acc;
+    else A[idx] = acc;
+  }
+  __syncthreads();
+  // 4. SVD
+  jacobi_svd(A, M, N, VV, rot, red);
+  for (int j = tid; j < N; j += blockDim.x) {
+    double s = 0;
+    for (int i = 0; i < M; ++i) s += A[i + M * j] * A[i + M * j];
+    svec[j] = sqrt(s);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 0; i < N; ++i) order[i] = i;
+    for (int i = 0; i < N; ++i)
+      for (int j = i + 1; j < N; ++j)
+        if (svec[order[j]] > svec[order[i]]) {
+          int t = order[i];
+          order[i] = order[j];
+          order[j] = t;
+        }
+    double tot = 0;
+    for (int i = 0; i < N; ++i) tot += svec[i] * svec[i];
+    int keep = N;
+    if (prm.maxdim > 0 && keep > prm.maxdim) keep = prm.maxdim;
+    double disc = 0;
+    for (int i = keep; i < N; ++i) disc += svec[order[i]] * svec[order[i]];
+    if (prm.cutoff >= 0 && tot > 0) {
+      while (keep > 1 && (disc + svec[order[keep - 1]] * svec[order[keep - 1]]) / tot <= prm.cutoff) {
+        disc += svec[order[keep - 1]] * svec[order[keep - 1]];
+        --keep;
+      }
+    }
+    int st = 0;
+    if (keep > chib) st = 1;  // the bond cannot grow on the device
+    truncerr[it.slot] = tot > 0 ? disc / tot : 0.0;
+    if (svals)
+      for (int i = 0; i < prm.max_keep; ++i)
+        svals[(long long)it.slot * prm.max_keep + i] = i < keep && i < N ? svec[order[i]] : 0.0;
+    if (kept) kept[it.slot] = keep;
+    status[it.slot] = st;
+    sh_keep = keep < chib ? keep : chib;
+  }
+  __syncthreads();
+  const int keep = sh_keep;
+  // left factor Lf (Mr x keep) = U sqrt(S) into Ru, right factor Rf (Nc x keep) = V sqrt(S) into Rv;
+  // A = U diag(s) (M x N), V = VV (N x N); if transposed, Theta = VV S U^T
+  for (int idx = tid; idx < Mr * keep; idx += blockDim.x) {
+    const int i = idx % Mr, j = idx / Mr;
+    const int o = order[j];
+    const double sv = svec[o];
+    Ru[idx] = !tr ? (sv > 0 ? A[i + M * o] / sqrt(sv) : 0.0) : VV[i + N * o] * sqrt(sv);
+  }
+  for (int idx = tid; idx < Nc * keep; idx += blockDim.x) {
+    const int i = idx % Nc, j = idx / Nc;
+    const int o = order[j];
+    const double sv = svec[o];
+    Rv[idx] = !tr ? VV[i + N * o] * sqrt(sv) : (sv > 0 ? A[i + M * o] / sqrt(sv) : 0.0);
+  }
+  __syncthreads();
+  // 5. rebuild both sides: Y = [F as r x (d keep); 0], apply Q, then M^{-1/2};
+  //    the largest-magnitude entry of every new-bond slice of psi_u' (first in
+  //    column-major order) is made positive and the same sign applied to psi_v'
+  __shared__ double bond_sign[AP_TH];
+  for (int side = 0; side < 2; ++side) {
+    const SideGeom& g = side == 0 ? gu : gv;
+    const double* F = side == 0 ? Ru : Rv;  // rows (r_i, s), r fastest
+    const double* scr = side == 0 ? scr_u : scr_v;
+    for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = scr[i];
+    const int cols = g.d * keep;  // column c' = s + d b'
+    for (int idx = tid; idx < g.E * cols; idx += blockDim.x) {
+      const int e = idx % g.E, c = idx / g.E;
+      const int s = c % g.d, b = c / g.d;
+      Y[idx] = e < g.r ? F[(e + g.r * s) + (long long)(g.r * g.d) * b] : 0.0;
+    }
+    __syncthreads();
+    apply_q(X, g.E, g.r, side == 0 ? vh : vhv, side == 0 ? beta : betav, Y, cols);
+    // M^{-1/2}: psi'[.. b ..] = sum_b' Y[.. b' ..] Si[b][b']
+    double* cur = Y;
+    double* oth = X;
+    for (int pa = 0; pa < g.next; ++pa) {
+      const double* Si = roots + ((side * AP_MAXEXT + pa) * 2 + 1) * AP_ENV * AP_ENV;
+      ext_mode_product(g, pa, cur, oth, cols, Si, true);
+      double* t = cur;
+      cur = oth;
+      oth = t;
+    }
+    if (side == 0 && tid < keep) {
+      const int j = tid;
+      double best = -1;
+      int best_off = 0x7fffffff;
+      double val = 0;
+      for (int e = 0; e < g.E; ++e)
+        for (int s2 = 0; s2 < g.d; ++s2) {
+          const double x = cur[e + (long long)g.E * (s2 + g.d * j)];
+          const int off = nat_off(g, s2, e, j);
+          if (fabs(x) > best || (fabs(x) == best && off < best_off)) {
+            best = fabs(x);
+            best_off = off;
+            val = x;
+          }
+        }
+      bond_sign[j] = val < 0 ? -1.0 : 1.0;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < g.E * cols; idx += blockDim.x) cur[idx] *= bond_sign[(idx / g.E) / g.d];
+    __syncthreads();
+    double nrm2 = 0;
+    for (int i = tid; i < g.E * cols; i += blockDim.x) nrm2 += cur[i] * cur[i];
+    nrm2 = block_sum_d(nrm2, red);
+    const double sc = (prm.normalize && nrm2 > 0) ? 1.0 / sqrt(nrm2) : 1.0;
+    for (int idx = tid; idx < g.E * g.n; idx += blockDim.x) {
+      const int e = idx % g.E, c = idx / g.E;
+      const int s = c % g.d, b = c / g.d;
+      const double val = b < keep ? cur[e + (long long)g.E * (s + g.d * b)] * sc : 0.0;
+      psi[g.psi_off + nat_off(g, s, e, b)] = val;
+    }
+    __syncthreads();
+  }
+}
+
+size_t apply_lds_bytes() {
+  return sizeof(double) * (2 * AP_XY + 2 * 1024 + 2 * AP_MAXEXT * 2 * AP_ENV * AP_ENV + 64 * 3 + 4 * 32 +
+                           64 * 5 + 16);
+}
+
+}  // namespace
+
+extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
+                                 int maxdim, double cutoff, int normalize, double* truncerr,
+                                 double* svals, int max_keep, int* kept) {
+  if (!net || n < 0 || (n > 0 && (!edge_ids || !gates || !truncerr))) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = net->ctx;
+  if (n == 0) return ITNBP_OK;
+  if (net->dtype != ITNBP_F64) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply needs FP64 site tensors");
+  if (svals && max_keep <= 0) return itnbp_fail(ctx, ITNBP_ERR_ARG, "max_keep must be > 0 with svals");
+  std::vector<ApplyItem> items(n);
+  std::vector<char> used(net->nv, 0);
+  long long goff = 0;
+  for (int i = 0; i < n; ++i) {
+    const int e = edge_ids[i];
+    if (e < 0 || e >= net->ne) return itnbp_fail(ctx, ITNBP_ERR_ARG, "edge id out of range");
+    const int u = net->edges[2 * e], v = net->edges[2 * e + 1];
+    if (used[u] || used[v]) return itnbp_fail(ctx, ITNBP_ERR_ARG, "the bonds of one apply call must be disjoint");
+    used[u] = used[v] = 1;
+    // product environment: every message into u but from v (and into v but from u) must be set
+    for (int side = 0; side < 2; ++side) {
+      const int x = side == 0 ? u : v;
+      const VDesc& V = net->vdesc[x];
+      if (V.k > AP_MAXEXT + 1) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply supports degree <= 4");
+      long long E = 1;
+      for (int a = net->inc_ptr[x]; a < net->inc_ptr[x + 1]; ++a) {
+        const int f = net->inc_edge[a];
+        if (f == e) continue;
+        if (!net->msg_set[msg_id_into(net, f, x)])
+          return itnbp_fail(ctx, ITNBP_ERR_STATE, "environment message not set");
+        if (net->chi[f] > AP_ENV) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply supports external bond dimensions <= 8");
+        E *= net->chi[f];
+      }
+      const long long nn = (long long)net->phys[x] * net->chi[e];
+      if (E * nn > AP_XY) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: site tensor too large for the LDS kernel");
+      if (std::min(E, nn) * net->phys[x] > AP_TH) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: Theta larger than 32x32");
+      if (std::min(E, nn) * nn > 1024) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: R factor larger than 1024");
+    }
+    items[i].u = u;
+    items[i].v = v;
+    items[i].ju = axis_of(net, u, e);
+    items[i].jv = axis_of(net, v, e);
+    items[i].gate = goff;
+    items[i].slot = i;
+    goff += (long long)net->phys[u] * net->phys[v] * net->phys[u] * net->phys[v];
+  }
+  DevBuf d_items, d_gates, d_scr, d_out;
+  auto cleanup = [&]() {
+    free_buf(d_items);
+    free_buf(d_gates);
+    free_buf(d_scr);
+    free_buf(d_out);
+  };
+  const int mk = svals ? max_keep : 1;
+  int rc = ensure_buf(ctx, d_items, n * sizeof(ApplyItem));
+  if (rc == ITNBP_OK) rc = ensure_buf(ctx, d_gates, goff * 8);
+  if (rc == ITNBP_OK) rc = ensure_buf(ctx, d_scr, (size_t)n * 2 * AP_XY * 8);
+  if (rc == ITNBP_OK) rc = ensure_buf(ctx, d_out, (size_t)n * (8 + 8 * mk + 4 + 4));
+  if (rc != ITNBP_OK) {
+    cleanup();
+    return rc;
+  }
+  char* ob = (char*)d_out.p;
+  double* d_te = (double*)ob;
+  double* d_sv = (double*)(ob + 8 * n);
+  int* d_kept = (int*)(ob + 8 * n + 8 * (size_t)n * mk);
+  int* d_st = d_kept + n;
+  hipError_t e1 = hipMemcpyAsync(d_items.p, items.data(), n * sizeof(ApplyItem), hipMemcpyHostToDevice, ctx->stream);
+  hipError_t e2 = hipMemcpyAsync(d_gates.p, gates, goff * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    cleanup();
+    return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: upload failed");
+  }
+  ApplyParams prm;
+  prm.maxdim = maxdim;
+  prm.cutoff = cutoff;
+  prm.normalize = normalize;
+  prm.eig_cutoff = 10 * 2.220446049250313e-16;
+  prm.max_keep = mk;
+  const size_t lds = apply_lds_bytes();
+  hipError_t ea = hipFuncSetAttribute((const void*)k_apply_gates, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (ea != hipSuccess) {
+    cleanup();
+    return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(ea));
+  }
+  hipLaunchKernelGGL(k_apply_gates, dim3(n), dim3(256), lds, ctx->stream, (const ApplyItem*)d_items.p,
+                     net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)d_gates.p, prm,
+                     (double*)d_scr.p, d_te, svals ? d_sv : nullptr, d_kept, d_st);
+  hipError_t ek = hipGetLastError();
+  std::vector<int> st(n);
+  std::vector<int> kp(n);
+  if (ek == hipSuccess) ek = hipMemcpyAsync(truncerr, d_te, n * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess && svals) ek = hipMemcpyAsync(svals, d_sv, (size_t)n * mk * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess) ek = hipMemcpyAsync(kp.data(), d_kept, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess) ek = hipMemcpyAsync(st.data(), d_st, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess) ek = hipStreamSynchronize(ctx->stream);
+  cleanup();
+  if (ek != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: ") + hipGetErrorString(ek));
+  if (kept) std::copy(kp.begin(), kp.end(), kept);
+  invalidate_fast_state(net);
+  for (int i = 0; i < n; ++i)
+    if (st[i])
+      return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
+                        "apply: the truncation keeps more values than the bond dimension; pass maxdim <= chi");
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(truncerr[i])) return itnbp_fail(ctx, ITNBP_ERR_NONFINITE, "apply: non-finite result");
+  return ITNBP_OK;
+}

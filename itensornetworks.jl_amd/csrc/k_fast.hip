@@ -1,12 +1,776 @@
-// Specialised sweep pipelines (placeholder: none yet).
+// Bulk-vertex sweep pipeline for bond dimension 32 on MFMA.
+//
+// For a site tensor psi[s, b1, b2, b3, b4] (d = 2, every chi = 32) the four
+// outgoing BP messages (updated_message, src/caches/abstractbeliefpropagationcache.jl:225-239)
+// are computed in two passes that share the absorbed pair:
+//
+//   pass A:  P[s, b1', b2', b3, b4] = sum_{b1,b2} M1[b1,b1'] M2[b2,b2'] psi[s,b1,b2,b3,b4]
+//            T3 = sum_r P_r M4 psi_r^T,   T4 = sum_r P_r^T M3 psi_r      (r = (s, b1', b2'))
+//   pass B:  the same with (3,4) absorbed and (1,2) open  ->  T1, T2
+//
+// where P_r and psi_r are the 32x32 (b3, b4) matrices of block r.  That is 12
+// 32^3-GEMM units per vertex instead of the 16 of four independent messages.
+// psi is kept in two block layouts (every 32x32 block contiguous), built once
+// per site-tensor upload:
+//   layout A: block (s + 2 b1 + 64 b2) holds [b3][b4];  layout B: block (s + 2 b3 + 64 b4) holds [b1][b2].
+// "absorb" reads layout B (A) blocks, forms M1^T X M2 for each, and scatters them
+// into P in layout A (B) through an LDS transpose; "fused" streams P and psi
+// blocks of one layout and accumulates both messages in MFMA accumulators,
+// feeding each intermediate accumulator straight back as the next B operand
+// (its row order is the K order the other operand is loaded with).
+//
+// BF16 flavour: v_mfma_f32_32x32x16_bf16 with FP32 accumulation; psi is exact
+// BF16, messages and intermediates are split into hi+lo BF16 terms
+// (hi*hi + hi*lo + lo*hi).  FP64 flavour: v_mfma_f64_16x16x4_f64 on LDS tiles.
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
 #include "itnbp_ops.hpp"
 
-bool fast_path_available(itnbp_net*, int, int) { return false; }
-const char* fast_path_name(itnbp_net*, int) { return "none"; }
-int fast_parallel_sweep(itnbp_net*, const std::vector<int>&, int, const void*, int, void*, void*,
-                        MsgStat*, int, bool* done) {
-  *done = false;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int FC = 32;            // bond dimension of the pipeline
+constexpr int FBLK = FC * FC;     // elements per block
+constexpr int FNR = 2 * FC * FC;  // blocks per vertex (d = 2)
+constexpr int FUSED_R_PER_WG = 128;
+constexpr int FUSED_CHUNKS = FNR / FUSED_R_PER_WG;  // 16
+
+struct FastItem {
+  long long psi_in;   // absorb input blocks (element offset into the layout buffer)
+  long long psi_out;  // fused blocks
+  long long P;        // intermediate
+  long long Ma, Mb;   // absorbed messages (rows, cols of the input blocks)
+  long long Mz, Mw;   // open-axis messages (rows, cols of the fused blocks)
+  long long out_z, out_w;
+  int slot_z, slot_w;
+  long long part;     // partial sums offset
+};
+
+__device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+// K slot (t, g, p) of a 32x32x16 operand <-> row of a 32x32 accumulator
+__device__ inline int kperm(int t, int g, int p) { return 16 * t + 8 * (p >> 2) + 4 * g + (p & 3); }
+
+#define MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16((a), (b), (c), 0, 0, 0)
+
+// ---------------------------------------------------------------------------
+// layout build: natural column-major psi -> layouts A and B
+// ---------------------------------------------------------------------------
+template <typename TS, typename TO>
+__global__ void k_fast_layout(const TS* __restrict__ psi, const long long* __restrict__ src_off,
+                              const long long* __restrict__ dst_off, TO* __restrict__ outA,
+                              TO* __restrict__ outB) {
+  const int it = blockIdx.y;
+  const TS* src = psi + src_off[it];
+  TO* A = outA + dst_off[it];
+  TO* B = outB + dst_off[it];
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC * FC;
+       n += gridDim.x * blockDim.x) {
+    const int s = n & 1;
+    const int b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = (n >> 11) & 31, b4 = (n >> 16) & 31;
+    const TO val = (TO)load_as<float>(src, n);
+    A[(long long)(s + 2 * b1 + 64 * b2) * FBLK + b3 * FC + b4] = val;
+    B[(long long)(s + 2 * b3 + 64 * b4) * FBLK + b1 * FC + b2] = val;
+  }
+}
+
+template <typename TS>
+__global__ void k_fast_layout64(const TS* __restrict__ psi, const long long* __restrict__ src_off,
+                                const long long* __restrict__ dst_off, double* __restrict__ outA,
+                                double* __restrict__ outB) {
+  const int it = blockIdx.y;
+  const TS* src = psi + src_off[it];
+  double* A = outA + dst_off[it];
+  double* B = outB + dst_off[it];
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC * FC;
+       n += gridDim.x * blockDim.x) {
+    const int s = n & 1;
+    const int b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = (n >> 11) & 31, b4 = (n >> 16) & 31;
+    const double val = load_as<double>(src, n);
+    A[(long long)(s + 2 * b1 + 64 * b2) * FBLK + b3 * FC + b4] = val;
+    B[(long long)(s + 2 * b3 + 64 * b4) * FBLK + b1 * FC + b2] = val;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 absorb: P[(s, x', y')][z][w] = (Ma^T X_{s,z,w} Mb)[x'][y']
+// grid (128, items): WG q -> s = q&1, z = (q>>1)&31, w in [16 (q>>6), +16)
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) void k_absorb_bf16(const FastItem* __restrict__ items,
+                                                     const __bf16* __restrict__ lay,
+                                                     const float* __restrict__ msg,
+                                                     float* __restrict__ P) {
+  __shared__ float S[16 * 1056];
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int s = q & 1, z = (q >> 1) & 31, w0 = (q >> 6) * 16;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  bf16x8 bMbh[2], bMbl[2], aMah[2], aMal[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      __bf16 h, lo;
+      split_bf16(msg[it.Mb + (16 * t + 8 * g + p) + FC * c], h, lo);  // Mb[y][y']
+      bMbh[t][p] = h;
+      bMbl[t][p] = lo;
+      split_bf16(msg[it.Ma + kperm(t, g, p) + FC * c], h, lo);        // Ma[x][x'], K permuted
+      aMah[t][p] = h;
+      aMal[t][p] = lo;
+    }
+#pragma unroll 1
+  for (int i = 0; i < 4; ++i) {
+    const int wl = 4 * wv + i, w = w0 + wl;
+    const __bf16* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * w)) * FBLK;
+    f32x16 V = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const bf16x8 aX = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+      V = MFMA_BF16(aX, bMbh[t], V);
+      if (NT > 1) V = MFMA_BF16(aX, bMbl[t], V);
+    }
+    bf16x8 bVh[2], bVl[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        __bf16 h, lo;
+        split_bf16(V[8 * t + p], h, lo);
+        bVh[t][p] = h;
+        bVl[t][p] = lo;
+      }
+    f32x16 Y = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      Y = MFMA_BF16(aMah[t], bVh[t], Y);
+      if (NT > 1) {
+        Y = MFMA_BF16(aMah[t], bVl[t], Y);
+        Y = MFMA_BF16(aMal[t], bVh[t], Y);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
+      S[wl * 1056 + xb + 33 * c] = Y[v];
+    }
+  }
+  __syncthreads();
+  for (int q2 = threadIdx.x; q2 < FBLK; q2 += blockDim.x) {
+    const int xb = q2 & 31, yb = q2 >> 5;
+    const long long r = s + 2 * (xb + FC * yb);
+    float* dst = P + it.P + r * FBLK + z * FC + w0;
+    const int o = xb + 33 * yb;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float4 v4 = make_float4(S[(4 * j) * 1056 + o], S[(4 * j + 1) * 1056 + o], S[(4 * j + 2) * 1056 + o],
+                              S[(4 * j + 3) * 1056 + o]);
+      *reinterpret_cast<float4*>(dst + 4 * j) = v4;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 fused: Tz += P_r Mw psi_r^T, Tw += P_r^T Mz psi_r over 128 blocks per WG
+// grid (16, items); partial[item][chunk][2][32*32], [x*32 + y]
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) void k_fused_bf16(const FastItem* __restrict__ items,
+                                                    const __bf16* __restrict__ lay,
+                                                    const float* __restrict__ msg,
+                                                    const float* __restrict__ P,
+                                                    float* __restrict__ part) {
+  __shared__ float Pb[4][32 * 33];
+  __shared__ __bf16 Sb[4][32 * 34];
+  __shared__ float Red[2 * FBLK];
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  bf16x8 aMwh[2], aMwl[2], aMzh[2], aMzl[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int k = 16 * t + 8 * g + p;
+      __bf16 h, lo;
+      split_bf16(msg[it.Mw + k + FC * c], h, lo);  // Mw[w][w'] with w = k, w' = c
+      aMwh[t][p] = h;
+      aMwl[t][p] = lo;
+      split_bf16(msg[it.Mz + c + FC * k], h, lo);  // Mz[z][z'] with z = c, z' = k
+      aMzh[t][p] = h;
+      aMzl[t][p] = lo;
+    }
+  f32x16 accTzT = {}, accTw = {};
+  float* pb = Pb[wv];
+  __bf16* sb = Sb[wv];
+#pragma unroll 1
+  for (int i = 0; i < FUSED_R_PER_WG / 4; ++i) {
+    const long long r = (long long)q * FUSED_R_PER_WG + wv * (FUSED_R_PER_WG / 4) + i;
+    const float* Pr = P + it.P + r * FBLK;
+    const __bf16* Yr = lay + it.psi_out + r * FBLK;
+    {
+      const int row = l >> 1, col0 = (l & 1) * 16;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 v4 = *reinterpret_cast<const float4*>(Pr + l * 16 + 4 * j);
+        float* d = pb + row * 33 + col0 + 4 * j;
+        d[0] = v4.x;
+        d[1] = v4.y;
+        d[2] = v4.z;
+        d[3] = v4.w;
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const bf16x8 v8 = *reinterpret_cast<const bf16x8*>(Yr + l * 16 + 8 * j);
+        __bf16* d = sb + row * 34 + col0 + 8 * j;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) d[p] = v8[p];
+      }
+    }
+    __syncthreads();
+    // step 1: X^T = Mw^T P_r^T
+    f32x16 XT = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 bh, bl;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        __bf16 h, lo;
+        split_bf16(pb[c * 33 + 16 * t + 8 * g + p], h, lo);
+        bh[p] = h;
+        bl[p] = lo;
+      }
+      XT = MFMA_BF16(aMwh[t], bh, XT);
+      if (NT > 1) {
+        XT = MFMA_BF16(aMwh[t], bl, XT);
+        XT = MFMA_BF16(aMwl[t], bh, XT);
+      }
+    }
+    // step 2: Tz^T += psi_r X^T
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 a, bh, bl;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        a[p] = sb[c * 34 + kperm(t, g, p)];
+        __bf16 h, lo;
+        split_bf16(XT[8 * t + p], h, lo);
+        bh[p] = h;
+        bl[p] = lo;
+      }
+      accTzT = MFMA_BF16(a, bh, accTzT);
+      if (NT > 1) accTzT = MFMA_BF16(a, bl, accTzT);
+    }
+    // step 3: Zm = Mz psi_r
+    f32x16 Zm = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 b;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) b[p] = sb[(16 * t + 8 * g + p) * 34 + c];
+      Zm = MFMA_BF16(aMzh[t], b, Zm);
+      if (NT > 1) Zm = MFMA_BF16(aMzl[t], b, Zm);
+    }
+    // step 4: Tw += P_r^T Zm
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 ah, al, bh, bl;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        __bf16 h, lo;
+        split_bf16(pb[kperm(t, g, p) * 33 + c], h, lo);
+        ah[p] = h;
+        al[p] = lo;
+        split_bf16(Zm[8 * t + p], h, lo);
+        bh[p] = h;
+        bl[p] = lo;
+      }
+      accTw = MFMA_BF16(ah, bh, accTw);
+      if (NT > 1) {
+        accTw = MFMA_BF16(ah, bl, accTw);
+        accTw = MFMA_BF16(al, bh, accTw);
+      }
+    }
+    __syncthreads();
+  }
+  // reduce the four waves
+  for (int w = 0; w < 4; ++w) {
+    if (wv == w) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w == 0) {
+          Red[c * FC + row] = accTzT[v];            // Tz[x = c][y = row]
+          Red[FBLK + row * FC + c] = accTw[v];      // Tw[x = row][y = c]
+        } else {
+          Red[c * FC + row] += accTzT[v];
+          Red[FBLK + row * FC + c] += accTw[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* dst = part + it.part + (long long)q * 2 * FBLK;
+  for (int o = threadIdx.x; o < 2 * FBLK; o += blockDim.x) dst[o] = Red[o];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 absorb / fused on v_mfma_f64_16x16x4_f64, operands from LDS
+// ---------------------------------------------------------------------------
+// 16x16x4 f64 operand layout: A[i][k] in lane (i + 16 k), B[k][j] in lane (j + 16 k);
+// the 16x16 accumulator holds D[4 v + l / 16][l % 16] in lane l, value v (measured).
+#define F64_DROW(l, v) (4 * (v) + ((l) >> 4))
+#define F64_DCOL(l, v) ((l) & 15)
+
+// C[32x32] (+)= A[32x32] * B[32x32]; A, B, C in LDS with row strides; one wave
+__device__ inline void wave_gemm32_f64(const double* A, int lda, const double* B, int ldb, double* C,
+                                       int ldc, bool accumulate) {
+  const int l = threadIdx.x & 63;
+  const int li = l & 15, lk = l >> 4;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = f64x4{0, 0, 0, 0};
+#pragma unroll
+  for (int k0 = 0; k0 < 32; k0 += 4) {
+    const double a0 = A[(li)*lda + k0 + lk];
+    const double a1 = A[(16 + li) * lda + k0 + lk];
+    const double b0 = B[(k0 + lk) * ldb + li];
+    const double b1 = B[(k0 + lk) * ldb + 16 + li];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * mi + F64_DROW(l, v), j = 16 * ni + F64_DCOL(l, v);
+        if (accumulate)
+          C[i * ldc + j] += acc[mi][ni][v];
+        else
+          C[i * ldc + j] = acc[mi][ni][v];
+      }
+}
+
+// one wave: C = A^T-free products use explicit transposed copies in LDS
+constexpr int LD64 = 33;
+
+// FP64 absorb: each wave handles 4 input blocks; WG q -> (s, z, 16 w) like the BF16 kernel
+__global__ __launch_bounds__(256) void k_absorb_f64(const FastItem* __restrict__ items,
+                                                    const double* __restrict__ lay,
+                                                    const double* __restrict__ msg,
+                                                    double* __restrict__ P) {
+  __shared__ double MaT[32 * LD64], Mb[32 * LD64];
+  __shared__ double X[4][32 * LD64], V[4][32 * LD64];
+  __shared__ double S[8][FBLK];  // staged outputs of 8 inputs (64 KiB)
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int s = q & 1, z = (q >> 1) & 31, w0 = (q >> 6) * 16;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    const int a = o & 31, b = o >> 5;  // column-major message: M[a + 32 b]
+    MaT[b * LD64 + a] = msg[it.Ma + o];  // Ma^T[x'][x] = Ma[x][x']
+    Mb[a * LD64 + b] = msg[it.Mb + o];   // Mb[y][y']
+  }
+  __syncthreads();
+  for (int half = 0; half < 2; ++half) {
+    for (int i = 0; i < 2; ++i) {
+      const int wl = 8 * half + 2 * wv + i;  // 0..15
+      const int w = w0 + wl;
+      const double* Xg = lay + it.psi_in + (long long)(s + 2 * (z + FC * w)) * FBLK;
+      double* x = X[wv];
+      for (int o = l; o < FBLK; o += 64) x[(o >> 5) * LD64 + (o & 31)] = Xg[o];
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+      wave_gemm32_f64(x, LD64, Mb, LD64, V[wv], LD64, false);      // V = X Mb
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+      wave_gemm32_f64(MaT, LD64, V[wv], LD64, x, LD64, false);     // Y = Ma^T V (into x)
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+      double* st = S[2 * wv + i];
+      for (int o = l; o < FBLK; o += 64) st[o] = x[(o >> 5) * LD64 + (o & 31)];
+    }
+    __syncthreads();
+    for (int q2 = threadIdx.x; q2 < FBLK; q2 += blockDim.x) {
+      const int xb = q2 & 31, yb = q2 >> 5;
+      const long long r = s + 2 * (xb + FC * yb);
+      double* dst = P + it.P + r * FBLK + z * FC + w0 + 8 * half;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) dst[j] = S[j][xb * FC + yb];
+    }
+    __syncthreads();
+  }
+}
+
+// FP64 fused: per block r, Tz += (P_r Mw) psi_r^T, Tw += P_r^T (Mz psi_r); one wave per block
+__global__ __launch_bounds__(256) void k_fused_f64(const FastItem* __restrict__ items,
+                                                   const double* __restrict__ lay,
+                                                   const double* __restrict__ msg,
+                                                   const double* __restrict__ P,
+                                                   double* __restrict__ part) {
+  __shared__ double Mw[32 * LD64], Mz[32 * LD64];
+  __shared__ double Pr[4][32 * LD64], PrT[4][32 * LD64], Yt[4][32 * LD64], Yr[4][32 * LD64];
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    const int a = o & 31, b = o >> 5;
+    Mw[a * LD64 + b] = msg[it.Mw + o];
+    Mz[a * LD64 + b] = msg[it.Mz + o];
+  }
+  __syncthreads();
+  const int li = l & 15, lk = l >> 4;
+  f64x4 tz[2][2], tw[2][2];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni) {
+      tz[mi][ni] = f64x4{0, 0, 0, 0};
+      tw[mi][ni] = f64x4{0, 0, 0, 0};
+    }
+  double* pr = Pr[wv];
+  double* prt = PrT[wv];
+  double* yt = Yt[wv];
+  double* yr = Yr[wv];
+#pragma unroll 1
+  for (int i = 0; i < FUSED_R_PER_WG / 4; ++i) {
+    const long long r = (long long)q * FUSED_R_PER_WG + wv * (FUSED_R_PER_WG / 4) + i;
+    const double* Pg = P + it.P + r * FBLK;
+    const double* Yg = lay + it.psi_out + r * FBLK;
+    for (int o = l; o < FBLK; o += 64) {
+      const int a = o >> 5, b = o & 31;
+      const double pv = Pg[o], yv = Yg[o];
+      pr[a * LD64 + b] = pv;
+      prt[b * LD64 + a] = pv;
+      yr[a * LD64 + b] = yv;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    wave_gemm32_f64(pr, LD64, Mw, LD64, yt, LD64, false);  // X = P_r Mw  (into yt)
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // Tz += X psi_r^T : B[k][j] = psi_r[j][k]
+#pragma unroll
+    for (int k0 = 0; k0 < 32; k0 += 4) {
+      const double a0 = yt[li * LD64 + k0 + lk], a1 = yt[(16 + li) * LD64 + k0 + lk];
+      const double b0 = yr[li * LD64 + k0 + lk], b1 = yr[(16 + li) * LD64 + k0 + lk];
+      tz[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, tz[0][0], 0, 0, 0);
+      tz[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, tz[0][1], 0, 0, 0);
+      tz[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, tz[1][0], 0, 0, 0);
+      tz[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, tz[1][1], 0, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    wave_gemm32_f64(Mz, LD64, yr, LD64, yt, LD64, false);  // Zm = Mz psi_r (into yt)
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // Tw += P_r^T Zm
+#pragma unroll
+    for (int k0 = 0; k0 < 32; k0 += 4) {
+      const double a0 = prt[li * LD64 + k0 + lk], a1 = prt[(16 + li) * LD64 + k0 + lk];
+      const double b0 = yt[(k0 + lk) * LD64 + li], b1 = yt[(k0 + lk) * LD64 + 16 + li];
+      tw[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, tw[0][0], 0, 0, 0);
+      tw[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, tw[0][1], 0, 0, 0);
+      tw[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, tw[1][0], 0, 0, 0);
+      tw[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, tw[1][1], 0, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // reduce the four waves through LDS (reuse Pr as 2 x 1024 doubles per wave? use PrT)
+  __syncthreads();
+  double* red = &Pr[0][0];  // 4*32*33 doubles >= 2*1024
+  for (int w = 0; w < 4; ++w) {
+    if (wv == w) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * mi + F64_DROW(l, v), j = 16 * ni + F64_DCOL(l, v);
+            if (w == 0) {
+              red[i * FC + j] = tz[mi][ni][v];
+              red[FBLK + i * FC + j] = tw[mi][ni][v];
+            } else {
+              red[i * FC + j] += tz[mi][ni][v];
+              red[FBLK + i * FC + j] += tw[mi][ni][v];
+            }
+          }
+    }
+    __syncthreads();
+  }
+  double* dst = part + it.part + (long long)q * 2 * FBLK;
+  for (int o = threadIdx.x; o < 2 * FBLK; o += blockDim.x) dst[o] = red[o];
+}
+
+// ---------------------------------------------------------------------------
+// finaliser: sum partials (FP64), normalise, message_diff / residual, write
+// ---------------------------------------------------------------------------
+template <typename TP, typename TM>
+__global__ __launch_bounds__(256) void k_fast_finalize(const FastItem* __restrict__ items,
+                                                       const TP* __restrict__ part,
+                                                       const TM* msg_in, TM* msg_out,
+                                                       MsgStat* __restrict__ stats, int normalize) {
+  __shared__ double T[FBLK];
+  __shared__ double red[8];
+  const FastItem it = items[blockIdx.y];
+  const int mat = blockIdx.x;
+  const TP* pp = part + it.part + (long long)mat * FBLK;
+  double n2 = 0;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    double sum = 0;
+    for (int ch = 0; ch < FUSED_CHUNKS; ++ch) sum += (double)pp[(long long)ch * 2 * FBLK + o];
+    const int x = o >> 5, y = o & 31;
+    T[x + FC * y] = sum;  // message layout M[b_ket + chi b_bra]
+    n2 += sum * sum;
+  }
+  auto bsum = [&](double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+    return s;
+  };
+  n2 = bsum(n2);
+  const double nrm = sqrt(n2);
+  const double sc = (normalize && nrm != 0.0) ? 1.0 / nrm : 1.0;
+  const long long off = mat == 0 ? it.out_z : it.out_w;
+  const int slot = mat == 0 ? it.slot_z : it.slot_w;
+  double dot = 0, o2 = 0, d2 = 0, bad = 0;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    const double nv = T[o] * sc;
+    const double ov = (double)msg_in[off + o];
+    if (!isfinite(nv)) bad = 1;
+    dot += ov * nv;
+    o2 += ov * ov;
+    d2 += (nv - ov) * (nv - ov);
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) msg_out[off + o] = (TM)(T[o] * sc);
+  dot = bsum(dot);
+  o2 = bsum(o2);
+  d2 = bsum(d2);
+  bad = bsum(bad);
+  if (threadIdx.x == 0 && slot >= 0) {
+    const double nn2 = n2 * sc * sc;
+    MsgStat st;
+    st.diff = 1.0 - (dot * dot) / (o2 * nn2);
+    st.res2 = nn2 > 0 ? d2 / nn2 : 0.0;
+    st.norm = nrm;
+    st.bad = (bad > 0 || !isfinite(nrm)) ? 1.0 : 0.0;
+    stats[slot] = st;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host state
+// ---------------------------------------------------------------------------
+struct FastState {
+  std::vector<int> bulk;            // eligible vertices
+  std::vector<FastItem> items;      // 2 per bulk vertex (passes A and B)
+  std::vector<int> out_msgs;        // message ids produced
+  DevBuf d_items, lay16, lay64, P32, P64, part32, part64, offs;
+  bool lay16_ok = false, lay64_ok = false;
+  std::vector<int> seq_key;         // sequence the slots were assigned for
+};
+
+FastState* state_of(itnbp_net* net) { return reinterpret_cast<FastState*>(net->fast); }
+
+bool eligible(const itnbp_net* net, int v) {
+  const VDesc& V = net->vdesc[v];
+  if (V.k != 4 || V.dims[0] != 2) return false;
+  for (int a = 1; a <= 4; ++a)
+    if (V.dims[a] != FC) return false;
+  return true;
+}
+
+int out_msg_id(const itnbp_net* net, int v, int a) {
+  const int e = net->inc_edge[net->inc_ptr[v] + a - 1];
+  return msg_id_into(net, e, net->edges[2 * e] == v ? net->edges[2 * e + 1] : net->edges[2 * e]);
+}
+
+}  // namespace
+
+bool fast_path_available(itnbp_net* net, int acc_f64, int schedule) {
+  if (schedule != ITNBP_SCHED_PARALLEL) return false;
+  // the BF16 kernels need psi exactly representable in BF16
+  if (!acc_f64 && net->dtype != ITNBP_BF16) return false;
+  if (getenv("ITNBP_DISABLE_FAST")) return false;
+  for (int v = 0; v < net->nv; ++v)
+    if (eligible(net, v)) return true;
+  return false;
+}
+
+const char* fast_path_name(itnbp_net*, int acc_f64) { return acc_f64 ? "mfma-f64-chi32" : "mfma-bf16x2-chi32"; }
+
+void release_fast_state(itnbp_net* net) {
+  FastState* st = state_of(net);
+  if (!st) return;
+  free_buf(st->d_items);
+  free_buf(st->lay16);
+  free_buf(st->lay64);
+  free_buf(st->P32);
+  free_buf(st->P64);
+  free_buf(st->part32);
+  free_buf(st->part64);
+  free_buf(st->offs);
+  delete st;
+  net->fast = nullptr;
+}
+
+void invalidate_fast_state(itnbp_net* net) {
+  FastState* st = state_of(net);
+  if (!st) return;
+  st->lay16_ok = false;
+  st->lay64_ok = false;
+}
+
+// Mark the sequence entries the pipeline produces and prepare its buffers.
+int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::vector<char>& handled) {
+  itnbp_ctx* ctx = net->ctx;
+  handled.assign(seq.size(), 0);
+  FastState* st = state_of(net);
+  if (!st) {
+    st = new FastState();
+    net->fast = st;
+    for (int v = 0; v < net->nv; ++v)
+      if (eligible(net, v)) st->bulk.push_back(v);
+  }
+  const long long per = (long long)FNR * FBLK;  // elements per vertex layout
+  // slots
+  std::vector<int> pos(2 * net->ne, -1);
+  for (size_t i = 0; i < seq.size(); ++i) pos[seq[i]] = (int)i;
+  st->items.clear();
+  const int nb = (int)st->bulk.size();
+  for (int bi = 0; bi < nb; ++bi) {
+    const int v = st->bulk[bi];
+    const VDesc& V = net->vdesc[v];
+    const int o1 = out_msg_id(net, v, 1), o2 = out_msg_id(net, v, 2), o3 = out_msg_id(net, v, 3),
+              o4 = out_msg_id(net, v, 4);
+    // every outgoing message of a bulk vertex must be in the sequence
+    if (pos[o1] < 0 || pos[o2] < 0 || pos[o3] < 0 || pos[o4] < 0) return ITNBP_ERR_UNSUPPORTED;
+    const long long lay = (long long)bi * per;
+    FastItem a{}, b{};
+    a.psi_in = nb * per + lay;  // layout B buffer follows layout A
+    a.psi_out = lay;
+    a.P = (long long)(2 * bi) * per;
+    a.Ma = V.in_msg[1];
+    a.Mb = V.in_msg[2];
+    a.Mz = V.in_msg[3];
+    a.Mw = V.in_msg[4];
+    a.out_z = net->msg_off[o3];
+    a.out_w = net->msg_off[o4];
+    a.slot_z = pos[o3];
+    a.slot_w = pos[o4];
+    a.part = (long long)(2 * bi) * FUSED_CHUNKS * 2 * FBLK;
+    b.psi_in = lay;
+    b.psi_out = nb * per + lay;
+    b.P = (long long)(2 * bi + 1) * per;
+    b.Ma = V.in_msg[3];
+    b.Mb = V.in_msg[4];
+    b.Mz = V.in_msg[1];
+    b.Mw = V.in_msg[2];
+    b.out_z = net->msg_off[o1];
+    b.out_w = net->msg_off[o2];
+    b.slot_z = pos[o1];
+    b.slot_w = pos[o2];
+    b.part = (long long)(2 * bi + 1) * FUSED_CHUNKS * 2 * FBLK;
+    st->items.push_back(a);
+    st->items.push_back(b);
+    for (int o : {o1, o2, o3, o4}) handled[pos[o]] = 1;
+  }
+  const size_t ni = st->items.size();
+  ITNBP_RETCHK(ensure_buf(ctx, st->d_items, ni * sizeof(FastItem)));
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_items.p, st->items.data(), ni * sizeof(FastItem), hipMemcpyHostToDevice,
+                                   ctx->stream));
+  // layouts
+  std::vector<long long> offs(2 * nb);
+  for (int bi = 0; bi < nb; ++bi) {
+    offs[bi] = net->psi_off[st->bulk[bi]];
+    offs[nb + bi] = (long long)bi * per;
+  }
+  ITNBP_RETCHK(ensure_buf(ctx, st->offs, offs.size() * 8));
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->offs.p, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const long long* d_src = (const long long*)st->offs.p;
+  const long long* d_dst = d_src + nb;
+  dim3 lgrid(256, nb);
+  if (!acc_f64 && !st->lay16_ok) {
+    ITNBP_RETCHK(ensure_buf(ctx, st->lay16, (size_t)2 * nb * per * 2));
+    __bf16* A = (__bf16*)st->lay16.p;
+    __bf16* B = A + nb * per;
+    switch (net->dtype) {
+      case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout<double, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B); break;
+      case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout<float, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B); break;
+      default: hipLaunchKernelGGL((k_fast_layout<bf16_t, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B); break;
+    }
+    ITNBP_HIPCHK(ctx, hipGetLastError());
+    st->lay16_ok = true;
+  }
+  if (acc_f64 && !st->lay64_ok) {
+    ITNBP_RETCHK(ensure_buf(ctx, st->lay64, (size_t)2 * nb * per * 8));
+    double* A = (double*)st->lay64.p;
+    double* B = A + nb * per;
+    switch (net->dtype) {
+      case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout64<double>), lgrid, dim3(256), 0, ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B); break;
+      case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout64<float>), lgrid, dim3(256), 0, ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B); break;
+      default: hipLaunchKernelGGL((k_fast_layout64<bf16_t>), lgrid, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B); break;
+    }
+    ITNBP_HIPCHK(ctx, hipGetLastError());
+    st->lay64_ok = true;
+  }
+  if (acc_f64) {
+    ITNBP_RETCHK(ensure_buf(ctx, st->P64, (size_t)2 * nb * per * 8));
+    ITNBP_RETCHK(ensure_buf(ctx, st->part64, ni * FUSED_CHUNKS * 2 * FBLK * 8));
+  } else {
+    ITNBP_RETCHK(ensure_buf(ctx, st->P32, (size_t)2 * nb * per * 4));
+    ITNBP_RETCHK(ensure_buf(ctx, st->part32, ni * FUSED_CHUNKS * 2 * FBLK * 4));
+  }
+  ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return ITNBP_OK;
 }
-void release_fast_state(itnbp_net*) {}
-void invalidate_fast_state(itnbp_net*) {}
+
+int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, MsgStat* stats,
+               int normalize) {
+  itnbp_ctx* ctx = net->ctx;
+  FastState* st = state_of(net);
+  if (!st || st->items.empty()) return ITNBP_OK;
+  const int ni = (int)st->items.size();
+  const FastItem* items = (const FastItem*)st->d_items.p;
+  if (!acc_f64) {
+    hipLaunchKernelGGL((k_absorb_bf16<2>), dim3(128, ni), dim3(256), 0, ctx->stream, items,
+                       (const __bf16*)st->lay16.p, (const float*)msg_in, (float*)st->P32.p);
+    hipLaunchKernelGGL((k_fused_bf16<2>), dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                       (const __bf16*)st->lay16.p, (const float*)msg_in, (const float*)st->P32.p,
+                       (float*)st->part32.p);
+    hipLaunchKernelGGL((k_fast_finalize<float, float>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
+                       (const float*)st->part32.p, (const float*)msg_in, (float*)msg_out, stats, normalize);
+  } else {
+    hipLaunchKernelGGL(k_absorb_f64, dim3(128, ni), dim3(256), 0, ctx->stream, items,
+                       (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
+    hipLaunchKernelGGL(k_fused_f64, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                       (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
+                       (double*)st->part64.p);
+    hipLaunchKernelGGL((k_fast_finalize<double, double>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
+                       (const double*)st->part64.p, (const double*)msg_in, (double*)msg_out, stats, normalize);
+  }
+  ITNBP_HIPCHK(ctx, hipGetLastError());
+  return ITNBP_OK;
+}

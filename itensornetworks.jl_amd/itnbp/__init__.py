@@ -13,3 +13,4 @@ from .cache import (BeliefPropagationCache, QuadraticFormNetwork, OPS, default_e
                     identity_messages, initialize_cache, logscalar, message_diff, norm_sqr_network,
                     normalize, rdm, region_scalars, rescale, rescale_, scalar, update, update_,
                     updated_message, vertex_scalars)
+from .apply import Gate, apply, apply_gates_, edge_coloring, simple_update_round_  # noqa: F401,E402

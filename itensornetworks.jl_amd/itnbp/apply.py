@@ -1,0 +1,147 @@
+"""BP-gauged gate application: ITensors.apply(o, psi; envs, maxdim, cutoff,
+normalize, callback) of src/apply.jl:97-160, on the device.
+
+A gate is ``Gate(matrix, sites)``: one site (d x d, O[s', s]) or two
+neighbouring sites ((d_u d_v) x (d_u d_v), O[(s_u', s_v'), (s_u, s_v)], s_u
+fastest).  Two-site gates need a product environment: a BeliefPropagationCache
+(its messages into the pair) or the dict returned by ``environment(bpc, [u, v])``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .cache import BeliefPropagationCache, QuadraticFormNetwork
+from .network import ITensorNetwork
+
+
+@dataclass
+class Gate:
+    matrix: np.ndarray
+    sites: tuple
+
+
+def _reorder_two_site(m, du, dv):
+    """O[(su', sv'), (su, sv)] -> O[(sv', su'), (sv, su)]."""
+    return m.reshape(du, dv, du, dv, order="F").transpose(1, 0, 3, 2).reshape(du * dv, du * dv, order="F")
+
+
+def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None, normalize=False,
+                 callback=None):
+    """Apply two-site gates in place on the cache's site tensors, every gate with
+    the cache's current messages as its (product) environment.  Gates are grouped
+    greedily, in order, into launches of disjoint bonds.  Returns the list of
+    truncation errors; ``callback(singular_values=..., truncation_error=...)`` is
+    called per gate (apply.jl:89)."""
+    g = bpc.graph
+    psi = bpc.tensornetwork()
+    pending = list(enumerate(gates))
+    errs = [None] * len(gates)
+    while pending:
+        used, batch, rest = set(), [], []
+        for i, gt in pending:
+            u, v = gt.sites
+            if not g.has_edge(u, v):
+                raise ValueError("Vertices where the gates are being applied must be neighbors for now.")
+            if u in used or v in used:
+                rest.append((i, gt))
+            else:
+                used.update((u, v))
+                batch.append((i, gt))
+        pending = rest
+        ids, mats = [], []
+        max_keep = 32
+        for i, gt in batch:
+            u, v = gt.sites
+            e = g.edge_id(u, v)
+            m = np.asarray(gt.matrix, dtype=np.float64)
+            du, dv = psi.phys_dim(u), psi.phys_dim(v)
+            if m.shape != (du * dv, du * dv):
+                raise ValueError("gate shape does not match the physical dimensions")
+            if g.edges[e] != (u, v):
+                m = _reorder_two_site(m, du, dv)
+            ids.append(e)
+            mats.append(m.reshape(-1, order="F"))
+        ids = np.array(ids, dtype=np.int32)
+        mats = np.ascontiguousarray(np.concatenate(mats))
+        te = np.empty(len(batch), dtype=np.float64)
+        sv = np.zeros(len(batch) * max_keep, dtype=np.float64)
+        kept = np.empty(len(batch), dtype=np.int32)
+        bpc.ctx.check(bpc.lib.itnbp_apply_gates(
+            bpc.h, len(batch), L.iptr(ids), L.dptr(mats), int(maxdim or 0),
+            -1.0 if cutoff is None else float(cutoff), int(bool(normalize)), L.dptr(te), L.dptr(sv),
+            max_keep, L.iptr(kept)))
+        bpc._psi_dirty = True
+        for k, (i, gt) in enumerate(batch):
+            errs[i] = float(te[k])
+            if callback is not None:
+                callback(singular_values=sv[k * max_keep:k * max_keep + kept[k]].copy(),
+                         truncation_error=float(te[k]))
+    return errs
+
+
+def _check_product_env(envs, u, v):
+    for (w, x) in envs:
+        if x not in (u, v) or w in (u, v):
+            raise RuntimeError(
+                "`apply` requires a product environment (`envs` with no shared edges); "
+                "contract `envs` to product form before calling.")
+
+
+def apply(o: Gate, psi: ITensorNetwork, *, envs=None, maxdim=None, cutoff=None, normalize=False,
+          callback=None):
+    """apply(o, psi; envs, maxdim, cutoff, normalize, callback) (apply.jl:97-146):
+    returns a new ITensorNetwork."""
+    sites = tuple(o.sites)
+    if len(sites) == 1:
+        v = sites[0]
+        out = psi.copy()
+        t = np.tensordot(np.asarray(o.matrix), psi[v], axes=([1], [0]))
+        if normalize:
+            t = t / np.linalg.norm(t)
+        out[v] = t
+        return out
+    if len(sites) > 2:
+        raise ValueError("Gates with more than 2 sites is not supported yet.")
+    if len(sites) < 1:
+        raise ValueError("Gate being applied does not share indices with tensor network.")
+    u, v = sites
+    if not psi.graph.has_edge(u, v):
+        raise ValueError("Vertices where the gates are being applied must be neighbors for now.")
+    if isinstance(envs, BeliefPropagationCache):
+        bpc = BeliefPropagationCache(QuadraticFormNetwork(psi), storage="f64", ctx=envs.ctx)
+        msgs = {e: m for e, m in envs.messages().items() if e[1] in (u, v) and e[0] not in (u, v)}
+    else:
+        msgs = dict(envs or {})
+        _check_product_env(msgs, u, v)
+        bpc = BeliefPropagationCache(QuadraticFormNetwork(psi), storage="f64")
+    bpc.set_messages_(msgs)
+    apply_gates_(bpc, [o], maxdim=maxdim, cutoff=cutoff, normalize=normalize, callback=callback)
+    return bpc.tensornetwork()
+
+
+def edge_coloring(graph):
+    """Greedy proper edge colouring in edge order: classes of disjoint bonds."""
+    classes = []
+    for (u, v) in graph.edges:
+        for c in classes:
+            if all(u not in e and v not in e for e in c):
+                c.append((u, v))
+                break
+        else:
+            classes.append([(u, v)])
+    return classes
+
+
+def simple_update_round_(bpc: BeliefPropagationCache, gate_fn, *, maxdim, cutoff=None, normalize=True):
+    """One BP-gauged gate on every bond, colour class by colour class, all with the
+    cache's current messages as environments.  Returns {edge: truncerr}."""
+    errs = {}
+    for cls in edge_coloring(bpc.graph):
+        gates = [Gate(gate_fn(u, v), (u, v)) for (u, v) in cls]
+        te = apply_gates_(bpc, gates, maxdim=maxdim, cutoff=cutoff, normalize=normalize)
+        errs.update(zip(cls, te))
+    return errs

@@ -209,7 +209,7 @@ class BeliefPropagationCache:
         self.ctx.check(self.lib.itnbp_set_site(self.h, i, L.vptr(buf)))
 
     def scalartype(self):
-        return {L.F64: np.float64, L.F32: np.float32, L.BF16: np.float32}[self.storage]
+        return {L.F64: np.float64, L.F32: np.float32, L.BF16: np.float64}[self.storage]
 
     # -- messages --------------------------------------------------------------
     def _mid(self, e):

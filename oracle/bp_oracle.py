@@ -608,6 +608,18 @@ def simple_update_bp(G, psi_u, psi_v, ax_u, ax_v, envs_u, envs_v, maxdim=None,
 
     pu = rebuild(psi_u, ax_u, Qu, Ru2, ext_u, Sinv_u)
     pv = rebuild(psi_v, ax_v, Qv, Rv2, ext_v, Sinv_v)
+    # deterministic gauge of the new bond: the largest-magnitude entry of every
+    # slice of psi_u' (first in column-major order) is made positive, and the
+    # same sign is applied to psi_v' (the pair product is unchanged)
+    for j in range(keep):
+        sl = np.take(pu, j, axis=ax_u).reshape(-1, order="F")
+        if np.sign(sl[np.argmax(np.abs(sl))]) < 0:
+            idx_u = [slice(None)] * pu.ndim
+            idx_u[ax_u] = j
+            pu[tuple(idx_u)] *= -1
+            idx_v = [slice(None)] * pv.ndim
+            idx_v[ax_v] = j
+            pv[tuple(idx_v)] *= -1
     if normalize:
         pu = pu / np.linalg.norm(pu)
         pv = pv / np.linalg.norm(pv)

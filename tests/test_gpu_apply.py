@@ -1,0 +1,100 @@
+"""GPU parity of the BP-gauged gate application (simple update, batched FP64
+Jacobi SVD) against the CPU oracle."""
+import numpy as np
+import pytest
+
+import itnbp
+from helpers import oracle_graph, oracle_psi, to_index
+from oracle import bp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def converged_cache(psi, sweeps=200):
+    return itnbp.update(itnbp.initialize_cache(psi), maxiter=sweeps, res_tol=1e-13, edge_sequence="parallel")
+
+
+def pair(g, psi, u, v):
+    au, av = g.axis(u, v), g.axis(v, u)
+    t = np.tensordot(psi[u], psi[v], axes=([au], [av]))
+    return np.moveaxis(t, psi[u].ndim - 1, 1)
+
+
+def test_untruncated_gate_is_exact_closed_form():
+    """With full-rank envs and no truncation, psi_u' psi_v' = G psi_u psi_v for any
+    env (apply.jl:33-95): the bond (1,1)-(2,1) has chi = 8 >= rank(Theta) = 4."""
+    g = itnbp.named_grid((2, 2))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(1), np.float64, g, link_space=[8, 2, 2, 2])
+    bpc = converged_cache(psi)
+    G = np.random.default_rng(2).standard_normal((4, 4))
+    u, v = (1, 1), (2, 1)
+    svals = []
+    new = itnbp.apply(itnbp.Gate(G, (u, v)), psi, envs=bpc, maxdim=8,
+                      callback=lambda singular_values, truncation_error: svals.append((singular_values, truncation_error)))
+    og = oracle_graph(g)
+    ref = pair(og, oracle_psi(psi), g.index[u], g.index[v])
+    rhs = np.einsum("klij,ij...->kl...", G.reshape(2, 2, 2, 2, order="F"), ref)
+    lhs = pair(og, oracle_psi(new), g.index[u], g.index[v])
+    np.testing.assert_allclose(lhs, rhs, atol=1e-11 * np.abs(rhs).max())
+    msgs = to_index(g, bpc.messages())
+    _, s_ref, err_ref = O.apply_gate(G, og, oracle_psi(psi), msgs, g.index[u], g.index[v], maxdim=8)
+    s, err = svals[0]
+    np.testing.assert_allclose(s, s_ref, rtol=1e-12)
+    assert err == 0.0 and err_ref == 0.0
+
+
+def test_simple_update_round_matches_oracle():
+    """One BP-gauged round of exp(-0.05 H_b), H_b = ZZ + 2.5 (X I/deg_u + I X/deg_v),
+    on every bond of a 4x4 chi=4 PEPS, truncated back to chi (config-2 pattern)."""
+    g = itnbp.named_grid((4, 4))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(3), np.float64, g, link_space=4)
+    bpc = converged_cache(psi)
+    og = oracle_graph(g)
+    opsi = oracle_psi(psi)
+    msgs = to_index(g, bpc.messages())
+    deg = {v: g.degree(v) for v in g.vertices}
+
+    def gate_named(u, v):
+        return O.expm_sym(O.ising_bond_hamiltonian(2.5, deg[u], deg[v]), -0.05)
+
+    errs = itnbp.simple_update_round_(bpc, gate_named, maxdim=4, normalize=True)
+    new = bpc.tensornetwork()
+    odeg = [len(og.incident[i]) for i in range(og.nv)]
+    ref_psi, ref_errs, _ = O.simple_update_round(
+        og, opsi, msgs, lambda a, b: O.expm_sym(O.ising_bond_hamiltonian(2.5, odeg[a], odeg[b]), -0.05),
+        maxdim=4, normalize=True)
+    for i, v in enumerate(g.vertices):
+        np.testing.assert_allclose(new[v], ref_psi[i], atol=1e-10)
+    for (a, b), e in errs.items():
+        assert e == pytest.approx(ref_errs[(g.index[a], g.index[b])], rel=1e-8, abs=1e-15)
+        assert e > 0
+
+
+def test_truncated_random_gate_fidelity_and_errors():
+    """test_apply.jl:12-66 pattern on a 2x2 chi=2 grid: truncation error > 0,
+    fidelity with the exact gate application > 0, non-product env throws."""
+    g = itnbp.named_grid((2, 2))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(1234), np.float64, g, link_space=2)
+    bpc = converged_cache(psi)
+    u, v = (2, 2), (1, 2)
+    og = oracle_graph(g)
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        Q, _ = np.linalg.qr(rng.standard_normal((4, 4)))
+        te = []
+        new = itnbp.apply(itnbp.Gate(Q, (u, v)), psi, envs=itnbp.environment(bpc, [u, v]), maxdim=2,
+                          normalize=True, callback=lambda singular_values, truncation_error: te.append(truncation_error))
+        assert te[0] > 0
+        exact = O.exact_state(og, oracle_psi(psi))
+        G4 = Q.reshape(2, 2, 2, 2, order="F")
+        iu, iv = g.index[u], g.index[v]
+        ex = np.moveaxis(np.tensordot(G4, np.moveaxis(exact, [iu, iv], [0, 1]), axes=([2, 3], [0, 1])), [0, 1], [iu, iv])
+        st = O.exact_state(og, oracle_psi(new))
+        f = np.vdot(st, ex) / np.sqrt(np.vdot(st, st) * np.vdot(ex, ex))
+        assert abs(f) ** 2 > 0
+    with pytest.raises(RuntimeError, match="product environment"):
+        bad = dict(itnbp.environment(bpc, [u, v]))
+        bad[(u, v)] = np.eye(2)
+        itnbp.apply(itnbp.Gate(np.eye(4), (u, v)), psi, envs=bad, maxdim=2)
+    with pytest.raises(ValueError, match="neighbors"):
+        itnbp.apply(itnbp.Gate(np.eye(4), ((1, 1), (2, 2))), psi, envs=bpc, maxdim=2)
