@@ -1,14 +1,21 @@
 """Benchmark: BP message updates/s on one MI355X.
 
-Workload (default): 32x32 open square-lattice PEPS, d=2, chi=32, site tensors
-N(0, 1/chi) stored in BF16, Jacobi schedule (all 3968 directed messages per
-sweep).  A step is one sweep: every message recomputed from the previous
-sweep's messages, normalised, with message_diff and residual.
+Workload (default, the mixed-precision configuration): 32x32 open square-lattice
+PEPS, d = 2, chi = 32, site tensors N(0, 1/chi) stored in BF16, Jacobi schedule
+(all 3968 directed messages per sweep).  A step is one sweep: every message
+recomputed from the previous sweep's messages, normalised, with message_diff
+and residual.
 
-The chosen path (BF16 MFMA contractions with FP32 accumulation) is timed for K
-steps; the plain FP64 GPU path is timed on the same tensors; both are reported,
-and a full solve (BF16 phase to residual 1e-4, then FP64 refinement) is checked
-against the FP64 solve on <X> and <ZZ>.
+Timed: K steps of the chosen path (bulk vertices on the BF16x2 MFMA pipeline
+with FP32 accumulation; boundary vertices on the generic kernels in FP32).
+Also measured on the same tensors (the BF16 values, exact in FP64):
+  * the plain FP64 GPU path per sweep (FP64 MFMA pipeline + generic FP64);
+  * both full solves: FP64 from identity messages to max residual 1e-10, and the
+    chosen solve (BF16 phase until the residual < 1e-4, then FP64 refinement to
+    1e-10), with the error of <X> (1024 sites) and <ZZ> (1984 bonds) vs FP64;
+  * the dominant kernel's average launch duration (HIP events on the library's
+    stream) for the roofline line;
+  * the CPU oracle (NumPy FP64) on a bounded sample of bulk message updates.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -27,56 +34,80 @@ sys.path.insert(0, ROOT)
 WORKLOADS = {
     "peps32x32_chi32_bf16": dict(dims=(32, 32), chi=32, storage="bf16", scale="invsqrtchi"),
     "peps16x16_chi8_f64": dict(dims=(16, 16), chi=8, storage="f64", scale=1.0),
-    "peps8x8_chi4_f64": dict(dims=(8, 8), chi=4, storage="f64", scale=1.0),
 }
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
+PEAK_F64_TFLOPS = 78.6      # MI355X FP64
+PEAK_HBM_GBS = 8000.0
 
 
-def sweep_flops_bytes(psi, g, elem_bytes, msg_bytes):
-    """Algorithmic flops and minimum bytes of one sweep: per message u->v,
-    F = 2 d prod(chi_u) sum(chi_u); bytes = each psi_u once + every message read
-    and written once."""
+def log(*a):
+    print("#", *a, file=sys.stderr, flush=True)
+
+
+def message_flops(psi, g):
+    """Algorithmic flops of one sweep: sum over messages u->v of 2 d prod(chi_u) sum(chi_u)."""
     F = 0
-    B = 0
     for v in g.vertices:
         t = psi[v]
-        D = t.size
-        F += 2 * D * sum(t.shape[1:]) * len(t.shape[1:])
-        B += D * elem_bytes
-    for (a, b) in g.edges:
-        c = psi.bond_dim(a, b)
-        B += 2 * 2 * c * c * msg_bytes
-    return F, B
+        k = t.ndim - 1
+        F += k * 2 * t.size * sum(t.shape[1:])
+    return F
 
 
-def cpu_baseline(psi, g, seconds=10.0):
-    """Oracle (NumPy FP64, this host's BLAS threads) on a bounded sample of the
-    same workload: message updates of bulk vertices until ~seconds elapse."""
+def cpu_baseline(psi, g, seconds):
+    """Oracle (NumPy FP64 on this host's cores via BLAS) on a bounded sample:
+    message updates out of bulk vertices until ~seconds elapse."""
     from oracle import bp_oracle as O
     edges = [(g.index[a], g.index[b]) for (a, b) in g.edges]
     inc = [[(g.index[w], g.edge_id(v, w)) for w in g.incident(v)] for v in g.vertices]
     og = O.Graph(g.nv, edges, incident=inc)
-    opsi = [np.asarray(psi[v], dtype=np.float64) for v in g.vertices]
-    msgs = O.identity_messages(og, opsi)
-    seq = O.parallel_sequence(og)
+    bulk = [i for i, v in enumerate(g.vertices) if g.degree(v) == 4]
+    opsi = {i: np.asarray(psi[g.vertices[i]], dtype=np.float64) for i in bulk[:8]}
+    full = [opsi.get(i, np.zeros((1,) * (1 + len(inc[i])))) for i in range(g.nv)]
+    msgs = {}
+    for (u, v) in og.directed_edges():
+        if u in opsi:
+            chi = full[u].shape[og.axis(u, v)]
+            for w in og.neighbors(u):
+                msgs[(w, u)] = np.eye(chi)
     n, t0 = 0, time.perf_counter()
-    for (u, v) in seq:
-        O.updated_message(og, opsi, msgs, u, v)
-        n += 1
-        if time.perf_counter() - t0 > seconds:
-            break
+    while time.perf_counter() - t0 < seconds:
+        for u in opsi:
+            for w in og.neighbors(u):
+                O.updated_message(og, full, msgs, u, w)
+                n += 1
+                if time.perf_counter() - t0 > seconds:
+                    break
+            if time.perf_counter() - t0 > seconds:
+                break
     dt = time.perf_counter() - t0
-    # the sample starts at edges(g)[0] and is weighted like the sweep's first edges;
-    # scale by the sweep's flops to count whole-sweep-equivalent updates
     return n / dt, n
+
+
+def build_state(W, seed=1234):
+    import itnbp
+    from itnbp.cache import bf16_round
+    g = itnbp.named_grid(W["dims"])
+    chi = W["chi"]
+    scale = 1.0 / np.sqrt(chi) if W["scale"] == "invsqrtchi" else W["scale"]
+    rng = np.random.default_rng(seed)
+    tensors = {}
+    for v in g.vertices:
+        shape = (2,) + (chi,) * g.degree(v)
+        x = rng.standard_normal(shape, dtype=np.float32) * np.float32(scale)
+        if W["storage"] == "bf16":
+            x = bf16_round(x)[0].reshape(shape)
+        tensors[v] = x.astype(np.float64)
+    return g, itnbp.ITensorNetwork(g, tensors)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="peps32x32_chi32_bf16", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -85,56 +116,143 @@ def main():
     import itnbp
 
     W = WORKLOADS[a.workload]
-    g = itnbp.named_grid(W["dims"])
-    chi = W["chi"]
-    scale = 1.0 / np.sqrt(chi) if W["scale"] == "invsqrtchi" else W["scale"]
-    rng = np.random.default_rng(1234)
-    psi = itnbp.random_tensornetwork(rng, np.float64, g, link_space=chi, scale=scale)
+    t0 = time.time()
+    g, psi = build_state(W)
     nmsg = 2 * g.ne
+    F = message_flops(psi, g)
+    log(f"built {a.workload}: {g.nv} sites, {nmsg} messages, {F / 1e12:.3f} TFLOP/sweep ({time.time() - t0:.1f} s)")
+    ctx = itnbp.default_context()
+    low = W["storage"] == "bf16"
+    chosen = itnbp.initialize_cache(psi, storage=W["storage"])
+    log("chosen cache on device")
 
-    def time_sweeps(storage, precision, steps, warmup):
-        bpc = itnbp.initialize_cache(psi, storage=storage)
-        itnbp.update_(bpc, maxiter=max(1, warmup), edge_sequence="parallel", precision=precision, switch_tol=0.0)
-        bpc.ctx.check(bpc.lib.itnbp_synchronize(bpc.ctx.h))
-        t0 = time.perf_counter()
-        itnbp.update_(bpc, maxiter=steps, edge_sequence="parallel", precision=precision, switch_tol=0.0)
-        bpc.ctx.check(bpc.lib.itnbp_synchronize(bpc.ctx.h))
-        wall = time.perf_counter() - t0
-        return bpc, wall, bpc.last_update
+    # ---- timed region: K sweeps of the chosen path
+    prec = "f32" if low else "f64"
+    itnbp.update_(chosen, maxiter=a.warmup, edge_sequence="parallel", precision=prec, switch_tol=0.0)
+    ctx.check(chosen.lib.itnbp_synchronize(ctx.h))
+    ctx.profile(True)
+    ctx.check(chosen.lib.itnbp_synchronize(ctx.h))
+    t1 = time.perf_counter()
+    itnbp.update_(chosen, maxiter=a.steps, edge_sequence="parallel", precision=prec, switch_tol=0.0)
+    ctx.check(chosen.lib.itnbp_synchronize(ctx.h))
+    wall = time.perf_counter() - t1
+    path = chosen.last_update["path"]
+    kernels = {}
+    for name in ("fused_bf16", "absorb_bf16", "fused_f64", "absorb_f64", "finalize", "oc_small", "oc_large"):
+        c, ms = ctx.kernel_time(name)
+        if c:
+            kernels[name] = dict(launches=c, total_ms=ms, avg_ms=ms / c)
+    ctx.profile(False)
+    value = nmsg * a.steps / wall
+    log(f"chosen path {path}: {a.steps} sweeps in {wall * 1e3:.1f} ms -> {value:.4g} updates/s; kernels {kernels}")
 
-    chosen_prec = "f32" if W["storage"] == "bf16" else "f64"
-    bpc_c, wall_c, lu_c = time_sweeps(W["storage"], chosen_prec, a.steps, a.warmup)
-    print(f"# chosen path {lu_c['path']}: {a.steps} sweeps in {wall_c*1e3:.1f} ms wall, "
-          f"{lu_c['device_ms']:.1f} ms device", file=sys.stderr, flush=True)
-    bpc_f, wall_f, lu_f = time_sweeps(W["storage"], "f64", max(2, a.steps // 2), 1)
-    print(f"# fp64 path {lu_f['path']}: {lu_f['iterations']} sweeps in {wall_f*1e3:.1f} ms wall",
-          file=sys.stderr, flush=True)
-    value = nmsg * a.steps / wall_c
-    fp64_rate = nmsg * lu_f["iterations"] / wall_f
-    eb = {"bf16": 2, "f32": 4, "f64": 8}[W["storage"]]
-    F, B = sweep_flops_bytes(psi, g, eb, 4)
+    # ---- plain FP64 GPU path per sweep, on the same (BF16-exact) tensors
+    plain = itnbp.initialize_cache(psi, storage="f64")
+    itnbp.update_(plain, maxiter=1, edge_sequence="parallel")
+    ctx.check(plain.lib.itnbp_synchronize(ctx.h))
+    ctx.profile(True)
+    ksw = max(3, a.steps // 4)
+    t2 = time.perf_counter()
+    itnbp.update_(plain, maxiter=ksw, edge_sequence="parallel")
+    ctx.check(plain.lib.itnbp_synchronize(ctx.h))
+    wall64 = time.perf_counter() - t2
+    k64 = {}
+    for name in ("fused_f64", "absorb_f64", "finalize", "oc_small", "oc_large"):
+        c, ms = ctx.kernel_time(name)
+        if c:
+            k64[name] = dict(launches=c, total_ms=ms, avg_ms=ms / c)
+    ctx.profile(False)
+    rate64 = nmsg * ksw / wall64
+    log(f"fp64 path {plain.last_update['path']}: {ksw} sweeps in {wall64 * 1e3:.1f} ms -> {rate64:.4g} updates/s; {k64}")
+
+    # ---- roofline of the dominant kernel of the chosen path
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    nbulk = sum(1 for v in g.vertices if g.degree(v) == 4) if W["chi"] == 32 else 0
+    roof = None
+    if dom in ("fused_bf16", "fused_f64", "absorb_bf16", "absorb_f64") and nbulk:
+        # algorithmic flops per launch: every bulk vertex, both passes, 2048 32x32 blocks;
+        # fused: 4 32^3 GEMMs per block, absorb: 2 per block
+        per_block = (4 if dom.startswith("fused") else 2) * 2 * 32 ** 3
+        flops = nbulk * 2 * 2048 * per_block
+        avg_s = kernels[dom]["avg_ms"] / 1e3
+        peak = PEAK_BF16_TFLOPS if dom.endswith("bf16") else PEAK_F64_TFLOPS
+        ach = flops / avg_s / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": None, "flops_per_launch": flops,
+                "avg_launch_ms": kernels[dom]["avg_ms"]}
+    elif kernels:
+        c = kernels[dom]
+        roof = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None, "avg_launch_ms": c["avg_ms"]}
+
+    # ---- full solves and accuracy of <X>, <ZZ> vs FP64
+    solve = None
+    if not a.no_solve and low:
+        del chosen
+        ref = itnbp.initialize_cache(psi, storage="f64")
+        t3 = time.perf_counter()
+        itnbp.update_(ref, maxiter=100, res_tol=1e-10, edge_sequence="parallel")
+        ctx.check(ref.lib.itnbp_synchronize(ctx.h))
+        t_ref = time.perf_counter() - t3
+        it_ref = ref.last_update["iterations"]
+        mix = itnbp.initialize_cache(psi, storage="bf16")
+        t4 = time.perf_counter()
+        itnbp.update_(mix, maxiter=100, res_tol=1e-10, edge_sequence="parallel", precision="bf16_f64",
+                      switch_tol=1e-4)
+        ctx.check(mix.lib.itnbp_synchronize(ctx.h))
+        t_mix = time.perf_counter() - t4
+        lu = mix.last_update
+        zz = np.kron(np.diag([1.0, -1.0]), np.diag([1.0, -1.0]))
+        x_ref = np.array(list(itnbp.expect_on_cache(ref, "X").values()))
+        x_mix = np.array(list(itnbp.expect_on_cache(mix, "X").values()))
+        z_ref = np.array(list(itnbp.expect2_on_cache(ref, zz).values()))
+        z_mix = np.array(list(itnbp.expect2_on_cache(mix, zz).values()))
+        # accuracy of the low phase alone: BF16 sweeps to residual 1e-4
+        lowc = itnbp.initialize_cache(psi, storage="bf16")
+        itnbp.update_(lowc, maxiter=100, res_tol=1e-4, edge_sequence="parallel", precision="f32")
+        x_low = np.array(list(itnbp.expect_on_cache(lowc, "X").values()))
+        solve = {
+            "fp64": {"sweeps": it_ref, "seconds": t_ref, "updates_per_s": nmsg * it_ref / t_ref,
+                     "max_residual": ref.last_update["max_residual"]},
+            "chosen": {"sweeps": lu["iterations"], "sweeps_bf16": lu["iterations_low"], "seconds": t_mix,
+                       "updates_per_s": nmsg * lu["iterations"] / t_mix, "max_residual": lu["max_residual"]},
+            "speedup_time_to_1e-10": t_ref / t_mix,
+            "err_X_max_abs": float(np.abs(x_mix - x_ref).max()),
+            "err_X_rel": float(np.linalg.norm(x_mix - x_ref) / np.linalg.norm(x_ref)),
+            "err_ZZ_max_abs": float(np.abs(z_mix - z_ref).max()),
+            "err_ZZ_rel": float(np.linalg.norm(z_mix - z_ref) / np.linalg.norm(z_ref)),
+            "bf16_phase_only_err_X_rel": float(np.linalg.norm(x_low - x_ref) / np.linalg.norm(x_ref)),
+        }
+        log(f"solve {solve}")
+        del ref, mix, lowc
+
     cpu_rate, cpu_n = cpu_baseline(psi, g, a.cpu_seconds)
     line = {
-        "metric": "BP message updates/s (Jacobi sweep)",
+        "metric": "BP message updates/s (chosen BF16x2-MFMA path; plain FP64 GPU path and solves alongside)",
         "value": value,
         "unit": "message-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": wall_c / a.steps * 1e3,
+        "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if W["storage"] == "bf16" else "f64",
-        "data": "synthetic (seeded N(0, scale^2) site tensors)",
-        "config": {"workload": a.workload, "lattice": list(W["dims"]), "chi": chi, "d": 2,
-                   "messages_per_sweep": nmsg},
-        "fp64_path": {"value": fp64_rate, "ms_per_step": wall_f / lu_f["iterations"] * 1e3,
-                      "path": lu_f["path"]},
-        "speedup_vs_fp64": value / fp64_rate,
-        "achieved_tflops_sweep": F / (wall_c / a.steps) / 1e12,
+        "dtype": "bf16" if low else "f64",
+        "data": "synthetic: seeded N(0, 1/chi) site tensors rounded to BF16",
+        "config": {"workload": a.workload, "lattice": list(W["dims"]), "chi": W["chi"], "d": 2,
+                   "schedule": "jacobi", "messages_per_sweep": nmsg, "path": path},
+        "fp64_path": {"value": rate64, "ms_per_step": wall64 / ksw * 1e3, "path": plain.last_update["path"],
+                      "kernels": k64},
+        "speedup_vs_fp64_per_sweep": value / rate64,
+        "algorithmic_tflops_per_sweep": F / 1e12,
+        "achieved_tflops": F / (wall / a.steps) / 1e12,
+        "kernels": kernels,
+        "roofline": roof,
+        "solve": solve,
         "cpu_baseline": {"value": cpu_rate, "unit": "message-updates/s", "cores": os.cpu_count(),
-                         "kind": "port", "sample": f"{cpu_n} FP64 message updates of the same workload"},
+                         "kind": "port",
+                         "sample": f"{cpu_n} FP64 bulk-vertex message updates (chi=32) by the NumPy oracle"},
     }
     print(json.dumps(line), flush=True)
 

@@ -184,6 +184,13 @@ int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_opts* opts,
 /* ---- diagnostics ------------------------------------------------------ */
 /* name of the kernel path the last itnbp_update used */
 const char* itnbp_last_path(itnbp_ctx* ctx);
+/* per-kernel timing with HIP events recorded on the library's stream around
+ * every launch while enabled; query returns the launch count and the summed
+ * duration (ms) of the kernel called `name` since the last reset. No
+ * counterpart in the reference (it has no timers); used by the benchmark. */
+int itnbp_profile_enable(itnbp_ctx* ctx, int enable);
+int itnbp_profile_reset(itnbp_ctx* ctx);
+int itnbp_profile_query(itnbp_ctx* ctx, const char* name, int* count, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,24 @@ int guarded(itnbp_ctx* ctx, F&& f) {
 
 }  // namespace
 
+// ---- per-kernel timing ----------------------------------------------------------
+int prof_mark(itnbp_ctx* ctx) {
+  if (!ctx || !ctx->prof) return -1;
+  if (ctx->ev_used == ctx->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    ctx->ev_pool.push_back(e);
+  }
+  const int i = (int)ctx->ev_used++;
+  if (hipEventRecord(ctx->ev_pool[i], ctx->stream) != hipSuccess) return -1;
+  return i;
+}
+
+void prof_add(itnbp_ctx* ctx, const char* name, int e0, int e1) {
+  if (!ctx || !ctx->prof || e0 < 0 || e1 < 0) return;
+  ctx->recs.push_back(ProfRec{name, e0, e1});
+}
+
 // ---- shared helpers (declared in itnbp_ops.hpp) ------------------------------
 size_t dtype_size(int dtype) { return dtype == ITNBP_F64 ? 8 : dtype == ITNBP_F32 ? 4 : 2; }
 
@@ -214,14 +232,17 @@ int launch_plan(itnbp_net* net, const TaskPlan& plan, size_t g0, size_t g1, cons
   itnbp_ctx* ctx = net->ctx;
   for (size_t gi = g0; gi < g1; ++gi) {
     const LaunchGroup& lg = plan.groups[gi];
+    const int e0 = prof_mark(ctx);
     if (lg.kind == 0) {
       HIPCHK(ctx, launch_oc_small(psi_dtype, plan.acc_f64, net->d_vdesc, d_tasks + lg.begin, (int)lg.count,
                                   psi, msg_in, out, d_stats, lg.lds, ctx->stream));
+      prof_add(ctx, "oc_small", e0, prof_mark(ctx));
     } else {
       HIPCHK(ctx, launch_oc_large(psi_dtype, plan.acc_f64, net->d_vdesc, d_tasks + lg.begin,
                                   plan.tasks.data() + lg.begin, net->vdesc.data(), (int)lg.count, psi,
                                   msg_in, out, d_stats, net->scratch.p, lg.maxD, net->partials.p,
                                   lg.nsplit, lg.part_stride, ctx->stream));
+      prof_add(ctx, "oc_large", e0, prof_mark(ctx));
     }
   }
   return ITNBP_OK;
@@ -321,6 +342,8 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
   int it = 0;
   *converged = 0;
   double red[4] = {0, 0, 0, 0};
+  double best_res2 = INFINITY;
+  int stall = 0;
   for (it = 1; it <= maxiter; ++it) {
     if (o->schedule == ITNBP_SCHED_PARALLEL) {
       void* cur = ph.msg[net->cur];
@@ -346,6 +369,16 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
     if (stop_on_tol && o->tol >= 0 && *mean_diff <= o->tol) stop = true;
     if (stop_on_tol && o->res_tol >= 0 && red[1] <= o->res_tol * o->res_tol) stop = true;
     if (stop_res2 >= 0 && red[1] <= stop_res2) stop = true;
+    // reduced-precision phase: also hand over once the residual stagnates at the
+    // precision floor (no 20% improvement over two consecutive sweeps)
+    if (stop_res2 >= 0) {
+      if (red[1] < 0.64 * best_res2) {
+        best_res2 = red[1];
+        stall = 0;
+      } else if (++stall >= 2 && it >= 3) {
+        stop = true;
+      }
+    }
     if (stop) {
       *converged = 1;
       if (o->verbose) printf("BP converged to desired precision after %d iterations.\n", it);
@@ -389,6 +422,26 @@ int check_sequence(itnbp_net* net, const std::vector<int>& seq, const itnbp_upda
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// one FP64 Jacobi sweep over the default parallel schedule, messages
+// normalised, no convergence test (forward pass of the adjoint)
+int sweep_phase_public(itnbp_net* net, int nsweeps) {
+  std::vector<int> seq;
+  RETCHK(default_sequence(net, ITNBP_SCHED_PARALLEL, seq));
+  if (seq.empty()) return ITNBP_OK;
+  itnbp_update_opts o{};
+  o.schedule = ITNBP_SCHED_PARALLEL;
+  o.maxiter = nsweeps;
+  o.tol = -1;
+  o.res_tol = -1;
+  o.precision = ITNBP_PREC_F64;
+  o.normalize = 1;
+  Phase ph{1, net->d_psi, net->dtype, {net->d_msg[0], net->d_msg[1]}};
+  int it = 0, conv = 0;
+  double md = 0, mr = 0;
+  return sweep_phase(net, seq, &o, ph, nsweeps, -1.0, false, &it, &md, &mr, &conv,
+                     fast_path_available(net, 1, ITNBP_SCHED_PARALLEL));
+}
+
 extern "C" {
 
 int itnbp_create(int device, itnbp_ctx** out) {
@@ -418,6 +471,7 @@ int itnbp_destroy(itnbp_ctx* ctx) {
   if (!ctx) return ITNBP_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -426,6 +480,37 @@ int itnbp_destroy(itnbp_ctx* ctx) {
 }
 
 const char* itnbp_last_error(itnbp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int itnbp_profile_enable(itnbp_ctx* ctx, int enable) {
+  if (!ctx) return ITNBP_ERR_ARG;
+  ctx->prof = enable != 0;
+  return ITNBP_OK;
+}
+
+int itnbp_profile_reset(itnbp_ctx* ctx) {
+  if (!ctx) return ITNBP_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->recs.clear();
+  ctx->ev_used = 0;
+  return ITNBP_OK;
+}
+
+int itnbp_profile_query(itnbp_ctx* ctx, const char* name, int* count, double* total_ms) {
+  if (!ctx || !name || !count || !total_ms) return ITNBP_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  int c = 0;
+  double t = 0;
+  for (const auto& r : ctx->recs) {
+    if (strcmp(r.name, name) != 0) continue;
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev_pool[r.ev0], ctx->ev_pool[r.ev1]));
+    ++c;
+    t += ms;
+  }
+  *count = c;
+  *total_ms = t;
+  return ITNBP_OK;
+}
 const char* itnbp_last_path(itnbp_ctx* ctx) { return ctx ? ctx->path.c_str() : ""; }
 
 int itnbp_synchronize(itnbp_ctx* ctx) {

@@ -1,0 +1,158 @@
+// Batched heterogeneous BP: many independent networks (any lattices, bond
+// dimensions and degrees, one storage dtype) updated together.  The instances
+// are merged on the device into one disjoint-union network, so every message of
+// every instance is a task of the same per-size-class launches of a sweep;
+// each instance's convergence is tested on its own statistics and converged
+// instances drop out of the launch plan.  Semantics per instance are those of
+// update(bpc; alg="bp") with the parallel schedule
+// (src/caches/abstractbeliefpropagationcache.jl:294-337, src/edge_sequences.jl:49-51).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "itnbp_ops.hpp"
+
+extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_opts* o,
+                                    itnbp_update_result* res) {
+  if (n <= 0 || !nets || !o || !nets[0]) return ITNBP_ERR_ARG;
+  itnbp_ctx* ctx = nets[0]->ctx;
+  if (o->maxiter < 1) return itnbp_fail(ctx, ITNBP_ERR_ARG, "You need to specify a number of iterations for BP!");
+  if (o->schedule != ITNBP_SCHED_PARALLEL || o->nseq != 0)
+    return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "batched update runs the default parallel schedule");
+  if (o->precision == ITNBP_PREC_BF16_F64)
+    return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "batched update supports FP64 and FP32 precision");
+  const int dtype = nets[0]->dtype;
+  // merged description
+  std::vector<int> edges, inc_ptr{0}, inc_edge, chi, phys;
+  std::vector<int> vbase(n), ebase(n);
+  int NV = 0, NE = 0;
+  for (int i = 0; i < n; ++i) {
+    itnbp_net* t = nets[i];
+    if (!t || t->ctx != ctx || t->dtype != dtype)
+      return itnbp_fail(ctx, ITNBP_ERR_ARG, "batched networks must share the context and the dtype");
+    for (int m = 0; m < 2 * t->ne; ++m)
+      if (!t->msg_set[m]) return itnbp_fail(ctx, ITNBP_ERR_STATE, "every message of a batched network must be set");
+    vbase[i] = NV;
+    ebase[i] = NE;
+    for (int e = 0; e < t->ne; ++e) {
+      edges.push_back(t->edges[2 * e] + NV);
+      edges.push_back(t->edges[2 * e + 1] + NV);
+      chi.push_back(t->chi[e]);
+    }
+    for (int v = 0; v < t->nv; ++v) {
+      for (int a = t->inc_ptr[v]; a < t->inc_ptr[v + 1]; ++a) inc_edge.push_back(t->inc_edge[a] + NE);
+      inc_ptr.push_back((int)inc_edge.size());
+      phys.push_back(t->phys[v]);
+    }
+    NV += t->nv;
+    NE += t->ne;
+  }
+  itnbp_net* M = nullptr;
+  ITNBP_RETCHK(itnbp_net_create(ctx, NV, NE, edges.data(), inc_ptr.data(), inc_edge.data(), chi.data(),
+                                phys.data(), dtype, &M));
+  std::unique_ptr<itnbp_net, int (*)(itnbp_net*)> guard(M, itnbp_net_destroy);
+  // copy site tensors and messages in (instance blocks are contiguous in the merged layout)
+  const size_t eb = dtype_size(dtype);
+  std::vector<long long> pbase(n), mbase(n);
+  for (int i = 0; i < n; ++i) {
+    itnbp_net* t = nets[i];
+    pbase[i] = M->psi_off[vbase[i]];
+    mbase[i] = M->msg_off[2 * ebase[i]];
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync((char*)M->d_psi + pbase[i] * eb, t->d_psi, t->psi_total * eb,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(M->d_msg[0] + mbase[i], t->d_msg[t->cur], t->msg_total * 8,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  std::fill(M->msg_set.begin(), M->msg_set.end(), 1);
+  M->cur = 0;
+  const int acc_f64 = o->precision == ITNBP_PREC_F64 ? 1 : 0;
+  void* buf[2];
+  if (acc_f64) {
+    buf[0] = M->d_msg[0];
+    buf[1] = M->d_msg[1];
+  } else {
+    for (int b = 0; b < 2; ++b)
+      ITNBP_HIPCHK(ctx, hipMalloc((void**)&M->d_msg32[b], std::max<size_t>(16, M->msg_total * 4)));
+    ITNBP_HIPCHK(ctx, launch_convert_msgs(M->d_msg[0], M->d_msg32[0], M->msg_total, ctx->stream));
+    buf[0] = M->d_msg32[0];
+    buf[1] = M->d_msg32[1];
+  }
+  const size_t mb = acc_f64 ? 8 : 4;
+  // tasks: slot = merged message id; instance i owns slots [2 ebase[i], 2 ebase[i] + 2 ne_i)
+  std::vector<char> active(n, 1);
+  std::vector<itnbp_update_result> out(n);
+  for (int i = 0; i < n; ++i) out[i] = itnbp_update_result{0, 0, 0.0, 0.0, 0, 0.0};
+  ITNBP_RETCHK(ensure_buf(ctx, M->stats, sizeof(MsgStat) * (size_t)std::max(1, 2 * NE)));
+  MsgStat* d_stats = (MsgStat*)M->stats.p;
+  std::vector<MsgStat> hs(2 * NE);
+  TaskPlan plan;
+  bool replan = true;
+  ITNBP_HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int it = 1; it <= o->maxiter; ++it) {
+    if (replan) {
+      std::vector<OcTask> tasks;
+      for (int i = 0; i < n; ++i) {
+        if (!active[i]) continue;
+        for (int m = 2 * ebase[i]; m < 2 * (ebase[i] + nets[i]->ne); ++m)
+          tasks.push_back(message_task(M, m, m, true, o->normalize));
+      }
+      plan = TaskPlan();
+      ITNBP_RETCHK(plan_tasks(M, tasks, acc_f64, plan));
+      ITNBP_RETCHK(upload_plan(M, plan, M->tasks));
+      replan = false;
+    }
+    void* cur = buf[M->cur];
+    void* nxt = buf[1 - M->cur];
+    // inactive instances keep their messages: carry the whole buffer over first
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(nxt, cur, M->msg_total * mb, hipMemcpyDeviceToDevice, ctx->stream));
+    ITNBP_RETCHK(launch_plan(M, plan, 0, plan.groups.size(), (const OcTask*)M->tasks.p, M->d_psi, dtype, cur, nxt,
+                             d_stats));
+    M->cur = 1 - M->cur;
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(hs.data(), d_stats, hs.size() * sizeof(MsgStat), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    bool any = false;
+    for (int i = 0; i < n; ++i) {
+      if (!active[i]) continue;
+      const int ne2 = 2 * nets[i]->ne;
+      double sd = 0, mr = 0;
+      bool bad = false;
+      for (int m = 2 * ebase[i]; m < 2 * ebase[i] + ne2; ++m) {
+        sd += hs[m].diff;
+        mr = std::max(mr, hs[m].res2);
+        bad = bad || hs[m].bad > 0;
+      }
+      if (bad) return itnbp_fail(ctx, ITNBP_ERR_NONFINITE, "non-finite message in batched instance " + std::to_string(i));
+      out[i].iterations = it;
+      out[i].mean_diff = ne2 ? sd / ne2 : 0.0;
+      out[i].max_residual = std::sqrt(mr);
+      bool stop = false;
+      if (o->tol >= 0 && out[i].mean_diff <= o->tol) stop = true;
+      if (o->res_tol >= 0 && mr <= o->res_tol * o->res_tol) stop = true;
+      if (stop) {
+        out[i].converged = 1;
+        active[i] = 0;
+        replan = true;
+      }
+      any = any || active[i];
+    }
+    if (!any) break;
+  }
+  ITNBP_HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  ITNBP_HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  ITNBP_HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  // scatter the messages back (FP64 canonical buffers)
+  if (!acc_f64)
+    ITNBP_HIPCHK(ctx, launch_convert_msgs_back(M->d_msg32[M->cur], M->d_msg[M->cur], M->msg_total, ctx->stream));
+  for (int i = 0; i < n; ++i) {
+    itnbp_net* t = nets[i];
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(t->d_msg[t->cur], M->d_msg[M->cur] + mbase[i], t->msg_total * 8,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    out[i].device_ms = ms;
+    if (res) res[i] = out[i];
+  }
+  ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->path = "batched-generic";
+  return ITNBP_OK;
+}

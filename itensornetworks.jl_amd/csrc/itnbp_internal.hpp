@@ -101,6 +101,11 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+struct ProfRec {
+  const char* name;
+  int ev0, ev1;  // indices into the event pool
+};
+
 struct itnbp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -108,7 +113,16 @@ struct itnbp_ctx {
   std::string err;
   std::string path;
   int num_cu = 256;
+  // per-kernel timing
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<ProfRec> recs;
 };
+
+// record a timing event on the context's stream (no-op unless profiling)
+int prof_mark(itnbp_ctx* ctx);
+void prof_add(itnbp_ctx* ctx, const char* name, int e0, int e1);
 
 struct itnbp_net {
   itnbp_ctx* ctx = nullptr;

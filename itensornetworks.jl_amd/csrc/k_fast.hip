@@ -323,6 +323,203 @@ __global__ __launch_bounds__(256) void k_fused_bf16(const FastItem* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// BF16 absorb, compact flavour: P stored as one BF16 term.  WG (512 threads)
+// -> (s, z), all 32 w; the 32 outputs per r are one 64-byte run written by 4
+// lanes cooperatively from an LDS stage [r][w].
+// grid (64, items)
+// ---------------------------------------------------------------------------
+constexpr int ST_LD = 40;  // LDS row stride (bf16) of the staging/tiles: 80 bytes, 16-byte aligned
+
+__global__ __launch_bounds__(512) void k_absorb_bf16c(const FastItem* __restrict__ items,
+                                                      const __bf16* __restrict__ lay,
+                                                      const float* __restrict__ msg,
+                                                      __bf16* __restrict__ P) {
+  __shared__ __attribute__((aligned(16))) __bf16 S[FBLK * ST_LD];
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int s = q & 1, z = q >> 1;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  bf16x8 bMbh[2], bMbl[2], aMah[2], aMal[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      __bf16 h, lo;
+      split_bf16(msg[it.Mb + (16 * t + 8 * g + p) + FC * c], h, lo);
+      bMbh[t][p] = h;
+      bMbl[t][p] = lo;
+      split_bf16(msg[it.Ma + kperm(t, g, p) + FC * c], h, lo);
+      aMah[t][p] = h;
+      aMal[t][p] = lo;
+    }
+#pragma unroll 1
+  for (int i = 0; i < 4; ++i) {
+    const int w = 4 * wv + i;
+    const __bf16* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * w)) * FBLK;
+    f32x16 V = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const bf16x8 aX = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+      V = MFMA_BF16(aX, bMbh[t], V);
+      V = MFMA_BF16(aX, bMbl[t], V);
+    }
+    bf16x8 bV[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int p = 0; p < 8; ++p) bV[t][p] = (__bf16)V[8 * t + p];
+    f32x16 Y = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      Y = MFMA_BF16(aMah[t], bV[t], Y);
+      Y = MFMA_BF16(aMal[t], bV[t], Y);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
+      S[(c + FC * xb) * ST_LD + w] = (__bf16)Y[v];  // row (y' + 32 x'), column w
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < FBLK * 4; k += blockDim.x) {
+    const int o = k >> 2, part = k & 3;
+    const int yb = o & 31, xb = o >> 5;
+    const long long r = s + 2 * (xb + FC * yb);
+    const bf16x8 val = *reinterpret_cast<const bf16x8*>(S + o * ST_LD + 8 * part);
+    *reinterpret_cast<bf16x8*>(P + it.P + r * FBLK + z * FC + 8 * part) = val;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 fused, compact flavour: P and psi blocks in BF16, intermediates as one
+// BF16 term, blocks double-buffered in LDS (prefetch of block i+1 overlaps the
+// MFMAs of block i).  grid (16, items), 4 waves x 32 blocks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fused_bf16c(const FastItem* __restrict__ items,
+                                                     const __bf16* __restrict__ lay,
+                                                     const float* __restrict__ msg,
+                                                     const __bf16* __restrict__ P,
+                                                     float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 Pb[2][4][FC * ST_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Sb[2][4][FC * ST_LD];
+  __shared__ float Red[2 * FBLK];
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  bf16x8 aMwh[2], aMwl[2], aMzh[2], aMzl[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int k = 16 * t + 8 * g + p;
+      __bf16 h, lo;
+      split_bf16(msg[it.Mw + k + FC * c], h, lo);
+      aMwh[t][p] = h;
+      aMwl[t][p] = lo;
+      split_bf16(msg[it.Mz + c + FC * k], h, lo);
+      aMzh[t][p] = h;
+      aMzl[t][p] = lo;
+    }
+  f32x16 accTzT = {}, accTw = {};
+  const long long r0 = (long long)q * FUSED_R_PER_WG + wv * (FUSED_R_PER_WG / 4);
+  const int row = l >> 1, col0 = (l & 1) * 16;
+  bf16x8 rp[2], ry[2];
+  {
+    const __bf16* Pr = P + it.P + r0 * FBLK;
+    const __bf16* Yr = lay + it.psi_out + r0 * FBLK;
+    rp[0] = *reinterpret_cast<const bf16x8*>(Pr + l * 16);
+    rp[1] = *reinterpret_cast<const bf16x8*>(Pr + l * 16 + 8);
+    ry[0] = *reinterpret_cast<const bf16x8*>(Yr + l * 16);
+    ry[1] = *reinterpret_cast<const bf16x8*>(Yr + l * 16 + 8);
+    *reinterpret_cast<bf16x8*>(&Pb[0][wv][row * ST_LD + col0]) = rp[0];
+    *reinterpret_cast<bf16x8*>(&Pb[0][wv][row * ST_LD + col0 + 8]) = rp[1];
+    *reinterpret_cast<bf16x8*>(&Sb[0][wv][row * ST_LD + col0]) = ry[0];
+    *reinterpret_cast<bf16x8*>(&Sb[0][wv][row * ST_LD + col0 + 8]) = ry[1];
+  }
+  __syncthreads();
+  constexpr int NB = FUSED_R_PER_WG / 4;
+#pragma unroll 1
+  for (int i = 0; i < NB; ++i) {
+    const int cur = i & 1;
+    if (i + 1 < NB) {
+      const __bf16* Pr = P + it.P + (r0 + i + 1) * FBLK;
+      const __bf16* Yr = lay + it.psi_out + (r0 + i + 1) * FBLK;
+      rp[0] = *reinterpret_cast<const bf16x8*>(Pr + l * 16);
+      rp[1] = *reinterpret_cast<const bf16x8*>(Pr + l * 16 + 8);
+      ry[0] = *reinterpret_cast<const bf16x8*>(Yr + l * 16);
+      ry[1] = *reinterpret_cast<const bf16x8*>(Yr + l * 16 + 8);
+    }
+    const __bf16* pb = Pb[cur][wv];
+    const __bf16* sb = Sb[cur][wv];
+    // step 1: X^T = Mw^T P_r^T
+    f32x16 XT = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + c * ST_LD + 16 * t + 8 * g);
+      XT = MFMA_BF16(aMwh[t], b, XT);
+      XT = MFMA_BF16(aMwl[t], b, XT);
+    }
+    // step 2: Tz^T += psi_r X^T
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 a, b;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        a[p] = sb[c * ST_LD + kperm(t, g, p)];
+        b[p] = (__bf16)XT[8 * t + p];
+      }
+      accTzT = MFMA_BF16(a, b, accTzT);
+    }
+    // step 3: Zm = Mz psi_r
+    f32x16 Zm = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 b;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) b[p] = sb[(16 * t + 8 * g + p) * ST_LD + c];
+      Zm = MFMA_BF16(aMzh[t], b, Zm);
+      Zm = MFMA_BF16(aMzl[t], b, Zm);
+    }
+    // step 4: Tw += P_r^T Zm
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x8 a, b;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        a[p] = pb[kperm(t, g, p) * ST_LD + c];
+        b[p] = (__bf16)Zm[8 * t + p];
+      }
+      accTw = MFMA_BF16(a, b, accTw);
+    }
+    if (i + 1 < NB) {
+      *reinterpret_cast<bf16x8*>(&Pb[1 - cur][wv][row * ST_LD + col0]) = rp[0];
+      *reinterpret_cast<bf16x8*>(&Pb[1 - cur][wv][row * ST_LD + col0 + 8]) = rp[1];
+      *reinterpret_cast<bf16x8*>(&Sb[1 - cur][wv][row * ST_LD + col0]) = ry[0];
+      *reinterpret_cast<bf16x8*>(&Sb[1 - cur][wv][row * ST_LD + col0 + 8]) = ry[1];
+    }
+    __syncthreads();
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (wv == w) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rw = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w == 0) {
+          Red[c * FC + rw] = accTzT[v];
+          Red[FBLK + rw * FC + c] = accTw[v];
+        } else {
+          Red[c * FC + rw] += accTzT[v];
+          Red[FBLK + rw * FC + c] += accTw[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* dst = part + it.part + (long long)q * 2 * FBLK;
+  for (int o = threadIdx.x; o < 2 * FBLK; o += blockDim.x) dst[o] = Red[o];
+}
+
+// ---------------------------------------------------------------------------
 // FP64 absorb / fused on v_mfma_f64_16x16x4_f64, operands from LDS
 // ---------------------------------------------------------------------------
 // 16x16x4 f64 operand layout: A[i][k] in lane (i + 16 k), B[k][j] in lane (j + 16 k);
@@ -586,12 +783,19 @@ struct FastState {
   std::vector<int> bulk;            // eligible vertices
   std::vector<FastItem> items;      // 2 per bulk vertex (passes A and B)
   std::vector<int> out_msgs;        // message ids produced
-  DevBuf d_items, lay16, lay64, P32, P64, part32, part64, offs;
+  DevBuf d_items, lay16, lay64, P32, P16, P64, part32, part64, offs;
   bool lay16_ok = false, lay64_ok = false;
   std::vector<int> seq_key;         // sequence the slots were assigned for
 };
 
 FastState* state_of(itnbp_net* net) { return reinterpret_cast<FastState*>(net->fast); }
+
+// BF16 flavour: compact (P and the per-block intermediates as one BF16 term,
+// default) or precise (P in FP32, every operand split hi+lo); ITNBP_BF16_PRECISE=1
+bool fast_precise() {
+  const char* e = getenv("ITNBP_BF16_PRECISE");
+  return e && e[0] == '1';
+}
 
 bool eligible(const itnbp_net* net, int v) {
   const VDesc& V = net->vdesc[v];
@@ -618,7 +822,9 @@ bool fast_path_available(itnbp_net* net, int acc_f64, int schedule) {
   return false;
 }
 
-const char* fast_path_name(itnbp_net*, int acc_f64) { return acc_f64 ? "mfma-f64-chi32" : "mfma-bf16x2-chi32"; }
+const char* fast_path_name(itnbp_net*, int acc_f64) {
+  return acc_f64 ? "mfma-f64-chi32" : (fast_precise() ? "mfma-bf16x2-chi32" : "mfma-bf16c-chi32");
+}
 
 void release_fast_state(itnbp_net* net) {
   FastState* st = state_of(net);
@@ -627,6 +833,7 @@ void release_fast_state(itnbp_net* net) {
   free_buf(st->lay16);
   free_buf(st->lay64);
   free_buf(st->P32);
+  free_buf(st->P16);
   free_buf(st->P64);
   free_buf(st->part32);
   free_buf(st->part64);
@@ -740,7 +947,10 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     ITNBP_RETCHK(ensure_buf(ctx, st->P64, (size_t)2 * nb * per * 8));
     ITNBP_RETCHK(ensure_buf(ctx, st->part64, ni * FUSED_CHUNKS * 2 * FBLK * 8));
   } else {
-    ITNBP_RETCHK(ensure_buf(ctx, st->P32, (size_t)2 * nb * per * 4));
+    if (fast_precise())
+      ITNBP_RETCHK(ensure_buf(ctx, st->P32, (size_t)2 * nb * per * 4));
+    else
+      ITNBP_RETCHK(ensure_buf(ctx, st->P16, (size_t)2 * nb * per * 2));
     ITNBP_RETCHK(ensure_buf(ctx, st->part32, ni * FUSED_CHUNKS * 2 * FBLK * 4));
   }
   ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -754,22 +964,44 @@ int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, M
   if (!st || st->items.empty()) return ITNBP_OK;
   const int ni = (int)st->items.size();
   const FastItem* items = (const FastItem*)st->d_items.p;
+  int e0 = prof_mark(ctx), e1, e2, e3;
   if (!acc_f64) {
-    hipLaunchKernelGGL((k_absorb_bf16<2>), dim3(128, ni), dim3(256), 0, ctx->stream, items,
-                       (const __bf16*)st->lay16.p, (const float*)msg_in, (float*)st->P32.p);
-    hipLaunchKernelGGL((k_fused_bf16<2>), dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
-                       (const __bf16*)st->lay16.p, (const float*)msg_in, (const float*)st->P32.p,
-                       (float*)st->part32.p);
+    if (fast_precise()) {
+      hipLaunchKernelGGL((k_absorb_bf16<2>), dim3(128, ni), dim3(256), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (float*)st->P32.p);
+      e1 = prof_mark(ctx);
+      hipLaunchKernelGGL((k_fused_bf16<2>), dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (const float*)st->P32.p,
+                         (float*)st->part32.p);
+    } else {
+      hipLaunchKernelGGL(k_absorb_bf16c, dim3(64, ni), dim3(512), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (__bf16*)st->P16.p);
+      e1 = prof_mark(ctx);
+      hipLaunchKernelGGL(k_fused_bf16c, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (const __bf16*)st->P16.p,
+                         (float*)st->part32.p);
+    }
+    e2 = prof_mark(ctx);
     hipLaunchKernelGGL((k_fast_finalize<float, float>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
                        (const float*)st->part32.p, (const float*)msg_in, (float*)msg_out, stats, normalize);
+    e3 = prof_mark(ctx);
+    prof_add(ctx, "absorb_bf16", e0, e1);
+    prof_add(ctx, "fused_bf16", e1, e2);
+    prof_add(ctx, "finalize", e2, e3);
   } else {
     hipLaunchKernelGGL(k_absorb_f64, dim3(128, ni), dim3(256), 0, ctx->stream, items,
                        (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
+    e1 = prof_mark(ctx);
     hipLaunchKernelGGL(k_fused_f64, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
                        (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
                        (double*)st->part64.p);
+    e2 = prof_mark(ctx);
     hipLaunchKernelGGL((k_fast_finalize<double, double>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
                        (const double*)st->part64.p, (const double*)msg_in, (double*)msg_out, stats, normalize);
+    e3 = prof_mark(ctx);
+    prof_add(ctx, "absorb_f64", e0, e1);
+    prof_add(ctx, "fused_f64", e1, e2);
+    prof_add(ctx, "finalize", e2, e3);
   }
   ITNBP_HIPCHK(ctx, hipGetLastError());
   return ITNBP_OK;

@@ -85,6 +85,9 @@ SIGNATURES = {
     "itnbp_energy_grad": (_I, [_P, _I, _I, _DP, _DP, _DP, _DP]),
     "itnbp_update_batched": (_I, [_I, C.POINTER(_P), C.POINTER(UpdateOpts), C.POINTER(UpdateResult)]),
     "itnbp_last_path": (C.c_char_p, [_P]),
+    "itnbp_profile_enable": (_I, [_P, _I]),
+    "itnbp_profile_reset": (_I, [_P]),
+    "itnbp_profile_query": (_I, [_P, C.c_char_p, _IP, _DP]),
 }
 
 _lib = None
@@ -142,6 +145,16 @@ class Context:
         if rc != OK:
             msg = self.lib.itnbp_last_error(self.h)
             raise ITNBPError(rc, msg.decode() if msg else "")
+
+    def profile(self, enable=True):
+        self.check(self.lib.itnbp_profile_enable(self.h, int(enable)))
+        self.check(self.lib.itnbp_profile_reset(self.h))
+
+    def kernel_time(self, name):
+        """(launch count, summed ms) of a library kernel since the last profile reset."""
+        n, t = C.c_int(), C.c_double()
+        self.check(self.lib.itnbp_profile_query(self.h, name.encode(), C.byref(n), C.byref(t)))
+        return n.value, t.value
 
     def last_path(self):
         return self.lib.itnbp_last_path(self.h).decode()

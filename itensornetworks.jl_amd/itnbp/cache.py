@@ -538,3 +538,70 @@ def normalize(psi, alg="bp", *, cache=None, update_cache=None, cache_update_kwar
         cache[0] = update_(cache[0], **(cache_update_kwargs or {}))
     cache[0] = rescale_(cache[0])
     return cache[0].tensornetwork()
+
+
+def energy_and_gradient(psi, nsweeps, ops1=None, ops2=None, *, checkpoint=1, ctx=None):
+    """BP energy E = sum_v <ops1[v]> + sum_e <ops2[e]> after ``nsweeps`` Jacobi sweeps
+    from identity messages, and dE/dpsi_v for every site tensor, by the library's
+    hand-written adjoint kernels (sweeps checkpointed every ``checkpoint``).  This is
+    the pullback the Julia shim wraps as a ChainRulesCore rrule; the reference has
+    no autodiff.  ``ops1``: {v: d x d} or callable; ``ops2``: {(u, v): (d_u d_v)^2} or
+    callable (u, v) -> matrix (first site fastest).  Returns (E, {v: grad})."""
+    bpc = BeliefPropagationCache(QuadraticFormNetwork(psi), storage="f64", ctx=ctx)
+    g = psi.graph
+    o1 = o2 = None
+    if ops1 is not None:
+        o1 = np.concatenate([np.asarray(ops1(v) if callable(ops1) else ops1[v], dtype=np.float64).reshape(-1, order="F")
+                             for v in g.vertices])
+    if ops2 is not None:
+        mats = []
+        for (u, v) in g.edges:
+            m = np.asarray(ops2(u, v) if callable(ops2) else ops2[(u, v)], dtype=np.float64)
+            mats.append(m.reshape(-1, order="F"))
+        o2 = np.concatenate(mats) if mats else None
+    E = C.c_double()
+    grad = np.empty(int(bpc.lib.itnbp_net_psi_numel(bpc.h)), dtype=np.float64)
+    bpc.ctx.check(bpc.lib.itnbp_energy_grad(bpc.h, int(nsweeps), int(checkpoint),
+                                            L.dptr(o1) if o1 is not None else None,
+                                            L.dptr(o2) if o2 is not None else None,
+                                            C.byref(E), L.dptr(grad)))
+    out, off = {}, 0
+    for v in g.vertices:
+        n = psi[v].size
+        out[v] = grad[off:off + n].reshape(psi[v].shape, order="F")
+        off += n
+    return E.value, out
+
+
+def update_batched_(caches, *, maxiter, tol=None, res_tol=None, precision="f64", normalize=True):
+    """Update many independent caches (any lattices and bond dimensions, one storage
+    dtype) together: every message of every instance in the same launches of a
+    Jacobi sweep; converged instances drop out.  In place; returns the per-cache
+    results (each also in cache.last_update)."""
+    if not caches:
+        return []
+    if maxiter is None:
+        raise ValueError("You need to specify a number of iterations for BP!")
+    n = len(caches)
+    hs = (C.c_void_p * n)(*[c.h.value for c in caches])
+    o = L.UpdateOpts()
+    o.schedule = L.SCHED_PARALLEL
+    o.maxiter = int(maxiter)
+    o.tol = -1.0 if tol is None else float(tol)
+    o.res_tol = -1.0 if res_tol is None else float(res_tol)
+    o.precision = PRECISIONS[precision]
+    o.switch_tol = 0.0
+    o.normalize = int(normalize)
+    o.nseq = 0
+    o.seq = None
+    o.verbose = 0
+    res = (L.UpdateResult * n)()
+    ctx = caches[0].ctx
+    ctx.check(ctx.lib.itnbp_update_batched(n, hs, C.byref(o), res))
+    out = []
+    for c, r in zip(caches, res):
+        c.last_update = dict(iterations=r.iterations, iterations_low=r.iterations_low, mean_diff=r.mean_diff,
+                             max_residual=r.max_residual, converged=bool(r.converged), device_ms=r.device_ms,
+                             path=ctx.last_path())
+        out.append(c.last_update)
+    return out

@@ -26,6 +26,9 @@ def _mode(X, M, ax):
 
 
 def _open(g, psi, msgs, u, open_axes):
+    """Open contraction of vertex u (bp_oracle.open_contraction in torch): a
+    (rows x rows) matrix whose row index is the column-major flattening of the
+    open axes of the ket, the column index that of the bra."""
     t = psi[u]
     phi = t
     for a, (w, _) in enumerate(g.incident[u], start=1):
@@ -36,37 +39,20 @@ def _open(g, psi, msgs, u, open_axes):
     T = torch.tensordot(phi, t, dims=(rest, rest))
     if not open_axes:
         return T.reshape(1, 1)
-    rows = int(np.prod([t.shape[a] for a in sorted(open_axes)]))
-    # column-major flattening of the open axes (first open axis fastest)
     nd = len(open_axes)
+    rows = int(np.prod([t.shape[a] for a in sorted(open_axes)]))
     perm = list(range(nd - 1, -1, -1)) + list(range(2 * nd - 1, nd - 1, -1))
-    return T.permute(*perm).reshape(rows, rows).transpose(0, 1).contiguous() if nd > 1 else T
-
-
-def _rdm2(g, psi, msgs, u, v):
-    du, dv = psi[u].shape[0], psi[v].shape[0]
-    chi = psi[u].shape[g.axis(u, v)]
-    Au = _open(g, psi, msgs, u, [0, g.axis(u, v)])
-    Av = _open(g, psi, msgs, v, [0, g.axis(v, u)])
-    # Au[(s + d b), (s' + d b')] -> [s, b, s', b']
-    Au = Au.reshape(chi, du, chi, du).permute(3, 2, 1, 0).permute(0, 1, 2, 3)
-    Au4 = Au.permute(1, 0, 3, 2)  # [b', s', b, s] -> fix below
-    return Au4, Av
+    return T.permute(*perm).reshape(rows, rows)
 
 
 def rdm2_torch(g, psi, msgs, u, v):
-    """rho[(su, sv), (su', sv')] (su fastest) as a (du dv) x (du dv) tensor."""
+    """rho[(su, sv), (su', sv')] (su fastest) of the pair (u, v)."""
     du, dv = psi[u].shape[0], psi[v].shape[0]
     chi = psi[u].shape[g.axis(u, v)]
-    Au = _open(g, psi, msgs, u, [0, g.axis(u, v)])  # rows (s + du b), cols (s' + du b')
-    Av = _open(g, psi, msgs, v, [0, g.axis(v, u)])
-    Au4 = Au.reshape(chi, du, chi, du)   # C-order view of a column-major index: [b', s', b, s]
-    Au4 = Au4.permute(3, 2, 1, 0)        # [s, b, s', b']
-    Av4 = Av.reshape(chi, dv, chi, dv).permute(3, 2, 1, 0)
-    rho = torch.einsum("ibjc,kblc->ikjl", Au4, Av4)  # [su, sv, su', sv']
-    # column-major flatten (su fastest) of rows and columns
-    return rho.permute(1, 0, 3, 2).reshape(du * dv, du * dv).transpose(0, 1).contiguous() \
-        if False else rho.permute(3, 2, 1, 0).reshape(du * dv, du * dv).transpose(0, 1).contiguous()
+    Au = _open(g, psi, msgs, u, [0, g.axis(u, v)]).reshape(chi, du, chi, du).permute(1, 0, 3, 2)
+    Av = _open(g, psi, msgs, v, [0, g.axis(v, u)]).reshape(chi, dv, chi, dv).permute(1, 0, 3, 2)
+    rho = torch.einsum("ibjc,kblc->ikjl", Au, Av)  # [su, sv, su', sv']
+    return rho.permute(1, 0, 3, 2).reshape(du * dv, du * dv)
 
 
 def bp_energy(g, psi, nsweeps, ops1=None, ops2=None):
