@@ -716,6 +716,220 @@ __global__ __launch_bounds__(256) void k_fused_f64(const FastItem* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// FP64, register-resident flavour.  v_mfma_f64_16x16x4_f64 lane layout (measured):
+// A[i][k] in lane (i + 16 k), B[k][j] in lane (j + 16 k), D[4 v + l/16][l % 16]
+// in lane l, value v.  A 32x32 product is 2x2 output tiles x 8 K-steps; an
+// accumulator tile column is exactly the B fragment of the next product:
+// K-step t of a 32-row operand held as acc[mi][ni][v] is acc[t / 4][ni][t % 4].
+// ---------------------------------------------------------------------------
+#define MFMA_F64(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+constexpr int LD8 = 33;   // padded LDS row stride (doubles) of a 32x32 block
+
+struct Acc32 {
+  f64x4 t[2][2];
+};
+
+__device__ inline void acc_zero(Acc32& a) {
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni) a.t[mi][ni] = f64x4{0, 0, 0, 0};
+}
+
+// C += A_frag * B_frag, with A given as 16 fragments af[mi][t] and B as
+// fragments bf[t][ni]
+__device__ inline void mm32(Acc32& C, const double (&af)[2][8], const double (&bf)[8][2]) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) C.t[mi][ni] = MFMA_F64(af[mi][t], bf[t][ni], C.t[mi][ni]);
+}
+
+// C += A_frag * (accumulator B)
+__device__ inline void mm32_accb(Acc32& C, const double (&af)[2][8], const Acc32& B) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) C.t[mi][ni] = MFMA_F64(af[mi][t], B.t[t >> 2][ni][t & 3], C.t[mi][ni]);
+}
+
+// FP64 absorb: WG (256 threads) -> (s, z, 8 consecutive w); each wave 2 inputs;
+// Y = Ma^T (X Mb) in registers, staged [y' + 32 x'][w] for 64-byte output runs.
+// grid (256, items)
+__global__ __launch_bounds__(256) void k_absorb_f64r(const FastItem* __restrict__ items,
+                                                     const double* __restrict__ lay,
+                                                     const double* __restrict__ msg,
+                                                     double* __restrict__ P) {
+  constexpr int SROW = 10;  // staging row stride (doubles): 80 bytes
+  __shared__ __attribute__((aligned(16))) double S[FBLK * SROW];
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int s = q & 1, z = (q >> 1) & 31, w0 = (q >> 6) * 8;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lg = l >> 4;
+  double bMb[8][2], aMaT[2][8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      // Mb[y][y'] = msg[Mb + y + 32 y'];  B fragment (k = 4t + lg, j = 16 h + li)
+      bMb[t][h] = msg[it.Mb + (4 * t + lg) + FC * (16 * h + li)];
+      // Ma^T[x'][x] = Ma[x][x'];  A fragment (i = 16 h + li, k = 4t + lg)
+      aMaT[h][t] = msg[it.Ma + (4 * t + lg) + FC * (16 * h + li)];
+    }
+#pragma unroll 1
+  for (int i = 0; i < 2; ++i) {
+    const int wl = 2 * wv + i;
+    const int w = w0 + wl;
+    const double* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * w)) * FBLK;
+    double aX[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) aX[h][t] = X[(16 * h + li) * FC + 4 * t + lg];
+    Acc32 V, Y;
+    acc_zero(V);
+    acc_zero(Y);
+    mm32(V, aX, bMb);
+    mm32_accb(Y, aMaT, V);
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int xb = 16 * mi + 4 * v + lg, yb = 16 * ni + li;
+          S[(yb + FC * xb) * SROW + wl] = Y.t[mi][ni][v];
+        }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < FBLK * 4; k += blockDim.x) {
+    const int o = k >> 2, part = k & 3;
+    const int yb = o & 31, xb = o >> 5;
+    const long long r = s + 2 * (xb + FC * yb);
+    const double2 v2 = *reinterpret_cast<const double2*>(S + o * SROW + 2 * part);
+    *reinterpret_cast<double2*>(P + it.P + r * FBLK + z * FC + w0 + 2 * part) = v2;
+  }
+}
+
+// FP64 fused: per block r (one wave), Tz^T += psi_r (Mw^T P_r^T), Tw += P_r^T (Mz psi_r);
+// the P_r and psi_r blocks are staged in LDS; grid (16, items)
+__global__ __launch_bounds__(256) void k_fused_f64r(const FastItem* __restrict__ items,
+                                                    const double* __restrict__ lay,
+                                                    const double* __restrict__ msg,
+                                                    const double* __restrict__ P,
+                                                    double* __restrict__ part) {
+  __shared__ double Pb[4][FC * LD8];
+  __shared__ double Sb[4][FC * LD8];
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lg = l >> 4;
+  double aMwT[2][8], aMz[2][8];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      // Mw^T[w'][w] = Mw[w][w'] : A fragment i = w' = 16h + li, k = w = 4t + lg
+      aMwT[h][t] = msg[it.Mw + (4 * t + lg) + FC * (16 * h + li)];
+      // Mz[z][z'] : A fragment i = z = 16h + li, k = z' = 4t + lg
+      aMz[h][t] = msg[it.Mz + (16 * h + li) + FC * (4 * t + lg)];
+    }
+  Acc32 tzT, tw;
+  acc_zero(tzT);
+  acc_zero(tw);
+  double* pb = Pb[wv];
+  double* sb = Sb[wv];
+  constexpr int NB = FUSED_R_PER_WG / 4;
+  const long long r0 = (long long)q * FUSED_R_PER_WG + wv * NB;
+#pragma unroll 1
+  for (int i = 0; i < NB; ++i) {
+    const double* Pg = P + it.P + (r0 + i) * FBLK;
+    const double* Yg = lay + it.psi_out + (r0 + i) * FBLK;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int o = 2 * l + 128 * j;  // two consecutive doubles per lane per step
+      const double2 pv = *reinterpret_cast<const double2*>(Pg + o);
+      const double2 yv = *reinterpret_cast<const double2*>(Yg + o);
+      const int a = o >> 5, b = o & 31;
+      pb[a * LD8 + b] = pv.x;
+      pb[a * LD8 + b + 1] = pv.y;
+      sb[a * LD8 + b] = yv.x;
+      sb[a * LD8 + b + 1] = yv.y;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // X^T = Mw^T P_r^T : B fragment (k = w = 4t + lg, j = z = 16h + li) = P_r[z][w]
+    {
+      double bPT[8][2];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) bPT[t][h] = pb[(16 * h + li) * LD8 + 4 * t + lg];
+      Acc32 XT;
+      acc_zero(XT);
+      mm32(XT, aMwT, bPT);
+      // Tz^T += psi_r X^T : A fragment (i = y = 16h + li, k = w' = 4t + lg) = psi_r[y][w']
+      double aY[2][8];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) aY[h][t] = sb[(16 * h + li) * LD8 + 4 * t + lg];
+      mm32_accb(tzT, aY, XT);
+    }
+    // Zm = Mz psi_r : B fragment (k = z' = 4t + lg, j = y = 16h + li) = psi_r[z'][y]
+    {
+      double bY[8][2];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) bY[t][h] = sb[(4 * t + lg) * LD8 + 16 * h + li];
+      Acc32 Zm;
+      acc_zero(Zm);
+      mm32(Zm, aMz, bY);
+      // Tw += P_r^T Zm : A fragment (i = x = 16h + li, k = z = 4t + lg) = P_r[z][x]
+      double aPT[2][8];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) aPT[h][t] = pb[(4 * t + lg) * LD8 + 16 * h + li];
+      mm32_accb(tw, aPT, Zm);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // reduce the four waves: Tz[x][y] from tz^T[y][x], Tw[x][y]
+  __syncthreads();
+  double* red = &Pb[0][0];  // 4 * 32 * 33 >= 2 * 1024
+  for (int w = 0; w < 4; ++w) {
+    if (wv == w) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * mi + 4 * v + lg, j = 16 * ni + li;
+            if (w == 0) {
+              red[j * FC + i] = tzT.t[mi][ni][v];
+              red[FBLK + i * FC + j] = tw.t[mi][ni][v];
+            } else {
+              red[j * FC + i] += tzT.t[mi][ni][v];
+              red[FBLK + i * FC + j] += tw.t[mi][ni][v];
+            }
+          }
+    }
+    __syncthreads();
+  }
+  double* dst = part + it.part + (long long)q * 2 * FBLK;
+  for (int o = threadIdx.x; o < 2 * FBLK; o += blockDim.x) dst[o] = red[o];
+}
+
+// ---------------------------------------------------------------------------
 // finaliser: sum partials (FP64), normalise, message_diff / residual, write
 // ---------------------------------------------------------------------------
 template <typename TP, typename TM>
@@ -792,6 +1006,13 @@ FastState* state_of(itnbp_net* net) { return reinterpret_cast<FastState*>(net->f
 
 // BF16 flavour: compact (P and the per-block intermediates as one BF16 term,
 // default) or precise (P in FP32, every operand split hi+lo); ITNBP_BF16_PRECISE=1
+// FP64 flavour: register-resident MFMA operands (default) or the LDS-tile
+// GEMMs (ITNBP_F64_LDS=1)
+bool fast_f64_lds() {
+  const char* e = getenv("ITNBP_F64_LDS");
+  return e && e[0] == '1';
+}
+
 bool fast_precise() {
   const char* e = getenv("ITNBP_BF16_PRECISE");
   return e && e[0] == '1';
@@ -989,12 +1210,21 @@ int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, M
     prof_add(ctx, "fused_bf16", e1, e2);
     prof_add(ctx, "finalize", e2, e3);
   } else {
-    hipLaunchKernelGGL(k_absorb_f64, dim3(128, ni), dim3(256), 0, ctx->stream, items,
-                       (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
-    e1 = prof_mark(ctx);
-    hipLaunchKernelGGL(k_fused_f64, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
-                       (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
-                       (double*)st->part64.p);
+    if (fast_f64_lds()) {
+      hipLaunchKernelGGL(k_absorb_f64, dim3(128, ni), dim3(256), 0, ctx->stream, items,
+                         (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
+      e1 = prof_mark(ctx);
+      hipLaunchKernelGGL(k_fused_f64, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                         (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
+                         (double*)st->part64.p);
+    } else {
+      hipLaunchKernelGGL(k_absorb_f64r, dim3(256, ni), dim3(256), 0, ctx->stream, items,
+                         (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
+      e1 = prof_mark(ctx);
+      hipLaunchKernelGGL(k_fused_f64r, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                         (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
+                         (double*)st->part64.p);
+    }
     e2 = prof_mark(ctx);
     hipLaunchKernelGGL((k_fast_finalize<double, double>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
                        (const double*)st->part64.p, (const double*)msg_in, (double*)msg_out, stats, normalize);

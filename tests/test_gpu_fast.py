@@ -31,7 +31,9 @@ def oracle_ref(net34):
     return m3, m40
 
 
-def test_f64_pipeline_matches_oracle(net34):
+@pytest.mark.parametrize("lds", ["0", "1"])
+def test_f64_pipeline_matches_oracle(net34, lds, monkeypatch):
+    monkeypatch.setenv("ITNBP_F64_LDS", lds)
     g, psi = net34
     bpc = itnbp.update(itnbp.initialize_cache(psi), maxiter=3, edge_sequence="parallel")
     assert "mfma-f64" in bpc.last_update["path"]
