@@ -109,6 +109,10 @@ def main():
     ap.add_argument("--workload", default="peps32x32_chi32_bf16", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed sweeps, copy the BF16 site tensors device-to-device once "
+                         "(a copy of known size for calibrating HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -137,6 +141,12 @@ def main():
     ctx.check(chosen.lib.itnbp_synchronize(ctx.h))
     wall = time.perf_counter() - t1
     path = chosen.last_update["path"]
+    if a.copy_probe:
+        probe = chosen.copy()
+        ctx.check(chosen.lib.itnbp_synchronize(ctx.h))
+        nbytes = int(chosen.lib.itnbp_net_psi_numel(chosen.h)) * (2 if low else 8)
+        log(f"copy probe: {nbytes} bytes of site tensors copied device-to-device")
+        del probe
     kernels = {}
     for name in ("fused_bf16", "absorb_bf16", "fused_f64", "absorb_f64", "finalize", "oc_small", "oc_large"):
         c, ms = ctx.kernel_time(name)
@@ -166,51 +176,75 @@ def main():
     log(f"fp64 path {plain.last_update['path']}: {ksw} sweeps in {wall64 * 1e3:.1f} ms -> {rate64:.4g} updates/s; {k64}")
 
     # ---- roofline of the dominant kernel of the chosen path
+    # units: 32x32 blocks (every eligible vertex x 2 passes x 2048 blocks per sweep, split
+    # over the launches the autotuner chose); per block the absorb kernel does 2 32^3
+    # GEMMs and moves 2 KiB of psi in + 2 KiB of P out (BF16 compact; P in FP32 for the
+    # precise flavour, FP64 throughout for the FP64 flavour), the fused kernel 4 GEMMs
+    # and 2 KiB P + 2 KiB psi in.
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     nbulk = sum(1 for v in g.vertices if g.degree(v) == 4) if W["chi"] == 32 else 0
     roof = None
-    if dom in ("fused_bf16", "fused_f64", "absorb_bf16", "absorb_f64") and nbulk:
-        # algorithmic flops per launch: every bulk vertex, both passes, 2048 32x32 blocks;
-        # fused: 4 32^3 GEMMs per block, absorb: 2 per block
-        per_block = (4 if dom.startswith("fused") else 2) * 2 * 32 ** 3
-        flops = nbulk * 2 * 2048 * per_block
+    if dom in ("fused_bf16", "absorb_bf16", "fused_f64", "absorb_f64") and nbulk:
+        launches_per_sweep = kernels[dom]["launches"] / a.steps
+        blocks = 2 * nbulk * 2048 / launches_per_sweep
+        fused = dom.startswith("fused")
+        flops = blocks * (4 if fused else 2) * 2 * 32 ** 3
+        precise = os.environ.get("ITNBP_BF16_PRECISE") == "1"
+        if dom.endswith("bf16"):
+            per_block = (4096 + 2048) if precise else 4096
+        else:
+            per_block = 2 * 8192
+        nbytes = blocks * per_block
         avg_s = kernels[dom]["avg_ms"] / 1e3
-        peak = PEAK_BF16_TFLOPS if dom.endswith("bf16") else PEAK_F64_TFLOPS
-        ach = flops / avg_s / 1e12
-        roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None, "flops_per_launch": flops,
-                "avg_launch_ms": kernels[dom]["avg_ms"]}
-    elif kernels:
-        c = kernels[dom]
-        roof = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": None, "traffic": None, "avg_launch_ms": c["avg_ms"]}
+        ai = flops / nbytes
+        ridge = (PEAK_BF16_TFLOPS if dom.endswith("bf16") else PEAK_F64_TFLOPS) * 1e12 / (PEAK_HBM_GBS * 1e9)
+        if ai < ridge:
+            ach = nbytes / avg_s / 1e9
+            roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM_GBS, "traffic": None}
+        else:
+            peak = PEAK_BF16_TFLOPS if dom.endswith("bf16") else PEAK_F64_TFLOPS
+            ach = flops / avg_s / 1e12
+            roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                    "frac": ach / peak, "traffic": None}
+        roof.update({"algorithmic_bytes_per_launch": nbytes, "flops_per_launch": flops,
+                     "blocks_per_launch": blocks, "arith_intensity": ai,
+                     "avg_launch_ms": kernels[dom]["avg_ms"],
+                     "achieved_tflops": flops / avg_s / 1e12})
 
-    # ---- full solves and accuracy of <X>, <ZZ> vs FP64
+    # ---- full solves and accuracy of <X>, <ZZ> vs FP64 (one-time per-network setup --
+    # block layouts, autotuning -- is done by a warm-up sweep before the clock starts)
     solve = None
     if not a.no_solve and low:
         del chosen
-        ref = itnbp.initialize_cache(psi, storage="f64")
-        t3 = time.perf_counter()
-        itnbp.update_(ref, maxiter=100, res_tol=1e-10, edge_sequence="parallel")
-        ctx.check(ref.lib.itnbp_synchronize(ctx.h))
-        t_ref = time.perf_counter() - t3
+
+        def timed_solve(storage, **kw):
+            c = itnbp.initialize_cache(psi, storage=storage)
+            itnbp.update_(c, maxiter=1, edge_sequence="parallel",
+                          precision="f32" if kw.get("precision") == "bf16_f64" else "f64")
+            if kw.get("precision") == "bf16_f64":
+                itnbp.update_(c, maxiter=1, edge_sequence="parallel")
+            c.ctx.check(c.lib.itnbp_init_identity_messages(c.h))
+            ctx.check(c.lib.itnbp_synchronize(ctx.h))
+            t = time.perf_counter()
+            itnbp.update_(c, maxiter=100, res_tol=1e-10, edge_sequence="parallel", **kw)
+            ctx.check(c.lib.itnbp_synchronize(ctx.h))
+            return c, time.perf_counter() - t
+
+        ref, t_ref = timed_solve("f64")
         it_ref = ref.last_update["iterations"]
-        mix = itnbp.initialize_cache(psi, storage="bf16")
-        t4 = time.perf_counter()
-        itnbp.update_(mix, maxiter=100, res_tol=1e-10, edge_sequence="parallel", precision="bf16_f64",
-                      switch_tol=1e-4)
-        ctx.check(mix.lib.itnbp_synchronize(ctx.h))
-        t_mix = time.perf_counter() - t4
+        mix, t_mix = timed_solve("bf16", precision="bf16_f64", switch_tol=1e-4)
         lu = mix.last_update
         zz = np.kron(np.diag([1.0, -1.0]), np.diag([1.0, -1.0]))
         x_ref = np.array(list(itnbp.expect_on_cache(ref, "X").values()))
         x_mix = np.array(list(itnbp.expect_on_cache(mix, "X").values()))
         z_ref = np.array(list(itnbp.expect2_on_cache(ref, zz).values()))
         z_mix = np.array(list(itnbp.expect2_on_cache(mix, zz).values()))
-        # accuracy of the low phase alone: BF16 sweeps to residual 1e-4
+        del mix
         lowc = itnbp.initialize_cache(psi, storage="bf16")
         itnbp.update_(lowc, maxiter=100, res_tol=1e-4, edge_sequence="parallel", precision="f32")
         x_low = np.array(list(itnbp.expect_on_cache(lowc, "X").values()))
+        del lowc
         solve = {
             "fp64": {"sweeps": it_ref, "seconds": t_ref, "updates_per_s": nmsg * it_ref / t_ref,
                      "max_residual": ref.last_update["max_residual"]},
@@ -224,9 +258,57 @@ def main():
             "bf16_phase_only_err_X_rel": float(np.linalg.norm(x_low - x_ref) / np.linalg.norm(x_ref)),
         }
         log(f"solve {solve}")
-        del ref, mix, lowc
+        del ref
+
+    # ---- secondary workloads: gradient time (8x8, chi=4, 40 sweeps) and the
+    # 16x16 chi=8 BP solve + one BP-gauged simple-update round
+    extras = {}
+    if not a.no_extras:
+        g5 = itnbp.named_grid((8, 8))
+        psi5 = itnbp.random_tensornetwork(np.random.default_rng(5), np.float64, g5, link_space=4)
+        X = np.array([[0.0, 1.0], [1.0, 0.0]])
+        ZZ = np.diag([1.0, -1.0, -1.0, 1.0])
+        ops1 = {v: 2.5 * X for v in g5.vertices}
+        ops2 = {e: ZZ for e in g5.edges}
+        itnbp.energy_and_gradient(psi5, 40, ops1, ops2, checkpoint=8)
+        t = time.perf_counter()
+        E5, _ = itnbp.energy_and_gradient(psi5, 40, ops1, ops2, checkpoint=8)
+        extras["grad_8x8_chi4_40sweeps"] = {"seconds": time.perf_counter() - t, "energy": E5,
+                                            "checkpoint": 8}
+        g2 = itnbp.named_grid((16, 16))
+        psi2 = itnbp.random_tensornetwork(np.random.default_rng(2), np.float64, g2, link_space=8)
+        c2 = itnbp.initialize_cache(psi2)
+        itnbp.update_(c2, maxiter=2, edge_sequence="parallel")
+        c2.ctx.check(c2.lib.itnbp_init_identity_messages(c2.h))
+        t = time.perf_counter()
+        itnbp.update_(c2, maxiter=500, tol=1e-10, edge_sequence="parallel")
+        ctx.check(c2.lib.itnbp_synchronize(ctx.h))
+        t_bp = time.perf_counter() - t
+        deg = {v: g2.degree(v) for v in g2.vertices}
+
+        def gate(u, v):
+            H = np.kron(np.diag([1.0, -1.0]), np.diag([1.0, -1.0])) + 2.5 * (
+                np.kron(np.eye(2), X) / deg[u] + np.kron(X, np.eye(2)) / deg[v])
+            w, U = np.linalg.eigh(H)
+            return (U * np.exp(-0.05 * w)) @ U.T
+
+        t = time.perf_counter()
+        errs = itnbp.simple_update_round_(c2, gate, maxdim=8, normalize=True)
+        ctx.check(c2.lib.itnbp_synchronize(ctx.h))
+        t_ap = time.perf_counter() - t
+        extras["bp_16x16_chi8_f64"] = {"sweeps": c2.last_update["iterations"], "seconds": t_bp,
+                                       "updates_per_s": 2 * g2.ne * c2.last_update["iterations"] / t_bp,
+                                       "path": c2.last_update["path"]}
+        extras["apply_round_16x16_chi8"] = {"bonds": len(errs), "seconds": t_ap, "svd_per_s": len(errs) / t_ap,
+                                            "max_truncerr": max(errs.values())}
+        log(f"extras {extras}")
 
     cpu_rate, cpu_n = cpu_baseline(psi, g, a.cpu_seconds)
+    try:
+        from threadpoolctl import threadpool_info
+        cpu_threads = max([p["num_threads"] for p in threadpool_info() if p.get("user_api") == "blas"] or [1])
+    except Exception:
+        cpu_threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
     line = {
         "metric": "BP message updates/s (chosen BF16x2-MFMA path; plain FP64 GPU path and solves alongside)",
         "value": value,
@@ -250,7 +332,8 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "solve": solve,
-        "cpu_baseline": {"value": cpu_rate, "unit": "message-updates/s", "cores": os.cpu_count(),
+        "extras": extras,
+        "cpu_baseline": {"value": cpu_rate, "unit": "message-updates/s", "cores": cpu_threads,
                          "kind": "port",
                          "sample": f"{cpu_n} FP64 bulk-vertex message updates (chi=32) by the NumPy oracle"},
     }

@@ -814,6 +814,8 @@ int itnbp_update(itnbp_net* net, const itnbp_update_opts* o, itnbp_update_result
     float ms = 0;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     r.device_ms = ms;
+    for (int a = 0; a < 2; ++a)
+      if (fast_chunk(net, a) > 0) ctx->path += std::string(a ? " [f64 chunk " : " [bf16 chunk ") + std::to_string(fast_chunk(net, a)) + "]";
     r.mean_diff = md;
     r.max_residual = mr;
     r.converged = conv;

@@ -36,6 +36,8 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
 int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, MsgStat* stats,
                int normalize);
 void release_fast_state(itnbp_net* net);
+// items per launch pair chosen by the autotuner (0 before the first sweep)
+int fast_chunk(itnbp_net* net, int acc_f64);
 void invalidate_fast_state(itnbp_net* net);
 
 #define ITNBP_RETCHK(call)          \

@@ -14,6 +14,8 @@
 //   large - mode products through a global scratch (one launch per absorbed
 //           axis over every task of a batch), then a split-K contraction with
 //           LDS tiles and a deterministic finaliser.
+#include <cstdlib>
+
 #include "itnbp_internal.hpp"
 
 namespace {
@@ -254,6 +256,82 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_mp_level(const VDesc* __restric
   }
 }
 
+// LDS-tiled variant of k_mp_level: a tile is either TH consecutive `hi` slabs
+// (A n contiguous elements each) or, for large A n, a run of TL consecutive `lo`
+// values for every x at one `hi`; the tile is loaded coalesced into LDS and
+// every output reads its n inputs from there.
+#define MP_TILE 4096
+template <typename TP, typename TM, typename TA>
+__global__ __launch_bounds__(ITNBP_BLOCK) void k_mp_tiled(const VDesc* __restrict__ vds,
+                                                          const OcTask* __restrict__ tasks,
+                                                          int level, const TP* __restrict__ psi,
+                                                          const TM* __restrict__ msg_in,
+                                                          TA* __restrict__ scratch,
+                                                          long long stride) {
+  __shared__ TA Xt[MP_TILE];
+  __shared__ TA Mst[64 * 64];
+  const OcTask t = tasks[blockIdx.y];
+  const VDesc V = vds[t.v];
+  const int a = nth_absorbed_axis(V, t.absorb_mask, level);
+  if (a == 0) return;
+  int A = 1;
+  for (int b = 0; b < a; ++b) A *= V.dims[b];
+  const int n = V.dims[a];
+  const int B = V.D / (A * n);
+  const TM* M = msg_in + V.in_msg[a];
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) Mst[i] = (TA)M[i];
+  TA* S0 = scratch + (long long)blockIdx.y * 2 * stride;
+  TA* Y = S0 + (level & 1) * stride;
+  const TA* Xs = S0 + ((level - 1) & 1) * stride;
+  const TP* ps = psi + V.psi_off;
+  const int An = A * n;
+  if (An <= MP_TILE) {
+    const int TH = MP_TILE / An;
+    const int ntiles = (B + TH - 1) / TH;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+      const int hi0 = tile * TH;
+      const int th = min(TH, B - hi0);
+      const int len = th * An;
+      const long long base = (long long)hi0 * An;
+      __syncthreads();
+      for (int i = threadIdx.x; i < len; i += blockDim.x)
+        Xt[i] = level == 0 ? load_as<TA>(ps, base + i) : Xs[base + i];
+      __syncthreads();
+      for (int o = threadIdx.x; o < len; o += blockDim.x) {
+        const int lo = o % A, rest = o / A, c = rest % n, hl = rest / n;
+        const TA* xr = Xt + lo + An * hl;
+        const TA* mc = Mst + n * c;
+        TA acc = 0;
+        for (int x = 0; x < n; ++x) acc += xr[A * x] * mc[x];
+        Y[base + o] = acc;
+      }
+    }
+  } else {
+    const int TL = MP_TILE / n;
+    const int ntl = (A + TL - 1) / TL;
+    const int ntiles = ntl * B;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+      const int hi = tile / ntl;
+      const int lo0 = (tile % ntl) * TL;
+      const int tl = min(TL, A - lo0);
+      __syncthreads();
+      for (int i = threadIdx.x; i < n * tl; i += blockDim.x) {
+        const int x = i / tl, j = i % tl;
+        const long long src = lo0 + j + (long long)A * (x + (long long)n * hi);
+        Xt[x * TL + j] = level == 0 ? load_as<TA>(ps, src) : Xs[src];
+      }
+      __syncthreads();
+      for (int o = threadIdx.x; o < n * tl; o += blockDim.x) {
+        const int c = o / tl, j = o % tl;
+        const TA* mc = Mst + n * c;
+        TA acc = 0;
+        for (int x = 0; x < n; ++x) acc += Xt[x * TL + j] * mc[x];
+        Y[lo0 + j + (long long)A * (c + (long long)n * hi)] = acc;
+      }
+    }
+  }
+}
+
 #define OC_RC 32  // rest indices per LDS tile
 
 // partial T over a slice of the rest index for every task
@@ -486,9 +564,15 @@ hipError_t large_impl(const VDesc* d_vd, const OcTask* d_tasks, const OcTask* h_
     maxD = std::max<long long>(maxD, h_vd[h_tasks[i].v].D);
   }
   const int chunks = (int)std::min<long long>(1024, (maxD + ITNBP_BLOCK * 4 - 1) / (ITNBP_BLOCK * 4));
+  static const bool untiled = getenv("ITNBP_MP_UNTILED") != nullptr;
   for (int lev = 0; lev < maxlev; ++lev) {
-    hipLaunchKernelGGL((k_mp_level<TP, TM, TA>), dim3(std::max(1, chunks), n), dim3(ITNBP_BLOCK), 0, s,
-                       d_vd, d_tasks, lev, (const TP*)psi, (const TM*)msg_in, (TA*)scratch, stride);
+    if (untiled)
+      hipLaunchKernelGGL((k_mp_level<TP, TM, TA>), dim3(std::max(1, chunks), n), dim3(ITNBP_BLOCK), 0, s,
+                         d_vd, d_tasks, lev, (const TP*)psi, (const TM*)msg_in, (TA*)scratch, stride);
+    else
+      hipLaunchKernelGGL((k_mp_tiled<TP, TM, TA>), dim3(std::max(1, std::min(256, (int)(maxD / MP_TILE) + 1)), n),
+                         dim3(ITNBP_BLOCK), 0, s, d_vd, d_tasks, lev, (const TP*)psi, (const TM*)msg_in,
+                         (TA*)scratch, stride);
   }
   hipLaunchKernelGGL((k_oc_partial<TP, TA>), dim3(nsplit, n), dim3(ITNBP_BLOCK), 0, s, d_vd, d_tasks,
                      (const TP*)psi, (const TA*)scratch, stride, (TA*)partials, part_stride);

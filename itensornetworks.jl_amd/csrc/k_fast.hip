@@ -999,7 +999,8 @@ struct FastState {
   std::vector<int> out_msgs;        // message ids produced
   DevBuf d_items, lay16, lay64, P32, P16, P64, part32, part64, offs;
   bool lay16_ok = false, lay64_ok = false;
-  std::vector<int> seq_key;         // sequence the slots were assigned for
+  int chunk[2] = {0, 0};            // items per absorb/fused launch pair, per precision (0: untuned)
+  std::vector<float> tune_ms[2];    // autotuner timings of the candidates
 };
 
 FastState* state_of(itnbp_net* net) { return reinterpret_cast<FastState*>(net->fast); }
@@ -1178,6 +1179,110 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   return ITNBP_OK;
 }
 
+// absorb + fused launches for items [i0, i1)
+static void launch_pair(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg_in, int i0, int i1) {
+  const int ni = i1 - i0;
+  const FastItem* items = (const FastItem*)st->d_items.p + i0;
+  const int e0 = prof_mark(ctx);
+  if (!acc_f64) {
+    if (fast_precise()) {
+      hipLaunchKernelGGL((k_absorb_bf16<2>), dim3(128, ni), dim3(256), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (float*)st->P32.p);
+    } else {
+      hipLaunchKernelGGL(k_absorb_bf16c, dim3(64, ni), dim3(512), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (__bf16*)st->P16.p);
+    }
+  } else if (fast_f64_lds()) {
+    hipLaunchKernelGGL(k_absorb_f64, dim3(128, ni), dim3(256), 0, ctx->stream, items,
+                       (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
+  } else {
+    hipLaunchKernelGGL(k_absorb_f64r, dim3(256, ni), dim3(256), 0, ctx->stream, items,
+                       (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
+  }
+  const int e1 = prof_mark(ctx);
+  if (!acc_f64) {
+    if (fast_precise()) {
+      hipLaunchKernelGGL((k_fused_bf16<2>), dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (const float*)st->P32.p,
+                         (float*)st->part32.p);
+    } else {
+      hipLaunchKernelGGL(k_fused_bf16c, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                         (const __bf16*)st->lay16.p, (const float*)msg_in, (const __bf16*)st->P16.p,
+                         (float*)st->part32.p);
+    }
+  } else if (fast_f64_lds()) {
+    hipLaunchKernelGGL(k_fused_f64, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                       (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
+                       (double*)st->part64.p);
+  } else {
+    hipLaunchKernelGGL(k_fused_f64r, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
+                       (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
+                       (double*)st->part64.p);
+  }
+  const int e2 = prof_mark(ctx);
+  prof_add(ctx, acc_f64 ? "absorb_f64" : "absorb_bf16", e0, e1);
+  prof_add(ctx, acc_f64 ? "fused_f64" : "fused_bf16", e1, e2);
+}
+
+static void launch_all_pairs(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg_in, int chunk) {
+  const int ni = (int)st->items.size();
+  for (int i0 = 0; i0 < ni; i0 += chunk) launch_pair(ctx, st, acc_f64, msg_in, i0, std::min(ni, i0 + chunk));
+}
+
+// Autotuner: the number of items per absorb/fused launch pair trades launch
+// count against reuse of the intermediate P in the last-level cache (a chunk
+// whose P fits in the 256 MiB Infinity Cache is read back from it).  Every
+// candidate is timed on one real sweep (same inputs; the outputs are
+// recomputed by the sweep that follows) and the fastest is kept per network
+// and precision.  ITNBP_FAST_CHUNK=<items> forces a value.
+static int tune_chunk(itnbp_net* net, FastState* st, int acc_f64, const void* msg_in) {
+  itnbp_ctx* ctx = net->ctx;
+  const int ni = (int)st->items.size();
+  if (const char* e = getenv("ITNBP_FAST_CHUNK")) {
+    const int c = atoi(e);
+    if (c > 0) return std::min(c, ni);
+  }
+  if (st->chunk[acc_f64] > 0) return st->chunk[acc_f64];
+  std::vector<int> cand{ni};
+  for (int c : {512, 192, 96, 48, 24})
+    if (c < ni) cand.push_back(c);
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return ni;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return ni;
+  }
+  const bool prof = ctx->prof;
+  ctx->prof = false;
+  int best = ni;
+  float best_ms = 1e30f;
+  st->tune_ms[acc_f64].clear();
+  launch_all_pairs(ctx, st, acc_f64, msg_in, ni);  // warm-up
+  for (int c : cand) {
+    (void)hipEventRecord(a, ctx->stream);
+    launch_all_pairs(ctx, st, acc_f64, msg_in, c);
+    (void)hipEventRecord(b, ctx->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 1e30f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    st->tune_ms[acc_f64].push_back(ms);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  ctx->prof = prof;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  st->chunk[acc_f64] = best;
+  return best;
+}
+
+int fast_chunk(itnbp_net* net, int acc_f64) {
+  FastState* st = state_of(net);
+  return st ? st->chunk[acc_f64 ? 1 : 0] : 0;
+}
+
 int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, MsgStat* stats,
                int normalize) {
   itnbp_ctx* ctx = net->ctx;
@@ -1185,54 +1290,17 @@ int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, M
   if (!st || st->items.empty()) return ITNBP_OK;
   const int ni = (int)st->items.size();
   const FastItem* items = (const FastItem*)st->d_items.p;
-  int e0 = prof_mark(ctx), e1, e2, e3;
+  const int chunk = tune_chunk(net, st, acc_f64, msg_in);
+  launch_all_pairs(ctx, st, acc_f64, msg_in, chunk);
+  const int e2 = prof_mark(ctx);
   if (!acc_f64) {
-    if (fast_precise()) {
-      hipLaunchKernelGGL((k_absorb_bf16<2>), dim3(128, ni), dim3(256), 0, ctx->stream, items,
-                         (const __bf16*)st->lay16.p, (const float*)msg_in, (float*)st->P32.p);
-      e1 = prof_mark(ctx);
-      hipLaunchKernelGGL((k_fused_bf16<2>), dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
-                         (const __bf16*)st->lay16.p, (const float*)msg_in, (const float*)st->P32.p,
-                         (float*)st->part32.p);
-    } else {
-      hipLaunchKernelGGL(k_absorb_bf16c, dim3(64, ni), dim3(512), 0, ctx->stream, items,
-                         (const __bf16*)st->lay16.p, (const float*)msg_in, (__bf16*)st->P16.p);
-      e1 = prof_mark(ctx);
-      hipLaunchKernelGGL(k_fused_bf16c, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
-                         (const __bf16*)st->lay16.p, (const float*)msg_in, (const __bf16*)st->P16.p,
-                         (float*)st->part32.p);
-    }
-    e2 = prof_mark(ctx);
     hipLaunchKernelGGL((k_fast_finalize<float, float>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
                        (const float*)st->part32.p, (const float*)msg_in, (float*)msg_out, stats, normalize);
-    e3 = prof_mark(ctx);
-    prof_add(ctx, "absorb_bf16", e0, e1);
-    prof_add(ctx, "fused_bf16", e1, e2);
-    prof_add(ctx, "finalize", e2, e3);
   } else {
-    if (fast_f64_lds()) {
-      hipLaunchKernelGGL(k_absorb_f64, dim3(128, ni), dim3(256), 0, ctx->stream, items,
-                         (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
-      e1 = prof_mark(ctx);
-      hipLaunchKernelGGL(k_fused_f64, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
-                         (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
-                         (double*)st->part64.p);
-    } else {
-      hipLaunchKernelGGL(k_absorb_f64r, dim3(256, ni), dim3(256), 0, ctx->stream, items,
-                         (const double*)st->lay64.p, (const double*)msg_in, (double*)st->P64.p);
-      e1 = prof_mark(ctx);
-      hipLaunchKernelGGL(k_fused_f64r, dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
-                         (const double*)st->lay64.p, (const double*)msg_in, (const double*)st->P64.p,
-                         (double*)st->part64.p);
-    }
-    e2 = prof_mark(ctx);
     hipLaunchKernelGGL((k_fast_finalize<double, double>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
                        (const double*)st->part64.p, (const double*)msg_in, (double*)msg_out, stats, normalize);
-    e3 = prof_mark(ctx);
-    prof_add(ctx, "absorb_f64", e0, e1);
-    prof_add(ctx, "fused_f64", e1, e2);
-    prof_add(ctx, "finalize", e2, e3);
   }
+  prof_add(ctx, "finalize", e2, prof_mark(ctx));
   ITNBP_HIPCHK(ctx, hipGetLastError());
   return ITNBP_OK;
 }
