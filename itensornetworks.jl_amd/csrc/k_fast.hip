@@ -781,16 +781,20 @@ __global__ __launch_bounds__(256) void k_absorb_f64r(const FastItem* __restrict_
       // Ma^T[x'][x] = Ma[x][x'];  A fragment (i = 16 h + li, k = 4t + lg)
       aMaT[h][t] = msg[it.Ma + (4 * t + lg) + FC * (16 * h + li)];
     }
-#pragma unroll 1
+  // both input blocks of this wave are requested before the first product
+  double aXs[2][2][8];
+#pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const int wl = 2 * wv + i;
-    const int w = w0 + wl;
-    const double* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * w)) * FBLK;
-    double aX[2][8];
+    const double* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * (w0 + 2 * wv + i))) * FBLK;
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int t = 0; t < 8; ++t) aX[h][t] = X[(16 * h + li) * FC + 4 * t + lg];
+      for (int t = 0; t < 8; ++t) aXs[i][h][t] = X[(16 * h + li) * FC + 4 * t + lg];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int wl = 2 * wv + i;
+    const double(&aX)[2][8] = aXs[i];
     Acc32 V, Y;
     acc_zero(V);
     acc_zero(Y);
@@ -846,23 +850,36 @@ __global__ __launch_bounds__(256) void k_fused_f64r(const FastItem* __restrict__
   double* sb = Sb[wv];
   constexpr int NB = FUSED_R_PER_WG / 4;
   const long long r0 = (long long)q * FUSED_R_PER_WG + wv * NB;
-#pragma unroll 1
-  for (int i = 0; i < NB; ++i) {
-    const double* Pg = P + it.P + (r0 + i) * FBLK;
-    const double* Yg = lay + it.psi_out + (r0 + i) * FBLK;
+  // software pipeline: the global loads of block i+1 are in flight (registers
+  // nP, nY) while block i is multiplied out of LDS
+  double2 nP[8], nY[8];
+  auto fetch = [&](long long r) {
+    const double* Pg = P + it.P + r * FBLK;
+    const double* Yg = lay + it.psi_out + r * FBLK;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      nP[j] = *reinterpret_cast<const double2*>(Pg + 2 * l + 128 * j);
+      nY[j] = *reinterpret_cast<const double2*>(Yg + 2 * l + 128 * j);
+    }
+  };
+  auto stage = [&]() {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int o = 2 * l + 128 * j;  // two consecutive doubles per lane per step
-      const double2 pv = *reinterpret_cast<const double2*>(Pg + o);
-      const double2 yv = *reinterpret_cast<const double2*>(Yg + o);
       const int a = o >> 5, b = o & 31;
-      pb[a * LD8 + b] = pv.x;
-      pb[a * LD8 + b + 1] = pv.y;
-      sb[a * LD8 + b] = yv.x;
-      sb[a * LD8 + b + 1] = yv.y;
+      pb[a * LD8 + b] = nP[j].x;
+      pb[a * LD8 + b + 1] = nP[j].y;
+      sb[a * LD8 + b] = nY[j].x;
+      sb[a * LD8 + b + 1] = nY[j].y;
     }
+  };
+  fetch(r0);
+  stage();
+#pragma unroll 1
+  for (int i = 0; i < NB; ++i) {
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
+    if (i + 1 < NB) fetch(r0 + i + 1);
     // X^T = Mw^T P_r^T : B fragment (k = w = 4t + lg, j = z = 16h + li) = P_r[z][w]
     {
       double bPT[8][2];
@@ -899,9 +916,11 @@ __global__ __launch_bounds__(256) void k_fused_f64r(const FastItem* __restrict__
         for (int t = 0; t < 8; ++t) aPT[h][t] = pb[(4 * t + lg) * LD8 + 16 * h + li];
       mm32_accb(tw, aPT, Zm);
     }
-    __builtin_amdgcn_s_waitcnt(0);
+    // every lane's LDS reads of block i are issued before any lane overwrites it
     __builtin_amdgcn_wave_barrier();
+    if (i + 1 < NB) stage();
   }
+  __builtin_amdgcn_s_waitcnt(0);
   // reduce the four waves: Tz[x][y] from tz^T[y][x], Tw[x][y]
   __syncthreads();
   double* red = &Pb[0][0];  // 4 * 32 * 33 >= 2 * 1024
