@@ -35,6 +35,21 @@ WORKLOADS = {
     "peps32x32_chi32_bf16": dict(dims=(32, 32), chi=32, storage="bf16", scale="invsqrtchi"),
     "peps16x16_chi8_f64": dict(dims=(16, 16), chi=8, storage="f64", scale=1.0),
 }
+KERNEL_NAMES = ("eigh", "rot12_bf16", "rot34_bf16", "gram12_bf16", "deg3_bf16", "rot12_f64", "rot34_f64",
+                "gram12_f64", "deg3_f64", "absorb_bf16", "fused_bf16", "absorb_f64", "fused_f64", "finalize",
+                "oc_small", "oc_large")
+# Per 32x32 block of a bulk (degree-4, chi = 32) site: algorithmic HBM bytes and
+# 32^3 GEMM units (2 * 32^3 flops each) of every streaming kernel, and how many
+# blocks of a site one sweep pushes through it (2048 blocks per site and pass).
+#   rot12: X in, U1^T X U2 out;  rot34: R in, Y out + two Grams (Y and Y^T
+#   rotations: 3 units);  gram12: Q in, two Grams (+ the exact identity product
+#   for D^T);  absorb / fused: the absorbed-pair passes (two per sweep).
+BLOCK_INFO = {
+    "rot12_bf16": (2048 + 2048, 2, 1), "rot34_bf16": (2048 + 2048, 5, 1), "gram12_bf16": (2048, 3, 1),
+    "rot12_f64": (8192 + 8192, 2, 1), "rot34_f64": (8192 + 8192, 5, 1), "gram12_f64": (8192, 2, 1),
+    "absorb_bf16": (2048 + 2048, 2, 2), "fused_bf16": (2048 + 2048, 4, 2),
+    "absorb_f64": (8192 + 8192, 2, 2), "fused_f64": (8192 + 8192, 4, 2),
+}
 PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 PEAK_F64_TFLOPS = 78.6      # MI355X FP64
 PEAK_HBM_GBS = 8000.0
@@ -148,7 +163,7 @@ def main():
         log(f"copy probe: {nbytes} bytes of site tensors copied device-to-device")
         del probe
     kernels = {}
-    for name in ("fused_bf16", "absorb_bf16", "fused_f64", "absorb_f64", "finalize", "oc_small", "oc_large"):
+    for name in KERNEL_NAMES:
         c, ms = ctx.kernel_time(name)
         if c:
             kernels[name] = dict(launches=c, total_ms=ms, avg_ms=ms / c)
@@ -167,7 +182,7 @@ def main():
     ctx.check(plain.lib.itnbp_synchronize(ctx.h))
     wall64 = time.perf_counter() - t2
     k64 = {}
-    for name in ("fused_f64", "absorb_f64", "finalize", "oc_small", "oc_large"):
+    for name in KERNEL_NAMES:
         c, ms = ctx.kernel_time(name)
         if c:
             k64[name] = dict(launches=c, total_ms=ms, avg_ms=ms / c)
@@ -175,38 +190,33 @@ def main():
     rate64 = nmsg * ksw / wall64
     log(f"fp64 path {plain.last_update['path']}: {ksw} sweeps in {wall64 * 1e3:.1f} ms -> {rate64:.4g} updates/s; {k64}")
 
-    # ---- roofline of the dominant kernel of the chosen path
-    # units: 32x32 blocks (every eligible vertex x 2 passes x 2048 blocks per sweep, split
-    # over the launches the autotuner chose); per block the absorb kernel does 2 32^3
-    # GEMMs and moves 2 KiB of psi in + 2 KiB of P out (BF16 compact; P in FP32 for the
-    # precise flavour, FP64 throughout for the FP64 flavour), the fused kernel 4 GEMMs
-    # and 2 KiB P + 2 KiB psi in.
-    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    # ---- roofline of the dominant kernel of the chosen path: algorithmic bytes
+    # (and GEMM flops) per launch from BLOCK_INFO x the blocks one launch covers,
+    # over the kernel's average launch time (HIP events on the library's stream)
+    dom = max((k for k in kernels if k in BLOCK_INFO), key=lambda k: kernels[k]["total_ms"], default=None)
     nbulk = sum(1 for v in g.vertices if g.degree(v) == 4) if W["chi"] == 32 else 0
     roof = None
-    if dom in ("fused_bf16", "absorb_bf16", "fused_f64", "absorb_f64") and nbulk:
+    if dom is not None and nbulk:
+        per_block, units, passes = BLOCK_INFO[dom]
+        if dom.startswith(("absorb_bf16", "fused_bf16")) and os.environ.get("ITNBP_BF16_PRECISE") == "1":
+            per_block += 2048   # P in FP32
         launches_per_sweep = kernels[dom]["launches"] / a.steps
-        blocks = 2 * nbulk * 2048 / launches_per_sweep
-        fused = dom.startswith("fused")
-        flops = blocks * (4 if fused else 2) * 2 * 32 ** 3
-        precise = os.environ.get("ITNBP_BF16_PRECISE") == "1"
-        if dom.endswith("bf16"):
-            per_block = (4096 + 2048) if precise else 4096
-        else:
-            per_block = 2 * 8192
+        blocks = passes * nbulk * 2048 / launches_per_sweep
+        flops = blocks * units * 2 * 32 ** 3
         nbytes = blocks * per_block
         avg_s = kernels[dom]["avg_ms"] / 1e3
         ai = flops / nbytes
-        ridge = (PEAK_BF16_TFLOPS if dom.endswith("bf16") else PEAK_F64_TFLOPS) * 1e12 / (PEAK_HBM_GBS * 1e9)
+        is_bf16 = dom.endswith("bf16")
+        peak_f = PEAK_BF16_TFLOPS if is_bf16 else PEAK_F64_TFLOPS
+        ridge = peak_f * 1e12 / (PEAK_HBM_GBS * 1e9)
         if ai < ridge:
             ach = nbytes / avg_s / 1e9
             roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": ach / PEAK_HBM_GBS, "traffic": None}
         else:
-            peak = PEAK_BF16_TFLOPS if dom.endswith("bf16") else PEAK_F64_TFLOPS
             ach = flops / avg_s / 1e12
-            roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                    "frac": ach / peak, "traffic": None}
+            roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak_f, "unit": "TFLOP/s",
+                    "frac": ach / peak_f, "traffic": None}
         roof.update({"algorithmic_bytes_per_launch": nbytes, "flops_per_launch": flops,
                      "blocks_per_launch": blocks, "arith_intensity": ai,
                      "avg_launch_ms": kernels[dom]["avg_ms"],
@@ -301,6 +311,32 @@ def main():
                                        "path": c2.last_update["path"]}
         extras["apply_round_16x16_chi8"] = {"bonds": len(errs), "seconds": t_ap, "svd_per_s": len(errs) / t_ap,
                                             "max_truncerr": max(errs.values())}
+        del c2
+        # 256 heterogeneous instances (L x L grids, L in 6..12, or heavy-hex), chi in
+        # {4, 6, 8, 12, 16}, FP32 N(0, 1/chi), 50 Jacobi sweeps in shared launches
+        rng3 = np.random.default_rng(3)
+        caches, nmsg3 = [], 0
+        for _ in range(256):
+            if rng3.random() < 0.5:
+                L = int(rng3.integers(6, 13))
+                g3 = itnbp.named_grid((L, L))
+            else:
+                L = int(rng3.integers(3, 7))
+                g3 = itnbp.heavy_hex(L, L)
+            chi3 = int(rng3.choice([4, 6, 8, 12, 16]))
+            psi3 = itnbp.random_tensornetwork(rng3, np.float32, g3, link_space=chi3, scale=1 / np.sqrt(chi3))
+            caches.append(itnbp.initialize_cache(psi3))
+            nmsg3 += 2 * g3.ne
+        ctx.check(caches[0].lib.itnbp_synchronize(ctx.h))
+        t = time.perf_counter()
+        res3 = itnbp.update_batched_(caches, maxiter=50, precision="f32")
+        ctx.check(caches[0].lib.itnbp_synchronize(ctx.h))
+        t_b = time.perf_counter() - t
+        extras["batched_256_instances_50_sweeps_f32"] = {
+            "messages_per_sweep": nmsg3, "seconds": t_b, "device_ms": res3[0]["device_ms"],
+            "updates_per_s": nmsg3 * 50 / t_b, "updates_per_s_device": nmsg3 * 50 / (res3[0]["device_ms"] / 1e3),
+            "path": res3[0]["path"]}
+        del caches
         log(f"extras {extras}")
 
     cpu_rate, cpu_n = cpu_baseline(psi, g, a.cpu_seconds)

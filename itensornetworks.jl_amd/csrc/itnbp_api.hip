@@ -650,6 +650,7 @@ int itnbp_net_copy(itnbp_net* src, itnbp_net** out) {
                              hipMemcpyDeviceToDevice, ctx->stream));
   dst->cur = 0;
   dst->msg_set = src->msg_set;
+  dst->msgs_sym = src->msgs_sym;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return ITNBP_OK;
 }
@@ -687,12 +688,26 @@ int itnbp_set_site(itnbp_net* net, int v, const void* data) {
   return ITNBP_OK;
 }
 
+// symmetric to within rounding: max |M - M^T| <= 1e-12 max |M|
+static bool message_symmetric(const double* M, int chi) {
+  double mx = 0, d = 0;
+  for (int i = 0; i < chi; ++i)
+    for (int j = 0; j < chi; ++j) {
+      mx = std::max(mx, std::fabs(M[i + chi * j]));
+      d = std::max(d, std::fabs(M[i + chi * j] - M[j + chi * i]));
+    }
+  return d <= 1e-12 * mx;
+}
+
 int itnbp_set_messages(itnbp_net* net, const double* data) {
   if (!net || !data) return ITNBP_ERR_ARG;
   itnbp_ctx* ctx = net->ctx;
   HIPCHK(ctx, hipMemcpyAsync(net->d_msg[net->cur], data, net->msg_total * 8, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::fill(net->msg_set.begin(), net->msg_set.end(), 1);
+  bool sym = true;
+  for (int m = 0; m < 2 * net->ne && sym; ++m) sym = message_symmetric(data + net->msg_off[m], net->chi[m / 2]);
+  net->msgs_sym = sym;
   return ITNBP_OK;
 }
 
@@ -711,6 +726,7 @@ int itnbp_set_message(itnbp_net* net, int m, const double* data) {
   HIPCHK(ctx, hipMemcpyAsync(net->d_msg[net->cur] + net->msg_off[m], data, n * 8, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   net->msg_set[m] = 1;
+  if (!message_symmetric(data, net->chi[m / 2])) net->msgs_sym = false;
   return ITNBP_OK;
 }
 

@@ -132,6 +132,10 @@ struct itnbp_net {
   long long psi_total = 0, msg_total = 0;
   std::vector<VDesc> vdesc;
   std::vector<char> msg_set;
+  // every message is symmetric (M = M^T up to rounding): true after identity
+  // initialisation and for sweeps from a symmetric state; the eigenbasis
+  // pipeline relies on it
+  bool msgs_sym = true;
   int max_chi = 0, max_k = 0;
   // device state
   void* d_psi = nullptr;       // in dtype

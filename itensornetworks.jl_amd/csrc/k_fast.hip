@@ -352,16 +352,22 @@ __global__ __launch_bounds__(512) void k_absorb_bf16c(const FastItem* __restrict
       aMah[t][p] = h;
       aMal[t][p] = lo;
     }
-#pragma unroll 1
+  // the wave's four input blocks (8 KiB) are all requested before the first product
+  bf16x8 aXs[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const __bf16* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * (4 * wv + i))) * FBLK;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) aXs[i][t] = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+  }
+#pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int w = 4 * wv + i;
-    const __bf16* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * w)) * FBLK;
     f32x16 V = {};
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      const bf16x8 aX = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
-      V = MFMA_BF16(aX, bMbh[t], V);
-      V = MFMA_BF16(aX, bMbl[t], V);
+      V = MFMA_BF16(aXs[i][t], bMbh[t], V);
+      V = MFMA_BF16(aXs[i][t], bMbl[t], V);
     }
     bf16x8 bV[2];
 #pragma unroll
@@ -1010,6 +1016,941 @@ __global__ __launch_bounds__(256) void k_fast_finalize(const FastItem* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Eigenbasis pipeline (symmetric messages).  With M_c = U_c diag(l_c) U_c^T for
+// the four incoming messages, rotate every bond of psi into the eigenbasis of
+// its message,
+//   Y[s, r1, r2, r3, r4] = sum_b psi[s, b1, b2, b3, b4] prod_c U_c[b_c, r_c],
+// and every outgoing message becomes a weighted Gram of one mode of Y rotated
+// back (exact for any orthogonal U_c; eigenvalues of either sign):
+//   T_c = U_c T'_c U_c^T,  T'_c[r, r'] = sum_{s, r_e (e != c)} prod_{e != c} l_e[r_e] Y[.., r, ..] Y[.., r', ..].
+// Three streaming kernels per sweep:
+//   rot12  = the absorb kernel with (U1, U2): psi layout-B block X[b1][b2] at (s, b3, b4)
+//            -> R block (s + 2 r1 + 64 r2) = [b3][b4] of U1^T X U2
+//   rot34  : R block Z -> Y block = U3^T Z U4 ([r3][r4]); T'3 += l1 l2 Y L4 Y^T and
+//            T'4 += l1 l2 Y^T L3 Y in MFMA accumulators; Y scattered into the
+//            Q block (s + 2 r3 + 64 r4) = [r1][r2]
+//   gram12 : Q block D -> T'1 += l3 l4 D L2 D^T, T'2 += l3 l4 D^T L1 D
+// then a finaliser rotates back, symmetrises, normalises and measures.  Per
+// vertex and sweep that is 20 MiB of BF16 traffic (psi in, R out and in, Q
+// out and in) instead of the 32 MiB of the absorbed-pair passes, and 9 GEMM
+// units of 32^3 per block instead of 12.
+// ---------------------------------------------------------------------------
+constexpr int EIG_NPART = 16;  // partial Grams per (vertex, bond)
+constexpr int ROT_NZ = 4;      // r1 values per rot34 workgroup (grid.x = 2 * 32 / ROT_NZ = EIG_NPART)
+
+struct EigItem {
+  long long R, Q;      // block buffers (element offsets)
+  long long X;         // degree-3 sites: layout-C blocks of psi
+  long long U[4];      // eigenvectors of the incoming message on bond c (column-major U[b + 32 r])
+  long long L[4];      // its eigenvalues
+  long long out[4];    // outgoing message offsets
+  int slot[4];
+  long long part;      // partial Grams [4 bonds][EIG_NPART][32 x 32]
+};
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Symmetric eigendecomposition of one 32x32 message per wave (cyclic parallel
+// Jacobi, 16 disjoint rotations per step, round-robin pairing), input
+// symmetrised as (M + M^T)/2.  grid (ceil(n/4)), 256 threads.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ moff, int n,
+                                                 const TM* __restrict__ msg, double* __restrict__ U64,
+                                                 double* __restrict__ L64, float* __restrict__ U32,
+                                                 float* __restrict__ L32) {
+  constexpr int LD = 33;
+  __shared__ double As[4][FC * LD];
+  __shared__ double Vs[4][FC * LD];
+  __shared__ double rc[4][16], rs[4][16];
+  __shared__ int rp[4][16], rq[4][16];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int e = blockIdx.x * 4 + wv;
+  if (e >= n) return;  // whole waves only; the kernel has no workgroup barrier
+  double* A = As[wv];
+  double* V = Vs[wv];
+  const TM* M = msg + moff[e];
+  for (int o = l; o < FBLK; o += 64) {
+    const int i = o & 31, j = o >> 5;
+    A[i * LD + j] = 0.5 * ((double)M[i + FC * j] + (double)M[j + FC * i]);
+    V[i * LD + j] = (i == j) ? 1.0 : 0.0;
+  }
+  wave_sync();
+  const int col = l & 31, half = l >> 5;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0, dg = 0;
+    for (int o = l; o < FBLK; o += 64) {
+      const int i = o >> 5, j = o & 31;
+      const double a = A[i * LD + j];
+      if (i != j) off += a * a;
+      else dg += a * a;
+    }
+    for (int m = 32; m > 0; m >>= 1) {
+      off += __shfl_xor(off, m);
+      dg += __shfl_xor(dg, m);
+    }
+    if (off <= 1e-30 * dg || off == 0.0) break;
+    for (int st = 0; st < FC - 1; ++st) {
+      if (l < 16) {
+        auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + st) % (FC - 1); };
+        int p = at(l), q = at(FC - 1 - l);
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1, s = 0;
+        const double apq = A[p * LD + q];
+        if (apq != 0.0) {
+          const double zeta = (A[q * LD + q] - A[p * LD + p]) / (2 * apq);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+          c = 1 / sqrt(1 + t * t);
+          s = c * t;
+        }
+        rc[wv][l] = c;
+        rs[wv][l] = s;
+        rp[wv][l] = p;
+        rq[wv][l] = q;
+      }
+      wave_sync();
+      // rows: A <- J^T A
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int i = half + 2 * jj;
+        const int p = rp[wv][i], q = rq[wv][i];
+        const double c = rc[wv][i], s = rs[wv][i];
+        const double ap = A[p * LD + col], aq = A[q * LD + col];
+        A[p * LD + col] = c * ap - s * aq;
+        A[q * LD + col] = s * ap + c * aq;
+      }
+      wave_sync();
+      // columns: A <- A J, V <- V J
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int i = half + 2 * jj;
+        const int p = rp[wv][i], q = rq[wv][i];
+        const double c = rc[wv][i], s = rs[wv][i];
+        const double ap = A[col * LD + p], aq = A[col * LD + q];
+        A[col * LD + p] = c * ap - s * aq;
+        A[col * LD + q] = s * ap + c * aq;
+        const double vp = V[col * LD + p], vq = V[col * LD + q];
+        V[col * LD + p] = c * vp - s * vq;
+        V[col * LD + q] = s * vp + c * vq;
+      }
+      wave_sync();
+    }
+  }
+  double* u64 = U64 + (long long)e * FBLK;
+  float* u32 = U32 ? U32 + (long long)e * FBLK : nullptr;
+  for (int o = l; o < FBLK; o += 64) {
+    const int b = o & 31, r = o >> 5;
+    const double v = V[b * LD + r];
+    u64[o] = v;  // U[b + 32 r]: column r is the eigenvector of eigenvalue r
+    if (u32) u32[o] = (float)v;
+  }
+  if (l < FC) {
+    const double lam = A[l * LD + l];
+    L64[(long long)e * FC + l] = lam;
+    if (L32) L32[(long long)e * FC + l] = (float)lam;
+  }
+}
+
+// rot34, BF16: WG (512 threads) -> (s, ROT_NZ consecutive r1, all 32 r2); wave
+// wv -> r2 in [4 wv, 4 wv + 4).  The four R blocks of the next r1 are in flight
+// while the current ones are multiplied.  grid (EIG_NPART, items)
+__global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ items,
+                                                    const __bf16* __restrict__ R,
+                                                    const float* __restrict__ U,
+                                                    const float* __restrict__ lam,
+                                                    __bf16* __restrict__ Q, float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 S[FBLK * ST_LD];
+  __shared__ __attribute__((aligned(16))) float lam34[2][FC];  // l3, l4
+  const EigItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int s = q & 1, z0 = (q >> 1) * ROT_NZ;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  if (threadIdx.x < 2 * FC) lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[it.L[2 + (threadIdx.x >> 5)] + (threadIdx.x & 31)];
+  bf16x8 aU3h[2], aU3l[2], bU4h[2], bU4l[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      __bf16 h, lo;
+      // U3 as A operand of U3^T V (lane r3, K = b3 in accumulator-row order) and as
+      // B operand of V^T U3 (lane r3, same K order): the same values
+      split_bf16(U[it.U[2] + kperm(t, g, p) + FC * c], h, lo);
+      aU3h[t][p] = h;
+      aU3l[t][p] = lo;
+      // U4 as B operand of Z U4 (lane r4, K = b4 natural)
+      split_bf16(U[it.U[3] + (16 * t + 8 * g + p) + FC * c], h, lo);
+      bU4h[t][p] = h;
+      bU4l[t][p] = lo;
+    }
+  __syncthreads();
+  // eigenvalue weights of accumulator rows kperm(t, g, 0..7): two runs of 4
+  auto lam_rows = [&](int which, int t, float (&o)[8]) {
+    const float4 a = *reinterpret_cast<const float4*>(&lam34[which][16 * t + 4 * g]);
+    const float4 b = *reinterpret_cast<const float4*>(&lam34[which][16 * t + 8 + 4 * g]);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  };
+  f32x16 T3 = {}, T4 = {};
+  bf16x8 cur[4][2], nxt[4][2];
+  auto fetch = [&](int z, bf16x8 (&dst)[4][2]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const __bf16* Z = R + it.R + (long long)(s + 2 * (z + FC * (4 * wv + i))) * FBLK;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) dst[i][t] = *reinterpret_cast<const bf16x8*>(Z + c * FC + 16 * t + 8 * g);
+    }
+  };
+  fetch(z0, cur);
+#pragma unroll 1
+  for (int zi = 0; zi < ROT_NZ; ++zi) {
+    const int z = z0 + zi;
+    if (zi + 1 < ROT_NZ) fetch(z + 1, nxt);
+    const float l1 = lam[it.L[0] + z];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int w = 4 * wv + i;
+      const float w12 = l1 * lam[it.L[1] + w];
+      f32x16 V = {};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        V = MFMA_BF16(cur[i][t], bU4h[t], V);
+        V = MFMA_BF16(cur[i][t], bU4l[t], V);
+      }
+      bf16x8 bV[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int p = 0; p < 8; ++p) bV[t][p] = (__bf16)V[8 * t + p];
+      f32x16 Y = {}, YT = {};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        Y = MFMA_BF16(aU3h[t], bV[t], Y);
+        Y = MFMA_BF16(aU3l[t], bV[t], Y);
+        YT = MFMA_BF16(bV[t], aU3h[t], YT);
+        YT = MFMA_BF16(bV[t], aU3l[t], YT);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        bf16x8 a3, b3, a4, b4;
+        float l3[8], l4[8];
+        lam_rows(0, t, l3);
+        lam_rows(1, t, l4);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          const float yt = YT[8 * t + p], y = Y[8 * t + p];
+          a3[p] = (__bf16)yt;
+          b3[p] = (__bf16)(w12 * l4[p] * yt);
+          a4[p] = (__bf16)y;
+          b4[p] = (__bf16)(w12 * l3[p] * y);
+        }
+        T3 = MFMA_BF16(a3, b3, T3);
+        T4 = MFMA_BF16(a4, b4, T4);
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
+        S[(c + FC * xb) * ST_LD + w] = (__bf16)Y[v];  // row (r4 + 32 r3), column r2
+      }
+      __builtin_amdgcn_sched_barrier(0);  // keep the four blocks' chains apart (register pressure)
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < FBLK * 4; k += blockDim.x) {
+      const int o = k >> 2, pt = k & 3;
+      const int yb = o & 31, xb = o >> 5;
+      const long long qb = s + 2 * (xb + FC * yb);
+      *reinterpret_cast<bf16x8*>(Q + it.Q + qb * FBLK + z * FC + 8 * pt) =
+          *reinterpret_cast<const bf16x8*>(S + o * ST_LD + 8 * pt);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cur[i][0] = nxt[i][0];
+      cur[i][1] = nxt[i][1];
+    }
+  }
+  float* Red = reinterpret_cast<float*>(S);
+  for (int w8 = 0; w8 < 8; ++w8) {
+    if (wv == w8) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rw = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w8 == 0) {
+          Red[rw * FC + c] = T3[v];
+          Red[FBLK + rw * FC + c] = T4[v];
+        } else {
+          Red[rw * FC + c] += T3[v];
+          Red[FBLK + rw * FC + c] += T4[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* d3 = part + it.part + (long long)(2 * EIG_NPART + q) * FBLK;
+  float* d4 = part + it.part + (long long)(3 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    d3[o] = Red[o];
+    d4[o] = Red[FBLK + o];
+  }
+}
+
+// gram12, BF16: WG (256 threads) -> 4 NB consecutive Q blocks, wave -> NB (32
+// for degree-4 sites, 1 for degree 3); the next 4 blocks of a wave are in
+// flight while the current 4 are multiplied.  D^T comes from D I on the MFMA
+// (exact).  Block qb has weight l3[(qb >> 1) & 31] l4[qb >> 6].
+// grid (EIG_NPART, items)
+template <int NB>
+__global__ __launch_bounds__(256) void k_gram12_bf16(const EigItem* __restrict__ items,
+                                                     const __bf16* __restrict__ Q,
+                                                     const float* __restrict__ lam,
+                                                     float* __restrict__ part) {
+  __shared__ float Red[2 * FBLK];
+  const EigItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  float lam1v[16], lam2v[16];
+  bf16x8 bI[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      lam1v[8 * t + p] = lam[it.L[0] + kperm(t, g, p)];   // K = r1 in accumulator-row order
+      lam2v[8 * t + p] = lam[it.L[1] + 16 * t + 8 * g + p];  // K = r2 natural
+      bI[t][p] = (__bf16)((16 * t + 8 * g + p) == c ? 1.0f : 0.0f);
+    }
+  f32x16 T1 = {}, T2 = {};
+  constexpr int PF = NB < 4 ? NB : 4;
+  const long long r0 = (long long)q * (4 * NB) + wv * NB;
+  bf16x8 cur[PF][2], nxt[PF][2];
+  auto fetch = [&](long long r, bf16x8 (&dst)[PF][2]) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const __bf16* D = Q + it.Q + (r + i) * FBLK;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) dst[i][t] = *reinterpret_cast<const bf16x8*>(D + c * FC + 16 * t + 8 * g);
+    }
+  };
+  fetch(r0, cur);
+#pragma unroll 1
+  for (int i0 = 0; i0 < NB; i0 += PF) {
+    if (i0 + PF < NB) fetch(r0 + i0 + PF, nxt);
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const long long qb = r0 + i0 + i;
+      const int r3 = (int)((qb >> 1) & 31), r4 = (int)(qb >> 6);
+      const float w34 = lam[it.L[2] + r3] * lam[it.L[3] + r4];
+      f32x16 Dt = {};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        bf16x8 b1;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) b1[p] = (__bf16)(w34 * lam2v[8 * t + p] * (float)cur[i][t][p]);
+        T1 = MFMA_BF16(cur[i][t], b1, T1);
+        Dt = MFMA_BF16(cur[i][t], bI[t], Dt);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        bf16x8 a2, b2;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          const float d = Dt[8 * t + p];
+          a2[p] = (__bf16)d;
+          b2[p] = (__bf16)(w34 * lam1v[8 * t + p] * d);
+        }
+        T2 = MFMA_BF16(a2, b2, T2);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      cur[i][0] = nxt[i][0];
+      cur[i][1] = nxt[i][1];
+    }
+  }
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rw = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w4 == 0) {
+          Red[rw * FC + c] = T1[v];
+          Red[FBLK + rw * FC + c] = T2[v];
+        } else {
+          Red[rw * FC + c] += T1[v];
+          Red[FBLK + rw * FC + c] += T2[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* d1 = part + it.part + (long long)(0 * EIG_NPART + q) * FBLK;
+  float* d2 = part + it.part + (long long)(1 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    d1[o] = Red[o];
+    d2[o] = Red[FBLK + o];
+  }
+}
+
+// rot34, FP64: WG (256 threads) -> (s, 8 consecutive r2, ROT_NZ64 consecutive r1);
+// wave wv -> r2 = w0 + 2 wv + {0, 1}.  One block of look-ahead: the loads of the
+// next block are in flight during the 160 MFMAs of the current one.
+// grid (EIG_NPART, items) = (2 s x 4 r2 groups x 2 r1 groups)
+constexpr int ROT_NZ64 = 16;
+
+__global__ __launch_bounds__(256) void k_rot34_f64(const EigItem* __restrict__ items,
+                                                   const double* __restrict__ R,
+                                                   const double* __restrict__ U,
+                                                   const double* __restrict__ lam,
+                                                   double* __restrict__ Q, double* __restrict__ part) {
+  constexpr int SROW = 10;
+  __shared__ __attribute__((aligned(16))) double S[FBLK * SROW];
+  const EigItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int s = q & 1, w0 = ((q >> 1) & 3) * 8, z0 = (q >> 3) * ROT_NZ64;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lg = l >> 4;
+  double aU3T[2][8], bU4[8][2], lam3v[8], lam4v[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      aU3T[h][t] = U[it.U[2] + (4 * t + lg) + FC * (16 * h + li)];  // U3^T[16h+li][4t+lg]
+      bU4[t][h] = U[it.U[3] + (4 * t + lg) + FC * (16 * h + li)];   // U4[4t+lg][16h+li]
+    }
+    lam3v[t] = lam[it.L[2] + 4 * t + lg];
+    lam4v[t] = lam[it.L[3] + 4 * t + lg];
+  }
+  Acc32 T3, T4;
+  acc_zero(T3);
+  acc_zero(T4);
+  double cur[2][8], nxt[2][8];
+  auto fetch = [&](int j, double (&dst)[2][8]) {  // block j of this wave: z = z0 + j / 2, w = w0 + 2 wv + j % 2
+    const int z = z0 + (j >> 1), w = w0 + 2 * wv + (j & 1);
+    const double* Z = R + it.R + (long long)(s + 2 * (z + FC * w)) * FBLK;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) dst[h][t] = Z[(16 * h + li) * FC + 4 * t + lg];
+  };
+  fetch(0, cur);
+#pragma unroll 1
+  for (int zi = 0; zi < ROT_NZ64; ++zi) {
+    const int z = z0 + zi;
+    const double l1 = lam[it.L[0] + z];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int j = 2 * zi + i;
+      if (j + 1 < 2 * ROT_NZ64) fetch(j + 1, nxt);
+      const int wl = 2 * wv + i;
+      const double w12 = l1 * lam[it.L[1] + w0 + wl];
+      Acc32 V, Y, YT;
+      acc_zero(V);
+      acc_zero(Y);
+      acc_zero(YT);
+      mm32(V, cur, bU4);       // V = Z U4
+      mm32_accb(Y, aU3T, V);   // Y = U3^T V
+#pragma unroll
+      for (int t = 0; t < 8; ++t)  // Y^T = V^T U3
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 2; ++ni)
+            YT.t[mi][ni] = MFMA_F64(V.t[t >> 2][mi][t & 3], aU3T[ni][t], YT.t[mi][ni]);
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 2; ++ni) {
+            // T3 += Y L4 Y^T (K = r4), T4 += Y^T L3 Y (K = r3)
+            T3.t[mi][ni] = MFMA_F64(YT.t[t >> 2][mi][t & 3], w12 * lam4v[t] * YT.t[t >> 2][ni][t & 3], T3.t[mi][ni]);
+            T4.t[mi][ni] = MFMA_F64(Y.t[t >> 2][mi][t & 3], w12 * lam3v[t] * Y.t[t >> 2][ni][t & 3], T4.t[mi][ni]);
+          }
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int xb = 16 * mi + 4 * v + lg, yb = 16 * ni + li;
+            S[(yb + FC * xb) * SROW + wl] = Y.t[mi][ni][v];
+          }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cur[h][t] = nxt[h][t];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < FBLK * 4; k += blockDim.x) {
+      const int o = k >> 2, pt = k & 3;
+      const int yb = o & 31, xb = o >> 5;
+      const long long qb = s + 2 * (xb + FC * yb);
+      *reinterpret_cast<double2*>(Q + it.Q + qb * FBLK + z * FC + w0 + 2 * pt) =
+          *reinterpret_cast<const double2*>(S + o * SROW + 2 * pt);
+    }
+    __syncthreads();
+  }
+  double* Red = S;
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rw = 16 * mi + 4 * v + lg, cl = 16 * ni + li;
+            if (w4 == 0) {
+              Red[rw * FC + cl] = T3.t[mi][ni][v];
+              Red[FBLK + rw * FC + cl] = T4.t[mi][ni][v];
+            } else {
+              Red[rw * FC + cl] += T3.t[mi][ni][v];
+              Red[FBLK + rw * FC + cl] += T4.t[mi][ni][v];
+            }
+          }
+    }
+    __syncthreads();
+  }
+  double* d3 = part + it.part + (long long)(2 * EIG_NPART + q) * FBLK;
+  double* d4 = part + it.part + (long long)(3 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    d3[o] = Red[o];
+    d4[o] = Red[FBLK + o];
+  }
+}
+
+// gram12, FP64: WG (256 threads) -> 4 NB consecutive Q blocks, wave -> NB, one
+// block of look-ahead; each block is read in both orientations (the second
+// read is served by the caches).  grid (EIG_NPART, items)
+template <int NB>
+__global__ __launch_bounds__(256) void k_gram12_f64(const EigItem* __restrict__ items,
+                                                    const double* __restrict__ Q,
+                                                    const double* __restrict__ lam,
+                                                    double* __restrict__ part) {
+  __shared__ double Red[2 * FBLK];
+  const EigItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lg = l >> 4;
+  double lam1v[8], lam2v[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    lam1v[t] = lam[it.L[0] + 4 * t + lg];
+    lam2v[t] = lam[it.L[1] + 4 * t + lg];
+  }
+  Acc32 T1, T2;
+  acc_zero(T1);
+  acc_zero(T2);
+  const long long r0 = (long long)q * (4 * NB) + wv * NB;
+  double cr[2][8], cc[2][8], nr[2][8], nc[2][8];
+  auto fetch = [&](long long r, double (&dr)[2][8], double (&dc)[2][8]) {
+    const double* D = Q + it.Q + r * FBLK;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        dr[h][t] = D[(16 * h + li) * FC + 4 * t + lg];  // D[16h+li][4t+lg]
+        dc[h][t] = D[(4 * t + lg) * FC + 16 * h + li];  // D[4t+lg][16h+li]
+      }
+  };
+  fetch(r0, cr, cc);
+#pragma unroll 1
+  for (int i = 0; i < NB; ++i) {
+    if (i + 1 < NB) fetch(r0 + i + 1, nr, nc);
+    const long long qb = r0 + i;
+    const int r3 = (int)((qb >> 1) & 31), r4 = (int)(qb >> 6);
+    const double w34 = lam[it.L[2] + r3] * lam[it.L[3] + r4];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+          // T1 += D L2 D^T (K = r2), T2 += D^T L1 D (K = r1)
+          T1.t[mi][ni] = MFMA_F64(cr[mi][t], w34 * lam2v[t] * cr[ni][t], T1.t[mi][ni]);
+          T2.t[mi][ni] = MFMA_F64(cc[mi][t], w34 * lam1v[t] * cc[ni][t], T2.t[mi][ni]);
+        }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        cr[h][t] = nr[h][t];
+        cc[h][t] = nc[h][t];
+      }
+  }
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rw = 16 * mi + 4 * v + lg, cl = 16 * ni + li;
+            if (w4 == 0) {
+              Red[rw * FC + cl] = T1.t[mi][ni][v];
+              Red[FBLK + rw * FC + cl] = T2.t[mi][ni][v];
+            } else {
+              Red[rw * FC + cl] += T1.t[mi][ni][v];
+              Red[FBLK + rw * FC + cl] += T2.t[mi][ni][v];
+            }
+          }
+    }
+    __syncthreads();
+  }
+  double* d1 = part + it.part + (long long)(0 * EIG_NPART + q) * FBLK;
+  double* d2 = part + it.part + (long long)(1 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    d1[o] = Red[o];
+    d2[o] = Red[FBLK + o];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Degree-3 sites (lattice boundary), every chi = 32.  psi[s, b1, b2, b3] in
+// layout C: block (s + 2 b3) = [b1][b2], 64 blocks.
+//   rot12_d3: X -> U1^T X U2, in place of the block index (R3)
+//   rot3_d3 : tile (s, r1): Z[b3][r2] = row r1 of R3 block (s + 2 b3); Y = U3^T Z
+//             ([r3][r2]); T'3 += Y diag(l1[r1] l2[r2]) Y^T; row r1 of the Q3 block
+//             (s + 2 r3) = Y[r3][:]
+//   gram12<1> with l4 = 1 (a table of ones) gives T'1 and T'2.
+// ---------------------------------------------------------------------------
+constexpr int D3_NBLK = 2 * FC;  // blocks per degree-3 site
+
+template <typename TS, typename TO, typename TA>
+__global__ void k_fast_layout3(const TS* __restrict__ psi, const long long* __restrict__ src_off,
+                               const long long* __restrict__ dst_off, TO* __restrict__ out) {
+  const int it = blockIdx.y;
+  const TS* src = psi + src_off[it];
+  TO* C = out + dst_off[it];
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC; n += gridDim.x * blockDim.x) {
+    const int s = n & 1, b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = n >> 11;
+    C[(long long)(s + 2 * b3) * FBLK + b1 * FC + b2] = (TO)load_as<TA>(src, n);
+  }
+}
+
+// grid (4, items), 256 threads: wave -> blocks 16 x + 4 wv + {0..3}
+__global__ __launch_bounds__(256) void k_rot12_d3_bf16(const EigItem* __restrict__ items,
+                                                       const __bf16* __restrict__ lay,
+                                                       const float* __restrict__ U,
+                                                       __bf16* __restrict__ R) {
+  const EigItem it = items[blockIdx.y];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  bf16x8 aU1h[2], aU1l[2], bU2h[2], bU2l[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      __bf16 h, lo;
+      split_bf16(U[it.U[0] + kperm(t, g, p) + FC * c], h, lo);  // U1^T, K = b1 (accumulator rows)
+      aU1h[t][p] = h;
+      aU1l[t][p] = lo;
+      split_bf16(U[it.U[1] + (16 * t + 8 * g + p) + FC * c], h, lo);  // U2, K = b2
+      bU2h[t][p] = h;
+      bU2l[t][p] = lo;
+    }
+  bf16x8 aX[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const __bf16* X = lay + it.X + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) aX[i][t] = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    f32x16 V = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      V = MFMA_BF16(aX[i][t], bU2h[t], V);
+      V = MFMA_BF16(aX[i][t], bU2l[t], V);
+    }
+    bf16x8 bV[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int p = 0; p < 8; ++p) bV[t][p] = (__bf16)V[8 * t + p];
+    f32x16 Y = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      Y = MFMA_BF16(aU1h[t], bV[t], Y);
+      Y = MFMA_BF16(aU1l[t], bV[t], Y);
+    }
+    __bf16* dst = R + it.R + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) dst[(8 * (v >> 2) + 4 * g + (v & 3)) * FC + c] = (__bf16)Y[v];
+  }
+}
+
+// grid (EIG_NPART, items), 256 threads: wave -> tile (s, r1) = (x & 1, x >> 1), x = 4 q + wv
+__global__ __launch_bounds__(256) void k_rot3_d3_bf16(const EigItem* __restrict__ items,
+                                                      const __bf16* __restrict__ R,
+                                                      const float* __restrict__ U,
+                                                      const float* __restrict__ lam,
+                                                      __bf16* __restrict__ Q, float* __restrict__ part) {
+  __shared__ float Red[FBLK];
+  const EigItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  const int x = 4 * q + wv, s = x & 1, r1 = x >> 1;
+  bf16x8 aU3h[2], aU3l[2], bZ[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      __bf16 h, lo;
+      split_bf16(U[it.U[2] + (16 * t + 8 * g + p) + FC * c], h, lo);  // U3^T, K = b3 natural
+      aU3h[t][p] = h;
+      aU3l[t][p] = lo;
+      // Z as B operand (lane r2, K = b3 natural)
+      bZ[t][p] = R[it.R + (long long)(s + 2 * (16 * t + 8 * g + p)) * FBLK + r1 * FC + c];
+    }
+  f32x16 Y = {}, YT = {};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    Y = MFMA_BF16(aU3h[t], bZ[t], Y);
+    Y = MFMA_BF16(aU3l[t], bZ[t], Y);
+    YT = MFMA_BF16(bZ[t], aU3h[t], YT);
+    YT = MFMA_BF16(bZ[t], aU3l[t], YT);
+  }
+  const float l1 = lam[it.L[0] + r1];
+  f32x16 T3 = {};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    bf16x8 a, b;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const float yt = YT[8 * t + p];
+      a[p] = (__bf16)yt;
+      b[p] = (__bf16)(l1 * lam[it.L[1] + kperm(t, g, p)] * yt);
+    }
+    T3 = MFMA_BF16(a, b, T3);
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int r3 = 8 * (v >> 2) + 4 * g + (v & 3);
+    Q[it.Q + (long long)(s + 2 * r3) * FBLK + r1 * FC + c] = (__bf16)Y[v];
+  }
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rw = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w4 == 0) Red[rw * FC + c] = T3[v];
+        else Red[rw * FC + c] += T3[v];
+      }
+    }
+    __syncthreads();
+  }
+  float* d3 = part + it.part + (long long)(2 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) d3[o] = Red[o];
+}
+
+// FP64 degree-3 kernels, same decomposition on v_mfma_f64_16x16x4_f64
+__global__ __launch_bounds__(256) void k_rot12_d3_f64(const EigItem* __restrict__ items,
+                                                      const double* __restrict__ lay,
+                                                      const double* __restrict__ U,
+                                                      double* __restrict__ R) {
+  const EigItem it = items[blockIdx.y];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lg = l >> 4;
+  double bU2[8][2], aU1T[2][8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      bU2[t][h] = U[it.U[1] + (4 * t + lg) + FC * (16 * h + li)];
+      aU1T[h][t] = U[it.U[0] + (4 * t + lg) + FC * (16 * h + li)];
+    }
+  double cur[2][8], nxt[2][8];
+  auto fetch = [&](int i, double (&dst)[2][8]) {
+    const double* X = lay + it.X + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) dst[h][t] = X[(16 * h + li) * FC + 4 * t + lg];
+  };
+  fetch(0, cur);
+#pragma unroll 1
+  for (int i = 0; i < 4; ++i) {
+    if (i + 1 < 4) fetch(i + 1, nxt);
+    Acc32 V, Y;
+    acc_zero(V);
+    acc_zero(Y);
+    mm32(V, cur, bU2);
+    mm32_accb(Y, aU1T, V);
+    double* dst = R + it.R + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dst[(16 * mi + 4 * v + lg) * FC + 16 * ni + li] = Y.t[mi][ni][v];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) cur[h][t] = nxt[h][t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rot3_d3_f64(const EigItem* __restrict__ items,
+                                                     const double* __restrict__ R,
+                                                     const double* __restrict__ U,
+                                                     const double* __restrict__ lam,
+                                                     double* __restrict__ Q, double* __restrict__ part) {
+  __shared__ double Red[FBLK];
+  const EigItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lg = l >> 4;
+  const int x = 4 * q + wv, s = x & 1, r1 = x >> 1;
+  double aU3T[2][8], bZ[8][2], wv2[8];
+  const double l1 = lam[it.L[0] + r1];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      aU3T[h][t] = U[it.U[2] + (4 * t + lg) + FC * (16 * h + li)];                        // U3[4t+lg][16h+li]
+      bZ[t][h] = R[it.R + (long long)(s + 2 * (4 * t + lg)) * FBLK + r1 * FC + 16 * h + li];  // Z[4t+lg][16h+li]
+    }
+    wv2[t] = l1 * lam[it.L[1] + 4 * t + lg];
+  }
+  Acc32 Y, YT, T3;
+  acc_zero(Y);
+  acc_zero(YT);
+  acc_zero(T3);
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) {
+        Y.t[mi][ni] = MFMA_F64(aU3T[mi][t], bZ[t][ni], Y.t[mi][ni]);   // Y = U3^T Z
+        YT.t[mi][ni] = MFMA_F64(bZ[t][mi], aU3T[ni][t], YT.t[mi][ni]);  // Y^T = Z^T U3
+      }
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+        T3.t[mi][ni] = MFMA_F64(YT.t[t >> 2][mi][t & 3], wv2[t] * YT.t[t >> 2][ni][t & 3], T3.t[mi][ni]);
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r3 = 16 * mi + 4 * v + lg;
+        Q[it.Q + (long long)(s + 2 * r3) * FBLK + r1 * FC + 16 * ni + li] = Y.t[mi][ni][v];
+      }
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rw = 16 * mi + 4 * v + lg, cl = 16 * ni + li;
+            if (w4 == 0) Red[rw * FC + cl] = T3.t[mi][ni][v];
+            else Red[rw * FC + cl] += T3.t[mi][ni][v];
+          }
+    }
+    __syncthreads();
+  }
+  double* d3 = part + it.part + (long long)(2 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) d3[o] = Red[o];
+}
+
+// Finaliser of the eigenbasis pipeline: T'_c = sum of the partial Grams (FP64),
+// T_c = U_c T'_c U_c^T, symmetrised, normalised, message_diff / residual
+// against the previous message.  grid (4, items), 256 threads.
+template <typename TP, typename TM>
+__global__ __launch_bounds__(256) void k_eig_finalize(const EigItem* __restrict__ items,
+                                                      const TP* __restrict__ part,
+                                                      const double* __restrict__ U64, const TM* msg_in,
+                                                      TM* msg_out, MsgStat* __restrict__ stats,
+                                                      int normalize) {
+  constexpr int LD = 33;
+  __shared__ double Tp[FC * LD], Us[FC * LD], X[FC * LD];
+  __shared__ double red[8];
+  const EigItem* itp = items + blockIdx.y;  // fields indexed by the bond: read in place
+  const int cb = blockIdx.x;
+  const TP* pp = part + itp->part + (long long)cb * EIG_NPART * FBLK;
+  const long long uoff = itp->U[cb];
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    double sum = 0;
+    for (int ch = 0; ch < EIG_NPART; ++ch) sum += (double)pp[(long long)ch * FBLK + o];
+    Tp[(o >> 5) * LD + (o & 31)] = sum;  // T'[r][r']
+    const double u = U64[uoff + o];   // U[b + 32 r]
+    Us[(o & 31) * LD + (o >> 5)] = u;    // Us[b][r]
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    const int b = o >> 5, rr = o & 31;
+    double acc = 0;
+    for (int r = 0; r < FC; ++r) acc += Us[b * LD + r] * Tp[r * LD + rr];
+    X[b * LD + rr] = acc;  // (U T')[b][r']
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    const int b = o >> 5, bb = o & 31;
+    double acc = 0;
+    for (int r = 0; r < FC; ++r) acc += X[b * LD + r] * Us[bb * LD + r];
+    Tp[b * LD + bb] = acc;  // T[b][b']
+  }
+  __syncthreads();
+  double n2 = 0;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    const int i = o & 31, j = o >> 5;  // message layout M[i + 32 j]
+    const double tv = 0.5 * (Tp[i * LD + j] + Tp[j * LD + i]);
+    X[i * LD + j] = tv;
+    n2 += tv * tv;
+  }
+  auto bsum = [&](double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+    return s;
+  };
+  n2 = bsum(n2);
+  const double nrm = sqrt(n2);
+  const double sc = (normalize && nrm != 0.0) ? 1.0 / nrm : 1.0;
+  const long long off = itp->out[cb];
+  const int slot = itp->slot[cb];
+  double dot = 0, o2 = 0, d2 = 0, bad = 0;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    const double nv = X[(o & 31) * LD + (o >> 5)] * sc;
+    const double ov = (double)msg_in[off + o];
+    if (!isfinite(nv)) bad = 1;
+    dot += ov * nv;
+    o2 += ov * ov;
+    d2 += (nv - ov) * (nv - ov);
+    msg_out[off + o] = (TM)nv;
+  }
+  dot = bsum(dot);
+  o2 = bsum(o2);
+  d2 = bsum(d2);
+  bad = bsum(bad);
+  if (threadIdx.x == 0 && slot >= 0) {
+    const double nn2 = n2 * sc * sc;
+    MsgStat st;
+    st.diff = 1.0 - (dot * dot) / (o2 * nn2);
+    st.res2 = nn2 > 0 ? d2 / nn2 : 0.0;
+    st.norm = nrm;
+    st.bad = (bad > 0 || !isfinite(nrm)) ? 1.0 : 0.0;
+    stats[slot] = st;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host state
 // ---------------------------------------------------------------------------
 struct FastState {
@@ -1020,6 +1961,18 @@ struct FastState {
   bool lay16_ok = false, lay64_ok = false;
   int chunk[2] = {0, 0};            // items per absorb/fused launch pair, per precision (0: untuned)
   std::vector<float> tune_ms[2];    // autotuner timings of the candidates
+  // eigenbasis pipeline
+  bool eig = false;                 // chosen for the current phase
+  std::vector<EigItem> eitems;      // 1 per bulk vertex
+  std::vector<FastItem> k1items;    // rot12 descriptors (the absorb kernel's item type)
+  std::vector<long long> eig_moff;  // message offset of every eigendecomposed message
+  DevBuf d_eitems, d_k1items, d_eig_moff, eigU64, eigL64, eigU32, eigL32;
+  // degree-3 sites (eigenbasis pipeline only)
+  std::vector<int> deg3;            // eligible degree-3 vertices
+  std::vector<int> deg3_used;       // those whose outgoing messages are all in the sequence
+  std::vector<EigItem> e3items;
+  DevBuf d_e3items, lay3_16, lay3_64, P3_16, P3_64, offs3;
+  bool lay3_16_ok = false, lay3_64_ok = false;
 };
 
 FastState* state_of(itnbp_net* net) { return reinterpret_cast<FastState*>(net->fast); }
@@ -1038,10 +1991,25 @@ bool fast_precise() {
   return e && e[0] == '1';
 }
 
+// the eigenbasis pipeline is used when every message is symmetric, unless
+// ITNBP_FAST_LEGACY=1 asks for the absorbed-pair passes
+bool use_eig(const itnbp_net* net) {
+  const char* e = getenv("ITNBP_FAST_LEGACY");
+  return net->msgs_sym && !(e && e[0] == '1');
+}
+
 bool eligible(const itnbp_net* net, int v) {
   const VDesc& V = net->vdesc[v];
   if (V.k != 4 || V.dims[0] != 2) return false;
   for (int a = 1; a <= 4; ++a)
+    if (V.dims[a] != FC) return false;
+  return true;
+}
+
+bool eligible3(const itnbp_net* net, int v) {
+  const VDesc& V = net->vdesc[v];
+  if (V.k != 3 || V.dims[0] != 2) return false;
+  for (int a = 1; a <= 3; ++a)
     if (V.dims[a] != FC) return false;
   return true;
 }
@@ -1058,12 +2026,14 @@ bool fast_path_available(itnbp_net* net, int acc_f64, int schedule) {
   // the BF16 kernels need psi exactly representable in BF16
   if (!acc_f64 && net->dtype != ITNBP_BF16) return false;
   if (getenv("ITNBP_DISABLE_FAST")) return false;
+  const bool eig = use_eig(net);
   for (int v = 0; v < net->nv; ++v)
-    if (eligible(net, v)) return true;
+    if (eligible(net, v) || (eig && eligible3(net, v))) return true;
   return false;
 }
 
-const char* fast_path_name(itnbp_net*, int acc_f64) {
+const char* fast_path_name(itnbp_net* net, int acc_f64) {
+  if (use_eig(net)) return acc_f64 ? "mfma-f64-eig-chi32" : "mfma-bf16-eig-chi32";
   return acc_f64 ? "mfma-f64-chi32" : (fast_precise() ? "mfma-bf16x2-chi32" : "mfma-bf16c-chi32");
 }
 
@@ -1079,6 +2049,19 @@ void release_fast_state(itnbp_net* net) {
   free_buf(st->part32);
   free_buf(st->part64);
   free_buf(st->offs);
+  free_buf(st->d_eitems);
+  free_buf(st->d_k1items);
+  free_buf(st->d_eig_moff);
+  free_buf(st->eigU64);
+  free_buf(st->eigL64);
+  free_buf(st->eigU32);
+  free_buf(st->eigL32);
+  free_buf(st->d_e3items);
+  free_buf(st->lay3_16);
+  free_buf(st->lay3_64);
+  free_buf(st->P3_16);
+  free_buf(st->P3_64);
+  free_buf(st->offs3);
   delete st;
   net->fast = nullptr;
 }
@@ -1088,6 +2071,8 @@ void invalidate_fast_state(itnbp_net* net) {
   if (!st) return;
   st->lay16_ok = false;
   st->lay64_ok = false;
+  st->lay3_16_ok = false;
+  st->lay3_64_ok = false;
 }
 
 // Mark the sequence entries the pipeline produces and prepare its buffers.
@@ -1098,8 +2083,10 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   if (!st) {
     st = new FastState();
     net->fast = st;
-    for (int v = 0; v < net->nv; ++v)
+    for (int v = 0; v < net->nv; ++v) {
       if (eligible(net, v)) st->bulk.push_back(v);
+      if (eligible3(net, v)) st->deg3.push_back(v);
+    }
   }
   const long long per = (long long)FNR * FBLK;  // elements per vertex layout
   // slots
@@ -1146,8 +2133,145 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   }
   const size_t ni = st->items.size();
   ITNBP_RETCHK(ensure_buf(ctx, st->d_items, ni * sizeof(FastItem)));
-  ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_items.p, st->items.data(), ni * sizeof(FastItem), hipMemcpyHostToDevice,
-                                   ctx->stream));
+  if (ni > 0)
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_items.p, st->items.data(), ni * sizeof(FastItem), hipMemcpyHostToDevice,
+                                     ctx->stream));
+  // eigenbasis pipeline: R = first half of the P buffer, Q = second half;
+  // eigen slot 4 bi + (c - 1) holds the message into bulk vertex bi on bond c
+  st->eig = use_eig(net);
+  st->eitems.clear();
+  st->k1items.clear();
+  st->eig_moff.clear();
+  st->e3items.clear();
+  st->deg3_used.clear();
+  if (st->eig) {
+    for (int bi = 0; bi < nb; ++bi) {
+      const int v = st->bulk[bi];
+      const VDesc& V = net->vdesc[v];
+      EigItem e{};
+      e.R = (long long)bi * per;
+      e.Q = (long long)(nb + bi) * per;
+      for (int c = 0; c < 4; ++c) {
+        const long long slot = 4LL * bi + c;
+        e.U[c] = slot * FBLK;
+        e.L[c] = slot * FC;
+        const int o = out_msg_id(net, v, c + 1);
+        e.out[c] = net->msg_off[o];
+        e.slot[c] = pos[o];
+        st->eig_moff.push_back(V.in_msg[c + 1]);
+      }
+      e.part = (long long)bi * 4 * EIG_NPART * FBLK;
+      st->eitems.push_back(e);
+      FastItem k{};
+      k.psi_in = (long long)(nb + bi) * per;  // layout B
+      k.P = e.R;
+      k.Ma = e.U[0];
+      k.Mb = e.U[1];
+      st->k1items.push_back(k);
+    }
+    ITNBP_RETCHK(ensure_buf(ctx, st->d_eitems, nb * sizeof(EigItem)));
+    ITNBP_RETCHK(ensure_buf(ctx, st->d_k1items, nb * sizeof(FastItem)));
+    if (nb > 0) {
+      ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_eitems.p, st->eitems.data(), nb * sizeof(EigItem),
+                                       hipMemcpyHostToDevice, ctx->stream));
+      ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_k1items.p, st->k1items.data(), nb * sizeof(FastItem),
+                                       hipMemcpyHostToDevice, ctx->stream));
+    }
+    // degree-3 sites: slots after the bulk ones; l4 points at a table of ones
+    // stored after the last eigenvalue slot
+    st->deg3_used.clear();
+    st->e3items.clear();
+    const long long per3 = (long long)D3_NBLK * FBLK;
+    for (int v : st->deg3) {
+      const VDesc& V = net->vdesc[v];
+      const int o1 = out_msg_id(net, v, 1), o2 = out_msg_id(net, v, 2), o3 = out_msg_id(net, v, 3);
+      if (pos[o1] < 0 || pos[o2] < 0 || pos[o3] < 0) continue;
+      const int j = (int)st->deg3_used.size();
+      st->deg3_used.push_back(v);
+      EigItem e{};
+      e.X = (long long)j * per3;
+      e.R = (long long)j * per3;
+      e.Q = (long long)(st->deg3.size() + j) * per3;
+      const int oo[3] = {o1, o2, o3};
+      for (int c = 0; c < 3; ++c) {
+        const long long slot = (long long)st->eig_moff.size();
+        e.U[c] = slot * FBLK;
+        e.L[c] = slot * FC;
+        e.out[c] = net->msg_off[oo[c]];
+        e.slot[c] = pos[oo[c]];
+        st->eig_moff.push_back(V.in_msg[c + 1]);
+        handled[pos[oo[c]]] = 1;
+      }
+      e.part = (long long)(nb + j) * 4 * EIG_NPART * FBLK;
+      st->e3items.push_back(e);
+    }
+    const size_t ne4 = st->eig_moff.size();
+    for (EigItem& e : st->e3items) e.L[3] = (long long)ne4 * FC;  // ones
+    const int n3 = (int)st->e3items.size();
+    if (n3 > 0) {
+      ITNBP_RETCHK(ensure_buf(ctx, st->d_e3items, n3 * sizeof(EigItem)));
+      ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_e3items.p, st->e3items.data(), n3 * sizeof(EigItem),
+                                       hipMemcpyHostToDevice, ctx->stream));
+    }
+    ITNBP_RETCHK(ensure_buf(ctx, st->d_eig_moff, ne4 * 8));
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_eig_moff.p, st->eig_moff.data(), ne4 * 8, hipMemcpyHostToDevice,
+                                     ctx->stream));
+    ITNBP_RETCHK(ensure_buf(ctx, st->eigU64, ne4 * FBLK * 8));
+    ITNBP_RETCHK(ensure_buf(ctx, st->eigL64, (ne4 + 1) * FC * 8));
+    static const double ones64[FC] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                      1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+    static const float ones32[FC] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                     1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync((double*)st->eigL64.p + ne4 * FC, ones64, sizeof(ones64),
+                                     hipMemcpyHostToDevice, ctx->stream));
+    if (!acc_f64) {
+      ITNBP_RETCHK(ensure_buf(ctx, st->eigU32, ne4 * FBLK * 4));
+      ITNBP_RETCHK(ensure_buf(ctx, st->eigL32, (ne4 + 1) * FC * 4));
+      ITNBP_HIPCHK(ctx, hipMemcpyAsync((float*)st->eigL32.p + ne4 * FC, ones32, sizeof(ones32),
+                                       hipMemcpyHostToDevice, ctx->stream));
+    }
+    // layout C of the degree-3 sites and their block buffers
+    if (n3 > 0) {
+      std::vector<long long> o3(2 * n3);
+      for (int j = 0; j < n3; ++j) {
+        o3[j] = net->psi_off[st->deg3_used[j]];
+        o3[n3 + j] = (long long)j * per3;
+      }
+      ITNBP_RETCHK(ensure_buf(ctx, st->offs3, o3.size() * 8));
+      ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->offs3.p, o3.data(), o3.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+      ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      const long long* s3 = (const long long*)st->offs3.p;
+      const long long* d3 = s3 + n3;
+      dim3 g3(64, n3);
+      if (!acc_f64) {
+        ITNBP_RETCHK(ensure_buf(ctx, st->lay3_16, (size_t)st->deg3.size() * per3 * 2));
+        ITNBP_RETCHK(ensure_buf(ctx, st->P3_16, (size_t)2 * st->deg3.size() * per3 * 2));
+        if (!st->lay3_16_ok) {
+          __bf16* C = (__bf16*)st->lay3_16.p;
+          switch (net->dtype) {
+            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, __bf16, float>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C); break;
+            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, __bf16, float>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C); break;
+            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, __bf16, float>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C); break;
+          }
+          ITNBP_HIPCHK(ctx, hipGetLastError());
+          st->lay3_16_ok = true;
+        }
+      } else {
+        ITNBP_RETCHK(ensure_buf(ctx, st->lay3_64, (size_t)st->deg3.size() * per3 * 8));
+        ITNBP_RETCHK(ensure_buf(ctx, st->P3_64, (size_t)2 * st->deg3.size() * per3 * 8));
+        if (!st->lay3_64_ok) {
+          double* C = (double*)st->lay3_64.p;
+          switch (net->dtype) {
+            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, double, double>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C); break;
+            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, double, double>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C); break;
+            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, double, double>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C); break;
+          }
+          ITNBP_HIPCHK(ctx, hipGetLastError());
+          st->lay3_64_ok = true;
+        }
+      }
+    }
+  }
   // layouts
   std::vector<long long> offs(2 * nb);
   for (int bi = 0; bi < nb; ++bi) {
@@ -1160,7 +2284,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   const long long* d_src = (const long long*)st->offs.p;
   const long long* d_dst = d_src + nb;
   dim3 lgrid(256, nb);
-  if (!acc_f64 && !st->lay16_ok) {
+  if (!acc_f64 && !st->lay16_ok && nb > 0) {
     ITNBP_RETCHK(ensure_buf(ctx, st->lay16, (size_t)2 * nb * per * 2));
     __bf16* A = (__bf16*)st->lay16.p;
     __bf16* B = A + nb * per;
@@ -1172,7 +2296,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     ITNBP_HIPCHK(ctx, hipGetLastError());
     st->lay16_ok = true;
   }
-  if (acc_f64 && !st->lay64_ok) {
+  if (acc_f64 && !st->lay64_ok && nb > 0) {
     ITNBP_RETCHK(ensure_buf(ctx, st->lay64, (size_t)2 * nb * per * 8));
     double* A = (double*)st->lay64.p;
     double* B = A + nb * per;
@@ -1184,15 +2308,18 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     ITNBP_HIPCHK(ctx, hipGetLastError());
     st->lay64_ok = true;
   }
+  // partial sums: the absorbed-pair passes need 2 items x FUSED_CHUNKS x 2 matrices
+  // per bulk vertex, the eigenbasis pipeline 4 bonds x EIG_NPART per (bulk or degree-3) site
+  const size_t nparts = std::max(ni * FUSED_CHUNKS * 2, (size_t)(nb + st->e3items.size()) * 4 * EIG_NPART);
   if (acc_f64) {
     ITNBP_RETCHK(ensure_buf(ctx, st->P64, (size_t)2 * nb * per * 8));
-    ITNBP_RETCHK(ensure_buf(ctx, st->part64, ni * FUSED_CHUNKS * 2 * FBLK * 8));
+    ITNBP_RETCHK(ensure_buf(ctx, st->part64, nparts * FBLK * 8));
   } else {
-    if (fast_precise())
+    if (fast_precise() && !st->eig)
       ITNBP_RETCHK(ensure_buf(ctx, st->P32, (size_t)2 * nb * per * 4));
     else
       ITNBP_RETCHK(ensure_buf(ctx, st->P16, (size_t)2 * nb * per * 2));
-    ITNBP_RETCHK(ensure_buf(ctx, st->part32, ni * FUSED_CHUNKS * 2 * FBLK * 4));
+    ITNBP_RETCHK(ensure_buf(ctx, st->part32, nparts * FBLK * 4));
   }
   ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return ITNBP_OK;
@@ -1302,12 +2429,113 @@ int fast_chunk(itnbp_net* net, int acc_f64) {
   return st ? st->chunk[acc_f64 ? 1 : 0] : 0;
 }
 
+// one sweep of the eigenbasis pipeline over every bulk and degree-3 site
+static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg_in, void* msg_out,
+                     MsgStat* stats, int normalize) {
+  const int nb = (int)st->eitems.size();
+  const int n3 = (int)st->e3items.size();
+  const int ne4 = (int)st->eig_moff.size();
+  if (ne4 == 0) return ITNBP_OK;
+  const EigItem* ei = (const EigItem*)st->d_eitems.p;
+  const EigItem* e3 = (const EigItem*)st->d_e3items.p;
+  const FastItem* k1 = (const FastItem*)st->d_k1items.p;
+  const int e0 = prof_mark(ctx);
+  if (acc_f64)
+    hipLaunchKernelGGL((k_msg_eigh<double>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
+                       (const long long*)st->d_eig_moff.p, ne4, (const double*)msg_in, (double*)st->eigU64.p,
+                       (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr);
+  else
+    hipLaunchKernelGGL((k_msg_eigh<float>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
+                       (const long long*)st->d_eig_moff.p, ne4, (const float*)msg_in, (double*)st->eigU64.p,
+                       (double*)st->eigL64.p, (float*)st->eigU32.p, (float*)st->eigL32.p);
+  const int e1 = prof_mark(ctx);
+  if (nb > 0) {
+    if (acc_f64)
+      hipLaunchKernelGGL(k_absorb_f64r, dim3(256, nb), dim3(256), 0, ctx->stream, k1, (const double*)st->lay64.p,
+                         (const double*)st->eigU64.p, (double*)st->P64.p);
+    else
+      hipLaunchKernelGGL(k_absorb_bf16c, dim3(64, nb), dim3(512), 0, ctx->stream, k1, (const __bf16*)st->lay16.p,
+                         (const float*)st->eigU32.p, (__bf16*)st->P16.p);
+  }
+  const int e2 = prof_mark(ctx);
+  if (nb > 0) {
+    if (acc_f64)
+      hipLaunchKernelGGL(k_rot34_f64, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei, (const double*)st->P64.p,
+                         (const double*)st->eigU64.p, (const double*)st->eigL64.p, (double*)st->P64.p,
+                         (double*)st->part64.p);
+    else
+      hipLaunchKernelGGL(k_rot34_bf16, dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei, (const __bf16*)st->P16.p,
+                         (const float*)st->eigU32.p, (const float*)st->eigL32.p, (__bf16*)st->P16.p,
+                         (float*)st->part32.p);
+  }
+  const int e3m = prof_mark(ctx);
+  if (nb > 0) {
+    if (acc_f64)
+      hipLaunchKernelGGL(k_gram12_f64<32>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei,
+                         (const double*)st->P64.p, (const double*)st->eigL64.p, (double*)st->part64.p);
+    else
+      hipLaunchKernelGGL(k_gram12_bf16<32>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei,
+                         (const __bf16*)st->P16.p, (const float*)st->eigL32.p, (float*)st->part32.p);
+  }
+  const int e4 = prof_mark(ctx);
+  if (n3 > 0) {
+    if (acc_f64) {
+      hipLaunchKernelGGL(k_rot12_d3_f64, dim3(4, n3), dim3(256), 0, ctx->stream, e3, (const double*)st->lay3_64.p,
+                         (const double*)st->eigU64.p, (double*)st->P3_64.p);
+      hipLaunchKernelGGL(k_rot3_d3_f64, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3,
+                         (const double*)st->P3_64.p, (const double*)st->eigU64.p, (const double*)st->eigL64.p,
+                         (double*)st->P3_64.p, (double*)st->part64.p);
+      hipLaunchKernelGGL(k_gram12_f64<1>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3,
+                         (const double*)st->P3_64.p, (const double*)st->eigL64.p, (double*)st->part64.p);
+    } else {
+      hipLaunchKernelGGL(k_rot12_d3_bf16, dim3(4, n3), dim3(256), 0, ctx->stream, e3, (const __bf16*)st->lay3_16.p,
+                         (const float*)st->eigU32.p, (__bf16*)st->P3_16.p);
+      hipLaunchKernelGGL(k_rot3_d3_bf16, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3,
+                         (const __bf16*)st->P3_16.p, (const float*)st->eigU32.p, (const float*)st->eigL32.p,
+                         (__bf16*)st->P3_16.p, (float*)st->part32.p);
+      hipLaunchKernelGGL(k_gram12_bf16<1>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3,
+                         (const __bf16*)st->P3_16.p, (const float*)st->eigL32.p, (float*)st->part32.p);
+    }
+  }
+  const int e5 = prof_mark(ctx);
+  if (acc_f64) {
+    if (nb > 0)
+      hipLaunchKernelGGL((k_eig_finalize<double, double>), dim3(4, nb), dim3(256), 0, ctx->stream, ei,
+                         (const double*)st->part64.p, (const double*)st->eigU64.p, (const double*)msg_in,
+                         (double*)msg_out, stats, normalize);
+    if (n3 > 0)
+      hipLaunchKernelGGL((k_eig_finalize<double, double>), dim3(3, n3), dim3(256), 0, ctx->stream, e3,
+                         (const double*)st->part64.p, (const double*)st->eigU64.p, (const double*)msg_in,
+                         (double*)msg_out, stats, normalize);
+  } else {
+    if (nb > 0)
+      hipLaunchKernelGGL((k_eig_finalize<float, float>), dim3(4, nb), dim3(256), 0, ctx->stream, ei,
+                         (const float*)st->part32.p, (const double*)st->eigU64.p, (const float*)msg_in,
+                         (float*)msg_out, stats, normalize);
+    if (n3 > 0)
+      hipLaunchKernelGGL((k_eig_finalize<float, float>), dim3(3, n3), dim3(256), 0, ctx->stream, e3,
+                         (const float*)st->part32.p, (const double*)st->eigU64.p, (const float*)msg_in,
+                         (float*)msg_out, stats, normalize);
+  }
+  const int e6 = prof_mark(ctx);
+  prof_add(ctx, "eigh", e0, e1);
+  prof_add(ctx, acc_f64 ? "rot12_f64" : "rot12_bf16", e1, e2);
+  prof_add(ctx, acc_f64 ? "rot34_f64" : "rot34_bf16", e2, e3m);
+  prof_add(ctx, acc_f64 ? "gram12_f64" : "gram12_bf16", e3m, e4);
+  prof_add(ctx, acc_f64 ? "deg3_f64" : "deg3_bf16", e4, e5);
+  prof_add(ctx, "finalize", e5, e6);
+  ITNBP_HIPCHK(ctx, hipGetLastError());
+  return ITNBP_OK;
+}
+
 int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, MsgStat* stats,
                int normalize) {
   itnbp_ctx* ctx = net->ctx;
   FastState* st = state_of(net);
-  if (!st || st->items.empty()) return ITNBP_OK;
+  if (!st) return ITNBP_OK;
+  if (st->eig) return eig_sweep(ctx, st, acc_f64, msg_in, msg_out, stats, normalize);
   const int ni = (int)st->items.size();
+  if (ni == 0) return ITNBP_OK;
   const FastItem* items = (const FastItem*)st->d_items.p;
   const int chunk = tune_chunk(net, st, acc_f64, msg_in);
   launch_all_pairs(ctx, st, acc_f64, msg_in, chunk);

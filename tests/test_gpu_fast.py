@@ -31,12 +31,14 @@ def oracle_ref(net34):
     return m3, m40
 
 
-@pytest.mark.parametrize("lds", ["0", "1"])
-def test_f64_pipeline_matches_oracle(net34, lds, monkeypatch):
+@pytest.mark.parametrize("pipeline,lds", [("eig", "0"), ("legacy", "0"), ("legacy", "1")])
+def test_f64_pipeline_matches_oracle(net34, pipeline, lds, monkeypatch):
     monkeypatch.setenv("ITNBP_F64_LDS", lds)
+    monkeypatch.setenv("ITNBP_FAST_LEGACY", "1" if pipeline == "legacy" else "0")
     g, psi = net34
     bpc = itnbp.update(itnbp.initialize_cache(psi), maxiter=3, edge_sequence="parallel")
     assert "mfma-f64" in bpc.last_update["path"]
+    assert ("eig" in bpc.last_update["path"]) == (pipeline == "eig")
     og, opsi = oracle_graph(g), oracle_psi(psi)
     ref, _, _, _ = O.bp_update(og, opsi, O.identity_messages(og, opsi), maxiter=3, schedule="parallel")
     got = to_index(g, bpc.messages())
@@ -44,14 +46,16 @@ def test_f64_pipeline_matches_oracle(net34, lds, monkeypatch):
     assert err < 1e-12, err
 
 
-@pytest.mark.parametrize("flavour,tol_low", [("precise", 2e-4), ("compact", 3e-3)])
+@pytest.mark.parametrize("flavour,tol_low", [("eig", 3e-3), ("precise", 2e-4), ("compact", 3e-3)])
 def test_bf16_pipeline_then_f64_refinement(net34, oracle_ref, flavour, tol_low, monkeypatch):
     monkeypatch.setenv("ITNBP_BF16_PRECISE", "1" if flavour == "precise" else "0")
+    monkeypatch.setenv("ITNBP_FAST_LEGACY", "0" if flavour == "eig" else "1")
     g, psi = net34
     m3, m40 = oracle_ref
     bpc = itnbp.initialize_cache(psi, storage="bf16")
     low = itnbp.update(bpc, maxiter=3, edge_sequence="parallel", precision="f32")
     assert "bf16" in low.last_update["path"]
+    assert ("eig" in low.last_update["path"]) == (flavour == "eig")
     got = to_index(g, low.messages())
     err_low = max(np.abs(got[e] - m3[e]).max() for e in m3)
     assert err_low < tol_low, err_low
@@ -61,3 +65,72 @@ def test_bf16_pipeline_then_f64_refinement(net34, oracle_ref, flavour, tol_low, 
     got = to_index(g, full.messages())
     err = max(np.abs(got[e] - m40[e]).max() for e in m40)
     assert err < 1e-10, err
+
+
+def _random_messages(g, psi, rng, symmetric):
+    """Random (indefinite when symmetric) messages for every directed edge."""
+    msgs = {}
+    for (u, v) in g.edges:
+        chi = psi[u].shape[g.axis(u, v)]
+        for e in ((u, v), (v, u)):
+            a = rng.standard_normal((chi, chi))
+            msgs[e] = (a + a.T) / 2 if symmetric else a
+    return msgs
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_user_messages_pick_pipeline_and_match_oracle(net34, symmetric):
+    """Symmetric user messages (eigenvalues of both signs) run on the eigenbasis
+    pipeline; a non-symmetric one sends the sweep to the absorbed-pair passes.
+    Both match the oracle started from the same messages."""
+    g, psi = net34
+    rng = np.random.default_rng(7 if symmetric else 8)
+    msgs = _random_messages(g, psi, rng, symmetric)
+    bpc = itnbp.initialize_cache(psi)
+    for e, m in msgs.items():
+        bpc.set_message_(e, m)
+    out = itnbp.update(bpc, maxiter=2, edge_sequence="parallel")
+    path = out.last_update["path"]
+    assert "mfma-f64" in path
+    assert ("eig" in path) == symmetric
+    og, opsi = oracle_graph(g), oracle_psi(psi)
+    start = {(g.index[a], g.index[b]): m for (a, b), m in msgs.items()}
+    ref, _, _, _ = O.bp_update(og, opsi, start, maxiter=2, schedule="parallel")
+    got = to_index(g, out.messages())
+    err = max(np.abs(got[e] - ref[e]).max() for e in ref)
+    assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("precision", ["f64", "bf16_f64"])
+def test_degree3_only_lattice(precision):
+    """Heavy-hex lattice with chi = 32: degree-3 and degree-2 sites only, so the
+    eigenbasis pipeline runs its degree-3 kernels without any bulk vertex."""
+    g = itnbp.heavy_hex(4, 3)
+    assert max(g.degree(v) for v in g.vertices) == 3
+    psi = itnbp.random_tensornetwork(np.random.default_rng(33), np.float64, g, link_space=32,
+                                     scale=1 / np.sqrt(32))
+    storage = "f64" if precision == "f64" else "bf16"
+    bpc = itnbp.initialize_cache(psi, storage=storage)
+    psi_b = bpc.tensornetwork()
+    og, opsi = oracle_graph(g), oracle_psi(psi_b)
+    if precision == "f64":
+        out = itnbp.update(bpc, maxiter=3, edge_sequence="parallel")
+        assert "mfma-f64-eig" in out.last_update["path"]
+        ref, _, _, _ = O.bp_update(og, opsi, O.identity_messages(og, opsi), maxiter=3, schedule="parallel")
+        tol = 1e-12
+    else:
+        low = itnbp.update(bpc, maxiter=3, edge_sequence="parallel", precision="f32")
+        assert "bf16-eig" in low.last_update["path"]
+        ref3, _, _, _ = O.bp_update(og, opsi, O.identity_messages(og, opsi), maxiter=3, schedule="parallel")
+        got = to_index(g, low.messages())
+        err_low = max(np.abs(got[e] - ref3[e]).max() for e in ref3)
+        assert err_low < 3e-3, err_low
+        out = itnbp.update(bpc, maxiter=60, res_tol=1e-11, edge_sequence="parallel", precision="bf16_f64",
+                           switch_tol=1e-4)
+        assert out.last_update["converged"]
+        it = out.last_update["iterations"]
+        ref, _, _, _ = O.bp_update(og, opsi, O.identity_messages(og, opsi), maxiter=80, schedule="parallel")
+        tol = 1e-9
+    got = to_index(g, out.messages())
+    err = max(np.abs(got[e] - ref[e]).max() for e in ref)
+    assert err < tol, err
