@@ -13,4 +13,5 @@ from .cache import (BeliefPropagationCache, QuadraticFormNetwork, OPS, default_e
                     identity_messages, initialize_cache, logscalar, message_diff, norm_sqr_network,
                     normalize, rdm, region_scalars, rescale, rescale_, scalar, update, update_,
                     updated_message, vertex_scalars, energy_and_gradient, update_batched_)
-from .apply import Gate, apply, apply_gates_, edge_coloring, simple_update_round_  # noqa: F401,E402
+from .apply import Gate, apply, apply_gates_, edge_coloring, simple_update_round_, tebd_bp_  # noqa: F401,E402
+from .io import load as load_network, save as save_network  # noqa: F401,E402

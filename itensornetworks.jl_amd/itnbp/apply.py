@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib as L
-from .cache import BeliefPropagationCache, QuadraticFormNetwork
+from .cache import BeliefPropagationCache, QuadraticFormNetwork, update_
 from .network import ITensorNetwork
 
 
@@ -145,3 +145,27 @@ def simple_update_round_(bpc: BeliefPropagationCache, gate_fn, *, maxdim, cutoff
         te = apply_gates_(bpc, gates, maxdim=maxdim, cutoff=cutoff, normalize=normalize)
         errs.update(zip(cls, te))
     return errs
+
+
+def tebd_bp_(bpc: BeliefPropagationCache, gate_fn, nsteps, *, maxdim, cutoff=None, normalize=True,
+             bp_maxiter=30, bp_tol=1e-10, edge_sequence="parallel", callback=None):
+    """Gate evolution with BP regauging between rounds, in place on the cache.
+
+    The reference's TEBD loop (src/tebd.jl:2-31) applies gates without
+    environments; this is the BP-gauged variant: every step first updates the
+    cache's messages (warm-started from the previous step's) and then applies
+    one gate per bond, colour class by colour class, with those messages as the
+    product environment of every gate.  ``gate_fn(u, v)`` returns the
+    (d_u d_v)^2 gate of bond (u, v) (first site fastest), e.g. exp(-dt H_uv).
+    A final update leaves the cache consistent with the evolved state.
+    ``callback(step, truncation_errors)`` is called after every gate round.
+    Returns the per-step maximum truncation error."""
+    hist = []
+    for step in range(int(nsteps)):
+        update_(bpc, maxiter=bp_maxiter, tol=bp_tol, edge_sequence=edge_sequence)
+        errs = simple_update_round_(bpc, gate_fn, maxdim=maxdim, cutoff=cutoff, normalize=normalize)
+        hist.append(max(errs.values()) if errs else 0.0)
+        if callback is not None:
+            callback(step, errs)
+    update_(bpc, maxiter=bp_maxiter, tol=bp_tol, edge_sequence=edge_sequence)
+    return hist
