@@ -132,6 +132,7 @@ class BeliefPropagationCache:
         self._host_psi = psi
         self._psi_dirty = False
         self.last_update = None
+        self.partitioning = None  # PartitionedNetwork when built over generalised partitions
         if _handle is not None:
             self.h = _handle
             return
@@ -171,6 +172,7 @@ class BeliefPropagationCache:
         c._host_psi = c.qf.ket
         c._psi_dirty = False
         c.last_update = self.last_update
+        c.partitioning = self.partitioning
         return c
 
     # -- structure -------------------------------------------------------------
@@ -287,11 +289,21 @@ def identity_messages(qf):
     return out
 
 
-def initialize_cache(tn, alg="bp", *, messages=None, storage=None, ctx=None):
+def initialize_cache(tn, alg="bp", *, messages=None, storage=None, ctx=None, partitioned_vertices=None):
     """initialize_cache(Algorithm"bp", ::QuadraticFormNetwork) (initialize_cache.jl:14-29):
-    empty messages on trees, identity messages on loopy graphs."""
+    empty messages on trees, identity messages on loopy graphs.
+    ``partitioned_vertices`` (BeliefPropagationCache(tn; partitioned_vertices),
+    beliefpropagationcache.jl:20-35) groups vertices into partitions: the cache
+    is then built over the partition network (see itnbp.partition)."""
     if alg != "bp":
         raise ValueError(f"unknown cache algorithm {alg!r}")
+    if partitioned_vertices is not None:
+        from .partition import PartitionedNetwork
+        psi = tn.ket if isinstance(tn, QuadraticFormNetwork) else tn
+        pn = PartitionedNetwork(psi, partitioned_vertices)
+        bpc = initialize_cache(pn.network, messages=messages, storage=storage, ctx=ctx)
+        bpc.partitioning = pn
+        return bpc
     qf = tn if isinstance(tn, QuadraticFormNetwork) else QuadraticFormNetwork(tn)
     bpc = BeliefPropagationCache(qf, storage=storage, ctx=ctx)
     if messages is None:
@@ -436,7 +448,21 @@ def rdm(bpc, verts):
 
 
 def expect_on_cache(bpc, op, vertices=None):
-    """<O_v> = tr(O rho_v)/tr(rho_v) for every vertex (expect.jl:5-19)."""
+    """<O_v> = tr(O rho_v)/tr(rho_v) for every vertex (expect.jl:5-19).  On a
+    cache over generalised partitions, rho_v is the partial trace of v's
+    partition RDM and ``vertices`` are vertices of the original network."""
+    if bpc.partitioning is not None:
+        pn = bpc.partitioning
+        vertices = list(pn.psi.graph.vertices) if vertices is None else list(vertices)
+        rho_p, out = {}, {}
+        for v in vertices:
+            p = pn.owner[v]
+            if p not in rho_p:
+                rho_p[p] = rdm(bpc, [p])
+            r = pn.member_rdm(rho_p[p], v)
+            o = op_matrix(op, pn.psi.phys_dim(v))
+            out[v] = float(np.trace(o @ r) / np.trace(r))
+        return out
     g = bpc.graph
     vertices = list(g.vertices) if vertices is None else list(vertices)
     ids = np.array([g.index[v] for v in vertices], dtype=np.int32)
