@@ -326,6 +326,12 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
       return fr;
     }
   }
+  // whole-vertex tasks for the small-chi vertices the pipeline did not take
+  bool use_small = false;
+  if (o->schedule == ITNBP_SCHED_PARALLEL) {
+    RETCHK(small_prepare(net, seq, ph.acc_f64, handled));
+    use_small = small_task_count(net) > 0;
+  }
   // one plan for the whole sweep; run r launches groups [run_groups[r], run_groups[r+1])
   TaskPlan plan;
   std::vector<size_t> run_groups{0};
@@ -351,6 +357,7 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
       if (partial)
         HIPCHK(ctx, hipMemcpyAsync(nxt, cur, net->msg_total * eb, hipMemcpyDeviceToDevice, ctx->stream));
       if (use_fast) RETCHK(fast_sweep(net, ph.acc_f64, cur, nxt, d_stats, o->normalize));
+      if (use_small) RETCHK(small_sweep(net, ph.acc_f64, ph.psi, ph.psi_dtype, cur, nxt, d_stats, o->normalize));
       RETCHK(launch_plan(net, plan, 0, plan.groups.size(), d_tasks, ph.psi, ph.psi_dtype, cur, nxt, d_stats));
       net->cur = 1 - net->cur;
     } else {
@@ -634,6 +641,7 @@ int itnbp_net_destroy(itnbp_net* net) {
   free_buf(net->red);
   free_buf(net->aux);
   release_fast_state(net);
+  release_small_state(net);
   delete net;
   return ITNBP_OK;
 }
@@ -830,6 +838,7 @@ int itnbp_update(itnbp_net* net, const itnbp_update_opts* o, itnbp_update_result
     float ms = 0;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     r.device_ms = ms;
+    if (small_task_count(net) > 0) ctx->path += " [vertex " + std::to_string(small_task_count(net)) + "]";
     for (int a = 0; a < 2; ++a)
       if (fast_chunk(net, a) > 0) ctx->path += std::string(a ? " [f64 chunk " : " [bf16 chunk ") + std::to_string(fast_chunk(net, a)) + "]";
     r.mean_diff = md;

@@ -146,6 +146,7 @@ struct itnbp_net {
   VDesc* d_vdesc = nullptr;
   DevBuf scratch, tasks, stats, partials, red, aux;
   void* fast = nullptr;  // state of the specialised pipeline (k_fast.hip)
+  void* small = nullptr; // state of the per-vertex small-chi kernel (k_vertex.hip)
 };
 
 size_t dtype_size(int dtype);

@@ -40,6 +40,17 @@ void release_fast_state(itnbp_net* net);
 int fast_chunk(itnbp_net* net, int acc_f64);
 void invalidate_fast_state(itnbp_net* net);
 
+// ---- per-vertex small-chi kernel (k_vertex.hip) ----------------------------
+// symmetric messages and not disabled (ITNBP_DISABLE_VERTEX=1)
+bool small_path_enabled(const itnbp_net* net);
+// mark the entries of seq produced by whole-vertex tasks (every chi <= 16, site
+// tensor within LDS, every outgoing message in seq and not yet handled)
+int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::vector<char>& handled);
+int small_task_count(itnbp_net* net);
+int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, void* msg_out,
+                MsgStat* stats, int normalize);
+void release_small_state(itnbp_net* net);
+
 #define ITNBP_RETCHK(call)          \
   do {                              \
     int _r = (call);                \

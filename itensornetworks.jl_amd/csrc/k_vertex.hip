@@ -1,0 +1,393 @@
+// Per-vertex eigenbasis message update for small bond dimensions (every
+// chi <= 16, the site tensor resident in LDS).  One workgroup computes all k
+// outgoing messages of a vertex in a Jacobi sweep
+// (updated_message, src/caches/abstractbeliefpropagationcache.jl:225-239):
+//
+//   1. psi_v -> LDS;
+//   2. every incoming message M_c (symmetric) -> M_c = U_c diag(l_c) U_c^T by a
+//      wave-level cyclic Jacobi (the four waves take one bond each);
+//   3. psi_v rotated in place into the eigenbasis of every bond (fibre by fibre);
+//   4. per bond c: T'_c[r, r'] = sum over the fibres along c of
+//      prod_{e != c} l_e[r_e] Y[.., r, ..] Y[.., r', ..];
+//   5. T_c = U_c T'_c U_c^T, symmetrised, normalised, message_diff / residual.
+//
+// That is d prod(chi) (k chi + k chi) multiply-adds per vertex for all k
+// messages, versus k (k - 1) chi d prod(chi) for k independent contractions, and
+// psi is read from HBM once per vertex instead of once per message.  The
+// absorbed form is recovered exactly for any orthogonal U_c (eigenvalues of
+// either sign), so the result is the reference's update for symmetric messages.
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+#include "itnbp_ops.hpp"
+
+namespace {
+
+constexpr int SV_MAXCHI = 16;
+constexpr int SV_LD = SV_MAXCHI + 1;
+constexpr int SV_THREADS = 256;
+
+struct SmallTask {
+  int v;
+  int pad;
+  long long out[ITNBP_MAXK];  // outgoing message on bond axis a + 1
+  int slot[ITNBP_MAXK];
+};
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Symmetric n x n (n <= 16) eigendecomposition in LDS by one wave: A (ld
+// SV_LD) is overwritten by diag(l), V receives the eigenvectors as columns.
+__device__ void wave_jacobi_small(double* A, double* V, int n, double* rc, double* rs, int* rp, int* rq) {
+  const int l = threadIdx.x & 63;
+  const int ne = n + (n & 1), np = ne / 2;
+  for (int o = l; o < n * n; o += 64) {
+    const int i = o / n, j = o % n;
+    V[i * SV_LD + j] = (i == j) ? 1.0 : 0.0;
+  }
+  wave_sync();
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0, dg = 0;
+    for (int o = l; o < n * n; o += 64) {
+      const int i = o / n, j = o % n;
+      const double a = A[i * SV_LD + j];
+      if (i != j) off += a * a;
+      else dg += a * a;
+    }
+    for (int m = 32; m > 0; m >>= 1) {
+      off += __shfl_xor(off, m);
+      dg += __shfl_xor(dg, m);
+    }
+    if (off <= 1e-30 * dg || off == 0.0) break;
+    for (int st = 0; st < ne - 1; ++st) {
+      if (l < np) {
+        auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + st) % (ne - 1); };
+        int p = at(l), q = at(ne - 1 - l);
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1, s = 0;
+        if (q < n) {
+          const double apq = A[p * SV_LD + q];
+          if (apq != 0.0) {
+            const double zeta = (A[q * SV_LD + q] - A[p * SV_LD + p]) / (2 * apq);
+            const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+            c = 1 / sqrt(1 + t * t);
+            s = c * t;
+          }
+        }
+        rc[l] = c;
+        rs[l] = s;
+        rp[l] = p;
+        rq[l] = q;
+      }
+      wave_sync();
+      for (int o = l; o < np * n; o += 64) {  // rows: A <- J^T A
+        const int i = o / n, col = o % n;
+        const int p = rp[i], q = rq[i];
+        if (q >= n) continue;
+        const double c = rc[i], s = rs[i];
+        const double ap = A[p * SV_LD + col], aq = A[q * SV_LD + col];
+        A[p * SV_LD + col] = c * ap - s * aq;
+        A[q * SV_LD + col] = s * ap + c * aq;
+      }
+      wave_sync();
+      for (int o = l; o < np * n; o += 64) {  // columns: A <- A J, V <- V J
+        const int i = o / n, row = o % n;
+        const int p = rp[i], q = rq[i];
+        if (q >= n) continue;
+        const double c = rc[i], s = rs[i];
+        const double ap = A[row * SV_LD + p], aq = A[row * SV_LD + q];
+        A[row * SV_LD + p] = c * ap - s * aq;
+        A[row * SV_LD + q] = s * ap + c * aq;
+        const double vp = V[row * SV_LD + p], vq = V[row * SV_LD + q];
+        V[row * SV_LD + p] = c * vp - s * vq;
+        V[row * SV_LD + q] = s * vp + c * vq;
+      }
+      wave_sync();
+    }
+  }
+}
+
+__device__ inline double block_sum(double v, double* red) {
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  for (int i = 0; i < SV_THREADS / 64; ++i) s += red[i];
+  return s;
+}
+
+// LDS carve-up (bytes): fixed part then Y[D] in TA
+constexpr size_t SV_FIXED = (size_t)ITNBP_MAXK * SV_LD * SV_LD * 8   // U_c
+                            + 4 * SV_LD * SV_LD * 8                   // Jacobi work, one per wave
+                            + (size_t)ITNBP_MAXK * SV_MAXCHI * 8      // l_c
+                            + 4 * 8 * (8 + 8 + 4 + 4)                 // rotations
+                            + SV_THREADS * 8                          // group partials of T'
+                            + 2 * SV_LD * SV_LD * 8                   // T', U T'
+                            + 64;                                     // block reduction
+
+template <typename TS, typename TA, typename TM>
+__global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restrict__ vdesc,
+                                                          const SmallTask* __restrict__ tasks,
+                                                          const TS* __restrict__ psi, const TM* msg_in,
+                                                          TM* msg_out, MsgStat* __restrict__ stats,
+                                                          int normalize) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* Us = reinterpret_cast<double*>(smem);
+  double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
+  double* lam = As + 4 * SV_LD * SV_LD;
+  double* rc = lam + ITNBP_MAXK * SV_MAXCHI;
+  double* rs = rc + 32;
+  int* rp = reinterpret_cast<int*>(rs + 32);
+  int* rq = rp + 32;
+  double* gpart = reinterpret_cast<double*>(rq + 32);
+  double* Tp = gpart + SV_THREADS;
+  double* X = Tp + SV_LD * SV_LD;
+  double* red = X + SV_LD * SV_LD;
+  TA* Y = reinterpret_cast<TA*>(smem + SV_FIXED);
+
+  __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
+  const SmallTask* tk = tasks + blockIdx.x;
+  const VDesc* V = vdesc + tk->v;
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+  const int k = V->k, D = V->D;
+  if (tid == 0) {
+    int s = 1;
+    for (int a = 0; a <= k; ++a) {
+      dims[a] = V->dims[a];
+      strides[a] = s;
+      s *= dims[a];
+    }
+  }
+  __syncthreads();
+  // 1. psi -> LDS
+  const TS* src = psi + V->psi_off;
+  for (int i = tid; i < D; i += SV_THREADS) Y[i] = load_as<TA>(src, i);
+  // 2. eigendecompositions, one bond per wave
+  for (int a = wv; a < k; a += 4) {
+    const int n = dims[a + 1];
+    double* A = As + wv * SV_LD * SV_LD;
+    const TM* M = msg_in + V->in_msg[a + 1];
+    for (int o = l; o < n * n; o += 64) {
+      const int i = o / n, j = o % n;
+      A[i * SV_LD + j] = 0.5 * ((double)M[i + n * j] + (double)M[j + n * i]);
+    }
+    wave_sync();
+    wave_jacobi_small(A, Us + a * SV_LD * SV_LD, n, rc + 8 * wv, rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
+    if (l < n) lam[a * SV_MAXCHI + l] = A[l * SV_LD + l];
+    wave_sync();
+  }
+  __syncthreads();
+  // 3. rotate every bond: Y <- Y x_c U_c (in place, one fibre per thread)
+  for (int a = 0; a < k; ++a) {
+    const int n = dims[a + 1], st = strides[a + 1];
+    const int F = D / n;
+    const double* U = Us + a * SV_LD * SV_LD;
+    for (int f = tid; f < F; f += SV_THREADS) {
+      const int base = (f % st) + (f / st) * st * n;
+      TA x[SV_MAXCHI];
+#pragma unroll
+      for (int j = 0; j < SV_MAXCHI; ++j) x[j] = j < n ? Y[base + j * st] : (TA)0;
+      for (int r = 0; r < n; ++r) {
+        TA acc = 0;
+#pragma unroll
+        for (int j = 0; j < SV_MAXCHI; ++j)
+          if (j < n) acc += (TA)U[j * SV_LD + r] * x[j];
+        Y[base + r * st] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  // 4-5. per bond: weighted Gram, rotate back, normalise, statistics
+  for (int a = 0; a < k; ++a) {
+    const int n = dims[a + 1], st = strides[a + 1], nn = n * n;
+    const int F = D / n;
+    const int G = SV_THREADS / nn;  // fibre groups (>= 1 since n <= 16)
+    const int g = tid / nn, e = tid % nn;
+    double acc = 0;
+    if (g < G) {
+      const int r = e % n, r2 = e / n;
+      for (int f = g; f < F; f += G) {
+        const int base = (f % st) + (f / st) * st * n;
+        double w = 1.0;
+        for (int b = 0; b < k; ++b)
+          if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
+        acc += w * (double)Y[base + r * st] * (double)Y[base + r2 * st];
+      }
+    }
+    gpart[tid] = acc;
+    __syncthreads();
+    if (tid < nn) {
+      double t = 0;
+      for (int gg = 0; gg < G; ++gg) t += gpart[gg * nn + tid];
+      Tp[(tid % n) * SV_LD + tid / n] = t;  // T'[r][r']
+    }
+    __syncthreads();
+    const double* U = Us + a * SV_LD * SV_LD;
+    if (tid < nn) {
+      const int b = tid % n, rr = tid / n;
+      double s = 0;
+      for (int r = 0; r < n; ++r) s += U[b * SV_LD + r] * Tp[r * SV_LD + rr];
+      X[b * SV_LD + rr] = s;  // (U T')[b][r']
+    }
+    __syncthreads();
+    double tv = 0;
+    if (tid < nn) {
+      const int i = tid % n, j = tid / n;  // message layout M[i + n j]
+      double sij = 0, sji = 0;
+      for (int r = 0; r < n; ++r) {
+        sij += X[i * SV_LD + r] * U[j * SV_LD + r];
+        sji += X[j * SV_LD + r] * U[i * SV_LD + r];
+      }
+      tv = 0.5 * (sij + sji);
+    }
+    const double n2 = block_sum(tv * tv, red);
+    const double nrm = sqrt(n2);
+    const double sc = (normalize && nrm != 0.0) ? 1.0 / nrm : 1.0;
+    const long long off = tk->out[a];
+    double dot = 0, o2 = 0, d2 = 0, bad = 0;
+    if (tid < nn) {
+      const double nv = tv * sc;
+      const double ov = (double)msg_in[off + tid];
+      if (!isfinite(nv)) bad = 1;
+      dot = ov * nv;
+      o2 = ov * ov;
+      d2 = (nv - ov) * (nv - ov);
+      msg_out[off + tid] = (TM)nv;
+    }
+    dot = block_sum(dot, red);
+    o2 = block_sum(o2, red);
+    d2 = block_sum(d2, red);
+    bad = block_sum(bad, red);
+    const int slot = tk->slot[a];
+    if (tid == 0 && slot >= 0) {
+      const double nn2 = n2 * sc * sc;
+      MsgStat stt;
+      stt.diff = 1.0 - (dot * dot) / (o2 * nn2);
+      stt.res2 = nn2 > 0 ? d2 / nn2 : 0.0;
+      stt.norm = nrm;
+      stt.bad = (bad > 0 || !isfinite(nrm)) ? 1.0 : 0.0;
+      stats[slot] = stt;
+    }
+    __syncthreads();
+  }
+}
+
+struct SmallState {
+  std::vector<SmallTask> tasks;
+  DevBuf d_tasks;
+  size_t lds = 0;
+};
+
+SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net->small); }
+
+template <typename TS, typename TA, typename TM>
+hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
+                       MsgStat* stats, int normalize) {
+  auto kern = k_vertex_eig<TS, TA, TM>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((unsigned)st->tasks.size()), dim3(SV_THREADS), st->lds, net->ctx->stream,
+                     (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, (const TS*)psi,
+                     (const TM*)msg_in, (TM*)msg_out, stats, normalize);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool small_path_enabled(const itnbp_net* net) {
+  const char* e = getenv("ITNBP_DISABLE_VERTEX");
+  return net->msgs_sym && !(e && e[0] == '1');
+}
+
+int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::vector<char>& handled) {
+  itnbp_ctx* ctx = net->ctx;
+  SmallState* st = small_of(net);
+  if (!st) {
+    st = new SmallState();
+    net->small = st;
+  }
+  st->tasks.clear();
+  st->lds = 0;
+  if (!small_path_enabled(net)) return ITNBP_OK;
+  std::vector<int> pos(2 * net->ne, -1);
+  for (size_t i = 0; i < seq.size(); ++i) pos[seq[i]] = (int)i;
+  const size_t eb = acc_f64 ? 8 : 4;
+  for (int v = 0; v < net->nv; ++v) {
+    const VDesc& V = net->vdesc[v];
+    if (V.k < 1 || V.k > ITNBP_MAXK) continue;
+    bool ok = true;
+    for (int a = 1; a <= V.k && ok; ++a) ok = V.dims[a] <= SV_MAXCHI;
+    const size_t lds = SV_FIXED + (size_t)V.D * eb;
+    if (!ok || lds + 256 > 160 * 1024) continue;  // + the static LDS of the kernel
+    SmallTask t{};
+    t.v = v;
+    for (int a = 1; a <= V.k && ok; ++a) {
+      const int e = net->inc_edge[net->inc_ptr[v] + a - 1];
+      const int w = net->edges[2 * e] == v ? net->edges[2 * e + 1] : net->edges[2 * e];
+      const int m = msg_id_into(net, e, w);
+      if (pos[m] < 0 || handled[pos[m]]) ok = false;
+      else {
+        t.out[a - 1] = net->msg_off[m];
+        t.slot[a - 1] = pos[m];
+      }
+    }
+    if (!ok) continue;
+    for (int a = 0; a < V.k; ++a) handled[t.slot[a]] = 1;
+    st->tasks.push_back(t);
+    st->lds = std::max(st->lds, lds);
+  }
+  if (!st->tasks.empty()) {
+    ITNBP_RETCHK(ensure_buf(ctx, st->d_tasks, st->tasks.size() * sizeof(SmallTask)));
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_tasks.p, st->tasks.data(), st->tasks.size() * sizeof(SmallTask),
+                                     hipMemcpyHostToDevice, ctx->stream));
+    ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return ITNBP_OK;
+}
+
+int small_task_count(itnbp_net* net) {
+  SmallState* st = small_of(net);
+  return st ? (int)st->tasks.size() : 0;
+}
+
+int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, void* msg_out,
+                MsgStat* stats, int normalize) {
+  itnbp_ctx* ctx = net->ctx;
+  SmallState* st = small_of(net);
+  if (!st || st->tasks.empty()) return ITNBP_OK;
+  const int e0 = prof_mark(ctx);
+  hipError_t e;
+  if (acc_f64) {
+    switch (psi_dtype) {
+      case ITNBP_F64: e = launch_impl<double, double, double>(net, st, psi, msg_in, msg_out, stats, normalize); break;
+      case ITNBP_F32: e = launch_impl<float, double, double>(net, st, psi, msg_in, msg_out, stats, normalize); break;
+      default: e = launch_impl<bf16_t, double, double>(net, st, psi, msg_in, msg_out, stats, normalize); break;
+    }
+  } else {
+    switch (psi_dtype) {
+      case ITNBP_F64: e = launch_impl<double, float, float>(net, st, psi, msg_in, msg_out, stats, normalize); break;
+      case ITNBP_F32: e = launch_impl<float, float, float>(net, st, psi, msg_in, msg_out, stats, normalize); break;
+      default: e = launch_impl<bf16_t, float, float>(net, st, psi, msg_in, msg_out, stats, normalize); break;
+    }
+  }
+  prof_add(ctx, "vertex_eig", e0, prof_mark(ctx));
+  ITNBP_HIPCHK(ctx, e);
+  return ITNBP_OK;
+}
+
+void release_small_state(itnbp_net* net) {
+  SmallState* st = small_of(net);
+  if (!st) return;
+  free_buf(st->d_tasks);
+  delete st;
+  net->small = nullptr;
+}
