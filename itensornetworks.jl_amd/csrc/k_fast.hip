@@ -1055,12 +1055,17 @@ __device__ inline void wave_sync() {
 
 // Symmetric eigendecomposition of one 32x32 message per wave (cyclic parallel
 // Jacobi, 16 disjoint rotations per step, round-robin pairing), input
-// symmetrised as (M + M^T)/2.  grid (ceil(n/4)), 256 threads.
+// symmetrised as S = (M + M^T)/2.  With warm = 1 the previous sweep's
+// eigenvectors U0 (in U64) seed it: Jacobi then diagonalises U0^T S U0, which
+// is nearly diagonal once BP settles, and the rotations accumulate onto U0.
+// Stops when the off-diagonal mass is below tol2 of the diagonal's (1e-28 for
+// the FP64 pipeline, 1e-14 for the BF16 one, whose operands carry ~16 bits) or
+// stops decreasing.  grid (ceil(n/4)), 256 threads.
 template <typename TM>
 __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ moff, int n,
                                                  const TM* __restrict__ msg, double* __restrict__ U64,
                                                  double* __restrict__ L64, float* __restrict__ U32,
-                                                 float* __restrict__ L32) {
+                                                 float* __restrict__ L32, int warm, double tol2) {
   constexpr int LD = 33;
   __shared__ double As[4][FC * LD];
   __shared__ double Vs[4][FC * LD];
@@ -1072,13 +1077,55 @@ __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ 
   double* A = As[wv];
   double* V = Vs[wv];
   const TM* M = msg + moff[e];
+  double* u64 = U64 + (long long)e * FBLK;
   for (int o = l; o < FBLK; o += 64) {
     const int i = o & 31, j = o >> 5;
     A[i * LD + j] = 0.5 * ((double)M[i + FC * j] + (double)M[j + FC * i]);
-    V[i * LD + j] = (i == j) ? 1.0 : 0.0;
+    V[i * LD + j] = warm ? u64[i + FC * j] : ((i == j) ? 1.0 : 0.0);  // V[b][r] = U0[b + 32 r]
   }
   wave_sync();
+  if (warm) {
+    // A <- U0^T S U0.  Lane l owns column j = l & 31 and rows i = (l >> 5) + 2 m
+    // (m = 0..15) of each product; the 16 accumulators are independent, so the
+    // loads of one k-step are all in flight together.
+    const int jc = l & 31, i0 = l >> 5;
+    double t[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) t[m] = 0;
+#pragma unroll 2
+    for (int q = 0; q < FC; ++q) {
+      const double u = V[q * LD + jc];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) t[m] += A[(i0 + 2 * m) * LD + q] * u;  // (S U0)[i][j]
+    }
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) A[(i0 + 2 * m) * LD + jc] = t[m];
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) t[m] = 0;
+#pragma unroll 2
+    for (int q = 0; q < FC; ++q) {
+      const double x = A[q * LD + jc];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) t[m] += V[q * LD + i0 + 2 * m] * x;  // (U0^T S U0)[i][j]
+    }
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int i = i0 + 2 * m;
+      if (i <= jc) A[i * LD + jc] = t[m];
+    }
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int i = i0 + 2 * m;
+      if (i > jc) A[i * LD + jc] = A[jc * LD + i];  // exact symmetry from the upper triangle
+    }
+    wave_sync();
+  }
   const int col = l & 31, half = l >> 5;
+  double prev = INFINITY;
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0, dg = 0;
     for (int o = l; o < FBLK; o += 64) {
@@ -1091,7 +1138,8 @@ __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ 
       off += __shfl_xor(off, m);
       dg += __shfl_xor(dg, m);
     }
-    if (off <= 1e-30 * dg || off == 0.0) break;
+    if (off <= tol2 * dg || off == 0.0 || off >= prev) break;
+    prev = off;
     for (int st = 0; st < FC - 1; ++st) {
       if (l < 16) {
         auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + st) % (FC - 1); };
@@ -1115,34 +1163,50 @@ __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ 
         rq[wv][l] = q;
       }
       wave_sync();
-      // rows: A <- J^T A
+      // the 8 (disjoint) pairs of this lane: every operand is loaded before any
+      // store, so the LDS latency is paid once per phase rather than per pair
+      int pp[8], qq[8];
+      double cc[8], ss[8];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
         const int i = half + 2 * jj;
-        const int p = rp[wv][i], q = rq[wv][i];
-        const double c = rc[wv][i], s = rs[wv][i];
-        const double ap = A[p * LD + col], aq = A[q * LD + col];
-        A[p * LD + col] = c * ap - s * aq;
-        A[q * LD + col] = s * ap + c * aq;
+        pp[jj] = rp[wv][i];
+        qq[jj] = rq[wv][i];
+        cc[jj] = rc[wv][i];
+        ss[jj] = rs[wv][i];
+      }
+      // rows: A <- J^T A
+      double x0[8], x1[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        x0[jj] = A[pp[jj] * LD + col];
+        x1[jj] = A[qq[jj] * LD + col];
+      }
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        A[pp[jj] * LD + col] = cc[jj] * x0[jj] - ss[jj] * x1[jj];
+        A[qq[jj] * LD + col] = ss[jj] * x0[jj] + cc[jj] * x1[jj];
       }
       wave_sync();
       // columns: A <- A J, V <- V J
+      double y0[8], y1[8];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
-        const int i = half + 2 * jj;
-        const int p = rp[wv][i], q = rq[wv][i];
-        const double c = rc[wv][i], s = rs[wv][i];
-        const double ap = A[col * LD + p], aq = A[col * LD + q];
-        A[col * LD + p] = c * ap - s * aq;
-        A[col * LD + q] = s * ap + c * aq;
-        const double vp = V[col * LD + p], vq = V[col * LD + q];
-        V[col * LD + p] = c * vp - s * vq;
-        V[col * LD + q] = s * vp + c * vq;
+        x0[jj] = A[col * LD + pp[jj]];
+        x1[jj] = A[col * LD + qq[jj]];
+        y0[jj] = V[col * LD + pp[jj]];
+        y1[jj] = V[col * LD + qq[jj]];
+      }
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        A[col * LD + pp[jj]] = cc[jj] * x0[jj] - ss[jj] * x1[jj];
+        A[col * LD + qq[jj]] = ss[jj] * x0[jj] + cc[jj] * x1[jj];
+        V[col * LD + pp[jj]] = cc[jj] * y0[jj] - ss[jj] * y1[jj];
+        V[col * LD + qq[jj]] = ss[jj] * y0[jj] + cc[jj] * y1[jj];
       }
       wave_sync();
     }
   }
-  double* u64 = U64 + (long long)e * FBLK;
   float* u32 = U32 ? U32 + (long long)e * FBLK : nullptr;
   for (int o = l; o < FBLK; o += 64) {
     const int b = o & 31, r = o >> 5;
@@ -1160,6 +1224,10 @@ __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ 
 // rot34, BF16: WG (512 threads) -> (s, ROT_NZ consecutive r1, all 32 r2); wave
 // wv -> r2 in [4 wv, 4 wv + 4).  The four R blocks of the next r1 are in flight
 // while the current ones are multiplied.  grid (EIG_NPART, items)
+// With GRAM = false it is rot12: the same block geometry (input block
+// s + 2 (z + 32 w), output block s + 2 (x' + 32 y') at [z][w]) with U1, U2 in the
+// U[2], U[3] slots of its items, psi layout B as R and the R buffer as Q.
+template <bool GRAM>
 __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ items,
                                                     const __bf16* __restrict__ R,
                                                     const float* __restrict__ U,
@@ -1171,7 +1239,8 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
   const int q = blockIdx.x;
   const int s = q & 1, z0 = (q >> 1) * ROT_NZ;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
-  if (threadIdx.x < 2 * FC) lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[it.L[2 + (threadIdx.x >> 5)] + (threadIdx.x & 31)];
+  if (GRAM && threadIdx.x < 2 * FC)
+    lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[it.L[2 + (threadIdx.x >> 5)] + (threadIdx.x & 31)];
   bf16x8 aU3h[2], aU3l[2], bU4h[2], bU4l[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
@@ -1211,11 +1280,11 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
   for (int zi = 0; zi < ROT_NZ; ++zi) {
     const int z = z0 + zi;
     if (zi + 1 < ROT_NZ) fetch(z + 1, nxt);
-    const float l1 = lam[it.L[0] + z];
+    const float l1 = GRAM ? lam[it.L[0] + z] : 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int w = 4 * wv + i;
-      const float w12 = l1 * lam[it.L[1] + w];
+      const float w12 = GRAM ? l1 * lam[it.L[1] + w] : 0.f;
       f32x16 V = {};
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
@@ -1232,11 +1301,13 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
       for (int t = 0; t < 2; ++t) {
         Y = MFMA_BF16(aU3h[t], bV[t], Y);
         Y = MFMA_BF16(aU3l[t], bV[t], Y);
-        YT = MFMA_BF16(bV[t], aU3h[t], YT);
-        YT = MFMA_BF16(bV[t], aU3l[t], YT);
+        if (GRAM) {
+          YT = MFMA_BF16(bV[t], aU3h[t], YT);
+          YT = MFMA_BF16(bV[t], aU3l[t], YT);
+        }
       }
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
+      for (int t = 0; t < 2 && GRAM; ++t) {
         bf16x8 a3, b3, a4, b4;
         float l3[8], l4[8];
         lam_rows(0, t, l3);
@@ -1274,6 +1345,7 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
       cur[i][1] = nxt[i][1];
     }
   }
+  if (!GRAM) return;
   float* Red = reinterpret_cast<float*>(S);
   for (int w8 = 0; w8 < 8; ++w8) {
     if (wv == w8) {
@@ -1401,6 +1473,8 @@ __global__ __launch_bounds__(256) void k_gram12_bf16(const EigItem* __restrict__
 // grid (EIG_NPART, items) = (2 s x 4 r2 groups x 2 r1 groups)
 constexpr int ROT_NZ64 = 16;
 
+// GRAM = false: rot12, as for the BF16 kernel
+template <bool GRAM>
 __global__ __launch_bounds__(256) void k_rot34_f64(const EigItem* __restrict__ items,
                                                    const double* __restrict__ R,
                                                    const double* __restrict__ U,
@@ -1421,8 +1495,8 @@ __global__ __launch_bounds__(256) void k_rot34_f64(const EigItem* __restrict__ i
       aU3T[h][t] = U[it.U[2] + (4 * t + lg) + FC * (16 * h + li)];  // U3^T[16h+li][4t+lg]
       bU4[t][h] = U[it.U[3] + (4 * t + lg) + FC * (16 * h + li)];   // U4[4t+lg][16h+li]
     }
-    lam3v[t] = lam[it.L[2] + 4 * t + lg];
-    lam4v[t] = lam[it.L[3] + 4 * t + lg];
+    lam3v[t] = GRAM ? lam[it.L[2] + 4 * t + lg] : 0.0;
+    lam4v[t] = GRAM ? lam[it.L[3] + 4 * t + lg] : 0.0;
   }
   Acc32 T3, T4;
   acc_zero(T3);
@@ -1440,13 +1514,13 @@ __global__ __launch_bounds__(256) void k_rot34_f64(const EigItem* __restrict__ i
 #pragma unroll 1
   for (int zi = 0; zi < ROT_NZ64; ++zi) {
     const int z = z0 + zi;
-    const double l1 = lam[it.L[0] + z];
+    const double l1 = GRAM ? lam[it.L[0] + z] : 0.0;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int j = 2 * zi + i;
       if (j + 1 < 2 * ROT_NZ64) fetch(j + 1, nxt);
       const int wl = 2 * wv + i;
-      const double w12 = l1 * lam[it.L[1] + w0 + wl];
+      const double w12 = GRAM ? l1 * lam[it.L[1] + w0 + wl] : 0.0;
       Acc32 V, Y, YT;
       acc_zero(V);
       acc_zero(Y);
@@ -1454,14 +1528,14 @@ __global__ __launch_bounds__(256) void k_rot34_f64(const EigItem* __restrict__ i
       mm32(V, cur, bU4);       // V = Z U4
       mm32_accb(Y, aU3T, V);   // Y = U3^T V
 #pragma unroll
-      for (int t = 0; t < 8; ++t)  // Y^T = V^T U3
+      for (int t = 0; t < 8 && GRAM; ++t)  // Y^T = V^T U3
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
           for (int ni = 0; ni < 2; ++ni)
             YT.t[mi][ni] = MFMA_F64(V.t[t >> 2][mi][t & 3], aU3T[ni][t], YT.t[mi][ni]);
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
+      for (int t = 0; t < 8 && GRAM; ++t)
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -1494,6 +1568,7 @@ __global__ __launch_bounds__(256) void k_rot34_f64(const EigItem* __restrict__ i
     }
     __syncthreads();
   }
+  if (!GRAM) return;
   double* Red = S;
   for (int w4 = 0; w4 < 4; ++w4) {
     if (wv == w4) {
@@ -1965,8 +2040,11 @@ struct FastState {
   bool eig = false;                 // chosen for the current phase
   std::vector<EigItem> eitems;      // 1 per bulk vertex
   std::vector<FastItem> k1items;    // rot12 descriptors (the absorb kernel's item type)
+  std::vector<EigItem> k1eitems;    // rot12 descriptors for the rot34 kernels without Grams
   std::vector<long long> eig_moff;  // message offset of every eigendecomposed message
-  DevBuf d_eitems, d_k1items, d_eig_moff, eigU64, eigL64, eigU32, eigL32;
+  DevBuf d_eitems, d_k1items, d_k1eitems, d_eig_moff, eigU64, eigL64, eigU32, eigL32;
+  bool eig_warm = false;            // eigU64 holds an orthogonal matrix per slot (seeds the next eigh)
+  long long eig_count = 0;          // eigh launches since the last allocation; every 16th starts cold
   // degree-3 sites (eigenbasis pipeline only)
   std::vector<int> deg3;            // eligible degree-3 vertices
   std::vector<int> deg3_used;       // those whose outgoing messages are all in the sequence
@@ -2051,6 +2129,7 @@ void release_fast_state(itnbp_net* net) {
   free_buf(st->offs);
   free_buf(st->d_eitems);
   free_buf(st->d_k1items);
+  free_buf(st->d_k1eitems);
   free_buf(st->d_eig_moff);
   free_buf(st->eigU64);
   free_buf(st->eigL64);
@@ -2141,6 +2220,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   st->eig = use_eig(net);
   st->eitems.clear();
   st->k1items.clear();
+  st->k1eitems.clear();
   st->eig_moff.clear();
   st->e3items.clear();
   st->deg3_used.clear();
@@ -2168,13 +2248,22 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
       k.Ma = e.U[0];
       k.Mb = e.U[1];
       st->k1items.push_back(k);
+      EigItem k1 = e;
+      k1.R = (long long)(nb + bi) * per;  // psi layout B
+      k1.Q = e.R;
+      k1.U[2] = e.U[0];
+      k1.U[3] = e.U[1];
+      st->k1eitems.push_back(k1);
     }
     ITNBP_RETCHK(ensure_buf(ctx, st->d_eitems, nb * sizeof(EigItem)));
     ITNBP_RETCHK(ensure_buf(ctx, st->d_k1items, nb * sizeof(FastItem)));
+    ITNBP_RETCHK(ensure_buf(ctx, st->d_k1eitems, nb * sizeof(EigItem)));
     if (nb > 0) {
       ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_eitems.p, st->eitems.data(), nb * sizeof(EigItem),
                                        hipMemcpyHostToDevice, ctx->stream));
       ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_k1items.p, st->k1items.data(), nb * sizeof(FastItem),
+                                       hipMemcpyHostToDevice, ctx->stream));
+      ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_k1eitems.p, st->k1eitems.data(), nb * sizeof(EigItem),
                                        hipMemcpyHostToDevice, ctx->stream));
     }
     // degree-3 sites: slots after the bulk ones; l4 points at a table of ones
@@ -2213,6 +2302,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
       ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_e3items.p, st->e3items.data(), n3 * sizeof(EigItem),
                                        hipMemcpyHostToDevice, ctx->stream));
     }
+    if (ne4 * FBLK * 8 > st->eigU64.bytes) st->eig_warm = false;  // reallocated below: no seed
     ITNBP_RETCHK(ensure_buf(ctx, st->d_eig_moff, ne4 * 8));
     ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_eig_moff.p, st->eig_moff.data(), ne4 * 8, hipMemcpyHostToDevice,
                                      ctx->stream));
@@ -2438,33 +2528,45 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
   if (ne4 == 0) return ITNBP_OK;
   const EigItem* ei = (const EigItem*)st->d_eitems.p;
   const EigItem* e3 = (const EigItem*)st->d_e3items.p;
-  const FastItem* k1 = (const FastItem*)st->d_k1items.p;
+  // seeded by the previous eigenvectors, except every 16th launch, which starts
+  // from the identity so that rounding in the accumulated rotations cannot build up
+  const int warm = (st->eig_warm && st->eig_count % 16 != 0) ? 1 : 0;
   const int e0 = prof_mark(ctx);
   if (acc_f64)
     hipLaunchKernelGGL((k_msg_eigh<double>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
                        (const long long*)st->d_eig_moff.p, ne4, (const double*)msg_in, (double*)st->eigU64.p,
-                       (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr);
+                       (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr, warm, 1e-28);
   else
     hipLaunchKernelGGL((k_msg_eigh<float>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
                        (const long long*)st->d_eig_moff.p, ne4, (const float*)msg_in, (double*)st->eigU64.p,
-                       (double*)st->eigL64.p, (float*)st->eigU32.p, (float*)st->eigL32.p);
+                       (double*)st->eigL64.p, (float*)st->eigU32.p, (float*)st->eigL32.p, warm, 1e-14);
+  st->eig_warm = true;
+  ++st->eig_count;
   const int e1 = prof_mark(ctx);
   if (nb > 0) {
-    if (acc_f64)
-      hipLaunchKernelGGL(k_absorb_f64r, dim3(256, nb), dim3(256), 0, ctx->stream, k1, (const double*)st->lay64.p,
-                         (const double*)st->eigU64.p, (double*)st->P64.p);
-    else
+    // rot12.  FP64: the rot34 kernel without Grams (items k1e map U1, U2 to the
+    // U[2], U[3] slots, psi layout B to R and the R buffer to Q), whose persistent
+    // prefetching loop hides the FP64 products.  BF16: the absorb kernel, whose
+    // two workgroups per CU stream faster than the rot34 loop's one.
+    if (acc_f64) {
+      const EigItem* k1e = (const EigItem*)st->d_k1eitems.p;
+      hipLaunchKernelGGL(k_rot34_f64<false>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, k1e,
+                         (const double*)st->lay64.p, (const double*)st->eigU64.p, (const double*)st->eigL64.p,
+                         (double*)st->P64.p, (double*)nullptr);
+    } else {
+      const FastItem* k1 = (const FastItem*)st->d_k1items.p;
       hipLaunchKernelGGL(k_absorb_bf16c, dim3(64, nb), dim3(512), 0, ctx->stream, k1, (const __bf16*)st->lay16.p,
                          (const float*)st->eigU32.p, (__bf16*)st->P16.p);
+    }
   }
   const int e2 = prof_mark(ctx);
   if (nb > 0) {
     if (acc_f64)
-      hipLaunchKernelGGL(k_rot34_f64, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei, (const double*)st->P64.p,
+      hipLaunchKernelGGL(k_rot34_f64<true>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei, (const double*)st->P64.p,
                          (const double*)st->eigU64.p, (const double*)st->eigL64.p, (double*)st->P64.p,
                          (double*)st->part64.p);
     else
-      hipLaunchKernelGGL(k_rot34_bf16, dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei, (const __bf16*)st->P16.p,
+      hipLaunchKernelGGL(k_rot34_bf16<true>, dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei, (const __bf16*)st->P16.p,
                          (const float*)st->eigU32.p, (const float*)st->eigL32.p, (__bf16*)st->P16.p,
                          (float*)st->part32.p);
   }

@@ -50,6 +50,7 @@ __device__ void wave_jacobi_small(double* A, double* V, int n, double* rc, doubl
     V[i * SV_LD + j] = (i == j) ? 1.0 : 0.0;
   }
   wave_sync();
+  double prev = INFINITY;
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0, dg = 0;
     for (int o = l; o < n * n; o += 64) {
@@ -62,7 +63,9 @@ __device__ void wave_jacobi_small(double* A, double* V, int n, double* rc, doubl
       off += __shfl_xor(off, m);
       dg += __shfl_xor(dg, m);
     }
-    if (off <= 1e-30 * dg || off == 0.0) break;
+    // converged (off-diagonal mass at rounding level) or no longer decreasing
+    if (off <= 1e-28 * dg || off == 0.0 || off >= prev) break;
+    prev = off;
     for (int st = 0; st < ne - 1; ++st) {
       if (l < np) {
         auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + st) % (ne - 1); };
@@ -125,7 +128,8 @@ __device__ inline double block_sum(double v, double* red) {
   return s;
 }
 
-// LDS carve-up (bytes): fixed part then Y[D] in TA
+// LDS carve-up (bytes): fixed part, the fibre tables (FMAX weights and base
+// offsets), then Y[D] in TA
 constexpr size_t SV_FIXED = (size_t)ITNBP_MAXK * SV_LD * SV_LD * 8   // U_c
                             + 4 * SV_LD * SV_LD * 8                   // Jacobi work, one per wave
                             + (size_t)ITNBP_MAXK * SV_MAXCHI * 8      // l_c
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
                                                           const SmallTask* __restrict__ tasks,
                                                           const TS* __restrict__ psi, const TM* msg_in,
                                                           TM* msg_out, MsgStat* __restrict__ stats,
-                                                          int normalize) {
+                                                          int normalize, int fmax) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* Us = reinterpret_cast<double*>(smem);
   double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
@@ -152,7 +156,9 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
   double* Tp = gpart + SV_THREADS;
   double* X = Tp + SV_LD * SV_LD;
   double* red = X + SV_LD * SV_LD;
-  TA* Y = reinterpret_cast<TA*>(smem + SV_FIXED);
+  double* fw = reinterpret_cast<double*>(smem + SV_FIXED);
+  int* fb = reinterpret_cast<int*>(fw + fmax);
+  TA* Y = reinterpret_cast<TA*>(smem + SV_FIXED + (((size_t)fmax * 12 + 15) & ~(size_t)15));
 
   __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
   const SmallTask* tk = tasks + blockIdx.x;
@@ -210,18 +216,39 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
   for (int a = 0; a < k; ++a) {
     const int n = dims[a + 1], st = strides[a + 1], nn = n * n;
     const int F = D / n;
+    // fibre table: base offset and weight prod_{b != a} l_b of every fibre along a
+    for (int f = tid; f < F; f += SV_THREADS) {
+      const int base = (f % st) + (f / st) * st * n;
+      double w = 1.0;
+      for (int b = 0; b < k; ++b)
+        if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
+      fb[f] = base;
+      fw[f] = w;
+    }
+    __syncthreads();
     const int G = SV_THREADS / nn;  // fibre groups (>= 1 since n <= 16)
     const int g = tid / nn, e = tid % nn;
     double acc = 0;
     if (g < G) {
       const int r = e % n, r2 = e / n;
-      for (int f = g; f < F; f += G) {
-        const int base = (f % st) + (f / st) * st * n;
-        double w = 1.0;
-        for (int b = 0; b < k; ++b)
-          if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
-        acc += w * (double)Y[base + r * st] * (double)Y[base + r2 * st];
+      // four fibres per step, each with its own accumulator, so that their LDS
+      // loads are in flight together
+      double a1 = 0, a2 = 0, a3 = 0;
+      int f = g;
+      for (; f + 3 * G < F; f += 4 * G) {
+        const int b0 = fb[f], b1 = fb[f + G], b2 = fb[f + 2 * G], b3 = fb[f + 3 * G];
+        const double w0 = fw[f], w1 = fw[f + G], w2 = fw[f + 2 * G], w3 = fw[f + 3 * G];
+        const double p0 = (double)Y[b0 + r * st] * (double)Y[b0 + r2 * st];
+        const double p1 = (double)Y[b1 + r * st] * (double)Y[b1 + r2 * st];
+        const double p2 = (double)Y[b2 + r * st] * (double)Y[b2 + r2 * st];
+        const double p3 = (double)Y[b3 + r * st] * (double)Y[b3 + r2 * st];
+        acc += w0 * p0;
+        a1 += w1 * p1;
+        a2 += w2 * p2;
+        a3 += w3 * p3;
       }
+      for (; f < F; f += G) acc += fw[f] * (double)Y[fb[f] + r * st] * (double)Y[fb[f] + r2 * st];
+      acc += (a1 + a2) + a3;
     }
     gpart[tid] = acc;
     __syncthreads();
@@ -285,6 +312,7 @@ struct SmallState {
   std::vector<SmallTask> tasks;
   DevBuf d_tasks;
   size_t lds = 0;
+  int fmax = 0;  // largest fibre count over the tasks' bonds
 };
 
 SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net->small); }
@@ -297,7 +325,7 @@ hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const vo
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3((unsigned)st->tasks.size()), dim3(SV_THREADS), st->lds, net->ctx->stream,
                      (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, (const TS*)psi,
-                     (const TM*)msg_in, (TM*)msg_out, stats, normalize);
+                     (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax);
   return hipGetLastError();
 }
 
@@ -317,17 +345,28 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
   }
   st->tasks.clear();
   st->lds = 0;
+  st->fmax = 0;
   if (!small_path_enabled(net)) return ITNBP_OK;
   std::vector<int> pos(2 * net->ne, -1);
   for (size_t i = 0; i < seq.size(); ++i) pos[seq[i]] = (int)i;
   const size_t eb = acc_f64 ? 8 : 4;
+  int dmax_all = 0;
   for (int v = 0; v < net->nv; ++v) {
     const VDesc& V = net->vdesc[v];
     if (V.k < 1 || V.k > ITNBP_MAXK) continue;
     bool ok = true;
-    for (int a = 1; a <= V.k && ok; ++a) ok = V.dims[a] <= SV_MAXCHI;
-    const size_t lds = SV_FIXED + (size_t)V.D * eb;
-    if (!ok || lds + 256 > 160 * 1024) continue;  // + the static LDS of the kernel
+    int nmin = SV_MAXCHI;
+    for (int a = 1; a <= V.k && ok; ++a) {
+      ok = V.dims[a] <= SV_MAXCHI;
+      nmin = std::min(nmin, V.dims[a]);
+    }
+    if (!ok) continue;
+    // one LDS carve-up serves every task of the launch: the largest fibre table
+    // and the largest site tensor
+    const int fmax = std::max(st->fmax, V.D / std::max(1, nmin));
+    const int dmax = std::max(dmax_all, V.D);
+    const size_t lds = SV_FIXED + (((size_t)fmax * 12 + 15) & ~(size_t)15) + (size_t)dmax * eb;
+    if (lds + 256 > 160 * 1024) continue;  // + the static LDS of the kernel
     SmallTask t{};
     t.v = v;
     for (int a = 1; a <= V.k && ok; ++a) {
@@ -343,7 +382,9 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
     if (!ok) continue;
     for (int a = 0; a < V.k; ++a) handled[t.slot[a]] = 1;
     st->tasks.push_back(t);
-    st->lds = std::max(st->lds, lds);
+    st->fmax = fmax;
+    dmax_all = dmax;
+    st->lds = lds;
   }
   if (!st->tasks.empty()) {
     ITNBP_RETCHK(ensure_buf(ctx, st->d_tasks, st->tasks.size() * sizeof(SmallTask)));
