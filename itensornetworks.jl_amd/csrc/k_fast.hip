@@ -1240,7 +1240,7 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
   const int s = q & 1, z0 = (q >> 1) * ROT_NZ;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   if (GRAM && threadIdx.x < 2 * FC)
-    lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[it.L[2 + (threadIdx.x >> 5)] + (threadIdx.x & 31)];
+    lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[((threadIdx.x >> 5) ? it.L[3] : it.L[2]) + (threadIdx.x & 31)];
   bf16x8 aU3h[2], aU3l[2], bU4h[2], bU4l[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
@@ -1258,28 +1258,27 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
       bU4l[t][p] = lo;
     }
   __syncthreads();
-  // eigenvalue weights of accumulator rows kperm(t, g, 0..7): two runs of 4
-  auto lam_rows = [&](int which, int t, float (&o)[8]) {
-    const float4 a = *reinterpret_cast<const float4*>(&lam34[which][16 * t + 4 * g]);
-    const float4 b = *reinterpret_cast<const float4*>(&lam34[which][16 * t + 8 + 4 * g]);
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
-  };
   f32x16 T3 = {}, T4 = {};
+  // the wave's four R blocks of the current and the next r1 (plain unrolled
+  // code, no array references, so that the blocks stay in registers)
   bf16x8 cur[4][2], nxt[4][2];
-  auto fetch = [&](int z, bf16x8 (&dst)[4][2]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const __bf16* Z = R + it.R + (long long)(s + 2 * (z + FC * (4 * wv + i))) * FBLK;
+  for (int i = 0; i < 4; ++i) {
+    const __bf16* Z = R + it.R + (long long)(s + 2 * (z0 + FC * (4 * wv + i))) * FBLK;
 #pragma unroll
-      for (int t = 0; t < 2; ++t) dst[i][t] = *reinterpret_cast<const bf16x8*>(Z + c * FC + 16 * t + 8 * g);
-    }
-  };
-  fetch(z0, cur);
+    for (int t = 0; t < 2; ++t) cur[i][t] = *reinterpret_cast<const bf16x8*>(Z + c * FC + 16 * t + 8 * g);
+  }
 #pragma unroll 1
   for (int zi = 0; zi < ROT_NZ; ++zi) {
     const int z = z0 + zi;
-    if (zi + 1 < ROT_NZ) fetch(z + 1, nxt);
+    if (zi + 1 < ROT_NZ) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const __bf16* Z = R + it.R + (long long)(s + 2 * (z + 1 + FC * (4 * wv + i))) * FBLK;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) nxt[i][t] = *reinterpret_cast<const bf16x8*>(Z + c * FC + 16 * t + 8 * g);
+      }
+    }
     const float l1 = GRAM ? lam[it.L[0] + z] : 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -1309,9 +1308,13 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
 #pragma unroll
       for (int t = 0; t < 2 && GRAM; ++t) {
         bf16x8 a3, b3, a4, b4;
-        float l3[8], l4[8];
-        lam_rows(0, t, l3);
-        lam_rows(1, t, l4);
+        // eigenvalue weights of accumulator rows kperm(t, g, 0..7): two runs of 4
+        const float4 l3a = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 4 * g]);
+        const float4 l3b = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 8 + 4 * g]);
+        const float4 l4a = *reinterpret_cast<const float4*>(&lam34[1][16 * t + 4 * g]);
+        const float4 l4b = *reinterpret_cast<const float4*>(&lam34[1][16 * t + 8 + 4 * g]);
+        const float l3[8] = {l3a.x, l3a.y, l3a.z, l3a.w, l3b.x, l3b.y, l3b.z, l3b.w};
+        const float l4[8] = {l4a.x, l4a.y, l4a.z, l4a.w, l4b.x, l4b.y, l4b.z, l4b.w};
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
           const float yt = YT[8 * t + p], y = Y[8 * t + p];
@@ -1598,15 +1601,16 @@ __global__ __launch_bounds__(256) void k_rot34_f64(const EigItem* __restrict__ i
   }
 }
 
-// gram12, FP64: WG (256 threads) -> 4 NB consecutive Q blocks, wave -> NB, one
-// block of look-ahead; each block is read in both orientations (the second
-// read is served by the caches).  grid (EIG_NPART, items)
+// gram12, FP64: WG (256 threads) -> 4 NB consecutive Q blocks, wave -> NB.  Each
+// block is loaded once (coalesced, one block of look-ahead in registers) into a
+// per-wave LDS tile, from which both operand orientations (D for T'1, D^T for
+// T'2) are read.  grid (EIG_NPART, items)
 template <int NB>
 __global__ __launch_bounds__(256) void k_gram12_f64(const EigItem* __restrict__ items,
                                                     const double* __restrict__ Q,
                                                     const double* __restrict__ lam,
                                                     double* __restrict__ part) {
-  __shared__ double Red[2 * FBLK];
+  __shared__ double Db[4][FC * LD8];
   const EigItem it = items[blockIdx.y];
   const int q = blockIdx.x;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1621,24 +1625,36 @@ __global__ __launch_bounds__(256) void k_gram12_f64(const EigItem* __restrict__ 
   acc_zero(T1);
   acc_zero(T2);
   const long long r0 = (long long)q * (4 * NB) + wv * NB;
-  double cr[2][8], cc[2][8], nr[2][8], nc[2][8];
-  auto fetch = [&](long long r, double (&dr)[2][8], double (&dc)[2][8]) {
-    const double* D = Q + it.Q + r * FBLK;
+  double* db = Db[wv];
+  double2 nD[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) nD[j] = *reinterpret_cast<const double2*>(Q + it.Q + r0 * FBLK + 2 * l + 128 * j);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int o = 2 * l + 128 * j, a = o >> 5, b = o & 31;
+    db[a * LD8 + b] = nD[j].x;
+    db[a * LD8 + b + 1] = nD[j].y;
+  }
+#pragma unroll 1
+  for (int i = 0; i < NB; ++i) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (i + 1 < NB) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        nD[j] = *reinterpret_cast<const double2*>(Q + it.Q + (r0 + i + 1) * FBLK + 2 * l + 128 * j);
+    }
+    const long long qb = r0 + i;
+    const int r3 = (int)((qb >> 1) & 31), r4 = (int)(qb >> 6);
+    const double w34 = lam[it.L[2] + r3] * lam[it.L[3] + r4];
+    double cr[2][8], cc[2][8];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
-        dr[h][t] = D[(16 * h + li) * FC + 4 * t + lg];  // D[16h+li][4t+lg]
-        dc[h][t] = D[(4 * t + lg) * FC + 16 * h + li];  // D[4t+lg][16h+li]
+        cr[h][t] = db[(16 * h + li) * LD8 + 4 * t + lg];  // D[16h+li][4t+lg]
+        cc[h][t] = db[(4 * t + lg) * LD8 + 16 * h + li];  // D[4t+lg][16h+li]
       }
-  };
-  fetch(r0, cr, cc);
-#pragma unroll 1
-  for (int i = 0; i < NB; ++i) {
-    if (i + 1 < NB) fetch(r0 + i + 1, nr, nc);
-    const long long qb = r0 + i;
-    const int r3 = (int)((qb >> 1) & 31), r4 = (int)(qb >> 6);
-    const double w34 = lam[it.L[2] + r3] * lam[it.L[3] + r4];
 #pragma unroll
     for (int t = 0; t < 8; ++t)
 #pragma unroll
@@ -1649,14 +1665,19 @@ __global__ __launch_bounds__(256) void k_gram12_f64(const EigItem* __restrict__ 
           T1.t[mi][ni] = MFMA_F64(cr[mi][t], w34 * lam2v[t] * cr[ni][t], T1.t[mi][ni]);
           T2.t[mi][ni] = MFMA_F64(cc[mi][t], w34 * lam1v[t] * cc[ni][t], T2.t[mi][ni]);
         }
+    // every lane's reads of this block are issued before the tile is overwritten
+    __builtin_amdgcn_wave_barrier();
+    if (i + 1 < NB) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        cr[h][t] = nr[h][t];
-        cc[h][t] = nc[h][t];
+      for (int j = 0; j < 8; ++j) {
+        const int o = 2 * l + 128 * j, a = o >> 5, b = o & 31;
+        db[a * LD8 + b] = nD[j].x;
+        db[a * LD8 + b + 1] = nD[j].y;
       }
+    }
   }
+  __syncthreads();
+  double* Red = &Db[0][0];  // 4 * 32 * 33 >= 2 * 1024
   for (int w4 = 0; w4 < 4; ++w4) {
     if (wv == w4) {
 #pragma unroll
