@@ -75,10 +75,11 @@ __device__ T block_sum(T v, T* red) {
 template <typename TM, typename TA>
 __device__ void finalize_task(const OcTask& t, const TA* Tb, int P2, const TM* msg_in, TM* out,
                               MsgStat* stats, double* redd) {
-  TA n2 = 0;
-  for (int o = threadIdx.x; o < P2; o += blockDim.x) n2 += Tb[o] * Tb[o];
-  n2 = block_sum(n2, reinterpret_cast<TA*>(redd));
-  const double nrm = sqrt((double)n2);
+  // the squared norm in FP64 whatever TA: an FP32 square underflows below 1e-19
+  double n2 = 0;
+  for (int o = threadIdx.x; o < P2; o += blockDim.x) n2 += (double)Tb[o] * (double)Tb[o];
+  n2 = block_sum(n2, redd);
+  const double nrm = sqrt(n2);
   const double sc = (t.normalize && nrm != 0.0) ? 1.0 / nrm : 1.0;
   double dot = 0, o2 = 0, d2 = 0, bad = 0;
   for (int o = threadIdx.x; o < P2; o += blockDim.x) {

@@ -23,6 +23,7 @@
 // BF16, messages and intermediates are split into hi+lo BF16 terms
 // (hi*hi + hi*lo + lo*hi).  FP64 flavour: v_mfma_f64_16x16x4_f64 on LDS tiles.
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 #include <vector>
 
@@ -49,6 +50,7 @@ struct FastItem {
   long long out_z, out_w;
   int slot_z, slot_w;
   long long part;     // partial sums offset
+  int site;           // index of the site's power-of-two scale (BF16/FP16 layouts)
 };
 
 __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
@@ -61,22 +63,93 @@ __device__ inline int kperm(int t, int g, int p) { return 16 * t + 8 * (p >> 2) 
 
 #define MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16((a), (b), (c), 0, 0, 0)
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// 16-bit operand types of the 32x32x16 MFMAs (identical lane layouts): BF16
+// (8 exponent bits) and FP16 (11-bit significand, 8x finer rounding)
+template <typename T>
+struct Mma;
+template <>
+struct Mma<__bf16> {
+  typedef bf16x8 v8;
+  static __device__ inline f32x16 mfma(v8 a, v8 b, f32x16 c) { return MFMA_BF16(a, b, c); }
+};
+template <>
+struct Mma<_Float16> {
+  typedef f16x8 v8;
+  static __device__ inline f32x16 mfma(v8 a, v8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+__device__ inline void split16(float x, T& hi, T& lo) {
+  hi = (T)x;
+  lo = (T)(x - (float)hi);
+}
+
 // ---------------------------------------------------------------------------
 // layout build: natural column-major psi -> layouts A and B
 // ---------------------------------------------------------------------------
+// Per-site power-of-two scales of the 16-bit layouts.  Site j's largest |psi|
+// becomes m in [1/2, 1) after multiplying by 2^-e[j], so that the FP16
+// intermediates (largest normal 65504, subnormals below 6.1e-5) hold the
+// rotated blocks at every scale of psi; a Gram built from the scaled site is
+// 2^-2e[j] times the true one and the finalisers multiply by 2^2e[j].
+// k_site_absmax: grid (64, sites), partial maxima pmax[64 j + x];
+// k_site_exp: e[j] from the 64 partials (0 for an all-zero or non-finite site).
+template <typename TS>
+__global__ __launch_bounds__(256) void k_site_absmax(const TS* __restrict__ psi, const long long* __restrict__ src_off,
+                                                     long long nelem, double* __restrict__ pmax) {
+  __shared__ double red[4];
+  const TS* src = psi + src_off[blockIdx.y];
+  double m = 0;
+  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nelem; n += (long long)gridDim.x * blockDim.x) {
+    const double a = fabs(load_as<double>(src, n));
+    m = (a > m || a != a) ? a : m;  // a NaN sticks
+  }
+  for (int k = 32; k > 0; k >>= 1) {
+    const double o = __shfl_xor(m, k);
+    m = (o > m || o != o) ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = (red[w] > m || red[w] != red[w]) ? red[w] : m;
+    pmax[(long long)blockIdx.y * 64 + blockIdx.x] = m;
+  }
+}
+
+__global__ void k_site_exp(const double* __restrict__ pmax, int n, int* __restrict__ sexp) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double m = 0;
+  for (int x = 0; x < 64; ++x) {
+    const double a = pmax[(long long)j * 64 + x];
+    m = (a > m || a != a) ? a : m;
+  }
+  int e = 0;
+  if (m > 0 && isfinite(m)) frexp(m, &e);
+  sexp[j] = e;
+}
+
+__device__ inline double site_scale(const int* sexp, int j) { return sexp ? ldexp(1.0, -sexp[j]) : 1.0; }
+__device__ inline double site_unscale2(const int* sexp, int j) { return sexp ? ldexp(1.0, 2 * sexp[j]) : 1.0; }
+
 template <typename TS, typename TO>
 __global__ void k_fast_layout(const TS* __restrict__ psi, const long long* __restrict__ src_off,
                               const long long* __restrict__ dst_off, TO* __restrict__ outA,
-                              TO* __restrict__ outB) {
+                              TO* __restrict__ outB, const int* __restrict__ sexp) {
   const int it = blockIdx.y;
   const TS* src = psi + src_off[it];
   TO* A = outA + dst_off[it];
   TO* B = outB + dst_off[it];
+  const double sc = site_scale(sexp, it);
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC * FC;
        n += gridDim.x * blockDim.x) {
     const int s = n & 1;
     const int b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = (n >> 11) & 31, b4 = (n >> 16) & 31;
-    const TO val = (TO)load_as<float>(src, n);
+    const TO val = (TO)(float)(load_as<double>(src, n) * sc);
     A[(long long)(s + 2 * b1 + 64 * b2) * FBLK + b3 * FC + b4] = val;
     B[(long long)(s + 2 * b3 + 64 * b4) * FBLK + b1 * FC + b2] = val;
   }
@@ -330,16 +403,22 @@ __global__ __launch_bounds__(256) void k_fused_bf16(const FastItem* __restrict__
 // ---------------------------------------------------------------------------
 constexpr int ST_LD = 40;  // LDS row stride (bf16) of the staging/tiles: 80 bytes, 16-byte aligned
 
+// TO: type of P and of the second product's operands (BF16 for the
+// absorbed-pair passes, FP16 for rot12 of the eigenbasis pipeline); the first
+// product multiplies the BF16 site tensor and stays in BF16.
+template <typename TO>
 __global__ __launch_bounds__(512) void k_absorb_bf16c(const FastItem* __restrict__ items,
                                                       const __bf16* __restrict__ lay,
                                                       const float* __restrict__ msg,
-                                                      __bf16* __restrict__ P) {
-  __shared__ __attribute__((aligned(16))) __bf16 S[FBLK * ST_LD];
+                                                      TO* __restrict__ P) {
+  typedef typename Mma<TO>::v8 o8;
+  __shared__ __attribute__((aligned(16))) TO S[FBLK * ST_LD];
   const FastItem it = items[blockIdx.y];
   const int q = blockIdx.x;
   const int s = q & 1, z = q >> 1;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
-  bf16x8 bMbh[2], bMbl[2], aMah[2], aMal[2];
+  bf16x8 bMbh[2], bMbl[2];
+  o8 aMah[2], aMal[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -348,9 +427,10 @@ __global__ __launch_bounds__(512) void k_absorb_bf16c(const FastItem* __restrict
       split_bf16(msg[it.Mb + (16 * t + 8 * g + p) + FC * c], h, lo);
       bMbh[t][p] = h;
       bMbl[t][p] = lo;
-      split_bf16(msg[it.Ma + kperm(t, g, p) + FC * c], h, lo);
-      aMah[t][p] = h;
-      aMal[t][p] = lo;
+      TO h2, lo2;
+      split16(msg[it.Ma + kperm(t, g, p) + FC * c], h2, lo2);
+      aMah[t][p] = h2;
+      aMal[t][p] = lo2;
     }
   // the wave's four input blocks (8 KiB) are all requested before the first product
   bf16x8 aXs[4][2];
@@ -369,21 +449,21 @@ __global__ __launch_bounds__(512) void k_absorb_bf16c(const FastItem* __restrict
       V = MFMA_BF16(aXs[i][t], bMbh[t], V);
       V = MFMA_BF16(aXs[i][t], bMbl[t], V);
     }
-    bf16x8 bV[2];
+    o8 bV[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int p = 0; p < 8; ++p) bV[t][p] = (__bf16)V[8 * t + p];
+      for (int p = 0; p < 8; ++p) bV[t][p] = (TO)V[8 * t + p];
     f32x16 Y = {};
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      Y = MFMA_BF16(aMah[t], bV[t], Y);
-      Y = MFMA_BF16(aMal[t], bV[t], Y);
+      Y = Mma<TO>::mfma(aMah[t], bV[t], Y);
+      Y = Mma<TO>::mfma(aMal[t], bV[t], Y);
     }
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
-      S[(c + FC * xb) * ST_LD + w] = (__bf16)Y[v];  // row (y' + 32 x'), column w
+      S[(c + FC * xb) * ST_LD + w] = (TO)Y[v];  // row (y' + 32 x'), column w
     }
   }
   __syncthreads();
@@ -391,8 +471,8 @@ __global__ __launch_bounds__(512) void k_absorb_bf16c(const FastItem* __restrict
     const int o = k >> 2, part = k & 3;
     const int yb = o & 31, xb = o >> 5;
     const long long r = s + 2 * (xb + FC * yb);
-    const bf16x8 val = *reinterpret_cast<const bf16x8*>(S + o * ST_LD + 8 * part);
-    *reinterpret_cast<bf16x8*>(P + it.P + r * FBLK + z * FC + 8 * part) = val;
+    const o8 val = *reinterpret_cast<const o8*>(S + o * ST_LD + 8 * part);
+    *reinterpret_cast<o8*>(P + it.P + r * FBLK + z * FC + 8 * part) = val;
   }
 }
 
@@ -961,16 +1041,19 @@ template <typename TP, typename TM>
 __global__ __launch_bounds__(256) void k_fast_finalize(const FastItem* __restrict__ items,
                                                        const TP* __restrict__ part,
                                                        const TM* msg_in, TM* msg_out,
-                                                       MsgStat* __restrict__ stats, int normalize) {
+                                                       MsgStat* __restrict__ stats, int normalize,
+                                                       const int* __restrict__ sexp) {
   __shared__ double T[FBLK];
   __shared__ double red[8];
   const FastItem it = items[blockIdx.y];
   const int mat = blockIdx.x;
   const TP* pp = part + it.part + (long long)mat * FBLK;
+  const double us = site_unscale2(sexp, it.site);
   double n2 = 0;
   for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
     double sum = 0;
     for (int ch = 0; ch < FUSED_CHUNKS; ++ch) sum += (double)pp[(long long)ch * 2 * FBLK + o];
+    sum *= us;
     const int x = o >> 5, y = o & 31;
     T[x + FC * y] = sum;  // message layout M[b_ket + chi b_bra]
     n2 += sum * sum;
@@ -1045,6 +1128,7 @@ struct EigItem {
   long long L[4];      // its eigenvalues
   long long out[4];    // outgoing message offsets
   int slot[4];
+  int site;            // index of the site's power-of-two scale (BF16/FP16 layouts)
   long long part;      // partial Grams [4 bonds][EIG_NPART][32 x 32]
 };
 
@@ -1227,13 +1311,14 @@ __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ 
 // With GRAM = false it is rot12: the same block geometry (input block
 // s + 2 (z + 32 w), output block s + 2 (x' + 32 y') at [z][w]) with U1, U2 in the
 // U[2], U[3] slots of its items, psi layout B as R and the R buffer as Q.
-template <bool GRAM>
+template <bool GRAM, typename TI>
 __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ items,
-                                                    const __bf16* __restrict__ R,
+                                                    const TI* __restrict__ R,
                                                     const float* __restrict__ U,
                                                     const float* __restrict__ lam,
-                                                    __bf16* __restrict__ Q, float* __restrict__ part) {
-  __shared__ __attribute__((aligned(16))) __bf16 S[FBLK * ST_LD];
+                                                    TI* __restrict__ Q, float* __restrict__ part) {
+  typedef typename Mma<TI>::v8 v8;
+  __shared__ __attribute__((aligned(16))) TI S[FBLK * ST_LD];
   __shared__ __attribute__((aligned(16))) float lam34[2][FC];  // l3, l4
   const EigItem it = items[blockIdx.y];
   const int q = blockIdx.x;
@@ -1241,19 +1326,19 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   if (GRAM && threadIdx.x < 2 * FC)
     lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[((threadIdx.x >> 5) ? it.L[3] : it.L[2]) + (threadIdx.x & 31)];
-  bf16x8 aU3h[2], aU3l[2], bU4h[2], bU4l[2];
+  v8 aU3h[2], aU3l[2], bU4h[2], bU4l[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      __bf16 h, lo;
+      TI h, lo;
       // U3 as A operand of U3^T V (lane r3, K = b3 in accumulator-row order) and as
       // B operand of V^T U3 (lane r3, same K order): the same values
-      split_bf16(U[it.U[2] + kperm(t, g, p) + FC * c], h, lo);
+      split16(U[it.U[2] + kperm(t, g, p) + FC * c], h, lo);
       aU3h[t][p] = h;
       aU3l[t][p] = lo;
       // U4 as B operand of Z U4 (lane r4, K = b4 natural)
-      split_bf16(U[it.U[3] + (16 * t + 8 * g + p) + FC * c], h, lo);
+      split16(U[it.U[3] + (16 * t + 8 * g + p) + FC * c], h, lo);
       bU4h[t][p] = h;
       bU4l[t][p] = lo;
     }
@@ -1261,12 +1346,12 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
   f32x16 T3 = {}, T4 = {};
   // the wave's four R blocks of the current and the next r1 (plain unrolled
   // code, no array references, so that the blocks stay in registers)
-  bf16x8 cur[4][2], nxt[4][2];
+  v8 cur[4][2], nxt[4][2];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const __bf16* Z = R + it.R + (long long)(s + 2 * (z0 + FC * (4 * wv + i))) * FBLK;
+    const TI* Z = R + it.R + (long long)(s + 2 * (z0 + FC * (4 * wv + i))) * FBLK;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) cur[i][t] = *reinterpret_cast<const bf16x8*>(Z + c * FC + 16 * t + 8 * g);
+    for (int t = 0; t < 2; ++t) cur[i][t] = *reinterpret_cast<const v8*>(Z + c * FC + 16 * t + 8 * g);
   }
 #pragma unroll 1
   for (int zi = 0; zi < ROT_NZ; ++zi) {
@@ -1274,9 +1359,9 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
     if (zi + 1 < ROT_NZ) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const __bf16* Z = R + it.R + (long long)(s + 2 * (z + 1 + FC * (4 * wv + i))) * FBLK;
+        const TI* Z = R + it.R + (long long)(s + 2 * (z + 1 + FC * (4 * wv + i))) * FBLK;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) nxt[i][t] = *reinterpret_cast<const bf16x8*>(Z + c * FC + 16 * t + 8 * g);
+        for (int t = 0; t < 2; ++t) nxt[i][t] = *reinterpret_cast<const v8*>(Z + c * FC + 16 * t + 8 * g);
       }
     }
     const float l1 = GRAM ? lam[it.L[0] + z] : 0.f;
@@ -1287,27 +1372,27 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
       f32x16 V = {};
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        V = MFMA_BF16(cur[i][t], bU4h[t], V);
-        V = MFMA_BF16(cur[i][t], bU4l[t], V);
+        V = Mma<TI>::mfma(cur[i][t], bU4h[t], V);
+        V = Mma<TI>::mfma(cur[i][t], bU4l[t], V);
       }
-      bf16x8 bV[2];
+      v8 bV[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int p = 0; p < 8; ++p) bV[t][p] = (__bf16)V[8 * t + p];
+        for (int p = 0; p < 8; ++p) bV[t][p] = (TI)V[8 * t + p];
       f32x16 Y = {}, YT = {};
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        Y = MFMA_BF16(aU3h[t], bV[t], Y);
-        Y = MFMA_BF16(aU3l[t], bV[t], Y);
+        Y = Mma<TI>::mfma(aU3h[t], bV[t], Y);
+        Y = Mma<TI>::mfma(aU3l[t], bV[t], Y);
         if (GRAM) {
-          YT = MFMA_BF16(bV[t], aU3h[t], YT);
-          YT = MFMA_BF16(bV[t], aU3l[t], YT);
+          YT = Mma<TI>::mfma(bV[t], aU3h[t], YT);
+          YT = Mma<TI>::mfma(bV[t], aU3l[t], YT);
         }
       }
 #pragma unroll
       for (int t = 0; t < 2 && GRAM; ++t) {
-        bf16x8 a3, b3, a4, b4;
+        v8 a3, b3, a4, b4;
         // eigenvalue weights of accumulator rows kperm(t, g, 0..7): two runs of 4
         const float4 l3a = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 4 * g]);
         const float4 l3b = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 8 + 4 * g]);
@@ -1318,18 +1403,18 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
           const float yt = YT[8 * t + p], y = Y[8 * t + p];
-          a3[p] = (__bf16)yt;
-          b3[p] = (__bf16)(w12 * l4[p] * yt);
-          a4[p] = (__bf16)y;
-          b4[p] = (__bf16)(w12 * l3[p] * y);
+          a3[p] = (TI)yt;
+          b3[p] = (TI)(w12 * l4[p] * yt);
+          a4[p] = (TI)y;
+          b4[p] = (TI)(w12 * l3[p] * y);
         }
-        T3 = MFMA_BF16(a3, b3, T3);
-        T4 = MFMA_BF16(a4, b4, T4);
+        T3 = Mma<TI>::mfma(a3, b3, T3);
+        T4 = Mma<TI>::mfma(a4, b4, T4);
       }
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
-        S[(c + FC * xb) * ST_LD + w] = (__bf16)Y[v];  // row (r4 + 32 r3), column r2
+        S[(c + FC * xb) * ST_LD + w] = (TI)Y[v];  // row (r4 + 32 r3), column r2
       }
       __builtin_amdgcn_sched_barrier(0);  // keep the four blocks' chains apart (register pressure)
     }
@@ -1338,8 +1423,8 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
       const int o = k >> 2, pt = k & 3;
       const int yb = o & 31, xb = o >> 5;
       const long long qb = s + 2 * (xb + FC * yb);
-      *reinterpret_cast<bf16x8*>(Q + it.Q + qb * FBLK + z * FC + 8 * pt) =
-          *reinterpret_cast<const bf16x8*>(S + o * ST_LD + 8 * pt);
+      *reinterpret_cast<v8*>(Q + it.Q + qb * FBLK + z * FC + 8 * pt) =
+          *reinterpret_cast<const v8*>(S + o * ST_LD + 8 * pt);
     }
     __syncthreads();
 #pragma unroll
@@ -1379,35 +1464,36 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
 // flight while the current 4 are multiplied.  D^T comes from D I on the MFMA
 // (exact).  Block qb has weight l3[(qb >> 1) & 31] l4[qb >> 6].
 // grid (EIG_NPART, items)
-template <int NB>
+template <int NB, typename TI>
 __global__ __launch_bounds__(256) void k_gram12_bf16(const EigItem* __restrict__ items,
-                                                     const __bf16* __restrict__ Q,
+                                                     const TI* __restrict__ Q,
                                                      const float* __restrict__ lam,
                                                      float* __restrict__ part) {
+  typedef typename Mma<TI>::v8 v8;
   __shared__ float Red[2 * FBLK];
   const EigItem it = items[blockIdx.y];
   const int q = blockIdx.x;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   float lam1v[16], lam2v[16];
-  bf16x8 bI[2];
+  v8 bI[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
       lam1v[8 * t + p] = lam[it.L[0] + kperm(t, g, p)];   // K = r1 in accumulator-row order
       lam2v[8 * t + p] = lam[it.L[1] + 16 * t + 8 * g + p];  // K = r2 natural
-      bI[t][p] = (__bf16)((16 * t + 8 * g + p) == c ? 1.0f : 0.0f);
+      bI[t][p] = (TI)((16 * t + 8 * g + p) == c ? 1.0f : 0.0f);
     }
   f32x16 T1 = {}, T2 = {};
   constexpr int PF = NB < 4 ? NB : 4;
   const long long r0 = (long long)q * (4 * NB) + wv * NB;
-  bf16x8 cur[PF][2], nxt[PF][2];
-  auto fetch = [&](long long r, bf16x8 (&dst)[PF][2]) {
+  v8 cur[PF][2], nxt[PF][2];
+  auto fetch = [&](long long r, v8 (&dst)[PF][2]) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
-      const __bf16* D = Q + it.Q + (r + i) * FBLK;
+      const TI* D = Q + it.Q + (r + i) * FBLK;
 #pragma unroll
-      for (int t = 0; t < 2; ++t) dst[i][t] = *reinterpret_cast<const bf16x8*>(D + c * FC + 16 * t + 8 * g);
+      for (int t = 0; t < 2; ++t) dst[i][t] = *reinterpret_cast<const v8*>(D + c * FC + 16 * t + 8 * g);
     }
   };
   fetch(r0, cur);
@@ -1422,22 +1508,22 @@ __global__ __launch_bounds__(256) void k_gram12_bf16(const EigItem* __restrict__
       f32x16 Dt = {};
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        bf16x8 b1;
+        v8 b1;
 #pragma unroll
-        for (int p = 0; p < 8; ++p) b1[p] = (__bf16)(w34 * lam2v[8 * t + p] * (float)cur[i][t][p]);
-        T1 = MFMA_BF16(cur[i][t], b1, T1);
-        Dt = MFMA_BF16(cur[i][t], bI[t], Dt);
+        for (int p = 0; p < 8; ++p) b1[p] = (TI)(w34 * lam2v[8 * t + p] * (float)cur[i][t][p]);
+        T1 = Mma<TI>::mfma(cur[i][t], b1, T1);
+        Dt = Mma<TI>::mfma(cur[i][t], bI[t], Dt);
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        bf16x8 a2, b2;
+        v8 a2, b2;
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
           const float d = Dt[8 * t + p];
-          a2[p] = (__bf16)d;
-          b2[p] = (__bf16)(w34 * lam1v[8 * t + p] * d);
+          a2[p] = (TI)d;
+          b2[p] = (TI)(w34 * lam1v[8 * t + p] * d);
         }
-        T2 = MFMA_BF16(a2, b2, T2);
+        T2 = Mma<TI>::mfma(a2, b2, T2);
       }
     }
 #pragma unroll
@@ -1717,34 +1803,42 @@ __global__ __launch_bounds__(256) void k_gram12_f64(const EigItem* __restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int D3_NBLK = 2 * FC;  // blocks per degree-3 site
 
-template <typename TS, typename TO, typename TA>
+template <typename TS, typename TO>
 __global__ void k_fast_layout3(const TS* __restrict__ psi, const long long* __restrict__ src_off,
-                               const long long* __restrict__ dst_off, TO* __restrict__ out) {
+                               const long long* __restrict__ dst_off, TO* __restrict__ out,
+                               const int* __restrict__ sexp) {
   const int it = blockIdx.y;
   const TS* src = psi + src_off[it];
   TO* C = out + dst_off[it];
+  const double sc = site_scale(sexp, it);
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC; n += gridDim.x * blockDim.x) {
     const int s = n & 1, b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = n >> 11;
-    C[(long long)(s + 2 * b3) * FBLK + b1 * FC + b2] = (TO)load_as<TA>(src, n);
+    const double v = load_as<double>(src, n) * sc;
+    if constexpr (std::is_same<TO, double>::value) C[(long long)(s + 2 * b3) * FBLK + b1 * FC + b2] = v;
+    else C[(long long)(s + 2 * b3) * FBLK + b1 * FC + b2] = (TO)(float)v;
   }
 }
 
 // grid (4, items), 256 threads: wave -> blocks 16 x + 4 wv + {0..3}
+template <typename TI>
 __global__ __launch_bounds__(256) void k_rot12_d3_bf16(const EigItem* __restrict__ items,
                                                        const __bf16* __restrict__ lay,
                                                        const float* __restrict__ U,
-                                                       __bf16* __restrict__ R) {
+                                                       TI* __restrict__ R) {
+  typedef typename Mma<TI>::v8 v8;
   const EigItem it = items[blockIdx.y];
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
-  bf16x8 aU1h[2], aU1l[2], bU2h[2], bU2l[2];
+  v8 aU1h[2], aU1l[2];
+  bf16x8 bU2h[2], bU2l[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
+      TI h1, l1;
+      split16(U[it.U[0] + kperm(t, g, p) + FC * c], h1, l1);  // U1^T, K = b1 (accumulator rows)
+      aU1h[t][p] = h1;
+      aU1l[t][p] = l1;
       __bf16 h, lo;
-      split_bf16(U[it.U[0] + kperm(t, g, p) + FC * c], h, lo);  // U1^T, K = b1 (accumulator rows)
-      aU1h[t][p] = h;
-      aU1l[t][p] = lo;
       split_bf16(U[it.U[1] + (16 * t + 8 * g + p) + FC * c], h, lo);  // U2, K = b2
       bU2h[t][p] = h;
       bU2l[t][p] = lo;
@@ -1764,41 +1858,43 @@ __global__ __launch_bounds__(256) void k_rot12_d3_bf16(const EigItem* __restrict
       V = MFMA_BF16(aX[i][t], bU2h[t], V);
       V = MFMA_BF16(aX[i][t], bU2l[t], V);
     }
-    bf16x8 bV[2];
+    v8 bV[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int p = 0; p < 8; ++p) bV[t][p] = (__bf16)V[8 * t + p];
+      for (int p = 0; p < 8; ++p) bV[t][p] = (TI)V[8 * t + p];
     f32x16 Y = {};
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      Y = MFMA_BF16(aU1h[t], bV[t], Y);
-      Y = MFMA_BF16(aU1l[t], bV[t], Y);
+      Y = Mma<TI>::mfma(aU1h[t], bV[t], Y);
+      Y = Mma<TI>::mfma(aU1l[t], bV[t], Y);
     }
-    __bf16* dst = R + it.R + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
+    TI* dst = R + it.R + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) dst[(8 * (v >> 2) + 4 * g + (v & 3)) * FC + c] = (__bf16)Y[v];
+    for (int v = 0; v < 16; ++v) dst[(8 * (v >> 2) + 4 * g + (v & 3)) * FC + c] = (TI)Y[v];
   }
 }
 
 // grid (EIG_NPART, items), 256 threads: wave -> tile (s, r1) = (x & 1, x >> 1), x = 4 q + wv
+template <typename TI>
 __global__ __launch_bounds__(256) void k_rot3_d3_bf16(const EigItem* __restrict__ items,
-                                                      const __bf16* __restrict__ R,
+                                                      const TI* __restrict__ R,
                                                       const float* __restrict__ U,
                                                       const float* __restrict__ lam,
-                                                      __bf16* __restrict__ Q, float* __restrict__ part) {
+                                                      TI* __restrict__ Q, float* __restrict__ part) {
+  typedef typename Mma<TI>::v8 v8;
   __shared__ float Red[FBLK];
   const EigItem it = items[blockIdx.y];
   const int q = blockIdx.x;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   const int x = 4 * q + wv, s = x & 1, r1 = x >> 1;
-  bf16x8 aU3h[2], aU3l[2], bZ[2];
+  v8 aU3h[2], aU3l[2], bZ[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      __bf16 h, lo;
-      split_bf16(U[it.U[2] + (16 * t + 8 * g + p) + FC * c], h, lo);  // U3^T, K = b3 natural
+      TI h, lo;
+      split16(U[it.U[2] + (16 * t + 8 * g + p) + FC * c], h, lo);  // U3^T, K = b3 natural
       aU3h[t][p] = h;
       aU3l[t][p] = lo;
       // Z as B operand (lane r2, K = b3 natural)
@@ -1807,28 +1903,28 @@ __global__ __launch_bounds__(256) void k_rot3_d3_bf16(const EigItem* __restrict_
   f32x16 Y = {}, YT = {};
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    Y = MFMA_BF16(aU3h[t], bZ[t], Y);
-    Y = MFMA_BF16(aU3l[t], bZ[t], Y);
-    YT = MFMA_BF16(bZ[t], aU3h[t], YT);
-    YT = MFMA_BF16(bZ[t], aU3l[t], YT);
+    Y = Mma<TI>::mfma(aU3h[t], bZ[t], Y);
+    Y = Mma<TI>::mfma(aU3l[t], bZ[t], Y);
+    YT = Mma<TI>::mfma(bZ[t], aU3h[t], YT);
+    YT = Mma<TI>::mfma(bZ[t], aU3l[t], YT);
   }
   const float l1 = lam[it.L[0] + r1];
   f32x16 T3 = {};
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    bf16x8 a, b;
+    v8 a, b;
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
       const float yt = YT[8 * t + p];
-      a[p] = (__bf16)yt;
-      b[p] = (__bf16)(l1 * lam[it.L[1] + kperm(t, g, p)] * yt);
+      a[p] = (TI)yt;
+      b[p] = (TI)(l1 * lam[it.L[1] + kperm(t, g, p)] * yt);
     }
-    T3 = MFMA_BF16(a, b, T3);
+    T3 = Mma<TI>::mfma(a, b, T3);
   }
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
     const int r3 = 8 * (v >> 2) + 4 * g + (v & 3);
-    Q[it.Q + (long long)(s + 2 * r3) * FBLK + r1 * FC + c] = (__bf16)Y[v];
+    Q[it.Q + (long long)(s + 2 * r3) * FBLK + r1 * FC + c] = (TI)Y[v];
   }
   for (int w4 = 0; w4 < 4; ++w4) {
     if (wv == w4) {
@@ -1970,7 +2066,7 @@ __global__ __launch_bounds__(256) void k_eig_finalize(const EigItem* __restrict_
                                                       const TP* __restrict__ part,
                                                       const double* __restrict__ U64, const TM* msg_in,
                                                       TM* msg_out, MsgStat* __restrict__ stats,
-                                                      int normalize) {
+                                                      int normalize, const int* __restrict__ sexp) {
   constexpr int LD = 33;
   __shared__ double Tp[FC * LD], Us[FC * LD], X[FC * LD];
   __shared__ double red[8];
@@ -2000,10 +2096,11 @@ __global__ __launch_bounds__(256) void k_eig_finalize(const EigItem* __restrict_
     Tp[b * LD + bb] = acc;  // T[b][b']
   }
   __syncthreads();
+  const double us = 0.5 * site_unscale2(sexp, itp->site);
   double n2 = 0;
   for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
     const int i = o & 31, j = o >> 5;  // message layout M[i + 32 j]
-    const double tv = 0.5 * (Tp[i * LD + j] + Tp[j * LD + i]);
+    const double tv = us * (Tp[i * LD + j] + Tp[j * LD + i]);
     X[i * LD + j] = tv;
     n2 += tv * tv;
   }
@@ -2072,6 +2169,9 @@ struct FastState {
   std::vector<EigItem> e3items;
   DevBuf d_e3items, lay3_16, lay3_64, P3_16, P3_64, offs3;
   bool lay3_16_ok = false, lay3_64_ok = false;
+  // power-of-two exponents of the 16-bit layouts: bulk site bi at bi, degree-3
+  // site k (index into deg3) at bulk.size() + k
+  DevBuf sexp, pmax;
 };
 
 FastState* state_of(itnbp_net* net) { return reinterpret_cast<FastState*>(net->fast); }
@@ -2162,6 +2262,8 @@ void release_fast_state(itnbp_net* net) {
   free_buf(st->P3_16);
   free_buf(st->P3_64);
   free_buf(st->offs3);
+  free_buf(st->sexp);
+  free_buf(st->pmax);
   delete st;
   net->fast = nullptr;
 }
@@ -2173,6 +2275,26 @@ void invalidate_fast_state(itnbp_net* net) {
   st->lay64_ok = false;
   st->lay3_16_ok = false;
   st->lay3_64_ok = false;
+}
+
+// Exponents of sites first .. first + n - 1 (psi offsets src, nelem elements each).
+int site_exponents(itnbp_net* net, FastState* st, const long long* src, int n, long long nelem, int first) {
+  itnbp_ctx* ctx = net->ctx;
+  const size_t ns = st->bulk.size() + st->deg3.size();
+  ITNBP_RETCHK(ensure_buf(ctx, st->sexp, ns * sizeof(int)));
+  ITNBP_RETCHK(ensure_buf(ctx, st->pmax, ns * 64 * sizeof(double)));
+  double* pm = (double*)st->pmax.p + (size_t)first * 64;
+  dim3 g(64, n);
+  switch (net->dtype) {
+    case ITNBP_F64: hipLaunchKernelGGL(k_site_absmax<double>, g, dim3(256), 0, ctx->stream, (const double*)net->d_psi, src, nelem, pm); break;
+    case ITNBP_F32: hipLaunchKernelGGL(k_site_absmax<float>, g, dim3(256), 0, ctx->stream, (const float*)net->d_psi, src, nelem, pm); break;
+    default: hipLaunchKernelGGL(k_site_absmax<bf16_t>, g, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, src, nelem, pm); break;
+  }
+  ITNBP_HIPCHK(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_site_exp, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, (const double*)pm, n,
+                     (int*)st->sexp.p + first);
+  ITNBP_HIPCHK(ctx, hipGetLastError());
+  return ITNBP_OK;
 }
 
 // Mark the sequence entries the pipeline produces and prepare its buffers.
@@ -2227,6 +2349,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     b.slot_z = pos[o1];
     b.slot_w = pos[o2];
     b.part = (long long)(2 * bi + 1) * FUSED_CHUNKS * 2 * FBLK;
+    a.site = b.site = bi;
     st->items.push_back(a);
     st->items.push_back(b);
     for (int o : {o1, o2, o3, o4}) handled[pos[o]] = 1;
@@ -2262,6 +2385,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
         st->eig_moff.push_back(V.in_msg[c + 1]);
       }
       e.part = (long long)bi * 4 * EIG_NPART * FBLK;
+      e.site = bi;
       st->eitems.push_back(e);
       FastItem k{};
       k.psi_in = (long long)(nb + bi) * per;  // layout B
@@ -2292,14 +2416,16 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     st->deg3_used.clear();
     st->e3items.clear();
     const long long per3 = (long long)D3_NBLK * FBLK;
-    for (int v : st->deg3) {
+    for (size_t k = 0; k < st->deg3.size(); ++k) {
+      const int v = st->deg3[k];
       const VDesc& V = net->vdesc[v];
       const int o1 = out_msg_id(net, v, 1), o2 = out_msg_id(net, v, 2), o3 = out_msg_id(net, v, 3);
       if (pos[o1] < 0 || pos[o2] < 0 || pos[o3] < 0) continue;
       const int j = (int)st->deg3_used.size();
       st->deg3_used.push_back(v);
       EigItem e{};
-      e.X = (long long)j * per3;
+      e.X = (long long)k * per3;  // layout C holds every degree-3 site
+      e.site = nb + (int)k;
       e.R = (long long)j * per3;
       e.Q = (long long)(st->deg3.size() + j) * per3;
       const int oo[3] = {o1, o2, o3};
@@ -2341,41 +2467,44 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
       ITNBP_HIPCHK(ctx, hipMemcpyAsync((float*)st->eigL32.p + ne4 * FC, ones32, sizeof(ones32),
                                        hipMemcpyHostToDevice, ctx->stream));
     }
-    // layout C of the degree-3 sites and their block buffers
+    // layout C of every degree-3 site and the block buffers of those in use
+    const int n3all = (int)st->deg3.size();
     if (n3 > 0) {
-      std::vector<long long> o3(2 * n3);
-      for (int j = 0; j < n3; ++j) {
-        o3[j] = net->psi_off[st->deg3_used[j]];
-        o3[n3 + j] = (long long)j * per3;
+      std::vector<long long> o3(2 * n3all);
+      for (int k = 0; k < n3all; ++k) {
+        o3[k] = net->psi_off[st->deg3[k]];
+        o3[n3all + k] = (long long)k * per3;
       }
       ITNBP_RETCHK(ensure_buf(ctx, st->offs3, o3.size() * 8));
       ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->offs3.p, o3.data(), o3.size() * 8, hipMemcpyHostToDevice, ctx->stream));
       ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
       const long long* s3 = (const long long*)st->offs3.p;
-      const long long* d3 = s3 + n3;
-      dim3 g3(64, n3);
+      const long long* d3 = s3 + n3all;
+      dim3 g3(64, n3all);
       if (!acc_f64) {
-        ITNBP_RETCHK(ensure_buf(ctx, st->lay3_16, (size_t)st->deg3.size() * per3 * 2));
-        ITNBP_RETCHK(ensure_buf(ctx, st->P3_16, (size_t)2 * st->deg3.size() * per3 * 2));
+        ITNBP_RETCHK(ensure_buf(ctx, st->lay3_16, (size_t)n3all * per3 * 2));
+        ITNBP_RETCHK(ensure_buf(ctx, st->P3_16, (size_t)2 * n3all * per3 * 2));
         if (!st->lay3_16_ok) {
+          ITNBP_RETCHK(site_exponents(net, st, s3, n3all, 2LL * FC * FC * FC, nb));
           __bf16* C = (__bf16*)st->lay3_16.p;
+          const int* e3 = (const int*)st->sexp.p + nb;
           switch (net->dtype) {
-            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, __bf16, float>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C); break;
-            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, __bf16, float>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C); break;
-            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, __bf16, float>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C); break;
+            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, __bf16>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C, e3); break;
+            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, __bf16>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C, e3); break;
+            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, __bf16>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C, e3); break;
           }
           ITNBP_HIPCHK(ctx, hipGetLastError());
           st->lay3_16_ok = true;
         }
       } else {
-        ITNBP_RETCHK(ensure_buf(ctx, st->lay3_64, (size_t)st->deg3.size() * per3 * 8));
-        ITNBP_RETCHK(ensure_buf(ctx, st->P3_64, (size_t)2 * st->deg3.size() * per3 * 8));
+        ITNBP_RETCHK(ensure_buf(ctx, st->lay3_64, (size_t)n3all * per3 * 8));
+        ITNBP_RETCHK(ensure_buf(ctx, st->P3_64, (size_t)2 * n3all * per3 * 8));
         if (!st->lay3_64_ok) {
           double* C = (double*)st->lay3_64.p;
           switch (net->dtype) {
-            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, double, double>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C); break;
-            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, double, double>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C); break;
-            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, double, double>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C); break;
+            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, double>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C, nullptr); break;
+            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, double>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C, nullptr); break;
+            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, double>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C, nullptr); break;
           }
           ITNBP_HIPCHK(ctx, hipGetLastError());
           st->lay3_64_ok = true;
@@ -2397,12 +2526,14 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   dim3 lgrid(256, nb);
   if (!acc_f64 && !st->lay16_ok && nb > 0) {
     ITNBP_RETCHK(ensure_buf(ctx, st->lay16, (size_t)2 * nb * per * 2));
+    ITNBP_RETCHK(site_exponents(net, st, d_src, nb, per, 0));
     __bf16* A = (__bf16*)st->lay16.p;
     __bf16* B = A + nb * per;
+    const int* e = (const int*)st->sexp.p;
     switch (net->dtype) {
-      case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout<double, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B); break;
-      case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout<float, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B); break;
-      default: hipLaunchKernelGGL((k_fast_layout<bf16_t, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B); break;
+      case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout<double, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B, e); break;
+      case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout<float, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B, e); break;
+      default: hipLaunchKernelGGL((k_fast_layout<bf16_t, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B, e); break;
     }
     ITNBP_HIPCHK(ctx, hipGetLastError());
     st->lay16_ok = true;
@@ -2446,7 +2577,7 @@ static void launch_pair(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* 
       hipLaunchKernelGGL((k_absorb_bf16<2>), dim3(128, ni), dim3(256), 0, ctx->stream, items,
                          (const __bf16*)st->lay16.p, (const float*)msg_in, (float*)st->P32.p);
     } else {
-      hipLaunchKernelGGL(k_absorb_bf16c, dim3(64, ni), dim3(512), 0, ctx->stream, items,
+      hipLaunchKernelGGL(k_absorb_bf16c<__bf16>, dim3(64, ni), dim3(512), 0, ctx->stream, items,
                          (const __bf16*)st->lay16.p, (const float*)msg_in, (__bf16*)st->P16.p);
     }
   } else if (fast_f64_lds()) {
@@ -2540,6 +2671,48 @@ int fast_chunk(itnbp_net* net, int acc_f64) {
   return st ? st->chunk[acc_f64 ? 1 : 0] : 0;
 }
 
+// Intermediates of the 16-bit eigenbasis pipeline: FP16 (default; 8x finer
+// rounding than BF16, range kept by the per-site scales) or BF16
+// (ITNBP_EIG_BF16=1)
+static bool eig_bf16_intermediates() {
+  const char* e = getenv("ITNBP_EIG_BF16");
+  return e && e[0] == '1';
+}
+
+// stage 1 rot12, 2 rot34 (+ T'3, T'4), 3 gram12 (T'1, T'2), 4 degree-3 sites
+template <typename TI>
+static void eig16_stage(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3) {
+  const EigItem* ei = (const EigItem*)st->d_eitems.p;
+  const EigItem* e3 = (const EigItem*)st->d_e3items.p;
+  const float* U = (const float*)st->eigU32.p;
+  const float* L = (const float*)st->eigL32.p;
+  TI* P = (TI*)st->P16.p;
+  TI* P3 = (TI*)st->P3_16.p;
+  float* part = (float*)st->part32.p;
+  if (stage == 1 && nb > 0)
+    hipLaunchKernelGGL(k_absorb_bf16c<TI>, dim3(64, nb), dim3(512), 0, ctx->stream,
+                       (const FastItem*)st->d_k1items.p, (const __bf16*)st->lay16.p, U, P);
+  if (stage == 2 && nb > 0)
+    hipLaunchKernelGGL((k_rot34_bf16<true, TI>), dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei,
+                       (const TI*)P, U, L, P, part);
+  if (stage == 3 && nb > 0)
+    hipLaunchKernelGGL((k_gram12_bf16<32, TI>), dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei, (const TI*)P,
+                       L, part);
+  if (stage == 4 && n3 > 0) {
+    hipLaunchKernelGGL(k_rot12_d3_bf16<TI>, dim3(4, n3), dim3(256), 0, ctx->stream, e3,
+                       (const __bf16*)st->lay3_16.p, U, P3);
+    hipLaunchKernelGGL(k_rot3_d3_bf16<TI>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const TI*)P3, U, L,
+                       P3, part);
+    hipLaunchKernelGGL((k_gram12_bf16<1, TI>), dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const TI*)P3,
+                       L, part);
+  }
+}
+
+static void eig16(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3) {
+  if (eig_bf16_intermediates()) eig16_stage<__bf16>(ctx, st, stage, nb, n3);
+  else eig16_stage<_Float16>(ctx, st, stage, nb, n3);
+}
+
 // one sweep of the eigenbasis pipeline over every bulk and degree-3 site
 static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg_in, void* msg_out,
                      MsgStat* stats, int normalize) {
@@ -2575,9 +2748,7 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
                          (const double*)st->lay64.p, (const double*)st->eigU64.p, (const double*)st->eigL64.p,
                          (double*)st->P64.p, (double*)nullptr);
     } else {
-      const FastItem* k1 = (const FastItem*)st->d_k1items.p;
-      hipLaunchKernelGGL(k_absorb_bf16c, dim3(64, nb), dim3(512), 0, ctx->stream, k1, (const __bf16*)st->lay16.p,
-                         (const float*)st->eigU32.p, (__bf16*)st->P16.p);
+      eig16(ctx, st, 1, nb, n3);
     }
   }
   const int e2 = prof_mark(ctx);
@@ -2587,9 +2758,7 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
                          (const double*)st->eigU64.p, (const double*)st->eigL64.p, (double*)st->P64.p,
                          (double*)st->part64.p);
     else
-      hipLaunchKernelGGL(k_rot34_bf16<true>, dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei, (const __bf16*)st->P16.p,
-                         (const float*)st->eigU32.p, (const float*)st->eigL32.p, (__bf16*)st->P16.p,
-                         (float*)st->part32.p);
+      eig16(ctx, st, 2, nb, n3);
   }
   const int e3m = prof_mark(ctx);
   if (nb > 0) {
@@ -2597,8 +2766,7 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
       hipLaunchKernelGGL(k_gram12_f64<32>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei,
                          (const double*)st->P64.p, (const double*)st->eigL64.p, (double*)st->part64.p);
     else
-      hipLaunchKernelGGL(k_gram12_bf16<32>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei,
-                         (const __bf16*)st->P16.p, (const float*)st->eigL32.p, (float*)st->part32.p);
+      eig16(ctx, st, 3, nb, n3);
   }
   const int e4 = prof_mark(ctx);
   if (n3 > 0) {
@@ -2611,13 +2779,7 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
       hipLaunchKernelGGL(k_gram12_f64<1>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3,
                          (const double*)st->P3_64.p, (const double*)st->eigL64.p, (double*)st->part64.p);
     } else {
-      hipLaunchKernelGGL(k_rot12_d3_bf16, dim3(4, n3), dim3(256), 0, ctx->stream, e3, (const __bf16*)st->lay3_16.p,
-                         (const float*)st->eigU32.p, (__bf16*)st->P3_16.p);
-      hipLaunchKernelGGL(k_rot3_d3_bf16, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3,
-                         (const __bf16*)st->P3_16.p, (const float*)st->eigU32.p, (const float*)st->eigL32.p,
-                         (__bf16*)st->P3_16.p, (float*)st->part32.p);
-      hipLaunchKernelGGL(k_gram12_bf16<1>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3,
-                         (const __bf16*)st->P3_16.p, (const float*)st->eigL32.p, (float*)st->part32.p);
+      eig16(ctx, st, 4, nb, n3);
     }
   }
   const int e5 = prof_mark(ctx);
@@ -2625,20 +2787,20 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
     if (nb > 0)
       hipLaunchKernelGGL((k_eig_finalize<double, double>), dim3(4, nb), dim3(256), 0, ctx->stream, ei,
                          (const double*)st->part64.p, (const double*)st->eigU64.p, (const double*)msg_in,
-                         (double*)msg_out, stats, normalize);
+                         (double*)msg_out, stats, normalize, (const int*)nullptr);
     if (n3 > 0)
       hipLaunchKernelGGL((k_eig_finalize<double, double>), dim3(3, n3), dim3(256), 0, ctx->stream, e3,
                          (const double*)st->part64.p, (const double*)st->eigU64.p, (const double*)msg_in,
-                         (double*)msg_out, stats, normalize);
+                         (double*)msg_out, stats, normalize, (const int*)nullptr);
   } else {
     if (nb > 0)
       hipLaunchKernelGGL((k_eig_finalize<float, float>), dim3(4, nb), dim3(256), 0, ctx->stream, ei,
                          (const float*)st->part32.p, (const double*)st->eigU64.p, (const float*)msg_in,
-                         (float*)msg_out, stats, normalize);
+                         (float*)msg_out, stats, normalize, (const int*)st->sexp.p);
     if (n3 > 0)
       hipLaunchKernelGGL((k_eig_finalize<float, float>), dim3(3, n3), dim3(256), 0, ctx->stream, e3,
                          (const float*)st->part32.p, (const double*)st->eigU64.p, (const float*)msg_in,
-                         (float*)msg_out, stats, normalize);
+                         (float*)msg_out, stats, normalize, (const int*)st->sexp.p);
   }
   const int e6 = prof_mark(ctx);
   prof_add(ctx, "eigh", e0, e1);
@@ -2665,10 +2827,12 @@ int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, M
   const int e2 = prof_mark(ctx);
   if (!acc_f64) {
     hipLaunchKernelGGL((k_fast_finalize<float, float>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
-                       (const float*)st->part32.p, (const float*)msg_in, (float*)msg_out, stats, normalize);
+                       (const float*)st->part32.p, (const float*)msg_in, (float*)msg_out, stats, normalize,
+                       (const int*)st->sexp.p);
   } else {
     hipLaunchKernelGGL((k_fast_finalize<double, double>), dim3(2, ni), dim3(256), 0, ctx->stream, items,
-                       (const double*)st->part64.p, (const double*)msg_in, (double*)msg_out, stats, normalize);
+                       (const double*)st->part64.p, (const double*)msg_in, (double*)msg_out, stats, normalize,
+                       (const int*)nullptr);
   }
   prof_add(ctx, "finalize", e2, prof_mark(ctx));
   ITNBP_HIPCHK(ctx, hipGetLastError());

@@ -46,10 +46,14 @@ def test_f64_pipeline_matches_oracle(net34, pipeline, lds, monkeypatch):
     assert err < 1e-12, err
 
 
-@pytest.mark.parametrize("flavour,tol_low", [("eig", 3e-3), ("precise", 2e-4), ("compact", 3e-3)])
+@pytest.mark.parametrize("flavour,tol_low", [("eig", 1e-3), ("eig_bf16", 3e-3), ("precise", 2e-4),
+                                             ("compact", 3e-3)])
 def test_bf16_pipeline_then_f64_refinement(net34, oracle_ref, flavour, tol_low, monkeypatch):
+    """BF16 site tensors; eig: FP16 intermediates (default), eig_bf16: BF16 ones."""
     monkeypatch.setenv("ITNBP_BF16_PRECISE", "1" if flavour == "precise" else "0")
-    monkeypatch.setenv("ITNBP_FAST_LEGACY", "0" if flavour == "eig" else "1")
+    monkeypatch.setenv("ITNBP_FAST_LEGACY", "0" if flavour.startswith("eig") else "1")
+    monkeypatch.setenv("ITNBP_EIG_BF16", "1" if flavour == "eig_bf16" else "0")
+    flavour = "eig" if flavour.startswith("eig") else flavour
     g, psi = net34
     m3, m40 = oracle_ref
     bpc = itnbp.initialize_cache(psi, storage="bf16")
@@ -58,6 +62,7 @@ def test_bf16_pipeline_then_f64_refinement(net34, oracle_ref, flavour, tol_low, 
     assert ("eig" in low.last_update["path"]) == (flavour == "eig")
     got = to_index(g, low.messages())
     err_low = max(np.abs(got[e] - m3[e]).max() for e in m3)
+    print(f"{flavour}: max message error after 3 low-precision sweeps {err_low:.3g}")
     assert err_low < tol_low, err_low
     full = itnbp.update(bpc, maxiter=40, res_tol=1e-11, edge_sequence="parallel", precision="bf16_f64",
                         switch_tol=1e-4)
@@ -65,6 +70,40 @@ def test_bf16_pipeline_then_f64_refinement(net34, oracle_ref, flavour, tol_low, 
     got = to_index(g, full.messages())
     err = max(np.abs(got[e] - m40[e]).max() for e in m40)
     assert err < 1e-10, err
+
+
+@pytest.mark.parametrize("scale", [2.0 ** -40, 1e-9, 3e5])
+def test_eig16_site_scales(net34, scale):
+    """The 16-bit eigenbasis pipeline multiplies each site by a power of two that
+    brings its largest entry into [1/2, 1) (the FP16 intermediates' range) and
+    divides the Grams by its square: normalised messages of psi * scale equal
+    those of psi (bit for bit for a power of two), and unnormalised ones carry
+    scale^2 (one Jacobi sweep against the FP64 pipeline)."""
+    g, psi = net34
+    base = itnbp.update(itnbp.initialize_cache(psi, storage="bf16"), maxiter=2, edge_sequence="parallel",
+                        precision="f32")
+    assert "bf16-eig" in base.last_update["path"]
+    psi_s = psi.copy()
+    for v in g.vertices:
+        psi_s[v] = psi[v] * scale
+    got = itnbp.update(itnbp.initialize_cache(psi_s, storage="bf16"), maxiter=2, edge_sequence="parallel",
+                       precision="f32")
+    a, b = base.messages(), got.messages()
+    err = max(np.abs(np.asarray(a[e]) - np.asarray(b[e])).max() for e in a)
+    print(f"scale {scale:g}: normalised messages differ by {err:.3g}")
+    if np.log2(scale) == np.round(np.log2(scale)):
+        assert err == 0.0, err
+    else:
+        assert err < 5e-3, err
+    raw = dict(message_update_alg={"alg": "contract", "normalize": False})
+    bpc = itnbp.initialize_cache(psi_s, storage="bf16")
+    lo = itnbp.update(bpc, maxiter=1, edge_sequence="parallel", precision="f32", **raw)
+    hi = itnbp.update(bpc, maxiter=1, edge_sequence="parallel", precision="f64", **raw)
+    assert "bf16-eig" in lo.last_update["path"] and "f64-eig" in hi.last_update["path"]
+    a, b = lo.messages(), hi.messages()
+    rel = max(np.abs(np.asarray(a[e], dtype=np.float64) - np.asarray(b[e])).max() / np.abs(b[e]).max() for e in b)
+    print(f"scale {scale:g}: unnormalised messages, relative difference {rel:.3g}")
+    assert rel < 1e-2, rel
 
 
 def _random_messages(g, psi, rng, symmetric):
