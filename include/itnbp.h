@@ -53,8 +53,16 @@ enum { ITNBP_SCHED_SEQUENTIAL = 0, ITNBP_SCHED_PARALLEL = 1 };
 enum {
   ITNBP_PREC_F64 = 0,         /* FP64 everywhere (the plain path) */
   ITNBP_PREC_F32 = 1,         /* FP32 messages and arithmetic */
-  ITNBP_PREC_BF16_F64 = 2     /* BF16 MFMA with FP32 accumulation until the
+  ITNBP_PREC_BF16_F64 = 2,    /* BF16 MFMA with FP32 accumulation until the
                                  residual < switch_tol, then FP64 refinement */
+  ITNBP_PREC_BF16X3 = 3,      /* compensated BF16 MFMA: every rounded operand split
+                                 into three BF16 terms (about 24 significant bits,
+                                 FP32-class results), FP32 intermediates and
+                                 messages; where the chi = 32 eigenbasis pipeline
+                                 does not apply, the absorbed-pair passes with
+                                 two-term splits or the FP32 kernels */
+  ITNBP_PREC_BF16X3_F64 = 4   /* ITNBP_PREC_BF16X3 until the residual < switch_tol,
+                                 then FP64 refinement */
 };
 
 typedef struct itnbp_ctx itnbp_ctx;

@@ -19,7 +19,8 @@ extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_
   if (o->maxiter < 1) return itnbp_fail(ctx, ITNBP_ERR_ARG, "You need to specify a number of iterations for BP!");
   if (o->schedule != ITNBP_SCHED_PARALLEL || o->nseq != 0)
     return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "batched update runs the default parallel schedule");
-  if (o->precision == ITNBP_PREC_BF16_F64)
+  // FP32 arithmetic serves ITNBP_PREC_F32 and ITNBP_PREC_BF16X3 (no 16-bit batched kernels)
+  if (o->precision == ITNBP_PREC_BF16_F64 || o->precision == ITNBP_PREC_BF16X3_F64)
     return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "batched update supports FP64 and FP32 precision");
   const int dtype = nets[0]->dtype;
   // merged description

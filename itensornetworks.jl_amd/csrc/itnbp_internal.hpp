@@ -136,6 +136,7 @@ struct itnbp_net {
   // initialisation and for sweeps from a symmetric state; the eigenbasis
   // pipeline relies on it
   bool msgs_sym = true;
+  int low_x3 = 0;        // reduced-precision phase with compensated (split-operand) arithmetic
   int max_chi = 0, max_k = 0;
   // device state
   void* d_psi = nullptr;       // in dtype

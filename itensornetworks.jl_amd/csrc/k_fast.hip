@@ -1941,6 +1941,462 @@ __global__ __launch_bounds__(256) void k_rot3_d3_bf16(const EigItem* __restrict_
   for (int o = threadIdx.x; o < FBLK; o += blockDim.x) d3[o] = Red[o];
 }
 
+// ---------------------------------------------------------------------------
+// Compensated flavour of the eigenbasis pipeline: R and Q in FP32, every MFMA
+// operand that is not exact in BF16 split into three BF16 terms
+// (x = h + m + l, about 24 significant bits) and every product a b taken as
+// the six terms of order <= 2: hh + hm + mh + hl + mm + lh (the site tensors
+// are BF16 and enter unsplit; a product with one exact operand has three).
+// ---------------------------------------------------------------------------
+constexpr int SX_LD = 33;  // FP32 staging row stride (odd: conflict-free column writes)
+
+struct bf3 {
+  bf16x8 h, m, l;
+};
+
+__device__ inline void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)x;
+  const float r = x - (float)h;
+  m = (__bf16)r;
+  l = (__bf16)(r - (float)m);
+}
+
+__device__ inline bf3 split8x3(const float (&x)[8]) {
+  bf3 o;
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    __bf16 h, m, l;
+    split3(x[p], h, m, l);
+    o.h[p] = h;
+    o.m[p] = m;
+    o.l[p] = l;
+  }
+  return o;
+}
+
+__device__ inline f32x16 mfma6(const bf3& a, const bf3& b, f32x16 acc) {
+  acc = MFMA_BF16(a.l, b.h, acc);
+  acc = MFMA_BF16(a.m, b.m, acc);
+  acc = MFMA_BF16(a.h, b.l, acc);
+  acc = MFMA_BF16(a.m, b.h, acc);
+  acc = MFMA_BF16(a.h, b.m, acc);
+  acc = MFMA_BF16(a.h, b.h, acc);
+  return acc;
+}
+
+// exact a (BF16) times split b
+__device__ inline f32x16 mfma3x(bf16x8 a, const bf3& b, f32x16 acc) {
+  acc = MFMA_BF16(a, b.l, acc);
+  acc = MFMA_BF16(a, b.m, acc);
+  acc = MFMA_BF16(a, b.h, acc);
+  return acc;
+}
+
+// rot12: R block (s + 2 (x' + 32 y')) row z, columns w = (U1^T X U2)[x'][y'] of
+// the 32 layout-B blocks X = (s + 2 (z + 32 w)).  Items: the absorb kernel's
+// (psi_in = layout B, P = R offset, Ma = U1, Mb = U2).
+// grid (64, items), 512 threads: WG -> (s, z) = (q & 1, q >> 1), wave -> w = 4 wv + {0..3}
+__global__ __launch_bounds__(512) void k_rot12_x3(const FastItem* __restrict__ items,
+                                                  const __bf16* __restrict__ lay,
+                                                  const float* __restrict__ U, float* __restrict__ R) {
+  __shared__ float S[FBLK * SX_LD];  // row (y' + 32 x'), column w
+  const FastItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int s = q & 1, z = q >> 1;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  bf3 bU2[2], aU1[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    float u2[8], u1[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      u2[p] = U[it.Mb + (16 * t + 8 * g + p) + FC * c];  // U2, K = b2 natural
+      u1[p] = U[it.Ma + kperm(t, g, p) + FC * c];        // U1^T, K = b1 in accumulator-row order
+    }
+    bU2[t] = split8x3(u2);
+    aU1[t] = split8x3(u1);
+  }
+  bf16x8 aXs[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const __bf16* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * (4 * wv + i))) * FBLK;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) aXs[i][t] = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int w = 4 * wv + i;
+    f32x16 V = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) V = mfma3x(aXs[i][t], bU2[t], V);
+    f32x16 Y = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      float v8[8];
+#pragma unroll
+      for (int p = 0; p < 8; ++p) v8[p] = V[8 * t + p];
+      Y = mfma6(aU1[t], split8x3(v8), Y);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
+      S[(c + FC * xb) * SX_LD + w] = Y[v];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < FBLK * 8; k += blockDim.x) {
+    const int o = k >> 3, pt = k & 7;
+    const int yb = o & 31, xb = o >> 5;
+    const long long r = s + 2 * (xb + FC * yb);
+    const float* src = S + o * SX_LD + 4 * pt;
+    *reinterpret_cast<float4*>(R + it.P + r * FBLK + z * FC + 4 * pt) = make_float4(src[0], src[1], src[2], src[3]);
+  }
+}
+
+// rot34 with the Grams T'3, T'4 (the compensated twin of k_rot34_bf16<true>).
+// grid (EIG_NPART, items), 512 threads: WG -> s = q & 1, r1 = z in
+// [ROT_NZ (q >> 1), +ROT_NZ); per z the 32 R blocks (s + 2 (z + 32 w)) in two
+// halves, wave wv -> w = 16 h + 2 wv + {0, 1}; the next half's two blocks are in
+// flight while the current two are multiplied.
+__global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ items, const float* __restrict__ R,
+                                                  const float* __restrict__ U, const float* __restrict__ lam,
+                                                  float* __restrict__ Q, float* __restrict__ part) {
+  __shared__ float S[FBLK * SX_LD];  // row (r4 + 32 r3), column r2
+  __shared__ __attribute__((aligned(16))) float lam34[2][FC];
+  const EigItem* itp = items + blockIdx.y;
+  const int q = blockIdx.x;
+  const int s = q & 1, z0 = (q >> 1) * ROT_NZ;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  const long long Ro = itp->R, Qo = itp->Q, L0 = itp->L[0], L1 = itp->L[1];
+  if (threadIdx.x < 2 * FC)
+    lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[((threadIdx.x >> 5) ? itp->L[3] : itp->L[2]) + (threadIdx.x & 31)];
+  bf3 aU3[2], bU4[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    float u3[8], u4[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      u3[p] = U[itp->U[2] + kperm(t, g, p) + FC * c];
+      u4[p] = U[itp->U[3] + (16 * t + 8 * g + p) + FC * c];
+    }
+    aU3[t] = split8x3(u3);
+    bU4[t] = split8x3(u4);
+  }
+  __syncthreads();
+  f32x16 T3 = {}, T4 = {};
+  float4 cur[2][4], nxt[2][4];
+  auto fetch = [&](int step, float4 (&dst)[2][4]) {
+    const int z = z0 + (step >> 1), h = step & 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int w = 16 * h + 2 * wv + i;
+      const float* Z = R + Ro + (long long)(s + 2 * (z + FC * w)) * FBLK + c * FC + 8 * g;
+      dst[i][0] = *reinterpret_cast<const float4*>(Z);
+      dst[i][1] = *reinterpret_cast<const float4*>(Z + 4);
+      dst[i][2] = *reinterpret_cast<const float4*>(Z + 16);
+      dst[i][3] = *reinterpret_cast<const float4*>(Z + 20);
+    }
+  };
+  constexpr int NSTEP = 2 * ROT_NZ;
+  fetch(0, cur);
+#pragma unroll 1
+  for (int step = 0; step < NSTEP; ++step) {
+    const int z = z0 + (step >> 1), h = step & 1;
+    if (step + 1 < NSTEP) fetch(step + 1, nxt);
+    const float l1 = lam[L0 + z];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int w = 16 * h + 2 * wv + i;
+      const float w12 = l1 * lam[L1 + w];
+      f32x16 V = {};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const float x8[8] = {cur[i][2 * t].x, cur[i][2 * t].y, cur[i][2 * t].z, cur[i][2 * t].w,
+                             cur[i][2 * t + 1].x, cur[i][2 * t + 1].y, cur[i][2 * t + 1].z, cur[i][2 * t + 1].w};
+        V = mfma6(split8x3(x8), bU4[t], V);
+      }
+      f32x16 Y = {}, YT = {};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        float v8[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) v8[p] = V[8 * t + p];
+        const bf3 bv = split8x3(v8);
+        Y = mfma6(aU3[t], bv, Y);
+        YT = mfma6(bv, aU3[t], YT);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const float4 l3a = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 4 * g]);
+        const float4 l3b = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 8 + 4 * g]);
+        const float4 l4a = *reinterpret_cast<const float4*>(&lam34[1][16 * t + 4 * g]);
+        const float4 l4b = *reinterpret_cast<const float4*>(&lam34[1][16 * t + 8 + 4 * g]);
+        const float l3[8] = {l3a.x, l3a.y, l3a.z, l3a.w, l3b.x, l3b.y, l3b.z, l3b.w};
+        const float l4[8] = {l4a.x, l4a.y, l4a.z, l4a.w, l4b.x, l4b.y, l4b.z, l4b.w};
+        float a8[8], b8[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          a8[p] = YT[8 * t + p];
+          b8[p] = w12 * l4[p] * a8[p];
+        }
+        T3 = mfma6(split8x3(a8), split8x3(b8), T3);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          a8[p] = Y[8 * t + p];
+          b8[p] = w12 * l3[p] * a8[p];
+        }
+        T4 = mfma6(split8x3(a8), split8x3(b8), T4);
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
+        S[(c + FC * xb) * SX_LD + w] = Y[v];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (h == 1) {
+      __syncthreads();
+      for (int k = threadIdx.x; k < FBLK * 8; k += blockDim.x) {
+        const int o = k >> 3, pt = k & 7;
+        const int yb = o & 31, xb = o >> 5;
+        const long long qb = s + 2 * (xb + FC * yb);
+        const float* src = S + o * SX_LD + 4 * pt;
+        *reinterpret_cast<float4*>(Q + Qo + qb * FBLK + z * FC + 4 * pt) = make_float4(src[0], src[1], src[2], src[3]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cur[i][j] = nxt[i][j];
+  }
+  float* Red = S;
+  for (int w8 = 0; w8 < 8; ++w8) {
+    if (wv == w8) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rw = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w8 == 0) {
+          Red[rw * FC + c] = T3[v];
+          Red[FBLK + rw * FC + c] = T4[v];
+        } else {
+          Red[rw * FC + c] += T3[v];
+          Red[FBLK + rw * FC + c] += T4[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* d3 = part + itp->part + (long long)(2 * EIG_NPART + q) * FBLK;
+  float* d4 = part + itp->part + (long long)(3 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    d3[o] = Red[o];
+    d4[o] = Red[FBLK + o];
+  }
+}
+
+// gram12 (T'1, T'2) of the FP32 Q blocks; the layout of k_gram12_bf16, two
+// blocks of look-ahead per wave
+template <int NB>
+__global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ items, const float* __restrict__ Q,
+                                                   const float* __restrict__ lam, float* __restrict__ part) {
+  __shared__ float Red[2 * FBLK];
+  const EigItem* itp = items + blockIdx.y;
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  const long long Qo = itp->Q, L2 = itp->L[2], L3 = itp->L[3];
+  float lam1v[16], lam2v[16];
+  bf16x8 bI[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      lam1v[8 * t + p] = lam[itp->L[0] + kperm(t, g, p)];
+      lam2v[8 * t + p] = lam[itp->L[1] + 16 * t + 8 * g + p];
+      bI[t][p] = (__bf16)((16 * t + 8 * g + p) == c ? 1.0f : 0.0f);
+    }
+  f32x16 T1 = {}, T2 = {};
+  constexpr int PF = NB < 2 ? NB : 2;
+  const long long r0 = (long long)q * (4 * NB) + wv * NB;
+  float4 cur[PF][4], nxt[PF][4];
+  auto fetch = [&](long long r, float4 (&dst)[PF][4]) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const float* D = Q + Qo + (r + i) * FBLK + c * FC + 8 * g;
+      dst[i][0] = *reinterpret_cast<const float4*>(D);
+      dst[i][1] = *reinterpret_cast<const float4*>(D + 4);
+      dst[i][2] = *reinterpret_cast<const float4*>(D + 16);
+      dst[i][3] = *reinterpret_cast<const float4*>(D + 20);
+    }
+  };
+  fetch(r0, cur);
+#pragma unroll 1
+  for (int i0 = 0; i0 < NB; i0 += PF) {
+    if (i0 + PF < NB) fetch(r0 + i0 + PF, nxt);
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const long long qb = r0 + i0 + i;
+      const int r3 = (int)((qb >> 1) & 31), r4 = (int)(qb >> 6);
+      const float w34 = lam[L2 + r3] * lam[L3 + r4];
+      f32x16 Dt = {};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const float d8[8] = {cur[i][2 * t].x, cur[i][2 * t].y, cur[i][2 * t].z, cur[i][2 * t].w,
+                             cur[i][2 * t + 1].x, cur[i][2 * t + 1].y, cur[i][2 * t + 1].z, cur[i][2 * t + 1].w};
+        float b8[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) b8[p] = w34 * lam2v[8 * t + p] * d8[p];
+        const bf3 a = split8x3(d8);
+        T1 = mfma6(a, split8x3(b8), T1);
+        Dt = MFMA_BF16(a.l, bI[t], Dt);  // D I: the transpose, exact
+        Dt = MFMA_BF16(a.m, bI[t], Dt);
+        Dt = MFMA_BF16(a.h, bI[t], Dt);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        float a8[8], b8[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          a8[p] = Dt[8 * t + p];
+          b8[p] = w34 * lam1v[8 * t + p] * a8[p];
+        }
+        T2 = mfma6(split8x3(a8), split8x3(b8), T2);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int i = 0; i < PF; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cur[i][j] = nxt[i][j];
+  }
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rw = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w4 == 0) {
+          Red[rw * FC + c] = T1[v];
+          Red[FBLK + rw * FC + c] = T2[v];
+        } else {
+          Red[rw * FC + c] += T1[v];
+          Red[FBLK + rw * FC + c] += T2[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* d1 = part + itp->part + (long long)(0 * EIG_NPART + q) * FBLK;
+  float* d2 = part + itp->part + (long long)(1 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
+    d1[o] = Red[o];
+    d2[o] = Red[FBLK + o];
+  }
+}
+
+// degree-3 sites, compensated: rot12_d3 and rot3_d3 of the 16-bit kernels on
+// FP32 R3 / Q3 blocks
+__global__ __launch_bounds__(256) void k_rot12_d3_x3(const EigItem* __restrict__ items,
+                                                     const __bf16* __restrict__ lay,
+                                                     const float* __restrict__ U, float* __restrict__ R) {
+  const EigItem* itp = items + blockIdx.y;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  bf3 aU1[2], bU2[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    float u1[8], u2[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      u1[p] = U[itp->U[0] + kperm(t, g, p) + FC * c];
+      u2[p] = U[itp->U[1] + (16 * t + 8 * g + p) + FC * c];
+    }
+    aU1[t] = split8x3(u1);
+    bU2[t] = split8x3(u2);
+  }
+  bf16x8 aX[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const __bf16* X = lay + itp->X + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) aX[i][t] = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    f32x16 V = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) V = mfma3x(aX[i][t], bU2[t], V);
+    f32x16 Y = {};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      float v8[8];
+#pragma unroll
+      for (int p = 0; p < 8; ++p) v8[p] = V[8 * t + p];
+      Y = mfma6(aU1[t], split8x3(v8), Y);
+    }
+    float* dst = R + itp->R + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) dst[(8 * (v >> 2) + 4 * g + (v & 3)) * FC + c] = Y[v];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rot3_d3_x3(const EigItem* __restrict__ items, const float* __restrict__ R,
+                                                    const float* __restrict__ U, const float* __restrict__ lam,
+                                                    float* __restrict__ Q, float* __restrict__ part) {
+  __shared__ float Red[FBLK];
+  const EigItem* itp = items + blockIdx.y;
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
+  const int x = 4 * q + wv, s = x & 1, r1 = x >> 1;
+  bf3 aU3[2], bZ[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    float u3[8], z8[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      u3[p] = U[itp->U[2] + (16 * t + 8 * g + p) + FC * c];  // U3^T, K = b3 natural
+      z8[p] = R[itp->R + (long long)(s + 2 * (16 * t + 8 * g + p)) * FBLK + r1 * FC + c];  // lane r2, K = b3
+    }
+    aU3[t] = split8x3(u3);
+    bZ[t] = split8x3(z8);
+  }
+  f32x16 Y = {}, YT = {};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    Y = mfma6(aU3[t], bZ[t], Y);
+    YT = mfma6(bZ[t], aU3[t], YT);
+  }
+  const float l1 = lam[itp->L[0] + r1];
+  f32x16 T3 = {};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    float a8[8], b8[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      a8[p] = YT[8 * t + p];
+      b8[p] = l1 * lam[itp->L[1] + kperm(t, g, p)] * a8[p];
+    }
+    T3 = mfma6(split8x3(a8), split8x3(b8), T3);
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int r3 = 8 * (v >> 2) + 4 * g + (v & 3);
+    Q[itp->Q + (long long)(s + 2 * r3) * FBLK + r1 * FC + c] = Y[v];
+  }
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rw = 8 * (v >> 2) + 4 * g + (v & 3);
+        if (w4 == 0) Red[rw * FC + c] = T3[v];
+        else Red[rw * FC + c] += T3[v];
+      }
+    }
+    __syncthreads();
+  }
+  float* d3 = part + itp->part + (long long)(2 * EIG_NPART + q) * FBLK;
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) d3[o] = Red[o];
+}
+
 // FP64 degree-3 kernels, same decomposition on v_mfma_f64_16x16x4_f64
 __global__ __launch_bounds__(256) void k_rot12_d3_f64(const EigItem* __restrict__ items,
                                                       const double* __restrict__ lay,
@@ -2167,8 +2623,11 @@ struct FastState {
   std::vector<int> deg3;            // eligible degree-3 vertices
   std::vector<int> deg3_used;       // those whose outgoing messages are all in the sequence
   std::vector<EigItem> e3items;
-  DevBuf d_e3items, lay3_16, lay3_64, P3_16, P3_64, offs3;
+  DevBuf d_e3items, lay3_16, lay3_64, P3_16, P3_32, P3_64, offs3;
   bool lay3_16_ok = false, lay3_64_ok = false;
+  // compensated 16-bit arithmetic (split operands, FP32 intermediates) in the
+  // current reduced-precision phase
+  bool x3 = false;
   // power-of-two exponents of the 16-bit layouts: bulk site bi at bi, degree-3
   // site k (index into deg3) at bulk.size() + k
   DevBuf sexp, pmax;
@@ -2231,9 +2690,14 @@ bool fast_path_available(itnbp_net* net, int acc_f64, int schedule) {
   return false;
 }
 
+// compensated reduced-precision arithmetic: requested by the update
+// (ITNBP_PREC_BF16X3*) or by ITNBP_BF16_PRECISE=1
+static bool precise_of(const itnbp_net* net) { return net->low_x3 || fast_precise(); }
+
 const char* fast_path_name(itnbp_net* net, int acc_f64) {
-  if (use_eig(net)) return acc_f64 ? "mfma-f64-eig-chi32" : "mfma-bf16-eig-chi32";
-  return acc_f64 ? "mfma-f64-chi32" : (fast_precise() ? "mfma-bf16x2-chi32" : "mfma-bf16c-chi32");
+  if (use_eig(net))
+    return acc_f64 ? "mfma-f64-eig-chi32" : (precise_of(net) ? "mfma-bf16x3-eig-chi32" : "mfma-bf16-eig-chi32");
+  return acc_f64 ? "mfma-f64-chi32" : (precise_of(net) ? "mfma-bf16x2-chi32" : "mfma-bf16c-chi32");
 }
 
 void release_fast_state(itnbp_net* net) {
@@ -2260,6 +2724,7 @@ void release_fast_state(itnbp_net* net) {
   free_buf(st->lay3_16);
   free_buf(st->lay3_64);
   free_buf(st->P3_16);
+  free_buf(st->P3_32);
   free_buf(st->P3_64);
   free_buf(st->offs3);
   free_buf(st->sexp);
@@ -2362,6 +2827,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   // eigenbasis pipeline: R = first half of the P buffer, Q = second half;
   // eigen slot 4 bi + (c - 1) holds the message into bulk vertex bi on bond c
   st->eig = use_eig(net);
+  st->x3 = !acc_f64 && precise_of(net);
   st->eitems.clear();
   st->k1items.clear();
   st->k1eitems.clear();
@@ -2483,7 +2949,8 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
       dim3 g3(64, n3all);
       if (!acc_f64) {
         ITNBP_RETCHK(ensure_buf(ctx, st->lay3_16, (size_t)n3all * per3 * 2));
-        ITNBP_RETCHK(ensure_buf(ctx, st->P3_16, (size_t)2 * n3all * per3 * 2));
+        if (st->x3) ITNBP_RETCHK(ensure_buf(ctx, st->P3_32, (size_t)2 * n3all * per3 * 4));
+        else ITNBP_RETCHK(ensure_buf(ctx, st->P3_16, (size_t)2 * n3all * per3 * 2));
         if (!st->lay3_16_ok) {
           ITNBP_RETCHK(site_exponents(net, st, s3, n3all, 2LL * FC * FC * FC, nb));
           __bf16* C = (__bf16*)st->lay3_16.p;
@@ -2557,7 +3024,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     ITNBP_RETCHK(ensure_buf(ctx, st->P64, (size_t)2 * nb * per * 8));
     ITNBP_RETCHK(ensure_buf(ctx, st->part64, nparts * FBLK * 8));
   } else {
-    if (fast_precise() && !st->eig)
+    if (st->x3)
       ITNBP_RETCHK(ensure_buf(ctx, st->P32, (size_t)2 * nb * per * 4));
     else
       ITNBP_RETCHK(ensure_buf(ctx, st->P16, (size_t)2 * nb * per * 2));
@@ -2573,7 +3040,7 @@ static void launch_pair(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* 
   const FastItem* items = (const FastItem*)st->d_items.p + i0;
   const int e0 = prof_mark(ctx);
   if (!acc_f64) {
-    if (fast_precise()) {
+    if (st->x3) {
       hipLaunchKernelGGL((k_absorb_bf16<2>), dim3(128, ni), dim3(256), 0, ctx->stream, items,
                          (const __bf16*)st->lay16.p, (const float*)msg_in, (float*)st->P32.p);
     } else {
@@ -2589,7 +3056,7 @@ static void launch_pair(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* 
   }
   const int e1 = prof_mark(ctx);
   if (!acc_f64) {
-    if (fast_precise()) {
+    if (st->x3) {
       hipLaunchKernelGGL((k_fused_bf16<2>), dim3(FUSED_CHUNKS, ni), dim3(256), 0, ctx->stream, items,
                          (const __bf16*)st->lay16.p, (const float*)msg_in, (const float*)st->P32.p,
                          (float*)st->part32.p);
@@ -2708,8 +3175,35 @@ static void eig16_stage(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3
   }
 }
 
+// the compensated flavour (FP32 R and Q, split BF16 operands)
+static void eigx3_stage(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3) {
+  const EigItem* ei = (const EigItem*)st->d_eitems.p;
+  const EigItem* e3 = (const EigItem*)st->d_e3items.p;
+  const float* U = (const float*)st->eigU32.p;
+  const float* L = (const float*)st->eigL32.p;
+  float* P = (float*)st->P32.p;
+  float* P3 = (float*)st->P3_32.p;
+  float* part = (float*)st->part32.p;
+  if (stage == 1 && nb > 0)
+    hipLaunchKernelGGL(k_rot12_x3, dim3(64, nb), dim3(512), 0, ctx->stream, (const FastItem*)st->d_k1items.p,
+                       (const __bf16*)st->lay16.p, U, P);
+  if (stage == 2 && nb > 0)
+    hipLaunchKernelGGL(k_rot34_x3, dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei, (const float*)P, U, L, P,
+                       part);
+  if (stage == 3 && nb > 0)
+    hipLaunchKernelGGL(k_gram12_x3<32>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei, (const float*)P, L, part);
+  if (stage == 4 && n3 > 0) {
+    hipLaunchKernelGGL(k_rot12_d3_x3, dim3(4, n3), dim3(256), 0, ctx->stream, e3, (const __bf16*)st->lay3_16.p, U,
+                       P3);
+    hipLaunchKernelGGL(k_rot3_d3_x3, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const float*)P3, U, L, P3,
+                       part);
+    hipLaunchKernelGGL(k_gram12_x3<1>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const float*)P3, L, part);
+  }
+}
+
 static void eig16(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3) {
-  if (eig_bf16_intermediates()) eig16_stage<__bf16>(ctx, st, stage, nb, n3);
+  if (st->x3) eigx3_stage(ctx, st, stage, nb, n3);
+  else if (eig_bf16_intermediates()) eig16_stage<__bf16>(ctx, st, stage, nb, n3);
   else eig16_stage<_Float16>(ctx, st, stage, nb, n3);
 }
 
@@ -2804,10 +3298,10 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
   }
   const int e6 = prof_mark(ctx);
   prof_add(ctx, "eigh", e0, e1);
-  prof_add(ctx, acc_f64 ? "rot12_f64" : "rot12_bf16", e1, e2);
-  prof_add(ctx, acc_f64 ? "rot34_f64" : "rot34_bf16", e2, e3m);
-  prof_add(ctx, acc_f64 ? "gram12_f64" : "gram12_bf16", e3m, e4);
-  prof_add(ctx, acc_f64 ? "deg3_f64" : "deg3_bf16", e4, e5);
+  prof_add(ctx, acc_f64 ? "rot12_f64" : (st->x3 ? "rot12_x3" : "rot12_bf16"), e1, e2);
+  prof_add(ctx, acc_f64 ? "rot34_f64" : (st->x3 ? "rot34_x3" : "rot34_bf16"), e2, e3m);
+  prof_add(ctx, acc_f64 ? "gram12_f64" : (st->x3 ? "gram12_x3" : "gram12_bf16"), e3m, e4);
+  prof_add(ctx, acc_f64 ? "deg3_f64" : (st->x3 ? "deg3_x3" : "deg3_bf16"), e4, e5);
   prof_add(ctx, "finalize", e5, e6);
   ITNBP_HIPCHK(ctx, hipGetLastError());
   return ITNBP_OK;

@@ -6,7 +6,9 @@ The library is required: importing the compute entry points without it raises
 from __future__ import annotations
 
 import ctypes as C
+import atexit
 import os
+import weakref
 
 import numpy as np
 
@@ -16,7 +18,7 @@ LIB_PATH = os.environ.get("ITNBP_LIB", os.path.join(HERE, "libitnbp.so"))
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_STATE = range(7)
 F64, F32, BF16 = 0, 1, 2
 SCHED_SEQUENTIAL, SCHED_PARALLEL = 0, 1
-PREC_F64, PREC_F32, PREC_BF16_F64 = 0, 1, 2
+PREC_F64, PREC_F32, PREC_BF16_F64, PREC_BF16X3, PREC_BF16X3_F64 = 0, 1, 2, 3, 4
 
 _ERRNAMES = {ERR_ARG: "invalid argument", ERR_HIP: "HIP error", ERR_NOMEM: "out of memory",
              ERR_NONFINITE: "non-finite result", ERR_UNSUPPORTED: "unsupported",
@@ -140,6 +142,9 @@ class Context:
             raise ITNBPError(rc, f"itnbp_create(device={device}) failed (no usable GPU?)")
         self.h = h
         self.device = device
+        # networks created on this context: released before the context itself
+        # (at close() or interpreter exit), whatever order the collector picks
+        self._nets = weakref.WeakSet()
 
     def check(self, rc):
         if rc != OK:
@@ -159,16 +164,28 @@ class Context:
     def last_path(self):
         return self.lib.itnbp_last_path(self.h).decode()
 
+    def close(self):
+        """Release every network of this context, then the context."""
+        for net in list(getattr(self, "_nets", ())):
+            net._release()
+        if getattr(self, "h", None):
+            self.lib.itnbp_destroy(self.h)
+            self.h = None
+
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                self.lib.itnbp_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 
 
 _default_ctx = None
+
+
+@atexit.register
+def _close_default_context():
+    if _default_ctx is not None:
+        _default_ctx.close()
 
 
 def default_context():

@@ -37,7 +37,8 @@ OPS = {
     "I": np.eye(2),
 }
 
-PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16_f64": L.PREC_BF16_F64}
+PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16_f64": L.PREC_BF16_F64, "bf16x3": L.PREC_BF16X3,
+              "bf16x3_f64": L.PREC_BF16X3_F64}
 
 
 def op_matrix(op, d=2):
@@ -135,6 +136,7 @@ class BeliefPropagationCache:
         self.partitioning = None  # PartitionedNetwork when built over generalised partitions
         if _handle is not None:
             self.h = _handle
+            self.ctx._nets.add(self)
             return
         edges, inc_ptr, inc_edge = g.csr()
         chis = np.array([psi.bond_dim(a, b) for (a, b) in g.edges], dtype=np.int32)
@@ -145,6 +147,7 @@ class BeliefPropagationCache:
             L.iptr(inc_edge) if g.ne else None, L.iptr(chis) if g.ne else None, L.iptr(phys),
             self.storage, C.byref(h)))
         self.h = h
+        self.ctx._nets.add(self)
         flat = _flat_sites(psi, self.storage)
         self.ctx.check(self.lib.itnbp_set_sites(self.h, L.vptr(flat)))
         if self.storage == L.BF16:
@@ -153,11 +156,14 @@ class BeliefPropagationCache:
             self.set_messages_(messages)
 
     # -- lifetime ------------------------------------------------------------
+    def _release(self):
+        if getattr(self, "h", None):
+            self.lib.itnbp_net_destroy(self.h)
+            self.h = None
+
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                self.lib.itnbp_net_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -168,6 +174,7 @@ class BeliefPropagationCache:
         c = BeliefPropagationCache.__new__(BeliefPropagationCache)
         c.qf = QuadraticFormNetwork(self.tensornetwork())
         c.ctx, c.lib, c.h = self.ctx, self.lib, h
+        self.ctx._nets.add(c)
         c.storage, c.storage_name = self.storage, self.storage_name
         c._host_psi = c.qf.ket
         c._psi_dirty = False
@@ -344,8 +351,10 @@ def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, v
     ``edge_sequence``: None / "forest_cover" (sequential, the default), "parallel",
     a list of directed edges (sequential) or a list of edge groups (parallel).
     Extensions: ``res_tol`` stops on max ||new-old||/||new||; ``precision`` is
-    "f64", "f32" or "bf16_f64" (BF16 MFMA sweeps until the residual drops below
-    ``switch_tol``, then FP64)."""
+    "f64", "f32" (16-bit MFMA sweeps where the chi = 32 pipelines apply, FP32
+    elsewhere), "bf16x3" (compensated 16-bit MFMA: operands split into three
+    BF16 terms, FP32 intermediates), or "bf16_f64" / "bf16x3_f64" (those sweeps
+    until the residual drops below ``switch_tol`` or stagnates, then FP64)."""
     if alg != "bp":
         raise ValueError(f"unknown update algorithm {alg!r}")
     if isinstance(maxiter, str):
