@@ -273,10 +273,13 @@ def main():
         del chosen
 
         def timed_solve(storage, **kw):
+            # one untimed sweep of every phase the solve will run (block layouts,
+            # buffers, autotuning), then identity messages and the clock
             c = itnbp.initialize_cache(psi, storage=storage)
-            itnbp.update_(c, maxiter=1, edge_sequence="parallel",
-                          precision="f32" if kw.get("precision") == "bf16_f64" else "f64")
-            if kw.get("precision") == "bf16_f64":
+            p = kw.get("precision", "f64")
+            low = {"bf16_f64": "f32", "bf16x3_f64": "bf16x3"}.get(p, p)
+            itnbp.update_(c, maxiter=1, edge_sequence="parallel", precision=low)
+            if p in ("bf16_f64", "bf16x3_f64"):
                 itnbp.update_(c, maxiter=1, edge_sequence="parallel")
             c.ctx.check(c.lib.itnbp_init_identity_messages(c.h))
             ctx.check(c.lib.itnbp_synchronize(ctx.h))

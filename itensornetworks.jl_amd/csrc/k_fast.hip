@@ -1984,6 +1984,28 @@ __device__ inline f32x16 mfma6(const bf3& a, const bf3& b, f32x16 acc) {
   return acc;
 }
 
+// v with the sign bits of mask flipped (exact negation of selected elements)
+__device__ inline bf16x8 flip_signs(bf16x8 v, uint4 m) {
+  uint4 u = __builtin_bit_cast(uint4, v);
+  u.x ^= m.x;
+  u.y ^= m.y;
+  u.z ^= m.z;
+  u.w ^= m.w;
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+// Weighted Gram term acc += sum_k w_k a[m][k] a[n][k] with one split: the
+// A operand is sqrt|w_k| a[.][k] (split into three terms) and the B operand the
+// same terms with the sign of w_k (mask) applied, so b = sign(w) sqrt|w| a exactly
+__device__ inline f32x16 wgram6(const float (&aw)[8], uint4 mask, f32x16 acc) {
+  const bf3 a = split8x3(aw);
+  bf3 b;
+  b.h = flip_signs(a.h, mask);
+  b.m = flip_signs(a.m, mask);
+  b.l = flip_signs(a.l, mask);
+  return mfma6(a, b, acc);
+}
+
 // exact a (BF16) times split b
 __device__ inline f32x16 mfma3x(bf16x8 a, const bf3& b, f32x16 acc) {
   acc = MFMA_BF16(a, b.l, acc);
@@ -2063,14 +2085,30 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
                                                   const float* __restrict__ U, const float* __restrict__ lam,
                                                   float* __restrict__ Q, float* __restrict__ part) {
   __shared__ float S[FBLK * SX_LD];  // row (r4 + 32 r3), column r2
-  __shared__ __attribute__((aligned(16))) float lam34[2][FC];
+  // sqrt|l3|, sqrt|l4| and the sign masks of l3, l4 for the K order of the Gram
+  // operands (rows kperm(t, g, 0..7)), per (bond, t, g)
+  __shared__ __attribute__((aligned(16))) float sq34[2][FC];
+  __shared__ __attribute__((aligned(16))) uint4 msk34[2][2][2];
   const EigItem* itp = items + blockIdx.y;
   const int q = blockIdx.x;
   const int s = q & 1, z0 = (q >> 1) * ROT_NZ;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   const long long Ro = itp->R, Qo = itp->Q, L0 = itp->L[0], L1 = itp->L[1];
-  if (threadIdx.x < 2 * FC)
-    lam34[threadIdx.x >> 5][threadIdx.x & 31] = lam[((threadIdx.x >> 5) ? itp->L[3] : itp->L[2]) + (threadIdx.x & 31)];
+  if (threadIdx.x < 2 * FC) {
+    const float lv = lam[((threadIdx.x >> 5) ? itp->L[3] : itp->L[2]) + (threadIdx.x & 31)];
+    sq34[threadIdx.x >> 5][threadIdx.x & 31] = sqrtf(fabsf(lv));
+  }
+  if (threadIdx.x < 8) {
+    const int bd = threadIdx.x >> 2, t = (threadIdx.x >> 1) & 1, gg = threadIdx.x & 1;
+    const long long lb = bd ? itp->L[3] : itp->L[2];
+    unsigned mw[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float lo = lam[lb + kperm(t, gg, 2 * j)], hi = lam[lb + kperm(t, gg, 2 * j + 1)];
+      mw[j] = (lo < 0.f ? 0x8000u : 0u) | (hi < 0.f ? 0x80000000u : 0u);
+    }
+    msk34[bd][t][gg] = make_uint4(mw[0], mw[1], mw[2], mw[3]);
+  }
   bf3 aU3[2], bU4[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -2109,6 +2147,8 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
     for (int i = 0; i < 2; ++i) {
       const int w = 16 * h + 2 * wv + i;
       const float w12 = l1 * lam[L1 + w];
+      const float sw12 = sqrtf(fabsf(w12));
+      const unsigned nw = w12 < 0.f ? 0x80008000u : 0u;
       f32x16 V = {};
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
@@ -2128,25 +2168,24 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        const float4 l3a = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 4 * g]);
-        const float4 l3b = *reinterpret_cast<const float4*>(&lam34[0][16 * t + 8 + 4 * g]);
-        const float4 l4a = *reinterpret_cast<const float4*>(&lam34[1][16 * t + 4 * g]);
-        const float4 l4b = *reinterpret_cast<const float4*>(&lam34[1][16 * t + 8 + 4 * g]);
-        const float l3[8] = {l3a.x, l3a.y, l3a.z, l3a.w, l3b.x, l3b.y, l3b.z, l3b.w};
-        const float l4[8] = {l4a.x, l4a.y, l4a.z, l4a.w, l4b.x, l4b.y, l4b.z, l4b.w};
-        float a8[8], b8[8];
+        const float4 s3a = *reinterpret_cast<const float4*>(&sq34[0][16 * t + 4 * g]);
+        const float4 s3b = *reinterpret_cast<const float4*>(&sq34[0][16 * t + 8 + 4 * g]);
+        const float4 s4a = *reinterpret_cast<const float4*>(&sq34[1][16 * t + 4 * g]);
+        const float4 s4b = *reinterpret_cast<const float4*>(&sq34[1][16 * t + 8 + 4 * g]);
+        const float s3[8] = {s3a.x, s3a.y, s3a.z, s3a.w, s3b.x, s3b.y, s3b.z, s3b.w};
+        const float s4[8] = {s4a.x, s4a.y, s4a.z, s4a.w, s4b.x, s4b.y, s4b.z, s4b.w};
+        uint4 m3 = msk34[0][t][g], m4 = msk34[1][t][g];
+        m3.x ^= nw; m3.y ^= nw; m3.z ^= nw; m3.w ^= nw;
+        m4.x ^= nw; m4.y ^= nw; m4.z ^= nw; m4.w ^= nw;
+        float a8[8];
+        // T'3: weights w12 l4 over the K index r4 of YT
 #pragma unroll
-        for (int p = 0; p < 8; ++p) {
-          a8[p] = YT[8 * t + p];
-          b8[p] = w12 * l4[p] * a8[p];
-        }
-        T3 = mfma6(split8x3(a8), split8x3(b8), T3);
+        for (int p = 0; p < 8; ++p) a8[p] = sw12 * s4[p] * YT[8 * t + p];
+        T3 = wgram6(a8, m4, T3);
+        // T'4: weights w12 l3 over the K index r3 of Y
 #pragma unroll
-        for (int p = 0; p < 8; ++p) {
-          a8[p] = Y[8 * t + p];
-          b8[p] = w12 * l3[p] * a8[p];
-        }
-        T4 = mfma6(split8x3(a8), split8x3(b8), T4);
+        for (int p = 0; p < 8; ++p) a8[p] = sw12 * s3[p] * Y[8 * t + p];
+        T4 = wgram6(a8, m3, T4);
       }
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
@@ -2206,16 +2245,24 @@ __global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ i
   const int q = blockIdx.x;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   const long long Qo = itp->Q, L2 = itp->L[2], L3 = itp->L[3];
-  float lam1v[16], lam2v[16];
+  // T'2 weights l1 over the K index r1 (rows kperm(t, g, p) of D^T): sqrt|l1| and
+  // the sign masks of the single-split Gram
+  float sl1v[16], lam2v[16];
+  uint4 m1[2];
   bf16x8 bI[2];
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int t = 0; t < 2; ++t) {
+    unsigned mw[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      lam1v[8 * t + p] = lam[itp->L[0] + kperm(t, g, p)];
+      const float l1 = lam[itp->L[0] + kperm(t, g, p)];
+      sl1v[8 * t + p] = sqrtf(fabsf(l1));
+      if (l1 < 0.f) mw[p >> 1] |= (p & 1) ? 0x80000000u : 0x8000u;
       lam2v[8 * t + p] = lam[itp->L[1] + 16 * t + 8 * g + p];
       bI[t][p] = (__bf16)((16 * t + 8 * g + p) == c ? 1.0f : 0.0f);
     }
+    m1[t] = make_uint4(mw[0], mw[1], mw[2], mw[3]);
+  }
   f32x16 T1 = {}, T2 = {};
   constexpr int PF = NB < 2 ? NB : 2;
   const long long r0 = (long long)q * (4 * NB) + wv * NB;
@@ -2253,15 +2300,15 @@ __global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ i
         Dt = MFMA_BF16(a.m, bI[t], Dt);
         Dt = MFMA_BF16(a.h, bI[t], Dt);
       }
+      const float sw34 = sqrtf(fabsf(w34));
+      const unsigned nw = w34 < 0.f ? 0x80008000u : 0u;
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        float a8[8], b8[8];
+        float a8[8];
 #pragma unroll
-        for (int p = 0; p < 8; ++p) {
-          a8[p] = Dt[8 * t + p];
-          b8[p] = w34 * lam1v[8 * t + p] * a8[p];
-        }
-        T2 = mfma6(split8x3(a8), split8x3(b8), T2);
+        for (int p = 0; p < 8; ++p) a8[p] = sw34 * sl1v[8 * t + p] * Dt[8 * t + p];
+        const uint4 mk = make_uint4(m1[t].x ^ nw, m1[t].y ^ nw, m1[t].z ^ nw, m1[t].w ^ nw);
+        T2 = wgram6(a8, mk, T2);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
