@@ -136,41 +136,64 @@ __global__ void k_site_exp(const double* __restrict__ pmax, int n, int* __restri
 __device__ inline double site_scale(const int* sexp, int j) { return sexp ? ldexp(1.0, -sexp[j]) : 1.0; }
 __device__ inline double site_unscale2(const int* sexp, int j) { return sexp ? ldexp(1.0, 2 * sexp[j]) : 1.0; }
 
+// Layout builds, coalesced through LDS.  A site of degree 4 (3) is the
+// column-major matrix [2048 rows r = s + 2 b1 + 64 b2] x [1024 (32) columns,
+// col = b3 + 32 b4 (b3)].
+// k_lay_cols: block (s + 2 col) = [b1][b2] of every column (layout B of a
+//   degree-4 site, layout C of a degree-3 site); grid (columns, sites).
+// k_lay_rows: layout A, A[r][b3 32 + b4] = psi[r][b3 + 32 b4]; WG -> (b3, 64 rows),
+//   grid (32 x 32, sites).
+// Reads run down columns (64 or 2048 consecutive elements), writes along
+// rows of 32 or 1024 consecutive elements.
+template <typename TO>
+__device__ inline TO to_out(double v) {
+  if constexpr (std::is_same<TO, double>::value) return v;
+  else return (TO)(float)v;
+}
+
 template <typename TS, typename TO>
-__global__ void k_fast_layout(const TS* __restrict__ psi, const long long* __restrict__ src_off,
-                              const long long* __restrict__ dst_off, TO* __restrict__ outA,
-                              TO* __restrict__ outB, const int* __restrict__ sexp) {
-  const int it = blockIdx.y;
-  const TS* src = psi + src_off[it];
-  TO* A = outA + dst_off[it];
-  TO* B = outB + dst_off[it];
+__global__ __launch_bounds__(256) void k_lay_cols(const TS* __restrict__ psi, const long long* __restrict__ src_off,
+                                                  const long long* __restrict__ dst_off, TO* __restrict__ out,
+                                                  const int* __restrict__ sexp) {
+  __shared__ double T[2][FC][FC + 1];  // [s][b1][b2]
+  const int it = blockIdx.y, col = blockIdx.x;
+  const TS* src = psi + src_off[it] + (long long)col * (2 * FBLK);
+  TO* dst = out + dst_off[it] + (long long)(2 * col) * FBLK;
   const double sc = site_scale(sexp, it);
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC * FC;
-       n += gridDim.x * blockDim.x) {
-    const int s = n & 1;
-    const int b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = (n >> 11) & 31, b4 = (n >> 16) & 31;
-    const TO val = (TO)(float)(load_as<double>(src, n) * sc);
-    A[(long long)(s + 2 * b1 + 64 * b2) * FBLK + b3 * FC + b4] = val;
-    B[(long long)(s + 2 * b3 + 64 * b4) * FBLK + b1 * FC + b2] = val;
+  for (int r = threadIdx.x; r < 2 * FBLK; r += blockDim.x)
+    T[r & 1][(r >> 1) & 31][r >> 6] = load_as<double>(src, r) * sc;
+  __syncthreads();
+  for (int o = threadIdx.x; o < 2 * FBLK; o += blockDim.x)
+    dst[o] = to_out<TO>(T[o >> 10][(o >> 5) & 31][o & 31]);
+}
+
+template <typename TS, typename TO>
+__global__ __launch_bounds__(256) void k_lay_rows(const TS* __restrict__ psi, const long long* __restrict__ src_off,
+                                                  const long long* __restrict__ dst_off, TO* __restrict__ out,
+                                                  const int* __restrict__ sexp) {
+  __shared__ double T[64][FC + 1];  // [row in tile][b4]
+  const int it = blockIdx.y;
+  const int b3 = blockIdx.x & 31, r0 = (blockIdx.x >> 5) * 64;
+  const TS* src = psi + src_off[it];
+  TO* dst = out + dst_off[it];
+  const double sc = site_scale(sexp, it);
+  for (int k = threadIdx.x; k < 64 * FC; k += blockDim.x) {
+    const int rl = k & 63, b4 = k >> 6;
+    T[rl][b4] = load_as<double>(src, (long long)(r0 + rl) + (long long)(2 * FBLK) * (b3 + FC * b4)) * sc;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 64 * FC; k += blockDim.x) {
+    const int b4 = k & 31, rl = k >> 5;
+    dst[(long long)(r0 + rl) * FBLK + b3 * FC + b4] = to_out<TO>(T[rl][b4]);
   }
 }
 
-template <typename TS>
-__global__ void k_fast_layout64(const TS* __restrict__ psi, const long long* __restrict__ src_off,
-                                const long long* __restrict__ dst_off, double* __restrict__ outA,
-                                double* __restrict__ outB) {
-  const int it = blockIdx.y;
-  const TS* src = psi + src_off[it];
-  double* A = outA + dst_off[it];
-  double* B = outB + dst_off[it];
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC * FC;
-       n += gridDim.x * blockDim.x) {
-    const int s = n & 1;
-    const int b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = (n >> 11) & 31, b4 = (n >> 16) & 31;
-    const double val = load_as<double>(src, n);
-    A[(long long)(s + 2 * b1 + 64 * b2) * FBLK + b3 * FC + b4] = val;
-    B[(long long)(s + 2 * b3 + 64 * b4) * FBLK + b1 * FC + b2] = val;
-  }
+// layouts A and B of bulk sites (offsets src_off, dst_off; B follows A at +bstride)
+template <typename TS, typename TO>
+static void launch_layout_ab(hipStream_t stream, const TS* psi, const long long* src_off, const long long* dst_off,
+                             TO* A, TO* B, int n, const int* sexp) {
+  hipLaunchKernelGGL((k_lay_rows<TS, TO>), dim3(32 * 32, n), dim3(256), 0, stream, psi, src_off, dst_off, A, sexp);
+  hipLaunchKernelGGL((k_lay_cols<TS, TO>), dim3(FC * FC, n), dim3(256), 0, stream, psi, src_off, dst_off, B, sexp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1803,22 +1826,6 @@ __global__ __launch_bounds__(256) void k_gram12_f64(const EigItem* __restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int D3_NBLK = 2 * FC;  // blocks per degree-3 site
 
-template <typename TS, typename TO>
-__global__ void k_fast_layout3(const TS* __restrict__ psi, const long long* __restrict__ src_off,
-                               const long long* __restrict__ dst_off, TO* __restrict__ out,
-                               const int* __restrict__ sexp) {
-  const int it = blockIdx.y;
-  const TS* src = psi + src_off[it];
-  TO* C = out + dst_off[it];
-  const double sc = site_scale(sexp, it);
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * FC * FC * FC; n += gridDim.x * blockDim.x) {
-    const int s = n & 1, b1 = (n >> 1) & 31, b2 = (n >> 6) & 31, b3 = n >> 11;
-    const double v = load_as<double>(src, n) * sc;
-    if constexpr (std::is_same<TO, double>::value) C[(long long)(s + 2 * b3) * FBLK + b1 * FC + b2] = v;
-    else C[(long long)(s + 2 * b3) * FBLK + b1 * FC + b2] = (TO)(float)v;
-  }
-}
-
 // grid (4, items), 256 threads: wave -> blocks 16 x + 4 wv + {0..3}
 template <typename TI>
 __global__ __launch_bounds__(256) void k_rot12_d3_bf16(const EigItem* __restrict__ items,
@@ -2993,7 +3000,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
       ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
       const long long* s3 = (const long long*)st->offs3.p;
       const long long* d3 = s3 + n3all;
-      dim3 g3(64, n3all);
+      dim3 g3(FC, n3all);
       if (!acc_f64) {
         ITNBP_RETCHK(ensure_buf(ctx, st->lay3_16, (size_t)n3all * per3 * 2));
         if (st->x3) ITNBP_RETCHK(ensure_buf(ctx, st->P3_32, (size_t)2 * n3all * per3 * 4));
@@ -3003,9 +3010,9 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
           __bf16* C = (__bf16*)st->lay3_16.p;
           const int* e3 = (const int*)st->sexp.p + nb;
           switch (net->dtype) {
-            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, __bf16>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C, e3); break;
-            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, __bf16>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C, e3); break;
-            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, __bf16>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C, e3); break;
+            case ITNBP_F64: hipLaunchKernelGGL((k_lay_cols<double, __bf16>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C, e3); break;
+            case ITNBP_F32: hipLaunchKernelGGL((k_lay_cols<float, __bf16>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C, e3); break;
+            default: hipLaunchKernelGGL((k_lay_cols<bf16_t, __bf16>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C, e3); break;
           }
           ITNBP_HIPCHK(ctx, hipGetLastError());
           st->lay3_16_ok = true;
@@ -3016,9 +3023,9 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
         if (!st->lay3_64_ok) {
           double* C = (double*)st->lay3_64.p;
           switch (net->dtype) {
-            case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout3<double, double>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C, nullptr); break;
-            case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout3<float, double>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C, nullptr); break;
-            default: hipLaunchKernelGGL((k_fast_layout3<bf16_t, double>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C, nullptr); break;
+            case ITNBP_F64: hipLaunchKernelGGL((k_lay_cols<double, double>), g3, dim3(256), 0, ctx->stream, (const double*)net->d_psi, s3, d3, C, (const int*)nullptr); break;
+            case ITNBP_F32: hipLaunchKernelGGL((k_lay_cols<float, double>), g3, dim3(256), 0, ctx->stream, (const float*)net->d_psi, s3, d3, C, (const int*)nullptr); break;
+            default: hipLaunchKernelGGL((k_lay_cols<bf16_t, double>), g3, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, s3, d3, C, (const int*)nullptr); break;
           }
           ITNBP_HIPCHK(ctx, hipGetLastError());
           st->lay3_64_ok = true;
@@ -3037,7 +3044,6 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   const long long* d_src = (const long long*)st->offs.p;
   const long long* d_dst = d_src + nb;
-  dim3 lgrid(256, nb);
   if (!acc_f64 && !st->lay16_ok && nb > 0) {
     ITNBP_RETCHK(ensure_buf(ctx, st->lay16, (size_t)2 * nb * per * 2));
     ITNBP_RETCHK(site_exponents(net, st, d_src, nb, per, 0));
@@ -3045,9 +3051,9 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     __bf16* B = A + nb * per;
     const int* e = (const int*)st->sexp.p;
     switch (net->dtype) {
-      case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout<double, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B, e); break;
-      case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout<float, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B, e); break;
-      default: hipLaunchKernelGGL((k_fast_layout<bf16_t, __bf16>), lgrid, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B, e); break;
+      case ITNBP_F64: launch_layout_ab(ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B, nb, e); break;
+      case ITNBP_F32: launch_layout_ab(ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B, nb, e); break;
+      default: launch_layout_ab(ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B, nb, e); break;
     }
     ITNBP_HIPCHK(ctx, hipGetLastError());
     st->lay16_ok = true;
@@ -3057,9 +3063,9 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
     double* A = (double*)st->lay64.p;
     double* B = A + nb * per;
     switch (net->dtype) {
-      case ITNBP_F64: hipLaunchKernelGGL((k_fast_layout64<double>), lgrid, dim3(256), 0, ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B); break;
-      case ITNBP_F32: hipLaunchKernelGGL((k_fast_layout64<float>), lgrid, dim3(256), 0, ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B); break;
-      default: hipLaunchKernelGGL((k_fast_layout64<bf16_t>), lgrid, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B); break;
+      case ITNBP_F64: launch_layout_ab(ctx->stream, (const double*)net->d_psi, d_src, d_dst, A, B, nb, (const int*)nullptr); break;
+      case ITNBP_F32: launch_layout_ab(ctx->stream, (const float*)net->d_psi, d_src, d_dst, A, B, nb, (const int*)nullptr); break;
+      default: launch_layout_ab(ctx->stream, (const bf16_t*)net->d_psi, d_src, d_dst, A, B, nb, (const int*)nullptr); break;
     }
     ITNBP_HIPCHK(ctx, hipGetLastError());
     st->lay64_ok = true;
