@@ -52,7 +52,9 @@ enum { ITNBP_SCHED_SEQUENTIAL = 0, ITNBP_SCHED_PARALLEL = 1 };
 /* arithmetic of the message contractions */
 enum {
   ITNBP_PREC_F64 = 0,         /* FP64 everywhere (the plain path) */
-  ITNBP_PREC_F32 = 1,         /* FP32 messages and arithmetic */
+  ITNBP_PREC_F32 = 1,         /* FP32 messages; FP32 arithmetic, except that with BF16
+                                 storage the chi = 32 pipelines run 16-bit MFMA
+                                 (FP16 intermediates, FP32 accumulation) */
   ITNBP_PREC_BF16_F64 = 2,    /* BF16 MFMA with FP32 accumulation until the
                                  residual < switch_tol, then FP64 refinement */
   ITNBP_PREC_BF16X3 = 3,      /* compensated BF16 MFMA: every rounded operand split
