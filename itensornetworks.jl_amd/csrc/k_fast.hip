@@ -2087,11 +2087,14 @@ __global__ __launch_bounds__(512) void k_rot12_x3(const FastItem* __restrict__ i
 // grid (EIG_NPART, items), 512 threads: WG -> s = q & 1, r1 = z in
 // [ROT_NZ (q >> 1), +ROT_NZ); per z the 32 R blocks (s + 2 (z + 32 w)) in two
 // halves, wave wv -> w = 16 h + 2 wv + {0, 1}; the next half's two blocks are in
-// flight while the current two are multiplied.
+// flight while the current two are multiplied.  Y goes to the FP32 staging
+// tile at column w ^ r3 (a swizzle that keeps the column writes, the row-wise
+// copy-out and the transposed read-back conflict-free); the wave reads its
+// block back transposed for T'3 instead of forming Y^T on the MFMA.
 __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ items, const float* __restrict__ R,
                                                   const float* __restrict__ U, const float* __restrict__ lam,
                                                   float* __restrict__ Q, float* __restrict__ part) {
-  __shared__ float S[FBLK * SX_LD];  // row (r4 + 32 r3), column r2
+  __shared__ float S[FBLK * SX_LD];  // row (r4 + 32 r3), column r2 ^ r3
   // sqrt|l3|, sqrt|l4| and the sign masks of l3, l4 for the K order of the Gram
   // operands (rows kperm(t, g, 0..7)), per (bond, t, g)
   __shared__ __attribute__((aligned(16))) float sq34[2][FC];
@@ -2163,15 +2166,19 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
                              cur[i][2 * t + 1].x, cur[i][2 * t + 1].y, cur[i][2 * t + 1].z, cur[i][2 * t + 1].w};
         V = mfma6(split8x3(x8), bU4[t], V);
       }
-      f32x16 Y = {}, YT = {};
+      f32x16 Y = {};
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         float v8[8];
 #pragma unroll
         for (int p = 0; p < 8; ++p) v8[p] = V[8 * t + p];
-        const bf3 bv = split8x3(v8);
-        Y = mfma6(aU3[t], bv, Y);
-        YT = mfma6(bv, aU3[t], YT);
+        Y = mfma6(aU3[t], split8x3(v8), Y);
+      }
+      // Y[r3][r4] (lane r4, rows r3 = xb) to the tile
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
+        S[(c + FC * xb) * SX_LD + (w ^ xb)] = Y[v];
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
@@ -2185,19 +2192,16 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
         m3.x ^= nw; m3.y ^= nw; m3.z ^= nw; m3.w ^= nw;
         m4.x ^= nw; m4.y ^= nw; m4.z ^= nw; m4.w ^= nw;
         float a8[8];
-        // T'3: weights w12 l4 over the K index r4 of YT
+        // T'3: weights w12 l4 over the K index r4 = kperm(t, g, p) of the transposed
+        // block, read back from the tile (lane r3 = c)
 #pragma unroll
-        for (int p = 0; p < 8; ++p) a8[p] = sw12 * s4[p] * YT[8 * t + p];
+        for (int p = 0; p < 8; ++p)
+          a8[p] = sw12 * s4[p] * S[(kperm(t, g, p) + FC * c) * SX_LD + (w ^ c)];
         T3 = wgram6(a8, m4, T3);
         // T'4: weights w12 l3 over the K index r3 of Y
 #pragma unroll
         for (int p = 0; p < 8; ++p) a8[p] = sw12 * s3[p] * Y[8 * t + p];
         T4 = wgram6(a8, m3, T4);
-      }
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
-        S[(c + FC * xb) * SX_LD + w] = Y[v];
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -2207,8 +2211,10 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
         const int o = k >> 3, pt = k & 7;
         const int yb = o & 31, xb = o >> 5;
         const long long qb = s + 2 * (xb + FC * yb);
-        const float* src = S + o * SX_LD + 4 * pt;
-        *reinterpret_cast<float4*>(Q + Qo + qb * FBLK + z * FC + 4 * pt) = make_float4(src[0], src[1], src[2], src[3]);
+        const float* src = S + o * SX_LD;
+        const int c0 = 4 * pt;
+        *reinterpret_cast<float4*>(Q + Qo + qb * FBLK + z * FC + c0) =
+            make_float4(src[c0 ^ xb], src[(c0 + 1) ^ xb], src[(c0 + 2) ^ xb], src[(c0 + 3) ^ xb]);
       }
       __syncthreads();
     }
