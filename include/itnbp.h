@@ -63,8 +63,11 @@ enum {
                                  messages; where the chi = 32 eigenbasis pipeline
                                  does not apply, the absorbed-pair passes with
                                  two-term splits or the FP32 kernels */
-  ITNBP_PREC_BF16X3_F64 = 4   /* ITNBP_PREC_BF16X3 until the residual < switch_tol,
+  ITNBP_PREC_BF16X3_F64 = 4,  /* ITNBP_PREC_BF16X3 until the residual < switch_tol,
                                  then FP64 refinement */
+  ITNBP_PREC_BF16_BF16X3 = 5  /* progressive: the ITNBP_PREC_F32 kernels until the
+                                 residual < switch_tol (or stops decreasing), then
+                                 ITNBP_PREC_BF16X3 */
 };
 
 typedef struct itnbp_ctx itnbp_ctx;

@@ -38,7 +38,7 @@ OPS = {
 }
 
 PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16_f64": L.PREC_BF16_F64, "bf16x3": L.PREC_BF16X3,
-              "bf16x3_f64": L.PREC_BF16X3_F64}
+              "bf16x3_f64": L.PREC_BF16X3_F64, "bf16_bf16x3": L.PREC_BF16_BF16X3}
 
 
 def op_matrix(op, d=2):
@@ -353,8 +353,9 @@ def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, v
     Extensions: ``res_tol`` stops on max ||new-old||/||new||; ``precision`` is
     "f64", "f32" (16-bit MFMA sweeps where the chi = 32 pipelines apply, FP32
     elsewhere), "bf16x3" (compensated 16-bit MFMA: operands split into three
-    BF16 terms, FP32 intermediates), or "bf16_f64" / "bf16x3_f64" (those sweeps
-    until the residual drops below ``switch_tol`` or stagnates, then FP64)."""
+    BF16 terms, FP32 intermediates), "bf16_f64" / "bf16x3_f64" (those sweeps
+    until the residual drops below ``switch_tol`` or stagnates, then FP64), or
+    "bf16_bf16x3" (the "f32" sweeps until then, followed by "bf16x3")."""
     if alg != "bp":
         raise ValueError(f"unknown update algorithm {alg!r}")
     if isinstance(maxiter, str):

@@ -145,6 +145,49 @@ def test_user_messages_pick_pipeline_and_match_oracle(net34, symmetric):
     assert err < 1e-12, err
 
 
+def test_progressive_precision_reaches_the_bf16x3_floor(net34, oracle_ref):
+    """Precision "bf16_bf16x3": one-term FP16 sweeps until the residual < 1e-2,
+    then three-term sweeps; after 40 sweeps the messages are within the
+    three-term flavour's bound of the oracle's fixed point."""
+    g, psi = net34
+    m3, m40 = oracle_ref
+    bpc = itnbp.initialize_cache(psi, storage="bf16")
+    out = itnbp.update(bpc, maxiter=40, res_tol=1e-9, edge_sequence="parallel", precision="bf16_bf16x3",
+                       switch_tol=1e-2)
+    path, lu = out.last_update["path"], out.last_update
+    assert path.startswith("mfma-bf16-eig-chi32+mfma-bf16x3-eig-chi32"), path
+    assert lu["iterations_low"] == lu["iterations"] >= 3
+    got = to_index(g, out.messages())
+    err = max(np.abs(got[e] - m40[e]).max() for e in m40)
+    print(f"progressive: {lu['iterations']} sweeps, residual {lu['max_residual']:.3g}, max message error {err:.3g}")
+    assert err < 1e-6, err
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_bf16x3_follows_message_symmetry(net34, symmetric):
+    """Precision "bf16x3" with user messages: symmetric ones run the three-term
+    eigenbasis kernels, a non-symmetric one the absorbed-pair passes with two-term
+    splits and FP32 intermediates. After two Jacobi sweeps on the same BF16 tensors
+    from random messages they are within 1e-6 (three terms, measured 1.2e-7) and
+    5e-6 (two terms, measured 1.2e-6) of the oracle."""
+    g, psi = net34
+    rng = np.random.default_rng(17 if symmetric else 18)
+    msgs = _random_messages(g, psi, rng, symmetric)
+    bpc = itnbp.initialize_cache(psi, storage="bf16")
+    for e, m in msgs.items():
+        bpc.set_message_(e, m)
+    out = itnbp.update(bpc, maxiter=2, edge_sequence="parallel", precision="bf16x3")
+    path = out.last_update["path"]
+    assert path.startswith("mfma-bf16x3-eig-chi32" if symmetric else "mfma-bf16x2-chi32"), path
+    og, opsi = oracle_graph(g), oracle_psi(bpc.tensornetwork())
+    start = {(g.index[a], g.index[b]): m for (a, b), m in msgs.items()}
+    ref, _, _, _ = O.bp_update(og, opsi, start, maxiter=2, schedule="parallel")
+    got = to_index(g, out.messages())
+    err = max(np.abs(got[e] - ref[e]).max() for e in ref)
+    print(f"symmetric={symmetric}: max message error {err:.3g}")
+    assert err < (1e-6 if symmetric else 5e-6), err
+
+
 @pytest.mark.parametrize("precision", ["f64", "bf16_f64", "bf16x3_f64"])
 def test_degree3_only_lattice(precision):
     """Heavy-hex lattice with chi = 32: degree-3 and degree-2 sites only, so the
