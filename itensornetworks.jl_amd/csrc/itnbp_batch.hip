@@ -66,6 +66,8 @@ extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_
   }
   std::fill(M->msg_set.begin(), M->msg_set.end(), 1);
   M->cur = 0;
+  // the per-vertex eigenbasis kernel needs symmetric messages in every instance
+  M->msgs_sym = std::all_of(nets, nets + n, [](const itnbp_net* t) { return t->msgs_sym; });
   const int acc_f64 = o->precision == ITNBP_PREC_F64 ? 1 : 0;
   void* buf[2];
   if (acc_f64) {
@@ -87,15 +89,28 @@ extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_
   MsgStat* d_stats = (MsgStat*)M->stats.p;
   std::vector<MsgStat> hs(2 * NE);
   TaskPlan plan;
-  bool replan = true;
+  bool replan = true, use_small = false;
+  int small_first = -1;  // vertex tasks of the first plan (reported in the path)
   ITNBP_HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int it = 1; it <= o->maxiter; ++it) {
     if (replan) {
+      // whole-vertex tasks (every chi <= 16, site in LDS) of the active instances on
+      // the per-vertex kernel, every other message on the generic plan; the
+      // statistics slot of a message is its merged id
+      std::vector<int> seq(2 * NE);
+      for (int m = 0; m < 2 * NE; ++m) seq[m] = m;
+      std::vector<char> handled(2 * NE, 0);
+      for (int i = 0; i < n; ++i)
+        if (!active[i])
+          std::fill(handled.begin() + 2 * ebase[i], handled.begin() + 2 * (ebase[i] + nets[i]->ne), 1);
+      ITNBP_RETCHK(small_prepare(M, seq, acc_f64, handled));
+      use_small = small_task_count(M) > 0;
+      if (small_first < 0) small_first = small_task_count(M);
       std::vector<OcTask> tasks;
       for (int i = 0; i < n; ++i) {
         if (!active[i]) continue;
         for (int m = 2 * ebase[i]; m < 2 * (ebase[i] + nets[i]->ne); ++m)
-          tasks.push_back(message_task(M, m, m, true, o->normalize));
+          if (!handled[m]) tasks.push_back(message_task(M, m, m, true, o->normalize));
       }
       plan = TaskPlan();
       ITNBP_RETCHK(plan_tasks(M, tasks, acc_f64, plan));
@@ -106,6 +121,7 @@ extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_
     void* nxt = buf[1 - M->cur];
     // inactive instances keep their messages: carry the whole buffer over first
     ITNBP_HIPCHK(ctx, hipMemcpyAsync(nxt, cur, M->msg_total * mb, hipMemcpyDeviceToDevice, ctx->stream));
+    if (use_small) ITNBP_RETCHK(small_sweep(M, acc_f64, M->d_psi, dtype, cur, nxt, d_stats, o->normalize));
     ITNBP_RETCHK(launch_plan(M, plan, 0, plan.groups.size(), (const OcTask*)M->tasks.p, M->d_psi, dtype, cur, nxt,
                              d_stats));
     M->cur = 1 - M->cur;
@@ -154,6 +170,6 @@ extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_
     if (res) res[i] = out[i];
   }
   ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  ctx->path = "batched-generic";
+  ctx->path = small_first > 0 ? "batched-generic [vertex " + std::to_string(small_first) + "]" : "batched-generic";
   return ITNBP_OK;
 }

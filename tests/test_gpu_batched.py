@@ -35,3 +35,29 @@ def test_batched_matches_per_instance_oracle(dtype, prec, tol):
         assert max(np.abs(got[e] - ref[e]).max() for e in ref) < tol
     # instances converge at different sweeps and drop out
     assert len({r["iterations"] for r in res}) > 1
+    # small-chi sites on the per-vertex kernel, the chi = 16 degree-4 centre on the generic one
+    assert "[vertex " in res[0]["path"], res[0]["path"]
+
+
+def test_batched_with_a_non_symmetric_message_uses_generic_kernels():
+    """One non-symmetric user message in one instance keeps every instance of the
+    batch on the generic kernels (the per-vertex kernel needs symmetric messages);
+    the results still match the oracle."""
+    inst = instances(np.float64, seed=4)[:3]
+    caches = [itnbp.BeliefPropagationCache(psi) for _, psi in inst]
+    starts = []
+    rng = np.random.default_rng(9)
+    for k, (c, (g, psi)) in enumerate(zip(caches, inst)):
+        msgs = itnbp.identity_messages(psi)
+        if k == 1:
+            e = next(iter(msgs))
+            msgs[e] = msgs[e] + 0.1 * rng.standard_normal(msgs[e].shape)
+        c.set_messages_(msgs)
+        starts.append({(g.index[a], g.index[b]): np.asarray(m) for (a, b), m in msgs.items()})
+    res = itnbp.update_batched_(caches, maxiter=6, precision="f64")
+    assert "[vertex" not in res[0]["path"], res[0]["path"]
+    for c, r, (g, psi), start in zip(caches, res, inst, starts):
+        og, opsi = oracle_graph(g), oracle_psi(psi)
+        ref, _, _, _ = O.bp_update(og, opsi, start, maxiter=r["iterations"], schedule="parallel")
+        got = to_index(g, c.messages())
+        assert max(np.abs(got[e] - ref[e]).max() for e in ref) < 1e-11
