@@ -33,6 +33,7 @@ struct SmallTask {
   int pad;
   long long out[ITNBP_MAXK];  // outgoing message on bond axis a + 1
   int slot[ITNBP_MAXK];
+  long long yoff;             // k_vertex_big: offset of the site's scratch tensor
 };
 
 __device__ inline void wave_sync() {
@@ -308,11 +309,184 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
   }
 }
 
+// The same update for sites whose tensor does not fit in LDS (every chi <= 16,
+// e.g. chi = 12 or 16 at degree 4): psi is copied into a scratch tensor Y in
+// global memory (L2 / Infinity Cache resident while the workgroup works on it),
+// rotated there in place fibre by fibre, and every weighted Gram streams Y
+// through LDS in tiles of VB_FT fibres (base offsets and weights computed by the
+// loader).  Steps 2 and 5 are those of k_vertex_eig.
+constexpr int VB_FT = 256;  // fibres per Gram tile
+
+template <typename TS, typename TA, typename TM>
+__global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restrict__ vdesc,
+                                                          const SmallTask* __restrict__ tasks,
+                                                          const TS* __restrict__ psi, const TM* msg_in,
+                                                          TM* msg_out, MsgStat* __restrict__ stats, int normalize,
+                                                          TA* __restrict__ scratch) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* Us = reinterpret_cast<double*>(smem);
+  double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
+  double* lam = As + 4 * SV_LD * SV_LD;
+  double* rc = lam + ITNBP_MAXK * SV_MAXCHI;
+  double* rs = rc + 32;
+  int* rp = reinterpret_cast<int*>(rs + 32);
+  int* rq = rp + 32;
+  double* gpart = reinterpret_cast<double*>(rq + 32);
+  double* Tp = gpart + SV_THREADS;
+  double* X = Tp + SV_LD * SV_LD;
+  double* red = X + SV_LD * SV_LD;
+  double* tw = reinterpret_cast<double*>(smem + SV_FIXED);      // tile weights [VB_FT]
+  TA* tv8 = reinterpret_cast<TA*>(tw + VB_FT);                 // tile values [VB_FT][SV_LD]
+
+  __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
+  const SmallTask* tk = tasks + blockIdx.x;
+  const VDesc* V = vdesc + tk->v;
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+  const int k = V->k, D = V->D;
+  if (tid == 0) {
+    int s = 1;
+    for (int a = 0; a <= k; ++a) {
+      dims[a] = V->dims[a];
+      strides[a] = s;
+      s *= dims[a];
+    }
+  }
+  __syncthreads();
+  TA* Y = scratch + tk->yoff;
+  // 1. psi -> Y
+  const TS* src = psi + V->psi_off;
+  for (int i = tid; i < D; i += SV_THREADS) Y[i] = load_as<TA>(src, i);
+  // 2. eigendecompositions, one bond per wave
+  for (int a = wv; a < k; a += 4) {
+    const int n = dims[a + 1];
+    double* A = As + wv * SV_LD * SV_LD;
+    const TM* M = msg_in + V->in_msg[a + 1];
+    for (int o = l; o < n * n; o += 64) {
+      const int i = o / n, j = o % n;
+      A[i * SV_LD + j] = 0.5 * ((double)M[i + n * j] + (double)M[j + n * i]);
+    }
+    wave_sync();
+    wave_jacobi_small(A, Us + a * SV_LD * SV_LD, n, rc + 8 * wv, rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
+    if (l < n) lam[a * SV_MAXCHI + l] = A[l * SV_LD + l];
+    wave_sync();
+  }
+  __threadfence_block();
+  __syncthreads();
+  // 3. rotate every bond in place, one fibre per thread
+  for (int a = 0; a < k; ++a) {
+    const int n = dims[a + 1], st = strides[a + 1];
+    const int F = D / n;
+    const double* U = Us + a * SV_LD * SV_LD;
+    for (int f = tid; f < F; f += SV_THREADS) {
+      const int base = (f % st) + (f / st) * st * n;
+      TA x[SV_MAXCHI];
+#pragma unroll
+      for (int j = 0; j < SV_MAXCHI; ++j) x[j] = j < n ? Y[base + j * st] : (TA)0;
+      for (int r = 0; r < n; ++r) {
+        TA acc = 0;
+#pragma unroll
+        for (int j = 0; j < SV_MAXCHI; ++j)
+          if (j < n) acc += (TA)U[j * SV_LD + r] * x[j];
+        Y[base + r * st] = acc;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+  // 4-5. per bond: weighted Gram over tiles of fibres, rotate back, normalise, statistics
+  for (int a = 0; a < k; ++a) {
+    const int n = dims[a + 1], st = strides[a + 1], nn = n * n;
+    const int F = D / n;
+    const int r = tid % n, r2 = tid / n;
+    double acc = 0;
+    for (int f0 = 0; f0 < F; f0 += VB_FT) {
+      const int nf = min(VB_FT, F - f0);
+      // loader: thread j of the tile -> fibre f0 + j (values and weight)
+      if (tid < nf) {
+        const int f = f0 + tid;
+        const int base = (f % st) + (f / st) * st * n;
+        double w = 1.0;
+        for (int b = 0; b < k; ++b)
+          if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
+        tw[tid] = w;
+        for (int x = 0; x < n; ++x) tv8[tid * SV_LD + x] = Y[base + x * st];
+      }
+      __syncthreads();
+      if (tid < nn) {
+        double a1 = 0, a2 = 0, a3 = 0;
+        int j = 0;
+        for (; j + 3 < nf; j += 4) {
+          acc += tw[j] * (double)tv8[j * SV_LD + r] * (double)tv8[j * SV_LD + r2];
+          a1 += tw[j + 1] * (double)tv8[(j + 1) * SV_LD + r] * (double)tv8[(j + 1) * SV_LD + r2];
+          a2 += tw[j + 2] * (double)tv8[(j + 2) * SV_LD + r] * (double)tv8[(j + 2) * SV_LD + r2];
+          a3 += tw[j + 3] * (double)tv8[(j + 3) * SV_LD + r] * (double)tv8[(j + 3) * SV_LD + r2];
+        }
+        for (; j < nf; ++j) acc += tw[j] * (double)tv8[j * SV_LD + r] * (double)tv8[j * SV_LD + r2];
+        acc += (a1 + a2) + a3;
+      }
+      __syncthreads();
+    }
+    if (tid < nn) Tp[r * SV_LD + r2] = acc;  // T'[r][r']
+    __syncthreads();
+    const double* U = Us + a * SV_LD * SV_LD;
+    if (tid < nn) {
+      const int b = tid % n, rr = tid / n;
+      double s = 0;
+      for (int q = 0; q < n; ++q) s += U[b * SV_LD + q] * Tp[q * SV_LD + rr];
+      X[b * SV_LD + rr] = s;  // (U T')[b][r']
+    }
+    __syncthreads();
+    double tval = 0;
+    if (tid < nn) {
+      const int i = tid % n, j = tid / n;  // message layout M[i + n j]
+      double sij = 0, sji = 0;
+      for (int q = 0; q < n; ++q) {
+        sij += X[i * SV_LD + q] * U[j * SV_LD + q];
+        sji += X[j * SV_LD + q] * U[i * SV_LD + q];
+      }
+      tval = 0.5 * (sij + sji);
+    }
+    const double n2 = block_sum(tval * tval, red);
+    const double nrm = sqrt(n2);
+    const double sc = (normalize && nrm != 0.0) ? 1.0 / nrm : 1.0;
+    const long long off = tk->out[a];
+    double dot = 0, o2 = 0, d2 = 0, bad = 0;
+    if (tid < nn) {
+      const double nv = tval * sc;
+      const double ov = (double)msg_in[off + tid];
+      if (!isfinite(nv)) bad = 1;
+      dot = ov * nv;
+      o2 = ov * ov;
+      d2 = (nv - ov) * (nv - ov);
+      msg_out[off + tid] = (TM)nv;
+    }
+    dot = block_sum(dot, red);
+    o2 = block_sum(o2, red);
+    d2 = block_sum(d2, red);
+    bad = block_sum(bad, red);
+    const int slot = tk->slot[a];
+    if (tid == 0 && slot >= 0) {
+      const double nn2 = n2 * sc * sc;
+      MsgStat stt;
+      stt.diff = 1.0 - (dot * dot) / (o2 * nn2);
+      stt.res2 = nn2 > 0 ? d2 / nn2 : 0.0;
+      stt.norm = nrm;
+      stt.bad = (bad > 0 || !isfinite(nrm)) ? 1.0 : 0.0;
+      stats[slot] = stt;
+    }
+    __syncthreads();
+  }
+}
+
 struct SmallState {
   std::vector<SmallTask> tasks;
   DevBuf d_tasks;
   size_t lds = 0;
   int fmax = 0;  // largest fibre count over the tasks' bonds
+  // sites too large for LDS: k_vertex_big with scratch tensors
+  std::vector<SmallTask> big;
+  DevBuf d_big, ybuf;
+  long long ytotal = 0;  // scratch elements
 };
 
 SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net->small); }
@@ -320,13 +494,28 @@ SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net-
 template <typename TS, typename TA, typename TM>
 hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
                        MsgStat* stats, int normalize) {
-  auto kern = k_vertex_eig<TS, TA, TM>;
-  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3((unsigned)st->tasks.size()), dim3(SV_THREADS), st->lds, net->ctx->stream,
-                     (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, (const TS*)psi,
-                     (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax);
-  return hipGetLastError();
+  if (!st->tasks.empty()) {
+    auto kern = k_vertex_eig<TS, TA, TM>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((unsigned)st->tasks.size()), dim3(SV_THREADS), st->lds, net->ctx->stream,
+                       (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, (const TS*)psi,
+                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (!st->big.empty()) {
+    auto kern = k_vertex_big<TS, TA, TM>;
+    const size_t lds = SV_FIXED + (size_t)VB_FT * 8 + (size_t)VB_FT * SV_LD * sizeof(TA);
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((unsigned)st->big.size()), dim3(SV_THREADS), lds, net->ctx->stream,
+                       (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_big.p, (const TS*)psi,
+                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, (TA*)st->ybuf.p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace
@@ -344,6 +533,8 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
     net->small = st;
   }
   st->tasks.clear();
+  st->big.clear();
+  st->ytotal = 0;
   st->lds = 0;
   st->fmax = 0;
   if (!small_path_enabled(net)) return ITNBP_OK;
@@ -366,7 +557,7 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
     const int fmax = std::max(st->fmax, V.D / std::max(1, nmin));
     const int dmax = std::max(dmax_all, V.D);
     const size_t lds = SV_FIXED + (((size_t)fmax * 12 + 15) & ~(size_t)15) + (size_t)dmax * eb;
-    if (lds + 256 > 160 * 1024) continue;  // + the static LDS of the kernel
+    const bool fits = lds + 256 <= 160 * 1024;  // + the static LDS of the kernel
     SmallTask t{};
     t.v = v;
     for (int a = 1; a <= V.k && ok; ++a) {
@@ -381,10 +572,23 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
     }
     if (!ok) continue;
     for (int a = 0; a < V.k; ++a) handled[t.slot[a]] = 1;
-    st->tasks.push_back(t);
-    st->fmax = fmax;
-    dmax_all = dmax;
-    st->lds = lds;
+    if (fits) {
+      st->tasks.push_back(t);
+      st->fmax = fmax;
+      dmax_all = dmax;
+      st->lds = lds;
+    } else {
+      t.yoff = st->ytotal;
+      st->ytotal += V.D;
+      st->big.push_back(t);
+    }
+  }
+  if (!st->big.empty()) {
+    ITNBP_RETCHK(ensure_buf(ctx, st->d_big, st->big.size() * sizeof(SmallTask)));
+    ITNBP_RETCHK(ensure_buf(ctx, st->ybuf, (size_t)st->ytotal * eb));
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_big.p, st->big.data(), st->big.size() * sizeof(SmallTask),
+                                     hipMemcpyHostToDevice, ctx->stream));
+    ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   }
   if (!st->tasks.empty()) {
     ITNBP_RETCHK(ensure_buf(ctx, st->d_tasks, st->tasks.size() * sizeof(SmallTask)));
@@ -397,14 +601,14 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
 
 int small_task_count(itnbp_net* net) {
   SmallState* st = small_of(net);
-  return st ? (int)st->tasks.size() : 0;
+  return st ? (int)(st->tasks.size() + st->big.size()) : 0;
 }
 
 int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, void* msg_out,
                 MsgStat* stats, int normalize) {
   itnbp_ctx* ctx = net->ctx;
   SmallState* st = small_of(net);
-  if (!st || st->tasks.empty()) return ITNBP_OK;
+  if (!st || (st->tasks.empty() && st->big.empty())) return ITNBP_OK;
   const int e0 = prof_mark(ctx);
   hipError_t e;
   if (acc_f64) {
@@ -429,6 +633,8 @@ void release_small_state(itnbp_net* net) {
   SmallState* st = small_of(net);
   if (!st) return;
   free_buf(st->d_tasks);
+  free_buf(st->d_big);
+  free_buf(st->ybuf);
   delete st;
   net->small = nullptr;
 }
