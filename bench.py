@@ -321,11 +321,12 @@ def main():
                 pool[storage] = itnbp.initialize_cache(psi, storage=storage)
             c = pool[storage]
             p = kw.get("precision", "f64")
-            if p == "bf16_bf16x3":
+            if p in ("bf16_bf16x3", "bf16_bf16x3_f64"):
                 itnbp.update_(c, maxiter=1, edge_sequence="parallel", precision="f32")
-            low = {"bf16_f64": "f32", "bf16x3_f64": "bf16x3", "bf16_bf16x3": "bf16x3"}.get(p, p)
+            low = {"bf16_f64": "f32", "bf16x3_f64": "bf16x3", "bf16_bf16x3": "bf16x3",
+                   "bf16_bf16x3_f64": "bf16x3"}.get(p, p)
             itnbp.update_(c, maxiter=1, edge_sequence="parallel", precision=low)
-            if p in ("bf16_f64", "bf16x3_f64"):
+            if p in ("bf16_f64", "bf16x3_f64", "bf16_bf16x3_f64"):
                 itnbp.update_(c, maxiter=1, edge_sequence="parallel")
             c.ctx.check(c.lib.itnbp_init_identity_messages(c.h))
             ctx.check(c.lib.itnbp_synchronize(ctx.h))
@@ -352,6 +353,10 @@ def main():
         lu3 = mx3.last_update
         x_mx3 = ex(mx3)
         log(f"solve to 1e-10: bf16x3 then fp64 {t_mx3 * 1e3:.1f} ms")
+        m3p, t_m3p = timed_solve("bf16", precision="bf16_bf16x3_f64", switch_tol=1e-7)
+        lu3p = m3p.last_update
+        x_m3p, z_m3p = ex(m3p), np.array(list(itnbp.expect2_on_cache(m3p, zz).values()))
+        log(f"solve to 1e-10: fp16, bf16x3, fp64 {t_m3p * 1e3:.1f} ms")
         lowc, _ = timed_solve("bf16", precision="f32", res_tol=1e-4)
         x_low = ex(lowc)
         x3c, _ = timed_solve("bf16", precision="bf16x3", res_tol=1e-8)
@@ -389,6 +394,11 @@ def main():
                                  "seconds": t_mx3, "max_residual": lu3["max_residual"],
                                  "err_X_rel": float(np.linalg.norm(x_mx3 - x_ref) / np.linalg.norm(x_ref)),
                                  "speedup_time_to_1e-10": t_ref / t_mx3},
+            "fp16_bf16x3_fp64": {"sweeps": lu3p["iterations"], "sweeps_low": lu3p["iterations_low"],
+                                 "seconds": t_m3p, "max_residual": lu3p["max_residual"],
+                                 "err_X_rel": float(np.linalg.norm(x_m3p - x_ref) / np.linalg.norm(x_ref)),
+                                 "err_ZZ_rel": float(np.linalg.norm(z_m3p - z_ref) / np.linalg.norm(z_ref)),
+                                 "speedup_time_to_1e-10": t_ref / t_m3p},
             "err_X_max_abs": float(np.abs(x_mix - x_ref).max()),
             "err_X_rel": float(np.linalg.norm(x_mix - x_ref) / np.linalg.norm(x_ref)),
             "err_ZZ_max_abs": float(np.abs(z_mix - z_ref).max()),
@@ -397,7 +407,7 @@ def main():
             "bf16x3_floor_err_X_rel": float(np.linalg.norm(x_x3 - x_ref) / np.linalg.norm(x_ref)),
         }
         log(f"solve {solve}")
-        del ref, mix, mx3, lowc, x3c
+        del ref, mix, mx3, m3p, lowc, x3c
         pool.clear()
 
     # ---- secondary workloads: gradient time (8x8, chi=4, 40 sweeps) and the

@@ -65,9 +65,12 @@ enum {
                                  two-term splits or the FP32 kernels */
   ITNBP_PREC_BF16X3_F64 = 4,  /* ITNBP_PREC_BF16X3 until the residual < switch_tol,
                                  then FP64 refinement */
-  ITNBP_PREC_BF16_BF16X3 = 5  /* progressive: the ITNBP_PREC_F32 kernels until the
+  ITNBP_PREC_BF16_BF16X3 = 5, /* progressive: the ITNBP_PREC_F32 kernels until the
                                  residual < switch_tol (or stops decreasing), then
                                  ITNBP_PREC_BF16X3 */
+  ITNBP_PREC_BF16_BF16X3_F64 = 6 /* the ITNBP_PREC_F32 kernels until the residual
+                                 < 1e-2, ITNBP_PREC_BF16X3 until < switch_tol (or
+                                 stagnating), then FP64 refinement */
 };
 
 typedef struct itnbp_ctx itnbp_ctx;

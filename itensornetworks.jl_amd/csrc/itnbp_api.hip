@@ -403,7 +403,8 @@ int default_sequence(const itnbp_net* net, int schedule, std::vector<int>& seq);
 int check_sequence(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_opts* o) {
   std::vector<char> set(net->msg_set.begin(), net->msg_set.end());
   const bool need_old = o->tol >= 0 || o->res_tol >= 0 || o->precision == ITNBP_PREC_BF16_F64 ||
-                        o->precision == ITNBP_PREC_BF16X3_F64 || o->precision == ITNBP_PREC_BF16_BF16X3;
+                        o->precision == ITNBP_PREC_BF16X3_F64 || o->precision == ITNBP_PREC_BF16_BF16X3 ||
+                        o->precision == ITNBP_PREC_BF16_BF16X3_F64;
   for (int m : seq) {
     if (m < 0 || m >= 2 * net->ne) return set_err(net->ctx, ITNBP_ERR_ARG, "message id out of range in edge sequence");
     const int u = msg_src(net, m), v = msg_dst(net, m);
@@ -808,9 +809,10 @@ int itnbp_update(itnbp_net* net, const itnbp_update_opts* o, itnbp_update_result
       r.iterations = it;
     } else if (o->precision == ITNBP_PREC_F32 || o->precision == ITNBP_PREC_BF16_F64 ||
                o->precision == ITNBP_PREC_BF16X3 || o->precision == ITNBP_PREC_BF16X3_F64 ||
-               o->precision == ITNBP_PREC_BF16_BF16X3) {
+               o->precision == ITNBP_PREC_BF16_BF16X3 || o->precision == ITNBP_PREC_BF16_BF16X3_F64) {
       const bool x3 = o->precision == ITNBP_PREC_BF16X3 || o->precision == ITNBP_PREC_BF16X3_F64 ||
-                      o->precision == ITNBP_PREC_BF16_BF16X3;
+                      o->precision == ITNBP_PREC_BF16_BF16X3 || o->precision == ITNBP_PREC_BF16_BF16X3_F64;
+      const bool progressive = o->precision == ITNBP_PREC_BF16_BF16X3 || o->precision == ITNBP_PREC_BF16_BF16X3_F64;
       for (int i = 0; i < 2; ++i)
         if (!net->d_msg32[i]) HIPCHK(ctx, hipMalloc((void**)&net->d_msg32[i], std::max<size_t>(16, net->msg_total * 4)));
       HIPCHK(ctx, launch_convert_msgs(net->d_msg[net->cur], net->d_msg32[net->cur], net->msg_total, ctx->stream));
@@ -822,11 +824,15 @@ int itnbp_update(itnbp_net* net, const itnbp_update_opts* o, itnbp_update_result
       // (or stagnates), then the three-term kernels
       int it_pre = 0;
       ctx->path.clear();
-      if (o->precision == ITNBP_PREC_BF16_BF16X3) {
+      if (progressive) {
         net->low_x3 = 0;
         const bool fast0 = fast_path_available(net, 0, o->schedule);
         ctx->path = std::string(fast0 ? fast_path_name(net, 0) : "generic-f32") + "+";
-        const double pre2 = o->switch_tol > 0 ? o->switch_tol * o->switch_tol : 0.0;
+        // BF16_BF16X3: switch_tol ends the one-term phase; BF16_BF16X3_F64: the
+        // one-term phase ends at residual 1e-2 (two decades above its floor) and
+        // switch_tol ends the three-term phase
+        const double pre = o->precision == ITNBP_PREC_BF16_BF16X3 ? o->switch_tol : 1e-2;
+        const double pre2 = pre > 0 ? pre * pre : 0.0;
         const int rc_pre = sweep_phase(net, seq, o, ph, o->maxiter, pre2, false, &it_pre, &md, &mr, &conv, fast0);
         RETCHK(rc_pre);
         conv = 0;

@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("ITNBP_LIB", os.path.join(HERE, "libitnbp.so"))
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_STATE = range(7)
 F64, F32, BF16 = 0, 1, 2
 SCHED_SEQUENTIAL, SCHED_PARALLEL = 0, 1
-PREC_F64, PREC_F32, PREC_BF16_F64, PREC_BF16X3, PREC_BF16X3_F64, PREC_BF16_BF16X3 = 0, 1, 2, 3, 4, 5
+PREC_F64, PREC_F32, PREC_BF16_F64, PREC_BF16X3, PREC_BF16X3_F64, PREC_BF16_BF16X3, PREC_BF16_BF16X3_F64 = range(7)
 
 _ERRNAMES = {ERR_ARG: "invalid argument", ERR_HIP: "HIP error", ERR_NOMEM: "out of memory",
              ERR_NONFINITE: "non-finite result", ERR_UNSUPPORTED: "unsupported",

@@ -38,7 +38,8 @@ OPS = {
 }
 
 PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16_f64": L.PREC_BF16_F64, "bf16x3": L.PREC_BF16X3,
-              "bf16x3_f64": L.PREC_BF16X3_F64, "bf16_bf16x3": L.PREC_BF16_BF16X3}
+              "bf16x3_f64": L.PREC_BF16X3_F64, "bf16_bf16x3": L.PREC_BF16_BF16X3,
+              "bf16_bf16x3_f64": L.PREC_BF16_BF16X3_F64}
 
 
 def op_matrix(op, d=2):
@@ -355,7 +356,9 @@ def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, v
     elsewhere), "bf16x3" (compensated 16-bit MFMA: operands split into three
     BF16 terms, FP32 intermediates), "bf16_f64" / "bf16x3_f64" (those sweeps
     until the residual drops below ``switch_tol`` or stagnates, then FP64), or
-    "bf16_bf16x3" (the "f32" sweeps until then, followed by "bf16x3")."""
+    "bf16_bf16x3" (the "f32" sweeps until then, followed by "bf16x3"), or
+    "bf16_bf16x3_f64" ("f32" sweeps to residual 1e-2, "bf16x3" to ``switch_tol``,
+    then FP64)."""
     if alg != "bp":
         raise ValueError(f"unknown update algorithm {alg!r}")
     if isinstance(maxiter, str):

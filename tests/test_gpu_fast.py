@@ -163,6 +163,24 @@ def test_progressive_precision_reaches_the_bf16x3_floor(net34, oracle_ref):
     assert err < 1e-6, err
 
 
+def test_three_phase_precision_refines_to_fp64(net34, oracle_ref):
+    """Precision "bf16_bf16x3_f64": FP16 sweeps to residual 1e-2, three-term sweeps
+    to switch_tol, then FP64 to 1e-11; the messages match the oracle's fixed point
+    to 1e-10."""
+    g, psi = net34
+    m3, m40 = oracle_ref
+    bpc = itnbp.initialize_cache(psi, storage="bf16")
+    out = itnbp.update(bpc, maxiter=40, res_tol=1e-11, edge_sequence="parallel", precision="bf16_bf16x3_f64",
+                       switch_tol=1e-6)
+    path, lu = out.last_update["path"], out.last_update
+    assert path.startswith("mfma-bf16-eig-chi32+mfma-bf16x3-eig-chi32+mfma-f64-eig-chi32"), path
+    assert lu["converged"] and 2 <= lu["iterations_low"] < lu["iterations"]
+    got = to_index(g, out.messages())
+    err = max(np.abs(got[e] - m40[e]).max() for e in m40)
+    print(f"three-phase: {lu['iterations_low']} + {lu['iterations'] - lu['iterations_low']} sweeps, error {err:.3g}")
+    assert err < 1e-10, err
+
+
 @pytest.mark.parametrize("symmetric", [True, False])
 def test_bf16x3_follows_message_symmetry(net34, symmetric):
     """Precision "bf16x3" with user messages: symmetric ones run the three-term
