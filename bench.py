@@ -353,7 +353,7 @@ def main():
         lu3 = mx3.last_update
         x_mx3 = ex(mx3)
         log(f"solve to 1e-10: bf16x3 then fp64 {t_mx3 * 1e3:.1f} ms")
-        m3p, t_m3p = timed_solve("bf16", precision="bf16_bf16x3_f64", switch_tol=1e-7)
+        m3p, t_m3p = timed_solve("bf16", precision="bf16_bf16x3_f64", switch_tol=1e-6)
         lu3p = m3p.last_update
         x_m3p, z_m3p = ex(m3p), np.array(list(itnbp.expect2_on_cache(m3p, zz).values()))
         log(f"solve to 1e-10: fp16, bf16x3, fp64 {t_m3p * 1e3:.1f} ms")

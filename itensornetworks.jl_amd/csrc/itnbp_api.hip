@@ -642,6 +642,7 @@ int itnbp_net_destroy(itnbp_net* net) {
   free_buf(net->partials);
   free_buf(net->red);
   free_buf(net->aux);
+  free_buf(net->mdesc);
   release_fast_state(net);
   release_small_state(net);
   delete net;
@@ -755,15 +756,47 @@ int itnbp_message_is_set(itnbp_net* net, int m) {
   return net->msg_set[m] ? 1 : 0;
 }
 
+namespace {
+// every message M_m (chi_m x chi_m at offset off_m) <- identity; one workgroup per message
+__global__ void k_identity_msgs(double* __restrict__ msg, const long long* __restrict__ off,
+                                const int* __restrict__ chi, int nmsg) {
+  const int m = blockIdx.x;
+  if (m >= nmsg) return;
+  const int c = chi[m];
+  double* M = msg + off[m];
+  for (int t = threadIdx.x; t < c * c; t += blockDim.x) M[t] = (t % (c + 1) == 0) ? 1.0 : 0.0;
+}
+}  // namespace
+
+// identity_messages (quadraticformnetwork.jl:96-124), written on the device
 int itnbp_init_identity_messages(itnbp_net* net) {
   if (!net) return ITNBP_ERR_ARG;
   return guarded(net->ctx, [&]() -> int {
-    std::vector<double> h(net->msg_total, 0.0);
-    for (int m = 0; m < 2 * net->ne; ++m) {
-      const int c = net->chi[m / 2];
-      for (int i = 0; i < c; ++i) h[net->msg_off[m] + i + (long long)c * i] = 1.0;
+    itnbp_ctx* ctx = net->ctx;
+    const int nmsg = 2 * net->ne;
+    if (nmsg > 0) {
+      const size_t bytes = (size_t)nmsg * (sizeof(long long) + sizeof(int));
+      if (net->mdesc.bytes < bytes) {
+        std::vector<char> h(bytes);
+        long long* off = reinterpret_cast<long long*>(h.data());
+        int* chi = reinterpret_cast<int*>(off + nmsg);
+        for (int m = 0; m < nmsg; ++m) {
+          off[m] = net->msg_off[m];
+          chi[m] = net->chi[m / 2];
+        }
+        RETCHK(ensure_buf(ctx, net->mdesc, bytes));
+        HIPCHK(ctx, hipMemcpyAsync(net->mdesc.p, h.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      }
+      const long long* d_off = (const long long*)net->mdesc.p;
+      const int* d_chi = (const int*)(d_off + nmsg);
+      hipLaunchKernelGGL(k_identity_msgs, dim3(nmsg), dim3(256), 0, ctx->stream, net->d_msg[net->cur], d_off, d_chi,
+                         nmsg);
+      HIPCHK(ctx, hipGetLastError());
     }
-    return itnbp_set_messages(net, h.data());
+    std::fill(net->msg_set.begin(), net->msg_set.end(), 1);
+    net->msgs_sym = true;
+    return ITNBP_OK;
   });
 }
 

@@ -146,6 +146,7 @@ struct itnbp_net {
   int cur = 0;
   VDesc* d_vdesc = nullptr;
   DevBuf scratch, tasks, stats, partials, red, aux;
+  DevBuf mdesc;          // per message: offset (long long) and chi (int), for device-side initialisation
   void* fast = nullptr;  // state of the specialised pipeline (k_fast.hip)
   void* small = nullptr; // state of the per-vertex small-chi kernel (k_vertex.hip)
 };
