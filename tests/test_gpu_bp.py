@@ -212,10 +212,14 @@ def test_vertex_kernel_and_generic_path_with_user_messages(symmetric):
 def test_vertex_kernel_matches_generic_kernels(precision, monkeypatch):
     """The same Jacobi sweeps with and without the per-vertex kernels: a 5x5 grid at
     chi = 6 (degrees 2..4) and a 2x6 grid at chi = 16 (degrees 2..3) in LDS; the
-    degree-4 sites of a 3x4 grid at chi = 12 and a 3x3 grid at chi = 16 too large
-    for LDS (the global-memory twin, k_vertex_big)."""
+    degree-4 sites of a 3x4 grid at chi = 12 and a 3x3 grid at chi = 16 and the
+    degree-6 centre of a 3x3 triangular lattice at chi = 6, too large for LDS
+    (scratch tensors in global memory, k_vertex_big)."""
+    g0 = itnbp.named_grid((3, 3))
+    tri = itnbp.NamedGraph(g0.vertices, list(g0.edges) + [((i, j), (i + 1, j + 1)) for i in (1, 2) for j in (1, 2)])
+    assert max(tri.degree(v) for v in tri.vertices) == 6
     for g, chi in ((itnbp.named_grid((5, 5)), 6), (itnbp.named_grid((2, 6)), 16), (itnbp.named_grid((3, 4)), 12),
-                   (itnbp.named_grid((3, 3)), 16)):
+                   (itnbp.named_grid((3, 3)), 16), (tri, 6)):
         psi = itnbp.random_tensornetwork(np.random.default_rng(5), np.float64, g, link_space=chi)
         a = itnbp.update(itnbp.initialize_cache(psi), maxiter=6, edge_sequence="parallel", precision=precision)
         assert "[vertex" in a.last_update["path"]
