@@ -27,30 +27,38 @@ int check_all_incoming_set(itnbp_net* net, int v, int skip_e) {
   return ITNBP_OK;
 }
 
-// rho_v for the listed vertices into aux (d_v^2 each), offsets returned
+// rho_v for the listed vertices into aux (d_v^2 each), offsets returned.  Sites
+// of the chi = 32 eigenbasis pipeline take the rotated-Gram route, the others
+// the generic open contraction.
 int rdm1_device(itnbp_net* net, int n, const int* verts, std::vector<long long>& offs, long long& tot) {
-  std::vector<OcTask> tasks;
   tot = 0;
   offs.clear();
   for (int i = 0; i < n; ++i) {
     const int v = verts[i];
     if (v < 0 || v >= net->nv) return itnbp_fail(net->ctx, ITNBP_ERR_ARG, "vertex out of range");
     ITNBP_RETCHK(check_all_incoming_set(net, v, -1));
-    OcTask t;
-    t.v = v;
-    t.j = 0;
-    t.s_open = 1;
-    t.absorb_mask = all_bonds_mask(net, v);
-    t.out_off = tot;
-    t.old_off = -1;
-    t.normalize = 0;
-    t.slot = i;
     offs.push_back(tot);
     tot += (long long)net->phys[v] * net->phys[v];
-    tasks.push_back(t);
   }
   ITNBP_RETCHK(ensure_buf(net->ctx, net->aux, std::max<long long>(1, tot) * 8));
-  ITNBP_RETCHK(ensure_buf(net->ctx, net->stats, sizeof(MsgStat) * std::max(1, n)));
+  std::vector<char> done;
+  ITNBP_RETCHK(fast_site_rdm(net, n, verts, offs.data(), (double*)net->aux.p, done));
+  std::vector<OcTask> tasks;
+  for (int i = 0; i < n; ++i) {
+    if (done[i]) continue;
+    OcTask t;
+    t.v = verts[i];
+    t.j = 0;
+    t.s_open = 1;
+    t.absorb_mask = all_bonds_mask(net, t.v);
+    t.out_off = offs[i];
+    t.old_off = -1;
+    t.normalize = 0;
+    t.slot = (int)tasks.size();
+    tasks.push_back(t);
+  }
+  if (tasks.empty()) return ITNBP_OK;
+  ITNBP_RETCHK(ensure_buf(net->ctx, net->stats, sizeof(MsgStat) * std::max<size_t>(1, tasks.size())));
   return run_tasks(net, tasks, 1, net->d_psi, net->dtype, net->d_msg[net->cur], net->aux.p,
                    (MsgStat*)net->stats.p, net->tasks);
 }
@@ -201,9 +209,21 @@ int itnbp_region_scalars(itnbp_net* net, double* vs, double* es) {
   if (!net) return ITNBP_ERR_ARG;
   itnbp_ctx* ctx = net->ctx;
   if (vs) {
-    std::vector<OcTask> tasks;
+    // vertex scalar = tr rho_v: sites of the chi = 32 eigenbasis pipeline from
+    // their density matrices (aux[nv + 4 v]), the others as closed contractions
+    std::vector<int> all(net->nv);
+    std::vector<long long> roff(net->nv);
     for (int v = 0; v < net->nv; ++v) {
       ITNBP_RETCHK(check_all_incoming_set(net, v, -1));
+      all[v] = v;
+      roff[v] = net->nv + 4LL * v;
+    }
+    ITNBP_RETCHK(ensure_buf(ctx, net->aux, std::max(1, 5 * net->nv) * 8));
+    std::vector<char> done;
+    ITNBP_RETCHK(fast_site_rdm(net, net->nv, all.data(), roff.data(), (double*)net->aux.p, done));
+    std::vector<OcTask> tasks;
+    for (int v = 0; v < net->nv; ++v) {
+      if (done[v]) continue;
       OcTask t;
       t.v = v;
       t.j = 0;
@@ -212,15 +232,18 @@ int itnbp_region_scalars(itnbp_net* net, double* vs, double* es) {
       t.out_off = v;
       t.old_off = -1;
       t.normalize = 0;
-      t.slot = v;
+      t.slot = (int)tasks.size();
       tasks.push_back(t);
     }
-    ITNBP_RETCHK(ensure_buf(ctx, net->aux, std::max(1, net->nv) * 8));
-    ITNBP_RETCHK(ensure_buf(ctx, net->stats, sizeof(MsgStat) * std::max(1, net->nv)));
-    ITNBP_RETCHK(run_tasks(net, tasks, 1, net->d_psi, net->dtype, net->d_msg[net->cur], net->aux.p,
-                           (MsgStat*)net->stats.p, net->tasks));
-    ITNBP_HIPCHK(ctx, hipMemcpyAsync(vs, net->aux.p, net->nv * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (!tasks.empty()) {
+      ITNBP_RETCHK(ensure_buf(ctx, net->stats, sizeof(MsgStat) * tasks.size()));
+      ITNBP_RETCHK(run_tasks(net, tasks, 1, net->d_psi, net->dtype, net->d_msg[net->cur], net->aux.p,
+                             (MsgStat*)net->stats.p, net->tasks));
+    }
+    std::vector<double> h(5 * (size_t)net->nv);
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(h.data(), net->aux.p, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
     ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int v = 0; v < net->nv; ++v) vs[v] = done[v] ? h[net->nv + 4LL * v] + h[net->nv + 4LL * v + 3] : h[v];
   }
   if (es && net->ne > 0) {
     for (int m = 0; m < 2 * net->ne; ++m)

@@ -39,6 +39,10 @@ void release_fast_state(itnbp_net* net);
 // items per launch pair chosen by the autotuner (0 before the first sweep)
 int fast_chunk(itnbp_net* net, int acc_f64);
 void invalidate_fast_state(itnbp_net* net);
+// one-site density matrices of eligible vertices on the FP64 eigenbasis
+// pipeline (rho of request i at out + offs[i]); done[i] = 1 for those served
+int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs, double* out,
+                  std::vector<char>& done);
 
 // ---- per-vertex small-chi kernel (k_vertex.hip) ----------------------------
 // symmetric messages and not disabled (ITNBP_DISABLE_VERTEX=1)

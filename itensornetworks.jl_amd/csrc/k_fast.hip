@@ -2660,6 +2660,96 @@ __global__ __launch_bounds__(256) void k_eig_finalize(const EigItem* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// One-site reduced density matrices (environment of a single vertex contracted
+// with its ket and bra, src/caches/beliefpropagationcache.jl:100-105 followed by
+// the open-index contraction of src/expect.jl:5-19).  With every incoming
+// message symmetric, M_c = U_c diag(lam_c) U_c^T and
+//   rho[s, s'] = sum_r prod_c lam_c(r_c) Y[s, r] Y[s', r],
+// Y = psi rotated into the eigenbasis of every incoming message: the Q buffer of
+// the eigenbasis pipeline.  Pair p of Q blocks holds s = 0, 1 of the same
+// (r3, r4) = (p & 31, p >> 5) (bulk sites, 1024 pairs) or r3 = p (degree-3 sites,
+// 32 pairs, lam_4 = a table of ones).  FP64 throughout.
+// ---------------------------------------------------------------------------
+constexpr int RHO_NPART = 16;  // partial sums per site (workgroups)
+
+// grid (RHO_NPART, items), 256 threads: wave gw = 4 q + wv takes pairs gw,
+// gw + 64, ...; lane l holds elements o = 2 l + 128 j (j < 8) of a block:
+// r1 = o >> 5, r2 = (o & 31) + {0, 1}.  rpart[item][q][3] = (rho00, rho01, rho11).
+__global__ __launch_bounds__(256) void k_rho_sgram_f64(const EigItem* __restrict__ items,
+                                                       const double* __restrict__ Q,
+                                                       const double* __restrict__ lam, int npair,
+                                                       double* __restrict__ rpart) {
+  __shared__ double red[4][3];
+  const EigItem it = items[blockIdx.y];
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double w12[8][2];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int o = 2 * l + 128 * j, r1 = o >> 5, r2 = o & 31;
+    const double l1 = lam[it.L[0] + r1];
+    w12[j][0] = l1 * lam[it.L[1] + r2];
+    w12[j][1] = l1 * lam[it.L[1] + r2 + 1];
+  }
+  double a00 = 0, a01 = 0, a11 = 0;
+#pragma unroll 1
+  for (int p = 4 * q + wv; p < npair; p += 4 * RHO_NPART) {
+    const double w34 = lam[it.L[2] + (p & 31)] * lam[it.L[3] + (p >> 5)];
+    const double* B0 = Q + it.Q + (long long)(2 * p) * FBLK;
+    const double* B1 = B0 + FBLK;
+    double2 x[8], y[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      x[j] = *reinterpret_cast<const double2*>(B0 + 2 * l + 128 * j);
+      y[j] = *reinterpret_cast<const double2*>(B1 + 2 * l + 128 * j);
+    }
+    double c00 = 0, c01 = 0, c11 = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const double xa = w12[j][0] * x[j].x, xb = w12[j][1] * x[j].y;
+      c00 += xa * x[j].x + xb * x[j].y;
+      c01 += xa * y[j].x + xb * y[j].y;
+      c11 += w12[j][0] * y[j].x * y[j].x + w12[j][1] * y[j].y * y[j].y;
+    }
+    a00 += w34 * c00;
+    a01 += w34 * c01;
+    a11 += w34 * c11;
+  }
+#pragma unroll
+  for (int sh = 32; sh >= 1; sh >>= 1) {
+    a00 += __shfl_xor(a00, sh);
+    a01 += __shfl_xor(a01, sh);
+    a11 += __shfl_xor(a11, sh);
+  }
+  if (l == 0) {
+    red[wv][0] = a00;
+    red[wv][1] = a01;
+    red[wv][2] = a11;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    rpart[((long long)blockIdx.y * RHO_NPART + q) * 3 + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+  }
+}
+
+// request i: rho (2 x 2, column-major rho[s + 2 s']) of site item[i] at out + off[i]
+__global__ void k_rho_finalize(const double* __restrict__ rpart, const int* __restrict__ item,
+                               const long long* __restrict__ off, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = rpart + (long long)item[i] * RHO_NPART * 3;
+  double a[3] = {0, 0, 0};
+  for (int q = 0; q < RHO_NPART; ++q)
+    for (int k = 0; k < 3; ++k) a[k] += p[3 * q + k];
+  double* r = out + off[i];
+  r[0] = a[0];
+  r[1] = a[1];
+  r[2] = a[1];
+  r[3] = a[2];
+}
+
+// ---------------------------------------------------------------------------
 // host state
 // ---------------------------------------------------------------------------
 struct FastState {
@@ -2691,6 +2781,8 @@ struct FastState {
   // power-of-two exponents of the 16-bit layouts: bulk site bi at bi, degree-3
   // site k (index into deg3) at bulk.size() + k
   DevBuf sexp, pmax;
+  // one-site density matrices: item lists, partial sums and request tables
+  DevBuf rdm;
 };
 
 FastState* state_of(itnbp_net* net) { return reinterpret_cast<FastState*>(net->fast); }
@@ -2789,6 +2881,7 @@ void release_fast_state(itnbp_net* net) {
   free_buf(st->offs3);
   free_buf(st->sexp);
   free_buf(st->pmax);
+  free_buf(st->rdm);
   delete st;
   net->fast = nullptr;
 }
@@ -3389,5 +3482,119 @@ int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, M
   }
   prof_add(ctx, "finalize", e2, prof_mark(ctx));
   ITNBP_HIPCHK(ctx, hipGetLastError());
+  return ITNBP_OK;
+}
+
+// One-site density matrices of the listed vertices on the FP64 eigenbasis
+// pipeline: every message symmetric and set, at least ITNBP_RDM_FAST_MIN
+// (default 16) eligible vertices among the requests (the eigendecomposition of
+// every message into an eligible site is a fixed cost).  rho of request i
+// (unnormalised, column-major 2 x 2) goes to out + offs[i]; done[i] = 1 for the
+// requests served here, the others are left to the generic kernels.
+int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs, double* out,
+                  std::vector<char>& done) {
+  itnbp_ctx* ctx = net->ctx;
+  done.assign(n, 0);
+  if (n <= 0 || !use_eig(net) || getenv("ITNBP_DISABLE_FAST")) return ITNBP_OK;
+  for (int m = 0; m < 2 * net->ne; ++m)
+    if (!net->msg_set[m]) return ITNBP_OK;
+  int min_sites = 16;
+  if (const char* e = getenv("ITNBP_RDM_FAST_MIN")) min_sites = std::max(1, atoi(e));
+  int cand = 0;
+  for (int i = 0; i < n; ++i) {
+    const int v = verts[i];
+    if (v >= 0 && v < net->nv && (eligible(net, v) || eligible3(net, v))) ++cand;
+  }
+  if (cand < min_sites) return ITNBP_OK;
+  std::vector<int> seq(2 * net->ne);
+  for (int m = 0; m < 2 * net->ne; ++m) seq[m] = m;
+  std::vector<char> handled;
+  const int fr = fast_prepare(net, seq, 1, handled);
+  if (fr == ITNBP_ERR_UNSUPPORTED) return ITNBP_OK;
+  ITNBP_RETCHK(fr);
+  FastState* st = state_of(net);
+  if (!st->eig || st->eig_moff.empty()) return ITNBP_OK;
+  std::vector<int> bidx(net->nv, -1), tidx(net->nv, -1);
+  for (size_t bi = 0; bi < st->bulk.size(); ++bi) bidx[st->bulk[bi]] = (int)bi;
+  for (size_t j = 0; j < st->deg3_used.size(); ++j) tidx[st->deg3_used[j]] = (int)j;
+  // unique sites: bulk items first, then degree-3 items
+  std::vector<int> sb, s3, which(net->nv, -1);
+  for (int i = 0; i < n; ++i) {
+    const int v = verts[i];
+    if (v < 0 || v >= net->nv || which[v] >= 0) continue;
+    if (bidx[v] >= 0) {
+      which[v] = 0;
+      sb.push_back(v);
+    } else if (tidx[v] >= 0) {
+      which[v] = 0;
+      s3.push_back(v);
+    }
+  }
+  const int nbs = (int)sb.size(), n3s = (int)s3.size();
+  if (nbs + n3s == 0) return ITNBP_OK;
+  std::vector<EigItem> items;  // [rot34 items nbs][rot12 items nbs][degree-3 items n3s]
+  for (int k = 0; k < nbs; ++k) {
+    which[sb[k]] = k;
+    items.push_back(st->eitems[bidx[sb[k]]]);
+  }
+  for (int k = 0; k < nbs; ++k) items.push_back(st->k1eitems[bidx[sb[k]]]);
+  for (int k = 0; k < n3s; ++k) {
+    which[s3[k]] = nbs + k;
+    items.push_back(st->e3items[tidx[s3[k]]]);
+  }
+  std::vector<int> ritem;
+  std::vector<long long> roff;
+  for (int i = 0; i < n; ++i) {
+    const int v = verts[i];
+    if (v < 0 || v >= net->nv || (bidx[v] < 0 && tidx[v] < 0)) continue;
+    done[i] = 1;
+    ritem.push_back(which[v]);
+    roff.push_back(offs[i]);
+  }
+  const int nr = (int)ritem.size();
+  const size_t b_items = items.size() * sizeof(EigItem);
+  const size_t at_part = (b_items + 255) / 256 * 256;
+  const size_t at_off = at_part + (size_t)(nbs + n3s) * RHO_NPART * 3 * 8;
+  const size_t at_item = at_off + (size_t)nr * 8;
+  ITNBP_RETCHK(ensure_buf(ctx, st->rdm, at_item + (size_t)nr * 4));
+  char* base = (char*)st->rdm.p;
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(base, items.data(), b_items, hipMemcpyHostToDevice, ctx->stream));
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(base + at_off, roff.data(), nr * 8, hipMemcpyHostToDevice, ctx->stream));
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(base + at_item, ritem.data(), nr * 4, hipMemcpyHostToDevice, ctx->stream));
+  const EigItem* d_e = (const EigItem*)base;
+  const EigItem* d_k1 = d_e + nbs;
+  const EigItem* d_3 = d_e + 2 * nbs;
+  double* rpart = (double*)(base + at_part);
+  const int ne4 = (int)st->eig_moff.size();
+  const double* U = (const double*)st->eigU64.p;
+  const double* L = (const double*)st->eigL64.p;
+  const int e0 = prof_mark(ctx);
+  // cold start: the rotations must be orthogonal to rounding
+  hipLaunchKernelGGL((k_msg_eigh<double>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
+                     (const long long*)st->d_eig_moff.p, ne4, (const double*)net->d_msg[net->cur],
+                     (double*)st->eigU64.p, (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr, 0, 1e-28);
+  st->eig_warm = true;
+  if (nbs > 0) {
+    double* P = (double*)st->P64.p;
+    hipLaunchKernelGGL(k_rot34_f64<false>, dim3(EIG_NPART, nbs), dim3(256), 0, ctx->stream, d_k1,
+                       (const double*)st->lay64.p, U, L, P, (double*)nullptr);
+    hipLaunchKernelGGL(k_rot34_f64<false>, dim3(EIG_NPART, nbs), dim3(256), 0, ctx->stream, d_e, (const double*)P, U,
+                       L, P, (double*)nullptr);
+    hipLaunchKernelGGL(k_rho_sgram_f64, dim3(RHO_NPART, nbs), dim3(256), 0, ctx->stream, d_e, (const double*)P, L,
+                       FNR / 2, rpart);
+  }
+  if (n3s > 0) {
+    double* P3 = (double*)st->P3_64.p;
+    hipLaunchKernelGGL(k_rot12_d3_f64, dim3(4, n3s), dim3(256), 0, ctx->stream, d_3, (const double*)st->lay3_64.p, U,
+                       P3);
+    hipLaunchKernelGGL(k_rot3_d3_f64, dim3(EIG_NPART, n3s), dim3(256), 0, ctx->stream, d_3, (const double*)P3, U, L,
+                       P3, (double*)st->part64.p);
+    hipLaunchKernelGGL(k_rho_sgram_f64, dim3(RHO_NPART, n3s), dim3(256), 0, ctx->stream, d_3, (const double*)P3, L,
+                       D3_NBLK / 2, rpart + (size_t)nbs * RHO_NPART * 3);
+  }
+  hipLaunchKernelGGL(k_rho_finalize, dim3((nr + 255) / 256), dim3(256), 0, ctx->stream, (const double*)rpart,
+                     (const int*)(base + at_item), (const long long*)(base + at_off), nr, out);
+  ITNBP_HIPCHK(ctx, hipGetLastError());
+  prof_add(ctx, "rdm1_eig", e0, prof_mark(ctx));
   return ITNBP_OK;
 }

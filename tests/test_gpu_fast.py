@@ -241,3 +241,34 @@ def test_degree3_only_lattice(precision):
     got = to_index(g, out.messages())
     err = max(np.abs(got[e] - ref[e]).max() for e in ref)
     assert err < tol, err
+
+
+def test_site_rdm_eigenbasis_matches_oracle_and_generic(net34, monkeypatch):
+    """One-site density matrices, <X>, <Z>, vertex scalars and logscalar of a chi = 32
+    3x4 grid (two bulk, six degree-3 sites) through the rotated-Gram route
+    (rho = sum_r prod lam Y[s, r] Y[s', r] in the eigenbasis of every incoming
+    message) against the oracle's open contraction and the generic kernels."""
+    g, psi = net34
+    bpc = itnbp.update(itnbp.initialize_cache(psi), maxiter=4, edge_sequence="parallel")
+    og, opsi = oracle_graph(g), oracle_psi(psi)
+    ref = to_index(g, bpc.messages())
+    ctx = bpc.ctx
+    monkeypatch.setenv("ITNBP_RDM_FAST_MIN", "1")
+    ctx.profile(True)
+    for v in g.vertices:
+        r = itnbp.rdm(bpc, [v])
+        rr = O.rdm1(og, opsi, ref, g.index[v])
+        np.testing.assert_allclose(r, rr, rtol=1e-11, atol=1e-12 * np.abs(rr).max())
+    vs, _ = itnbp.region_scalars(bpc)
+    n_fast, _ = ctx.kernel_time("rdm1_eig")
+    ctx.profile(False)
+    assert n_fast == len(g.vertices) - 4 + 1  # every bulk and degree-3 site, then the scalars
+    for i in range(g.nv):
+        assert vs[i] == pytest.approx(O.vertex_scalar(og, opsi, ref, i), rel=1e-11)
+    assert itnbp.logscalar(bpc) == pytest.approx(O.logscalar(og, opsi, ref), rel=1e-11)
+    fast = {op: itnbp.expect_on_cache(bpc, op) for op in ("X", "Z")}
+    monkeypatch.setenv("ITNBP_DISABLE_FAST", "1")
+    for op in ("X", "Z"):
+        gen = itnbp.expect_on_cache(bpc, op)
+        err = max(abs(fast[op][v] - gen[v]) for v in g.vertices)
+        assert err < 1e-12, (op, err)
