@@ -142,9 +142,31 @@ int itnbp_rdm2(itnbp_net* net, int n, const int* edge_ids, double* out) {
     otot += (long long)net->phys[u] * net->phys[v] * net->phys[u] * net->phys[v];
   }
   ITNBP_RETCHK(ensure_buf(ctx, net->aux, (tot + otot) * 8));
-  ITNBP_RETCHK(ensure_buf(ctx, net->stats, sizeof(MsgStat) * 2 * n));
-  ITNBP_RETCHK(run_tasks(net, tasks, 1, net->d_psi, net->dtype, net->d_msg[net->cur], net->aux.p,
-                         (MsgStat*)net->stats.p, net->tasks));
+  // half-pairs of chi = 32 sites from the eigenbasis rotations, the rest as
+  // generic open contractions
+  {
+    std::vector<int> hs(2 * n), ha(2 * n);
+    std::vector<long long> ho(2 * n);
+    for (size_t k = 0; k < tasks.size(); ++k) {
+      hs[k] = tasks[k].v;
+      ha[k] = tasks[k].j;
+      ho[k] = tasks[k].out_off;
+    }
+    std::vector<char> done;
+    ITNBP_RETCHK(fast_pair_halves(net, 2 * n, hs.data(), ha.data(), ho.data(), (double*)net->aux.p, done));
+    std::vector<OcTask> rest;
+    for (size_t k = 0; k < tasks.size(); ++k)
+      if (!done[k]) {
+        rest.push_back(tasks[k]);
+        rest.back().slot = (int)rest.size() - 1;
+      }
+    tasks.swap(rest);
+  }
+  if (!tasks.empty()) {
+    ITNBP_RETCHK(ensure_buf(ctx, net->stats, sizeof(MsgStat) * tasks.size()));
+    ITNBP_RETCHK(run_tasks(net, tasks, 1, net->d_psi, net->dtype, net->d_msg[net->cur], net->aux.p,
+                           (MsgStat*)net->stats.p, net->tasks));
+  }
   // small index tables
   DevBuf tab;
   const size_t tb = n * (2 * sizeof(long long) + 3 * sizeof(int));

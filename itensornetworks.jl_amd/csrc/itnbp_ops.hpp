@@ -43,6 +43,10 @@ void invalidate_fast_state(itnbp_net* net);
 // pipeline (rho of request i at out + offs[i]); done[i] = 1 for those served
 int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs, double* out,
                   std::vector<char>& done);
+// half-pair environments (site, 1-based open axis) of eligible sites for
+// two-site RDMs, in the generic open-contraction layout at out + offs[i]
+int fast_pair_halves(itnbp_net* net, int n, const int* sites, const int* axes, const long long* offs, double* out,
+                     std::vector<char>& done);
 
 // ---- per-vertex small-chi kernel (k_vertex.hip) ----------------------------
 // symmetric messages and not disabled (ITNBP_DISABLE_VERTEX=1)

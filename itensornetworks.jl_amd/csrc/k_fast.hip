@@ -1334,6 +1334,11 @@ __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ 
 // With GRAM = false it is rot12: the same block geometry (input block
 // s + 2 (z + 32 w), output block s + 2 (x' + 32 y') at [z][w]) with U1, U2 in the
 // U[2], U[3] slots of its items, psi layout B as R and the R buffer as Q.
+// Staging tile row stride (16-bit elements): 72 bytes, so that the tile
+// (73.7 KB) lets two workgroups share a CU, and the row stride is 18 banks
+// (2-way conflicts on the column writes instead of 4-way at 80 bytes)
+constexpr int ROT_LD = 36;
+
 template <bool GRAM, typename TI>
 __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ items,
                                                     const TI* __restrict__ R,
@@ -1341,7 +1346,7 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
                                                     const float* __restrict__ lam,
                                                     TI* __restrict__ Q, float* __restrict__ part) {
   typedef typename Mma<TI>::v8 v8;
-  __shared__ __attribute__((aligned(16))) TI S[FBLK * ST_LD];
+  __shared__ __attribute__((aligned(16))) TI S[FBLK * ROT_LD];
   __shared__ __attribute__((aligned(16))) float lam34[2][FC];  // l3, l4
   const EigItem it = items[blockIdx.y];
   const int q = blockIdx.x;
@@ -1437,7 +1442,7 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int xb = 8 * (v >> 2) + 4 * g + (v & 3);
-        S[(c + FC * xb) * ST_LD + w] = (TI)Y[v];  // row (r4 + 32 r3), column r2
+        S[(c + FC * xb) * ROT_LD + w] = (TI)Y[v];  // row (r4 + 32 r3), column r2
       }
       __builtin_amdgcn_sched_barrier(0);  // keep the four blocks' chains apart (register pressure)
     }
@@ -1446,8 +1451,10 @@ __global__ __launch_bounds__(512) void k_rot34_bf16(const EigItem* __restrict__ 
       const int o = k >> 2, pt = k & 3;
       const int yb = o & 31, xb = o >> 5;
       const long long qb = s + 2 * (xb + FC * yb);
-      *reinterpret_cast<v8*>(Q + it.Q + qb * FBLK + z * FC + 8 * pt) =
-          *reinterpret_cast<const v8*>(S + o * ST_LD + 8 * pt);
+      // 8-byte aligned tile rows: two 8-byte reads, one 16-byte store
+      const uint2* src = reinterpret_cast<const uint2*>(S + o * ROT_LD + 8 * pt);
+      const uint2 lo = src[0], hi = src[1];
+      *reinterpret_cast<uint4*>(Q + it.Q + qb * FBLK + z * FC + 8 * pt) = make_uint4(lo.x, lo.y, hi.x, hi.y);
     }
     __syncthreads();
 #pragma unroll
@@ -2749,6 +2756,211 @@ __global__ void k_rho_finalize(const double* __restrict__ rpart, const int* __re
   r[3] = a[2];
 }
 
+// Half-pair environments for two-site RDMs (the pair environment of
+// test/test_belief_propagation.jl:64-91 with one bond open): for open bond c,
+//   G[(s, r_c), (s', r_c')] = sum over the other indices of prod lam * Y Y',
+// a 64 x 64 Gram of the rotated site with K running over every other index, then
+// A = (I x U_c) G (I x U_c)^T.  The K stream is cut into 32 x 32 tiles, two per
+// step (s = 0 and s = 1 of the same rows and K range):
+//   bulk, c = 0: tile (r3, r4) = Q block [r1][r2], rows r1, K = r2
+//   bulk, c = 1: the same block read transposed, rows r2, K = r1
+//   bulk, c = 2: tile (r4, m): row m of the blocks (r3, r4), rows r3, K = r2
+//   bulk, c = 3: tile (r3, m): row m of the blocks (r3, r4), rows r4, K = r2
+//   degree 3 (blocks s + 2 r3): c = 0, 1 as bulk with tile r3; c = 2 tile m, rows r3
+struct PairTask {
+  long long Q;     // rotated site: Q (bulk) or Q3 (degree-3) element offset
+  long long L[4];  // eigenvalue offsets of the site's bonds (L[3] = ones for degree 3)
+  long long U;     // eigenvectors of the incoming message on the open bond
+  long long out;   // output offset of the 64 x 64 half-pair
+  int c;           // open bond, 0-based
+  int deg3;
+};
+constexpr int PG_NPART = 16;  // workgroups (partial Grams) per task
+
+// grid (PG_NPART, tasks), 256 threads: wave gw = 4 q + wv takes tiles gw,
+// gw + 64, ...; each tile pair is staged in the wave's two LDS tiles (one tile
+// pair of look-ahead in registers).  part[task][q] = (G00, G01, G11), 32 x 32 each
+// (row-major [r][r']).
+__global__ __launch_bounds__(256) void k_pair_gram_f64(const PairTask* __restrict__ tasks,
+                                                       const double* __restrict__ Qbulk,
+                                                       const double* __restrict__ Q3,
+                                                       const double* __restrict__ lam,
+                                                       double* __restrict__ part) {
+  __shared__ double Tb[4][2][FC * LD8];
+  const PairTask tk = tasks[blockIdx.y];
+  const double* Q = (tk.deg3 ? Q3 : Qbulk) + tk.Q;
+  const int q = blockIdx.x;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lg = l >> 4;
+  const int c = tk.c;
+  const bool tr = c == 1;
+  const int ntile = tk.deg3 ? FC : FC * FC;
+  const long long kw = c == 1 ? tk.L[0] : tk.L[1];
+  double lamK[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) lamK[t] = lam[kw + 4 * t + lg];
+  // s = 0 tile of step t: element [a][b] (loaded orientation) at base + a rs + b
+  auto geom = [&](int t, long long& base, long long& rs, double& ws) {
+    const int lo = t & 31, hi = t >> 5;
+    if (tk.deg3) {
+      if (c <= 1) {
+        base = 2LL * t * FBLK;
+        rs = FC;
+        ws = lam[tk.L[2] + t];
+      } else {
+        base = (long long)FC * t;
+        rs = 2LL * FBLK;
+        ws = lam[tk.L[0] + t];
+      }
+    } else if (c <= 1) {
+      base = 2LL * (lo + FC * hi) * FBLK;
+      rs = FC;
+      ws = lam[tk.L[2] + lo] * lam[tk.L[3] + hi];
+    } else if (c == 2) {
+      base = 2LL * FC * lo * FBLK + FC * hi;
+      rs = 2LL * FBLK;
+      ws = lam[tk.L[0] + hi] * lam[tk.L[3] + lo];
+    } else {
+      base = 2LL * lo * FBLK + FC * hi;
+      rs = 2LL * FC * FBLK;
+      ws = lam[tk.L[0] + hi] * lam[tk.L[2] + lo];
+    }
+  };
+  Acc32 C00, C01, C11;
+  acc_zero(C00);
+  acc_zero(C01);
+  acc_zero(C11);
+  double* t0 = Tb[wv][0];
+  double* t1 = Tb[wv][1];
+  double2 n0[8], n1[8];
+  auto fetch = [&](int t) {
+    long long base, rs;
+    double ws;
+    geom(t, base, rs, ws);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int o = 2 * l + 128 * j, a = o >> 5, b = o & 31;
+      n0[j] = *reinterpret_cast<const double2*>(Q + base + a * rs + b);
+      n1[j] = *reinterpret_cast<const double2*>(Q + base + FBLK + a * rs + b);
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int o = 2 * l + 128 * j, a = o >> 5, b = o & 31;
+      t0[a * LD8 + b] = n0[j].x;
+      t0[a * LD8 + b + 1] = n0[j].y;
+      t1[a * LD8 + b] = n1[j].x;
+      t1[a * LD8 + b + 1] = n1[j].y;
+    }
+  };
+  const int t_first = 4 * q + wv, t_step = 4 * PG_NPART;
+  if (t_first < ntile) {
+    fetch(t_first);
+    stage();
+  }
+#pragma unroll 1
+  for (int t = t_first; t < ntile; t += t_step) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    const bool more = t + t_step < ntile;
+    if (more) fetch(t + t_step);
+    long long base, rs;
+    double ws;
+    geom(t, base, rs, ws);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      double a0[2], a1[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // operand (row 16h + li, K 4k + lg) of the tile in row/K orientation
+        const int ia = tr ? (4 * k + lg) * LD8 + 16 * h + li : (16 * h + li) * LD8 + 4 * k + lg;
+        a0[h] = t0[ia];
+        a1[h] = t1[ia];
+      }
+      const double w = ws * lamK[k];
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+          C00.t[mi][ni] = MFMA_F64(a0[mi], w * a0[ni], C00.t[mi][ni]);
+          C01.t[mi][ni] = MFMA_F64(a0[mi], w * a1[ni], C01.t[mi][ni]);
+          C11.t[mi][ni] = MFMA_F64(a1[mi], w * a1[ni], C11.t[mi][ni]);
+        }
+    }
+    // every lane's reads of this pair are issued before the tiles are overwritten
+    __builtin_amdgcn_wave_barrier();
+    if (more) stage();
+  }
+  __syncthreads();
+  double* Red = &Tb[0][0][0];  // 4 * 2 * 32 * 33 >= 3 * 1024
+  for (int w4 = 0; w4 < 4; ++w4) {
+    if (wv == w4) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int o = (16 * mi + 4 * v + lg) * FC + 16 * ni + li;
+            if (w4 == 0) {
+              Red[o] = C00.t[mi][ni][v];
+              Red[FBLK + o] = C01.t[mi][ni][v];
+              Red[2 * FBLK + o] = C11.t[mi][ni][v];
+            } else {
+              Red[o] += C00.t[mi][ni][v];
+              Red[FBLK + o] += C01.t[mi][ni][v];
+              Red[2 * FBLK + o] += C11.t[mi][ni][v];
+            }
+          }
+    }
+    __syncthreads();
+  }
+  double* dst = part + ((long long)blockIdx.y * PG_NPART + q) * 3 * FBLK;
+  for (int o = threadIdx.x; o < 3 * FBLK; o += blockDim.x) dst[o] = Red[o];
+}
+
+// Sum the partial Grams (fixed order), rotate back: X = G (I x U)^T, then
+// A = (I x U) X, written as A[(s + 2 b) + 64 (s' + 2 b')].  grid (tasks), 256 threads.
+__global__ __launch_bounds__(256) void k_pair_finalize(const PairTask* __restrict__ tasks,
+                                                       const double* __restrict__ part,
+                                                       const double* __restrict__ U64, double* __restrict__ out) {
+  __shared__ double G[3][FBLK];   // G00, G01, G11 as [r][r']
+  __shared__ double Us[FBLK];     // U[b + 32 r]
+  __shared__ double X[4][FBLK];   // X_{ss'}[r][b'] = sum_r' G_{ss'}[r][r'] U[b' + 32 r']
+  const PairTask tk = tasks[blockIdx.x];
+  const double* p = part + (long long)blockIdx.x * PG_NPART * 3 * FBLK;
+  for (int o = threadIdx.x; o < 3 * FBLK; o += blockDim.x) {
+    double a = 0;
+    for (int q = 0; q < PG_NPART; ++q) a += p[(long long)q * 3 * FBLK + o];
+    G[o / FBLK][o % FBLK] = a;
+  }
+  for (int o = threadIdx.x; o < FBLK; o += blockDim.x) Us[o] = U64[tk.U + o];
+  __syncthreads();
+  for (int o = threadIdx.x; o < 4 * FBLK; o += blockDim.x) {
+    const int ss = o >> 10, r = (o >> 5) & 31, b2 = o & 31;
+    const int sa = ss & 1, sb = ss >> 1;
+    double a = 0;
+    for (int r2 = 0; r2 < FC; ++r2) {
+      // G_{sa sb}[r][r2]; G10 = G01^T
+      const double g = (sa == sb) ? G[sa ? 2 : 0][r * FC + r2] : (sa == 0 ? G[1][r * FC + r2] : G[1][r2 * FC + r]);
+      a += g * Us[b2 + FC * r2];
+    }
+    X[ss][r * FC + b2] = a;
+  }
+  __syncthreads();
+  double* A = out + tk.out;
+  for (int o = threadIdx.x; o < 4 * FBLK; o += blockDim.x) {
+    // A row x = s + 2 b, column y = s' + 2 b'
+    const int x = o & 63, y = o >> 6;
+    const int sa = x & 1, b = x >> 1, sb = y & 1, b2 = y >> 1;
+    const double* Xs = X[sa + 2 * sb];
+    double a = 0;
+    for (int r = 0; r < FC; ++r) a += Us[b + FC * r] * Xs[r * FC + b2];
+    A[o] = a;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host state
 // ---------------------------------------------------------------------------
@@ -3485,26 +3697,29 @@ int fast_sweep(itnbp_net* net, int acc_f64, const void* msg_in, void* msg_out, M
   return ITNBP_OK;
 }
 
-// One-site density matrices of the listed vertices on the FP64 eigenbasis
-// pipeline: every message symmetric and set, at least ITNBP_RDM_FAST_MIN
-// (default 16) eligible vertices among the requests (the eigendecomposition of
-// every message into an eligible site is a fixed cost).  rho of request i
-// (unnormalised, column-major 2 x 2) goes to out + offs[i]; done[i] = 1 for the
-// requests served here, the others are left to the generic kernels.
-int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs, double* out,
-                  std::vector<char>& done) {
+// ---- density matrices on the FP64 eigenbasis rotations ----------------------
+// Every message symmetric and set, at least ITNBP_RDM_FAST_MIN (default 16)
+// eligible sites among the requests: the eigendecomposition of every message
+// into an eligible site is a fixed cost.
+struct RotSites {
+  std::vector<int> sb, s3;    // bulk and degree-3 sites rotated
+  std::vector<int> item;      // vertex -> index into [bulk items, degree-3 items], -1 if none
+  std::vector<EigItem> eitems;  // the rotated sites' items, bulk then degree-3
+  char* tail = nullptr;       // device scratch after the item lists
+};
+
+static int rotated_sites_setup(itnbp_net* net, const std::vector<int>& want, size_t tail_bytes, RotSites& rs,
+                               bool& ok) {
   itnbp_ctx* ctx = net->ctx;
-  done.assign(n, 0);
-  if (n <= 0 || !use_eig(net) || getenv("ITNBP_DISABLE_FAST")) return ITNBP_OK;
+  ok = false;
+  if (want.empty() || !use_eig(net) || getenv("ITNBP_DISABLE_FAST")) return ITNBP_OK;
   for (int m = 0; m < 2 * net->ne; ++m)
     if (!net->msg_set[m]) return ITNBP_OK;
   int min_sites = 16;
   if (const char* e = getenv("ITNBP_RDM_FAST_MIN")) min_sites = std::max(1, atoi(e));
   int cand = 0;
-  for (int i = 0; i < n; ++i) {
-    const int v = verts[i];
+  for (int v : want)
     if (v >= 0 && v < net->nv && (eligible(net, v) || eligible3(net, v))) ++cand;
-  }
   if (cand < min_sites) return ITNBP_OK;
   std::vector<int> seq(2 * net->ne);
   for (int m = 0; m < 2 * net->ne; ++m) seq[m] = m;
@@ -3517,58 +3732,42 @@ int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs
   std::vector<int> bidx(net->nv, -1), tidx(net->nv, -1);
   for (size_t bi = 0; bi < st->bulk.size(); ++bi) bidx[st->bulk[bi]] = (int)bi;
   for (size_t j = 0; j < st->deg3_used.size(); ++j) tidx[st->deg3_used[j]] = (int)j;
-  // unique sites: bulk items first, then degree-3 items
-  std::vector<int> sb, s3, which(net->nv, -1);
-  for (int i = 0; i < n; ++i) {
-    const int v = verts[i];
-    if (v < 0 || v >= net->nv || which[v] >= 0) continue;
-    if (bidx[v] >= 0) {
-      which[v] = 0;
-      sb.push_back(v);
-    } else if (tidx[v] >= 0) {
-      which[v] = 0;
-      s3.push_back(v);
-    }
+  rs.sb.clear();
+  rs.s3.clear();
+  rs.item.assign(net->nv, -1);
+  std::vector<char> seen(net->nv, 0);
+  for (int v : want) {
+    if (v < 0 || v >= net->nv || seen[v]) continue;
+    seen[v] = 1;
+    if (bidx[v] >= 0) rs.sb.push_back(v);
+    else if (tidx[v] >= 0) rs.s3.push_back(v);
   }
-  const int nbs = (int)sb.size(), n3s = (int)s3.size();
+  const int nbs = (int)rs.sb.size(), n3s = (int)rs.s3.size();
   if (nbs + n3s == 0) return ITNBP_OK;
-  std::vector<EigItem> items;  // [rot34 items nbs][rot12 items nbs][degree-3 items n3s]
+  // device lists: [rot34 items nbs][degree-3 items n3s][rot12 items nbs]
+  rs.eitems.clear();
   for (int k = 0; k < nbs; ++k) {
-    which[sb[k]] = k;
-    items.push_back(st->eitems[bidx[sb[k]]]);
+    rs.item[rs.sb[k]] = k;
+    rs.eitems.push_back(st->eitems[bidx[rs.sb[k]]]);
   }
-  for (int k = 0; k < nbs; ++k) items.push_back(st->k1eitems[bidx[sb[k]]]);
   for (int k = 0; k < n3s; ++k) {
-    which[s3[k]] = nbs + k;
-    items.push_back(st->e3items[tidx[s3[k]]]);
+    rs.item[rs.s3[k]] = nbs + k;
+    rs.eitems.push_back(st->e3items[tidx[rs.s3[k]]]);
   }
-  std::vector<int> ritem;
-  std::vector<long long> roff;
-  for (int i = 0; i < n; ++i) {
-    const int v = verts[i];
-    if (v < 0 || v >= net->nv || (bidx[v] < 0 && tidx[v] < 0)) continue;
-    done[i] = 1;
-    ritem.push_back(which[v]);
-    roff.push_back(offs[i]);
-  }
-  const int nr = (int)ritem.size();
-  const size_t b_items = items.size() * sizeof(EigItem);
-  const size_t at_part = (b_items + 255) / 256 * 256;
-  const size_t at_off = at_part + (size_t)(nbs + n3s) * RHO_NPART * 3 * 8;
-  const size_t at_item = at_off + (size_t)nr * 8;
-  ITNBP_RETCHK(ensure_buf(ctx, st->rdm, at_item + (size_t)nr * 4));
+  std::vector<EigItem> all = rs.eitems;
+  for (int k = 0; k < nbs; ++k) all.push_back(st->k1eitems[bidx[rs.sb[k]]]);
+  const size_t b_items = all.size() * sizeof(EigItem);
+  const size_t at_tail = (b_items + 255) / 256 * 256;
+  ITNBP_RETCHK(ensure_buf(ctx, st->rdm, at_tail + tail_bytes));
   char* base = (char*)st->rdm.p;
-  ITNBP_HIPCHK(ctx, hipMemcpyAsync(base, items.data(), b_items, hipMemcpyHostToDevice, ctx->stream));
-  ITNBP_HIPCHK(ctx, hipMemcpyAsync(base + at_off, roff.data(), nr * 8, hipMemcpyHostToDevice, ctx->stream));
-  ITNBP_HIPCHK(ctx, hipMemcpyAsync(base + at_item, ritem.data(), nr * 4, hipMemcpyHostToDevice, ctx->stream));
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(base, all.data(), b_items, hipMemcpyHostToDevice, ctx->stream));
+  rs.tail = base + at_tail;
   const EigItem* d_e = (const EigItem*)base;
-  const EigItem* d_k1 = d_e + nbs;
-  const EigItem* d_3 = d_e + 2 * nbs;
-  double* rpart = (double*)(base + at_part);
+  const EigItem* d_3 = d_e + nbs;
+  const EigItem* d_k1 = d_e + nbs + n3s;
   const int ne4 = (int)st->eig_moff.size();
   const double* U = (const double*)st->eigU64.p;
   const double* L = (const double*)st->eigL64.p;
-  const int e0 = prof_mark(ctx);
   // cold start: the rotations must be orthogonal to rounding
   hipLaunchKernelGGL((k_msg_eigh<double>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
                      (const long long*)st->d_eig_moff.p, ne4, (const double*)net->d_msg[net->cur],
@@ -3580,8 +3779,6 @@ int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs
                        (const double*)st->lay64.p, U, L, P, (double*)nullptr);
     hipLaunchKernelGGL(k_rot34_f64<false>, dim3(EIG_NPART, nbs), dim3(256), 0, ctx->stream, d_e, (const double*)P, U,
                        L, P, (double*)nullptr);
-    hipLaunchKernelGGL(k_rho_sgram_f64, dim3(RHO_NPART, nbs), dim3(256), 0, ctx->stream, d_e, (const double*)P, L,
-                       FNR / 2, rpart);
   }
   if (n3s > 0) {
     double* P3 = (double*)st->P3_64.p;
@@ -3589,12 +3786,106 @@ int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs
                        P3);
     hipLaunchKernelGGL(k_rot3_d3_f64, dim3(EIG_NPART, n3s), dim3(256), 0, ctx->stream, d_3, (const double*)P3, U, L,
                        P3, (double*)st->part64.p);
-    hipLaunchKernelGGL(k_rho_sgram_f64, dim3(RHO_NPART, n3s), dim3(256), 0, ctx->stream, d_3, (const double*)P3, L,
-                       D3_NBLK / 2, rpart + (size_t)nbs * RHO_NPART * 3);
   }
+  ITNBP_HIPCHK(ctx, hipGetLastError());
+  ok = true;
+  return ITNBP_OK;
+}
+
+// One-site density matrices of the listed vertices: rho of request i
+// (unnormalised, column-major 2 x 2) goes to out + offs[i]; done[i] = 1 for the
+// requests served here, the others are left to the generic kernels.
+int fast_site_rdm(itnbp_net* net, int n, const int* verts, const long long* offs, double* out,
+                  std::vector<char>& done) {
+  itnbp_ctx* ctx = net->ctx;
+  done.assign(n, 0);
+  if (n <= 0) return ITNBP_OK;
+  std::vector<int> want(verts, verts + n);
+  // requests served: eligible sites (item >= 0 after the setup)
+  const size_t tail = (size_t)n * (RHO_NPART * 3 * 8 + 8 + 4) + 256;
+  RotSites rs;
+  bool ok = false;
+  const int e0 = prof_mark(ctx);
+  ITNBP_RETCHK(rotated_sites_setup(net, want, tail, rs, ok));
+  if (!ok) return ITNBP_OK;
+  FastState* st = state_of(net);
+  const int nbs = (int)rs.sb.size(), n3s = (int)rs.s3.size();
+  std::vector<int> ritem;
+  std::vector<long long> roff;
+  for (int i = 0; i < n; ++i) {
+    const int v = verts[i];
+    if (v < 0 || v >= net->nv || rs.item[v] < 0) continue;
+    done[i] = 1;
+    ritem.push_back(rs.item[v]);
+    roff.push_back(offs[i]);
+  }
+  const int nr = (int)ritem.size();
+  double* rpart = (double*)rs.tail;
+  char* tb = rs.tail + (size_t)(nbs + n3s) * RHO_NPART * 3 * 8;
+  long long* d_off = (long long*)tb;
+  int* d_item = (int*)(tb + (size_t)nr * 8);
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(d_off, roff.data(), nr * 8, hipMemcpyHostToDevice, ctx->stream));
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(d_item, ritem.data(), nr * 4, hipMemcpyHostToDevice, ctx->stream));
+  const EigItem* d_e = (const EigItem*)st->rdm.p;
+  const double* L = (const double*)st->eigL64.p;
+  if (nbs > 0)
+    hipLaunchKernelGGL(k_rho_sgram_f64, dim3(RHO_NPART, nbs), dim3(256), 0, ctx->stream, d_e,
+                       (const double*)st->P64.p, L, FNR / 2, rpart);
+  if (n3s > 0)
+    hipLaunchKernelGGL(k_rho_sgram_f64, dim3(RHO_NPART, n3s), dim3(256), 0, ctx->stream, d_e + nbs,
+                       (const double*)st->P3_64.p, L, D3_NBLK / 2, rpart + (size_t)nbs * RHO_NPART * 3);
   hipLaunchKernelGGL(k_rho_finalize, dim3((nr + 255) / 256), dim3(256), 0, ctx->stream, (const double*)rpart,
-                     (const int*)(base + at_item), (const long long*)(base + at_off), nr, out);
+                     (const int*)d_item, (const long long*)d_off, nr, out);
   ITNBP_HIPCHK(ctx, hipGetLastError());
   prof_add(ctx, "rdm1_eig", e0, prof_mark(ctx));
+  return ITNBP_OK;
+}
+
+// Half-pair environments of (site, axis) requests for two-site RDMs:
+// A[(s + 2 b) + 64 (s' + 2 b')] = sum over the other bonds of psi psi' times their
+// messages, the layout of the generic open contraction with one open bond,
+// written at out + offs[i]; done[i] = 1 for the requests served here.
+int fast_pair_halves(itnbp_net* net, int n, const int* sites, const int* axes, const long long* offs, double* out,
+                     std::vector<char>& done) {
+  itnbp_ctx* ctx = net->ctx;
+  done.assign(n, 0);
+  if (n <= 0) return ITNBP_OK;
+  std::vector<int> want(sites, sites + n);
+  const size_t tail = (size_t)n * (sizeof(PairTask) + (size_t)PG_NPART * 3 * FBLK * 8) + 256;
+  RotSites rs;
+  bool ok = false;
+  const int e0 = prof_mark(ctx);
+  ITNBP_RETCHK(rotated_sites_setup(net, want, tail, rs, ok));
+  if (!ok) return ITNBP_OK;
+  FastState* st = state_of(net);
+  std::vector<PairTask> tasks;
+  for (int i = 0; i < n; ++i) {
+    const int v = sites[i];
+    if (v < 0 || v >= net->nv || rs.item[v] < 0) continue;
+    const EigItem& e = rs.eitems[rs.item[v]];
+    const int deg3 = rs.item[v] >= (int)rs.sb.size() ? 1 : 0;
+    const int c = axes[i] - 1;
+    if (c < 0 || c >= (deg3 ? 3 : 4)) continue;
+    PairTask t{};
+    t.Q = e.Q;
+    for (int k = 0; k < 4; ++k) t.L[k] = e.L[k];
+    t.U = e.U[c];
+    t.out = offs[i];
+    t.c = c;
+    t.deg3 = deg3;
+    tasks.push_back(t);
+    done[i] = 1;
+  }
+  const int nt = (int)tasks.size();
+  if (nt == 0) return ITNBP_OK;
+  PairTask* d_t = (PairTask*)rs.tail;
+  double* part = (double*)(rs.tail + ((size_t)nt * sizeof(PairTask) + 255) / 256 * 256);
+  ITNBP_HIPCHK(ctx, hipMemcpyAsync(d_t, tasks.data(), nt * sizeof(PairTask), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_pair_gram_f64, dim3(PG_NPART, nt), dim3(256), 0, ctx->stream, (const PairTask*)d_t,
+                     (const double*)st->P64.p, (const double*)st->P3_64.p, (const double*)st->eigL64.p, part);
+  hipLaunchKernelGGL(k_pair_finalize, dim3(nt), dim3(256), 0, ctx->stream, (const PairTask*)d_t, (const double*)part,
+                     (const double*)st->eigU64.p, out);
+  ITNBP_HIPCHK(ctx, hipGetLastError());
+  prof_add(ctx, "rdm2_eig", e0, prof_mark(ctx));
   return ITNBP_OK;
 }

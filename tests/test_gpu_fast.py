@@ -272,3 +272,30 @@ def test_site_rdm_eigenbasis_matches_oracle_and_generic(net34, monkeypatch):
         gen = itnbp.expect_on_cache(bpc, op)
         err = max(abs(fast[op][v] - gen[v]) for v in g.vertices)
         assert err < 1e-12, (op, err)
+
+
+def test_pair_rdm_eigenbasis_matches_oracle_and_generic(net34, monkeypatch):
+    """Two-site density matrices and <ZZ> of a chi = 32 3x4 grid through the
+    eigenbasis half-pairs (a 64 x 64 Gram of the rotated site over every index but
+    (s, open bond), rotated back by that bond's eigenvectors; corners stay on the
+    generic kernels) against the oracle and the generic kernels."""
+    g, psi = net34
+    bpc = itnbp.update(itnbp.initialize_cache(psi), maxiter=4, edge_sequence="parallel")
+    og, opsi = oracle_graph(g), oracle_psi(psi)
+    ref = to_index(g, bpc.messages())
+    ctx = bpc.ctx
+    monkeypatch.setenv("ITNBP_RDM_FAST_MIN", "1")
+    ctx.profile(True)
+    for (a, b) in g.edges:
+        r = itnbp.rdm(bpc, [a, b])
+        rr = O.rdm2(og, opsi, ref, g.index[a], g.index[b])
+        np.testing.assert_allclose(r, rr, rtol=1e-10, atol=1e-11 * np.abs(rr).max())
+    n_fast, _ = ctx.kernel_time("rdm2_eig")
+    ctx.profile(False)
+    assert n_fast == len(g.edges)
+    zz = O.two_site(O.Z, O.Z)
+    fast = itnbp.expect2_on_cache(bpc, zz)
+    monkeypatch.setenv("ITNBP_DISABLE_FAST", "1")
+    gen = itnbp.expect2_on_cache(bpc, zz)
+    err = max(abs(fast[e] - gen[e]) for e in gen)
+    assert err < 1e-12, err
