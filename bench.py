@@ -342,8 +342,21 @@ def main():
         zz = np.kron(np.diag([1.0, -1.0]), np.diag([1.0, -1.0]))
         ref, t_ref = timed_solve("f64")
         it_ref, res_ref = ref.last_update["iterations"], ref.last_update["max_residual"]
+        # observables on the converged cache (FP64): <X> on every site, <ZZ> on
+        # every bond, logscalar; wall time of each call, results copied to the host
+        ex(ref)
+        t = time.perf_counter()
         x_ref = ex(ref)
+        t_x = time.perf_counter() - t
+        t = time.perf_counter()
         z_ref = np.array(list(itnbp.expect2_on_cache(ref, zz).values()))
+        t_zz = time.perf_counter() - t
+        t = time.perf_counter()
+        itnbp.logscalar(ref)
+        t_ls = time.perf_counter() - t
+        observables = {"expect_X_ms": t_x * 1e3, "sites": g.nv, "expect_ZZ_ms": t_zz * 1e3, "bonds": g.ne,
+                       "logscalar_ms": t_ls * 1e3}
+        log(f"observables {observables}")
         mix, t_mix = timed_solve("bf16", precision="bf16_f64", switch_tol=1e-4)
         lu = mix.last_update
         x_mix = ex(mix)
@@ -382,6 +395,7 @@ def main():
         solve = {
             "fp64": {"sweeps": it_ref, "seconds": t_ref, "updates_per_s": nmsg * it_ref / t_ref,
                      "max_residual": res_ref},
+            "observables": observables,
             "to_err_X_1e-6": {"fp64": acc64, "chosen": accprog, "bf16x3": accx3, "fp16_then_fp64": accmix,
                               "speedup": (acc64["seconds"] / accprog["seconds"]) if acc64 and accprog else None,
                               "speedup_bf16x3": (acc64["seconds"] / accx3["seconds"]) if acc64 and accx3 else None,
