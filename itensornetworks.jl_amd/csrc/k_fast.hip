@@ -1975,15 +1975,37 @@ __device__ inline void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
   l = (__bf16)(r - (float)m);
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// the two BF16 halves of a packed pair, widened to FP32 (exact)
+__device__ inline f32x2 widen_bf16x2(bf16x2 v) {
+  const unsigned u = __builtin_bit_cast(unsigned, v);
+  f32x2 r;
+  r.x = __builtin_bit_cast(float, u << 16);
+  r.y = __builtin_bit_cast(float, u & 0xffff0000u);
+  return r;
+}
+
+// split3 of eight values, two at a time: each rounding is one packed
+// conversion whose result is already the operand's register pair
 __device__ inline bf3 split8x3(const float (&x)[8]) {
   bf3 o;
 #pragma unroll
-  for (int p = 0; p < 8; ++p) {
-    __bf16 h, m, l;
-    split3(x[p], h, m, l);
-    o.h[p] = h;
-    o.m[p] = m;
-    o.l[p] = l;
+  for (int p = 0; p < 8; p += 2) {
+    f32x2 v;
+    v.x = x[p];
+    v.y = x[p + 1];
+    const bf16x2 h = __builtin_convertvector(v, bf16x2);
+    const f32x2 r = v - widen_bf16x2(h);
+    const bf16x2 m = __builtin_convertvector(r, bf16x2);
+    const bf16x2 lo = __builtin_convertvector(r - widen_bf16x2(m), bf16x2);
+    o.h[p] = h.x;
+    o.h[p + 1] = h.y;
+    o.m[p] = m.x;
+    o.m[p + 1] = m.y;
+    o.l[p] = lo.x;
+    o.l[p + 1] = lo.y;
   }
   return o;
 }
