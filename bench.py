@@ -77,10 +77,10 @@ TRAFFIC_SOURCE = {k: "profiles/eig_pipeline_peps32x32_chi32.md (FETCH_SIZE + WRI
                   for k in MEASURED_TRAFFIC_PER_SITE}
 MEASURED_TRAFFIC_PER_SITE.update({
     "rot12_x3": (3657.6 + 7200.0) * MIB / 900, "rot34_x3": (7265.8 + 7312.5) * MIB / 900,
-    "gram12_x3": (7207.3 + 112.5) * MIB / 900,
+    "gram12_x3": (7207.4 + 112.5) * MIB / 900, "rot34_bf16": (3663.1 + 4086.4) * MIB / 900,
 })
 TRAFFIC_SOURCE.update({k: "profiles/x3_pipeline_peps32x32_chi32.md (FETCH_SIZE + WRITE_SIZE, calibrated)"
-                       for k in ("rot12_x3", "rot34_x3", "gram12_x3")})
+                       for k in ("rot12_x3", "rot34_x3", "gram12_x3", "rot34_bf16")})
 PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 PEAK_F64_TFLOPS = 78.6      # MI355X FP64
 PEAK_HBM_GBS = 8000.0
