@@ -198,6 +198,29 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
     const int n = dims[a + 1], st = strides[a + 1];
     const int F = D / n;
     const double* U = Us + a * SV_LD * SV_LD;
+    if (n <= 8) {
+      // U in registers (zero-padded to 8 x 8), read once per bond
+      TA u[8][8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[j][r] = (j < n && r < n) ? (TA)U[j * SV_LD + r] : (TA)0;
+      for (int f = tid; f < F; f += SV_THREADS) {
+        const int base = (f % st) + (f / st) * st * n;
+        TA x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = j < n ? Y[base + j * st] : (TA)0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          TA acc = 0;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) acc += u[j][r] * x[j];
+          if (r < n) Y[base + r * st] = acc;
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     for (int f = tid; f < F; f += SV_THREADS) {
       const int base = (f % st) + (f / st) * st * n;
       TA x[SV_MAXCHI];
@@ -227,10 +250,43 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
       fw[f] = w;
     }
     __syncthreads();
+    if (n <= 8) {
+      // lane (r, gq) of wave wv: row r = l & 7 of T' over the fibres of group
+      // wv * 8 + gq (gq = l >> 3), all n columns at once; fibre groups summed with
+      // shuffles inside the wave, then across the four waves
+      const int r = l & 7, gg = wv * 8 + (l >> 3);
+      double acc8[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc8[j] = 0;
+      if (r < n) {
+#pragma unroll 2
+        for (int f = gg; f < F; f += 32) {
+          const int b = fb[f];
+          const double wy = fw[f] * (double)Y[b + r * st];
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (j < n) acc8[j] += wy * (double)Y[b + j * st];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        for (int m = 8; m < 64; m <<= 1) acc8[j] += __shfl_xor(acc8[j], m);
+      if (l < 8) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) gpart[wv * 64 + l * 8 + j] = acc8[j];
+      }
+      __syncthreads();
+      if (tid < nn) {
+        const int rr = tid % n, r2 = tid / n;
+        Tp[rr * SV_LD + r2] = (gpart[rr * 8 + r2] + gpart[64 + rr * 8 + r2]) +
+                              (gpart[128 + rr * 8 + r2] + gpart[192 + rr * 8 + r2]);
+      }
+      __syncthreads();
+    }
     const int G = SV_THREADS / nn;  // fibre groups (>= 1 since n <= 16)
     const int g = tid / nn, e = tid % nn;
     double acc = 0;
-    if (g < G) {
+    if (n > 8 && g < G) {
       const int r = e % n, r2 = e / n;
       // four fibres per step, each with its own accumulator, so that their LDS
       // loads are in flight together
@@ -251,14 +307,16 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
       for (; f < F; f += G) acc += fw[f] * (double)Y[fb[f] + r * st] * (double)Y[fb[f] + r2 * st];
       acc += (a1 + a2) + a3;
     }
-    gpart[tid] = acc;
-    __syncthreads();
-    if (tid < nn) {
-      double t = 0;
-      for (int gg = 0; gg < G; ++gg) t += gpart[gg * nn + tid];
-      Tp[(tid % n) * SV_LD + tid / n] = t;  // T'[r][r']
+    if (n > 8) {
+      gpart[tid] = acc;
+      __syncthreads();
+      if (tid < nn) {
+        double t = 0;
+        for (int gg = 0; gg < G; ++gg) t += gpart[gg * nn + tid];
+        Tp[(tid % n) * SV_LD + tid / n] = t;  // T'[r][r']
+      }
+      __syncthreads();
     }
-    __syncthreads();
     const double* U = Us + a * SV_LD * SV_LD;
     if (tid < nn) {
       const int b = tid % n, rr = tid / n;
