@@ -259,8 +259,26 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
 #pragma unroll
       for (int j = 0; j < 8; ++j) acc8[j] = 0;
       if (r < n) {
-#pragma unroll 2
-        for (int f = gg; f < F; f += 32) {
+        // four fibres per step, their loads issued together
+        int f = gg;
+        for (; f + 96 < F; f += 128) {
+          int b[4];
+          double wy[4], y[4][8];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            b[i] = fb[f + 32 * i];
+            wy[i] = fw[f + 32 * i];
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) y[i][j] = j < n ? (double)Y[b[i] + j * st] : 0.0;
+            wy[i] *= (double)Y[b[i] + r * st];
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) acc8[j] += (wy[0] * y[0][j] + wy[1] * y[1][j]) + (wy[2] * y[2][j] + wy[3] * y[3][j]);
+        }
+        for (; f < F; f += 32) {
           const int b = fb[f];
           const double wy = fw[f] * (double)Y[b + r * st];
 #pragma unroll
