@@ -34,6 +34,7 @@ struct SmallTask {
   long long out[ITNBP_MAXK];  // outgoing message on bond axis a + 1
   int slot[ITNBP_MAXK];
   long long yoff;             // k_vertex_big: offset of the site's scratch tensor
+  int useed[ITNBP_MAXK];      // id of the incoming message on bond axis a + 1 (eigenvector seed slot)
 };
 
 __device__ inline void wave_sync() {
@@ -43,13 +44,17 @@ __device__ inline void wave_sync() {
 
 // Symmetric n x n (n <= 16) eigendecomposition in LDS by one wave: A (ld
 // SV_LD) is overwritten by diag(l), V receives the eigenvectors as columns.
-__device__ void wave_jacobi_small(double* A, double* V, int n, double* rc, double* rs, int* rp, int* rq) {
+// With init_v = false, V holds an orthogonal seed onto which the rotations
+// accumulate.
+__device__ void wave_jacobi_small(double* A, double* V, int n, double* rc, double* rs, int* rp, int* rq,
+                                  bool init_v = true) {
   const int l = threadIdx.x & 63;
   const int ne = n + (n & 1), np = ne / 2;
-  for (int o = l; o < n * n; o += 64) {
-    const int i = o / n, j = o % n;
-    V[i * SV_LD + j] = (i == j) ? 1.0 : 0.0;
-  }
+  if (init_v)
+    for (int o = l; o < n * n; o += 64) {
+      const int i = o / n, j = o % n;
+      V[i * SV_LD + j] = (i == j) ? 1.0 : 0.0;
+    }
   wave_sync();
   double prev = INFINITY;
   for (int sweep = 0; sweep < 30; ++sweep) {
@@ -119,6 +124,73 @@ __device__ void wave_jacobi_small(double* A, double* V, int n, double* rc, doubl
   }
 }
 
+constexpr int SEED_LD = SV_MAXCHI;  // row stride of a stored eigenvector seed
+
+// n x n product by one wave into registers (at most 4 elements per lane for
+// n <= 16): C[i][j] = sum_k X(i, k) Y(k, j), X read transposed when tx
+__device__ inline void wave_mm(const double* X, bool tx, const double* Y, int n, double (&c)[4]) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int o = l + 64 * q;
+    c[q] = 0.0;
+    if (o < n * n) {
+      const int i = o / n, j = o % n;
+      double s = 0.0;
+      for (int kk = 0; kk < n; ++kk) s += (tx ? X[kk * SV_LD + i] : X[i * SV_LD + kk]) * Y[kk * SV_LD + j];
+      c[q] = s;
+    }
+  }
+}
+
+// Eigendecomposition of the symmetrised message M (n x n, column-major) on one
+// bond by one wave: eigenvectors to U (columns, ld SV_LD), eigenvalues to lamv,
+// A (ld SV_LD) as work.  With warm, the previous sweep's eigenvectors U0 (in
+// seed, ld SEED_LD) start it: Jacobi diagonalises the symmetrised U0^T S U0 and
+// its rotations accumulate onto U0; near a BP fixed point that matrix is
+// nearly diagonal and takes one or two sweeps instead of a cold start's six to
+// ten.  The eigenvectors are stored back to seed for the next sweep.
+template <typename TM>
+__device__ void bond_eigh(const TM* M, int n, double* A, double* U, double* lamv, double* seed, int warm,
+                          double* rc, double* rs, int* rp, int* rq) {
+  const int l = threadIdx.x & 63;
+  for (int o = l; o < n * n; o += 64) {
+    const int i = o / n, j = o % n;
+    A[i * SV_LD + j] = 0.5 * ((double)M[i + n * j] + (double)M[j + n * i]);
+    if (warm) U[i * SV_LD + j] = seed[i * SEED_LD + j];
+  }
+  wave_sync();
+  if (warm) {
+    double c[4];
+    wave_mm(A, false, U, n, c);  // S U0
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (l + 64 * q < n * n) A[((l + 64 * q) / n) * SV_LD + (l + 64 * q) % n] = c[q];
+    wave_sync();
+    wave_mm(U, true, A, n, c);  // U0^T (S U0)
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (l + 64 * q < n * n) A[((l + 64 * q) / n) * SV_LD + (l + 64 * q) % n] = c[q];
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // symmetrise
+      const int o = l + 64 * q;
+      if (o < n * n) c[q] = 0.5 * (A[(o / n) * SV_LD + o % n] + A[(o % n) * SV_LD + o / n]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (l + 64 * q < n * n) A[((l + 64 * q) / n) * SV_LD + (l + 64 * q) % n] = c[q];
+    wave_sync();
+  }
+  wave_jacobi_small(A, U, n, rc, rs, rp, rq, !warm);
+  if (l < n) lamv[l] = A[l * SV_LD + l];
+  for (int o = l; o < n * n; o += 64) seed[(o / n) * SEED_LD + o % n] = U[(o / n) * SV_LD + o % n];
+  wave_sync();
+}
+
 __device__ inline double block_sum(double v, double* red) {
   for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
   __syncthreads();
@@ -144,7 +216,8 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
                                                           const SmallTask* __restrict__ tasks,
                                                           const TS* __restrict__ psi, const TM* msg_in,
                                                           TM* msg_out, MsgStat* __restrict__ stats,
-                                                          int normalize, int fmax) {
+                                                          int normalize, int fmax, double* __restrict__ seeds,
+                                                          int warm) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* Us = reinterpret_cast<double*>(smem);
   double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
@@ -179,19 +252,10 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
   const TS* src = psi + V->psi_off;
   for (int i = tid; i < D; i += SV_THREADS) Y[i] = load_as<TA>(src, i);
   // 2. eigendecompositions, one bond per wave
-  for (int a = wv; a < k; a += 4) {
-    const int n = dims[a + 1];
-    double* A = As + wv * SV_LD * SV_LD;
-    const TM* M = msg_in + V->in_msg[a + 1];
-    for (int o = l; o < n * n; o += 64) {
-      const int i = o / n, j = o % n;
-      A[i * SV_LD + j] = 0.5 * ((double)M[i + n * j] + (double)M[j + n * i]);
-    }
-    wave_sync();
-    wave_jacobi_small(A, Us + a * SV_LD * SV_LD, n, rc + 8 * wv, rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
-    if (l < n) lam[a * SV_MAXCHI + l] = A[l * SV_LD + l];
-    wave_sync();
-  }
+  for (int a = wv; a < k; a += 4)
+    bond_eigh(msg_in + V->in_msg[a + 1], dims[a + 1], As + wv * SV_LD * SV_LD, Us + a * SV_LD * SV_LD,
+              lam + a * SV_MAXCHI, seeds + (long long)tk->useed[a] * SEED_LD * SEED_LD, warm, rc + 8 * wv,
+              rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
   __syncthreads();
   // 3. rotate every bond: Y <- Y x_c U_c (in place, one fibre per thread)
   for (int a = 0; a < k; ++a) {
@@ -398,7 +462,8 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
                                                           const SmallTask* __restrict__ tasks,
                                                           const TS* __restrict__ psi, const TM* msg_in,
                                                           TM* msg_out, MsgStat* __restrict__ stats, int normalize,
-                                                          TA* __restrict__ scratch) {
+                                                          TA* __restrict__ scratch, double* __restrict__ seeds,
+                                                          int warm) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* Us = reinterpret_cast<double*>(smem);
   double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
@@ -433,19 +498,10 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
   const TS* src = psi + V->psi_off;
   for (int i = tid; i < D; i += SV_THREADS) Y[i] = load_as<TA>(src, i);
   // 2. eigendecompositions, one bond per wave
-  for (int a = wv; a < k; a += 4) {
-    const int n = dims[a + 1];
-    double* A = As + wv * SV_LD * SV_LD;
-    const TM* M = msg_in + V->in_msg[a + 1];
-    for (int o = l; o < n * n; o += 64) {
-      const int i = o / n, j = o % n;
-      A[i * SV_LD + j] = 0.5 * ((double)M[i + n * j] + (double)M[j + n * i]);
-    }
-    wave_sync();
-    wave_jacobi_small(A, Us + a * SV_LD * SV_LD, n, rc + 8 * wv, rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
-    if (l < n) lam[a * SV_MAXCHI + l] = A[l * SV_LD + l];
-    wave_sync();
-  }
+  for (int a = wv; a < k; a += 4)
+    bond_eigh(msg_in + V->in_msg[a + 1], dims[a + 1], As + wv * SV_LD * SV_LD, Us + a * SV_LD * SV_LD,
+              lam + a * SV_MAXCHI, seeds + (long long)tk->useed[a] * SEED_LD * SEED_LD, warm, rc + 8 * wv,
+              rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
   __threadfence_block();
   __syncthreads();
   // 3. rotate every bond in place, one fibre per thread
@@ -563,6 +619,11 @@ struct SmallState {
   std::vector<SmallTask> big;
   DevBuf d_big, ybuf;
   long long ytotal = 0;  // scratch elements
+  // eigenvector seeds, one SEED_LD x SEED_LD slot per message id; valid after the
+  // first sweep since the last prepare, and every 16th sweep starts cold so that
+  // rounding in the accumulated rotations cannot build up
+  DevBuf seeds;
+  long long sweeps = 0;
 };
 
 SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net->small); }
@@ -570,13 +631,15 @@ SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net-
 template <typename TS, typename TA, typename TM>
 hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
                        MsgStat* stats, int normalize) {
+  const int warm = (st->sweeps % 16) != 0 ? 1 : 0;
+  double* seeds = (double*)st->seeds.p;
   if (!st->tasks.empty()) {
     auto kern = k_vertex_eig<TS, TA, TM>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3((unsigned)st->tasks.size()), dim3(SV_THREADS), st->lds, net->ctx->stream,
                        (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, (const TS*)psi,
-                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax);
+                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax, seeds, warm);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -587,7 +650,7 @@ hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const vo
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3((unsigned)st->big.size()), dim3(SV_THREADS), lds, net->ctx->stream,
                        (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_big.p, (const TS*)psi,
-                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, (TA*)st->ybuf.p);
+                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, (TA*)st->ybuf.p, seeds, warm);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -613,6 +676,7 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
   st->ytotal = 0;
   st->lds = 0;
   st->fmax = 0;
+  st->sweeps = 0;
   if (!small_path_enabled(net)) return ITNBP_OK;
   std::vector<int> pos(2 * net->ne, -1);
   for (size_t i = 0; i < seq.size(); ++i) pos[seq[i]] = (int)i;
@@ -644,6 +708,7 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
       else {
         t.out[a - 1] = net->msg_off[m];
         t.slot[a - 1] = pos[m];
+        t.useed[a - 1] = msg_id_into(net, e, v);
       }
     }
     if (!ok) continue;
@@ -659,6 +724,8 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
       st->big.push_back(t);
     }
   }
+  if (!st->tasks.empty() || !st->big.empty())
+    ITNBP_RETCHK(ensure_buf(ctx, st->seeds, (size_t)std::max(1, 2 * net->ne) * SEED_LD * SEED_LD * 8));
   if (!st->big.empty()) {
     ITNBP_RETCHK(ensure_buf(ctx, st->d_big, st->big.size() * sizeof(SmallTask)));
     ITNBP_RETCHK(ensure_buf(ctx, st->ybuf, (size_t)st->ytotal * eb));
@@ -702,6 +769,7 @@ int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, con
   }
   prof_add(ctx, "vertex_eig", e0, prof_mark(ctx));
   ITNBP_HIPCHK(ctx, e);
+  ++st->sweeps;
   return ITNBP_OK;
 }
 
@@ -711,6 +779,7 @@ void release_small_state(itnbp_net* net) {
   free_buf(st->d_tasks);
   free_buf(st->d_big);
   free_buf(st->ybuf);
+  free_buf(st->seeds);
   delete st;
   net->small = nullptr;
 }
