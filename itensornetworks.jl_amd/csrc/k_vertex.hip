@@ -494,9 +494,7 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
   }
   __syncthreads();
   TA* Y = scratch + tk->yoff;
-  // 1. psi -> Y
   const TS* src = psi + V->psi_off;
-  for (int i = tid; i < D; i += SV_THREADS) Y[i] = load_as<TA>(src, i);
   // 2. eigendecompositions, one bond per wave
   for (int a = wv; a < k; a += 4)
     bond_eigh(msg_in + V->in_msg[a + 1], dims[a + 1], As + wv * SV_LD * SV_LD, Us + a * SV_LD * SV_LD,
@@ -504,22 +502,40 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
               rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
   __threadfence_block();
   __syncthreads();
-  // 3. rotate every bond in place, one fibre per thread
+  // 3. rotate every bond, NFB fibres per thread in flight (their loads issued
+  // together, each U element read once for all of them); bond 0 reads psi and
+  // writes Y, the later bonds rotate Y in place
+  constexpr int NFB = sizeof(TA) == 4 ? 4 : 2;
   for (int a = 0; a < k; ++a) {
     const int n = dims[a + 1], st = strides[a + 1];
     const int F = D / n;
     const double* U = Us + a * SV_LD * SV_LD;
-    for (int f = tid; f < F; f += SV_THREADS) {
-      const int base = (f % st) + (f / st) * st * n;
-      TA x[SV_MAXCHI];
+    for (int f0 = tid; f0 < F; f0 += SV_THREADS * NFB) {
+      int base[NFB];
+      TA x[NFB][SV_MAXCHI];
 #pragma unroll
-      for (int j = 0; j < SV_MAXCHI; ++j) x[j] = j < n ? Y[base + j * st] : (TA)0;
-      for (int r = 0; r < n; ++r) {
-        TA acc = 0;
+      for (int i = 0; i < NFB; ++i) {
+        const int f = f0 + i * SV_THREADS;
+        base[i] = f < F ? (f % st) + (f / st) * st * n : -1;
 #pragma unroll
         for (int j = 0; j < SV_MAXCHI; ++j)
-          if (j < n) acc += (TA)U[j * SV_LD + r] * x[j];
-        Y[base + r * st] = acc;
+          x[i][j] = (base[i] >= 0 && j < n) ? (a == 0 ? load_as<TA>(src, base[i] + j * st) : Y[base[i] + j * st])
+                                             : (TA)0;
+      }
+      for (int r = 0; r < n; ++r) {
+        TA acc[NFB];
+#pragma unroll
+        for (int i = 0; i < NFB; ++i) acc[i] = 0;
+#pragma unroll
+        for (int j = 0; j < SV_MAXCHI; ++j)
+          if (j < n) {
+            const TA u = (TA)U[j * SV_LD + r];
+#pragma unroll
+            for (int i = 0; i < NFB; ++i) acc[i] += u * x[i][j];
+          }
+#pragma unroll
+        for (int i = 0; i < NFB; ++i)
+          if (base[i] >= 0) Y[base[i] + r * st] = acc[i];
       }
     }
     __threadfence_block();
@@ -529,8 +545,12 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
   for (int a = 0; a < k; ++a) {
     const int n = dims[a + 1], st = strides[a + 1], nn = n * n;
     const int F = D / n;
-    const int r = tid % n, r2 = tid / n;
-    double acc = 0;
+    // lane (rl, grp): row rl = l & 15 of T' over the tile fibres j = grp mod 16
+    // (grp = (l >> 4) + 4 wv), every column at once
+    const int rl = l & 15, grp = (l >> 4) + 4 * wv;
+    double acc16[SV_MAXCHI];
+#pragma unroll
+    for (int x = 0; x < SV_MAXCHI; ++x) acc16[x] = 0;
     for (int f0 = 0; f0 < F; f0 += VB_FT) {
       const int nf = min(VB_FT, F - f0);
       // loader: thread j of the tile -> fibre f0 + j (values and weight)
@@ -544,21 +564,33 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
         for (int x = 0; x < n; ++x) tv8[tid * SV_LD + x] = Y[base + x * st];
       }
       __syncthreads();
-      if (tid < nn) {
-        double a1 = 0, a2 = 0, a3 = 0;
-        int j = 0;
-        for (; j + 3 < nf; j += 4) {
-          acc += tw[j] * (double)tv8[j * SV_LD + r] * (double)tv8[j * SV_LD + r2];
-          a1 += tw[j + 1] * (double)tv8[(j + 1) * SV_LD + r] * (double)tv8[(j + 1) * SV_LD + r2];
-          a2 += tw[j + 2] * (double)tv8[(j + 2) * SV_LD + r] * (double)tv8[(j + 2) * SV_LD + r2];
-          a3 += tw[j + 3] * (double)tv8[(j + 3) * SV_LD + r] * (double)tv8[(j + 3) * SV_LD + r2];
+      if (rl < n) {
+#pragma unroll 2
+        for (int j = grp; j < nf; j += 16) {
+          const double wy = tw[j] * (double)tv8[j * SV_LD + rl];
+#pragma unroll
+          for (int x = 0; x < SV_MAXCHI; ++x)
+            if (x < n) acc16[x] += wy * (double)tv8[j * SV_LD + x];
         }
-        for (; j < nf; ++j) acc += tw[j] * (double)tv8[j * SV_LD + r] * (double)tv8[j * SV_LD + r2];
-        acc += (a1 + a2) + a3;
       }
       __syncthreads();
     }
-    if (tid < nn) Tp[r * SV_LD + r2] = acc;  // T'[r][r']
+    // the 16 fibre groups: four per wave by shuffles, then the four waves in LDS
+    // (the tile area, free after the loop: 4 x 256 doubles)
+#pragma unroll
+    for (int x = 0; x < SV_MAXCHI; ++x)
+      for (int m = 16; m < 64; m <<= 1) acc16[x] += __shfl_xor(acc16[x], m);
+    double* red4 = reinterpret_cast<double*>(tv8);
+    if (l < 16) {
+#pragma unroll
+      for (int x = 0; x < SV_MAXCHI; ++x) red4[wv * 256 + l * 16 + x] = acc16[x];
+    }
+    __syncthreads();
+    if (tid < nn) {
+      const int rr = tid % n, r2 = tid / n;
+      Tp[rr * SV_LD + r2] = (red4[rr * 16 + r2] + red4[256 + rr * 16 + r2]) +
+                            (red4[512 + rr * 16 + r2] + red4[768 + rr * 16 + r2]);  // T'[r][r']
+    }
     __syncthreads();
     const double* U = Us + a * SV_LD * SV_LD;
     if (tid < nn) {
