@@ -551,19 +551,34 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
     double acc16[SV_MAXCHI];
 #pragma unroll
     for (int x = 0; x < SV_MAXCHI; ++x) acc16[x] = 0;
-    for (int f0 = 0; f0 < F; f0 += VB_FT) {
-      const int nf = min(VB_FT, F - f0);
-      // loader: thread j of the tile -> fibre f0 + j (values and weight)
-      if (tid < nf) {
+    // loader: thread j of a tile -> fibre f0 + j (values and weight), fetched
+    // into registers one tile ahead so that the loads overlap the Gram of the
+    // current tile
+    TA pv[SV_MAXCHI];
+    double pw = 0;
+    auto fetch = [&](int f0) {
+      if (tid < min(VB_FT, F - f0)) {
         const int f = f0 + tid;
         const int base = (f % st) + (f / st) * st * n;
         double w = 1.0;
         for (int b = 0; b < k; ++b)
           if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
-        tw[tid] = w;
-        for (int x = 0; x < n; ++x) tv8[tid * SV_LD + x] = Y[base + x * st];
+        pw = w;
+#pragma unroll
+        for (int x = 0; x < SV_MAXCHI; ++x) pv[x] = x < n ? Y[base + x * st] : (TA)0;
+      }
+    };
+    fetch(0);
+    for (int f0 = 0; f0 < F; f0 += VB_FT) {
+      const int nf = min(VB_FT, F - f0);
+      if (tid < nf) {
+        tw[tid] = pw;
+#pragma unroll
+        for (int x = 0; x < SV_MAXCHI; ++x)
+          if (x < n) tv8[tid * SV_LD + x] = pv[x];
       }
       __syncthreads();
+      if (f0 + VB_FT < F) fetch(f0 + VB_FT);
       if (rl < n) {
 #pragma unroll 2
         for (int j = grp; j < nf; j += 16) {
