@@ -37,7 +37,9 @@ def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None
     truncation errors; ``callback(singular_values=..., truncation_error=...)`` is
     called per gate (apply.jl:89)."""
     g = bpc.graph
-    psi = bpc.tensornetwork()
+    # physical dimensions only (gates keep them): the host description, with no
+    # download of the site tensors the previous gates changed
+    ket = bpc.qf.ket
     pending = list(enumerate(gates))
     errs = [None] * len(gates)
     while pending:
@@ -58,7 +60,7 @@ def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None
             u, v = gt.sites
             e = g.edge_id(u, v)
             m = np.asarray(gt.matrix, dtype=np.float64)
-            du, dv = psi.phys_dim(u), psi.phys_dim(v)
+            du, dv = ket.phys_dim(u), ket.phys_dim(v)
             if m.shape != (du * dv, du * dv):
                 raise ValueError("gate shape does not match the physical dimensions")
             if g.edges[e] != (u, v):
@@ -124,16 +126,23 @@ def apply(o: Gate, psi: ITensorNetwork, *, envs=None, maxdim=None, cutoff=None, 
 
 
 def edge_coloring(graph):
-    """Greedy proper edge colouring in edge order: classes of disjoint bonds."""
-    classes = []
-    for (u, v) in graph.edges:
-        for c in classes:
-            if all(u not in e and v not in e for e in c):
-                c.append((u, v))
-                break
-        else:
-            classes.append([(u, v)])
-    return classes
+    """Greedy proper edge colouring in edge order: classes of disjoint bonds.
+    Computed once per graph (graphs are not modified after construction) and
+    kept on it."""
+    cached = getattr(graph, "_edge_classes", None)
+    if cached is None:
+        cached, used = [], []
+        for (u, v) in graph.edges:
+            for c, us in zip(cached, used):
+                if u not in us and v not in us:
+                    c.append((u, v))
+                    us.update((u, v))
+                    break
+            else:
+                cached.append([(u, v)])
+                used.append({u, v})
+        graph._edge_classes = cached
+    return [list(c) for c in cached]
 
 
 def simple_update_round_(bpc: BeliefPropagationCache, gate_fn, *, maxdim, cutoff=None, normalize=True):
