@@ -455,7 +455,12 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
 // rotated there in place fibre by fibre, and every weighted Gram streams Y
 // through LDS in tiles of VB_FT fibres (base offsets and weights computed by the
 // loader).  Steps 2 and 5 are those of k_vertex_eig.
-constexpr int VB_FT = 256;  // fibres per Gram tile
+constexpr int VB_FT = 192;  // fibres per Gram tile
+// fixed LDS of k_vertex_big: that of k_vertex_eig without the Jacobi work
+// areas, which alias the Gram tile (idle while the messages are decomposed), so
+// that four FP32 workgroups share a CU
+constexpr size_t VB_FIXED = SV_FIXED - 4 * SV_LD * SV_LD * 8;
+static_assert(VB_FT * SV_LD * sizeof(float) >= 4 * SV_LD * SV_LD * 8, "Jacobi work must fit in the tile area");
 
 template <typename TS, typename TA, typename TM>
 __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restrict__ vdesc,
@@ -466,8 +471,7 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
                                                           int warm) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* Us = reinterpret_cast<double*>(smem);
-  double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
-  double* lam = As + 4 * SV_LD * SV_LD;
+  double* lam = Us + ITNBP_MAXK * SV_LD * SV_LD;
   double* rc = lam + ITNBP_MAXK * SV_MAXCHI;
   double* rs = rc + 32;
   int* rp = reinterpret_cast<int*>(rs + 32);
@@ -476,8 +480,9 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
   double* Tp = gpart + SV_THREADS;
   double* X = Tp + SV_LD * SV_LD;
   double* red = X + SV_LD * SV_LD;
-  double* tw = reinterpret_cast<double*>(smem + SV_FIXED);      // tile weights [VB_FT]
+  double* tw = reinterpret_cast<double*>(smem + VB_FIXED);      // tile weights [VB_FT]
   TA* tv8 = reinterpret_cast<TA*>(tw + VB_FT);                 // tile values [VB_FT][SV_LD]
+  double* As = reinterpret_cast<double*>(tv8);                  // Jacobi work (step 2 only)
 
   __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
   const SmallTask* tk = tasks + blockIdx.x;
@@ -692,7 +697,7 @@ hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const vo
   }
   if (!st->big.empty()) {
     auto kern = k_vertex_big<TS, TA, TM>;
-    const size_t lds = SV_FIXED + (size_t)VB_FT * 8 + (size_t)VB_FT * SV_LD * sizeof(TA);
+    const size_t lds = VB_FIXED + (size_t)VB_FT * 8 + (size_t)VB_FT * SV_LD * sizeof(TA);
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3((unsigned)st->big.size()), dim3(SV_THREADS), lds, net->ctx->stream,
