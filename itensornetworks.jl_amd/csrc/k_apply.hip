@@ -13,8 +13,9 @@
 //      psi_v' with sqrt(S) V^T (src/apply.jl:90-93); optional normalisation
 //      (src/apply.jl:134-137).
 // One workgroup per bond; the bonds of one launch are disjoint.  Small dense
-// linear algebra runs in LDS: parallel (round-robin) two-sided Jacobi for the
-// symmetric eigenproblems, Householder QR, one-sided (Hestenes) Jacobi SVD.
+// linear algebra: two-sided Jacobi in registers for the environment
+// eigenproblems (one wave each), Householder QR and one-sided (Hestenes) Jacobi
+// SVD in LDS.
 #include <cmath>
 #include <vector>
 
@@ -72,132 +73,100 @@ __device__ inline void rr_pair(int n, int st, int i, int& p, int& q) {
   }
 }
 
-// Symmetric eigendecomposition A = V diag(w) V^T, A n x n (row-major, ld n) in
-// LDS, overwritten; parallel two-sided Jacobi.  n <= 64.
-__device__ void jacobi_eigh(double* A, double* V, double* w, int n, double* rot) {
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n * n; i += blockDim.x) V[i] = (i / n == i % n) ? 1.0 : 0.0;
-  __syncthreads();
-  if (n == 1) {
-    if (tid == 0) w[0] = A[0];
-    __syncthreads();
-    return;
-  }
-  const int ne = n + (n & 1);  // pad to even with a dummy index n
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    // off-diagonal norm
-    double off = 0, diag = 0;
-    for (int i = tid; i < n * n; i += blockDim.x) {
-      const double a = A[i];
-      if (i / n != i % n) off += a * a;
-      else diag += a * a;
-    }
-    __shared__ double redj[8];
-    off = block_sum_d(off, redj);
-    diag = block_sum_d(diag, redj);
-    if (off <= 1e-30 * diag || off == 0.0) break;
-    for (int st = 0; st < ne - 1; ++st) {
-      // rotations
-      for (int i = tid; i < ne / 2; i += blockDim.x) {
-        int p, q;
-        rr_pair(ne, st, i, p, q);
-        double c = 1, s = 0;
-        if (q < n) {
-          const double apq = A[p * n + q];
-          if (apq != 0.0) {
-            const double zeta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
-            const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
-            c = 1 / sqrt(1 + t * t);
-            s = c * t;
-          }
-        }
-        rot[4 * i] = p;
-        rot[4 * i + 1] = q;
-        rot[4 * i + 2] = c;
-        rot[4 * i + 3] = s;
+// sqrt(M) and M^{-1/2} of a symmetrised chi x chi message (column-major in
+// global), chi <= 8, by one wave: lane i ne + j (ne = chi rounded up to even)
+// holds A[i][j] and V[i][j] in registers.  Cyclic two-sided Jacobi with
+// round-robin pairs (rr_pair's schedule), every lane deriving the rotations of
+// its row pair and column pair from the pre-step matrix through shuffles, so a
+// step needs no LDS and no barrier; stops when the off-diagonal mass is below
+// 1e-30 of the diagonal's (at most 40 sweeps).  Then the eigenvalue truncation
+// of map_eigvals (cutoff = 10 eps, src/apply.jl:9-25), recomputed identically
+// by every lane.  S and Si row-major, ld chi.
+__device__ void env_roots_wave(const double* M, int chi, double cutoff, double* S, double* Si) {
+  const int l = threadIdx.x & 63;
+  const int ne = chi + (chi & 1);
+  const bool live = l < ne * ne;
+  const int i = live ? l / ne : 0, j = live ? l % ne : 0;
+  const bool in = live && i < chi && j < chi;
+  double a = in ? 0.5 * (M[i + chi * j] + M[j + chi * i]) : 0.0;
+  double v = (live && i == j) ? 1.0 : 0.0;
+  if (chi > 1) {
+    auto rot = [&](int p, int q, double& c, double& sn) {
+      const double app = __shfl(a, p * ne + p), aqq = __shfl(a, q * ne + q), apq = __shfl(a, p * ne + q);
+      c = 1.0;
+      sn = 0.0;
+      if (q < chi && apq != 0.0) {
+        const double zeta = (aqq - app) / (2 * apq);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+        c = 1 / sqrt(1 + t * t);
+        sn = c * t;
       }
-      __syncthreads();
-      // rows: A' = J^T A
-      for (int idx = tid; idx < (ne / 2) * n; idx += blockDim.x) {
-        const int i = idx / n, col = idx % n;
-        const int p = (int)rot[4 * i], q = (int)rot[4 * i + 1];
-        if (q >= n) continue;
-        const double c = rot[4 * i + 2], s = rot[4 * i + 3];
-        const double ap = A[p * n + col], aq = A[q * n + col];
-        A[p * n + col] = c * ap - s * aq;
-        A[q * n + col] = s * ap + c * aq;
+    };
+    for (int sweep = 0; sweep < 40; ++sweep) {
+      double off = (in && i != j) ? a * a : 0.0, diag = (in && i == j) ? a * a : 0.0;
+      for (int m = 32; m > 0; m >>= 1) {
+        off += __shfl_xor(off, m);
+        diag += __shfl_xor(diag, m);
       }
-      __syncthreads();
-      // columns: A'' = A' J, V = V J
-      for (int idx = tid; idx < (ne / 2) * n; idx += blockDim.x) {
-        const int i = idx / n, row = idx % n;
-        const int p = (int)rot[4 * i], q = (int)rot[4 * i + 1];
-        if (q >= n) continue;
-        const double c = rot[4 * i + 2], s = rot[4 * i + 3];
-        const double ap = A[row * n + p], aq = A[row * n + q];
-        A[row * n + p] = c * ap - s * aq;
-        A[row * n + q] = s * ap + c * aq;
-        const double vp = V[row * n + p], vq = V[row * n + q];
-        V[row * n + p] = c * vp - s * vq;
-        V[row * n + q] = s * vp + c * vq;
+      if (off <= 1e-30 * diag || off == 0.0) break;
+      for (int st = 0; st < ne - 1; ++st) {
+        // partner of index x in step st of the round-robin schedule (rr_pair)
+        auto partner = [&](int x) {
+          const int pos = x == 0 ? 0 : 1 + ((x - 1 - st) % (ne - 1) + (ne - 1)) % (ne - 1);
+          const int pp = ne - 1 - pos;
+          return pp == 0 ? 0 : 1 + (pp - 1 + st) % (ne - 1);
+        };
+        const int ri = partner(i), cj = partner(j);
+        double cr, sr, cc, sc;
+        rot(i < ri ? i : ri, i < ri ? ri : i, cr, sr);
+        rot(j < cj ? j : cj, j < cj ? cj : j, cc, sc);
+        const double ao = __shfl(a, ri * ne + j);  // rows: A <- J^T A
+        const double ar = i < ri ? cr * a - sr * ao : sr * ao + cr * a;
+        const double aro = __shfl(ar, i * ne + cj), vo = __shfl(v, i * ne + cj);  // columns: A J, V J
+        a = j < cj ? cc * ar - sc * aro : sc * aro + cc * ar;
+        v = j < cj ? cc * v - sc * vo : sc * vo + cc * v;
       }
-      __syncthreads();
     }
   }
-  for (int i = tid; i < n; i += blockDim.x) w[i] = A[i * n + i];
-  __syncthreads();
-}
-
-// sqrt(M) and M^{-1/2} of a symmetrised chi x chi message (column-major in global)
-__device__ void env_roots(const double* M, int chi, double cutoff, double* S, double* Si, double* A,
-                          double* V, double* w, double* rot) {
-  const int tid = threadIdx.x;
-  for (int i = tid; i < chi * chi; i += blockDim.x) {
-    const int r = i / chi, c = i % chi;
-    A[i] = 0.5 * (M[r + chi * c] + M[c + chi * r]);
-  }
-  __syncthreads();
-  jacobi_eigh(A, V, w, chi, rot);
+  double w[AP_ENV];
+#pragma unroll
+  for (int m = 0; m < AP_ENV; ++m) w[m] = __shfl(a, (m < ne ? m : 0) * ne + (m < ne ? m : 0));
   // truncation: sort |lambda| descending; drop from the small end while the
   // dropped weight / total <= cutoff (keep >= 1)
-  __shared__ int keepmask[AP_ENV * 8];
-  if (tid == 0) {
-    int order[64];
-    for (int i = 0; i < chi; ++i) order[i] = i;
-    for (int i = 0; i < chi; ++i)
-      for (int j = i + 1; j < chi; ++j)
-        if (fabs(w[order[j]]) > fabs(w[order[i]])) {
-          int t = order[i];
-          order[i] = order[j];
-          order[j] = t;
-        }
-    double tot = 0;
-    for (int i = 0; i < chi; ++i) tot += fabs(w[i]);
-    int keep = chi;
-    double disc = 0;
-    while (keep > 1 && tot > 0 && (disc + fabs(w[order[keep - 1]])) / tot <= cutoff) {
-      disc += fabs(w[order[keep - 1]]);
-      --keep;
-    }
-    for (int i = 0; i < chi; ++i) keepmask[i] = 0;
-    for (int i = 0; i < keep; ++i) keepmask[order[i]] = 1;
+  int order[AP_ENV];
+  for (int m = 0; m < chi; ++m) order[m] = m;
+  for (int x = 0; x < chi; ++x)
+    for (int y = x + 1; y < chi; ++y)
+      if (fabs(w[order[y]]) > fabs(w[order[x]])) {
+        const int t = order[x];
+        order[x] = order[y];
+        order[y] = t;
+      }
+  double tot = 0;
+  for (int m = 0; m < chi; ++m) tot += fabs(w[m]);
+  int keep = chi;
+  double disc = 0;
+  while (keep > 1 && tot > 0 && (disc + fabs(w[order[keep - 1]])) / tot <= cutoff) {
+    disc += fabs(w[order[keep - 1]]);
+    --keep;
   }
-  __syncthreads();
-  for (int i = tid; i < chi * chi; i += blockDim.x) {
-    const int r = i / chi, c = i % chi;
-    double s = 0, si = 0;
-    for (int m = 0; m < chi; ++m) {
-      if (!keepmask[m]) continue;
-      const double lam = w[m];
-      const double vv = V[r * chi + m] * V[c * chi + m];
-      const double sq = sqrt(lam);  // NaN for a kept negative eigenvalue
-      s += vv * sq;
-      si += vv / sq;
+  unsigned kmask = 0;
+  for (int m = 0; m < keep; ++m) kmask |= 1u << order[m];
+  // S[r][c] = sum_m V[r][m] V[c][m] sqrt(l_m) over the kept m (lane (r, c) = (i, j))
+  double sv = 0, siv = 0;
+  for (int m = 0; m < AP_ENV; ++m) {
+    const int mm = m < ne ? m : 0;
+    const double vr = __shfl(v, i * ne + mm), vc = __shfl(v, j * ne + mm);
+    if (m < chi && ((kmask >> m) & 1u)) {
+      const double sq = sqrt(w[m]);  // NaN for a kept negative eigenvalue
+      sv += vr * vc * sq;
+      siv += vr * vc / sq;
     }
-    S[i] = s;   // row-major [b][b']
-    Si[i] = si;
   }
-  __syncthreads();
+  if (in) {
+    S[i * chi + j] = sv;
+    Si[i * chi + j] = siv;
+  }
 }
 
 struct SideGeom {
@@ -428,17 +397,19 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   double* scr_u = scratch + (long long)blockIdx.x * 2 * AP_XY;
   double* scr_v = scr_u + AP_XY;
 
-  // 1. environment roots (sqrt and inverse sqrt), both sides
-  for (int side = 0; side < 2; ++side) {
-    const SideGeom& g = side == 0 ? gu : gv;
-    const VDesc& V = vds[side == 0 ? it.u : it.v];
-    for (int pa = 0; pa < g.next; ++pa) {
+  // 1. environment roots (sqrt and inverse sqrt), both sides, one environment per wave
+  {
+    const int wv = tid >> 6;
+    for (int q = wv; q < gu.next + gv.next; q += (int)blockDim.x / 64) {
+      const int side = q < gu.next ? 0 : 1;
+      const int pa = side == 0 ? q : q - gu.next;
+      const SideGeom& g = side == 0 ? gu : gv;
+      const VDesc& V = vds[side == 0 ? it.u : it.v];
       const int a = g.ext[pa];
-      const int chi = g.dims[a];
       double* S = roots + ((side * AP_MAXEXT + pa) * 2) * AP_ENV * AP_ENV;
-      double* Si = S + AP_ENV * AP_ENV;
-      env_roots(msg + V.in_msg[a], chi, prm.eig_cutoff, S, Si, ea, ev, ew, rot);
+      env_roots_wave(msg + V.in_msg[a], g.dims[a], prm.eig_cutoff, S, S + AP_ENV * AP_ENV);
     }
+    __syncthreads();
   }
   // 2. X = psi with sqrt(env) on the external bonds, QR; reflectors to scratch
   for (int side = 0; side < 2; ++side) {
@@ -708,10 +679,12 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
     cleanup();
     return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(ea));
   }
+  const int pm0 = prof_mark(ctx);
   hipLaunchKernelGGL(k_apply_gates, dim3(n), dim3(256), lds, ctx->stream, (const ApplyItem*)d_items.p,
                      net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)d_gates.p, prm,
                      (double*)d_scr.p, d_te, svals ? d_sv : nullptr, d_kept, d_st);
   hipError_t ek = hipGetLastError();
+  prof_add(ctx, "apply_gates", pm0, prof_mark(ctx));
   std::vector<int> st(n);
   std::vector<int> kp(n);
   if (ek == hipSuccess) ek = hipMemcpyAsync(truncerr, d_te, n * 8, hipMemcpyDeviceToHost, ctx->stream);
