@@ -298,7 +298,8 @@ struct Phase {
 
 int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_opts* o,
                 const Phase& ph, int maxiter, double stop_res2, bool stop_on_tol, int* iters,
-                double* mean_diff, double* max_res, int* converged, bool use_fast_in) {
+                double* mean_diff, double* max_res, int* converged, bool use_fast_in,
+                double* trace = nullptr, long long trace_stride = 0, int trace_every = 1) {
   bool use_fast = use_fast_in;
   itnbp_ctx* ctx = net->ctx;
   const int n = (int)seq.size();
@@ -365,6 +366,10 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
       for (size_t r = 0; r + 1 < run_groups.size(); ++r)
         RETCHK(launch_plan(net, plan, run_groups[r], run_groups[r + 1], d_tasks, ph.psi, ph.psi_dtype, cur, cur, d_stats));
     }
+    // FP64 messages after every trace_every-th sweep to trace + (it / trace_every) trace_stride
+    if (trace && it % trace_every == 0)
+      HIPCHK(ctx, hipMemcpyAsync(trace + (long long)(it / trace_every) * trace_stride, ph.msg[net->cur],
+                                 net->msg_total * 8, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(ctx, launch_reduce_stats(d_stats, n, d_red, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(red, d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -431,9 +436,11 @@ int check_sequence(itnbp_net* net, const std::vector<int>& seq, const itnbp_upda
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-// one FP64 Jacobi sweep over the default parallel schedule, messages
-// normalised, no convergence test (forward pass of the adjoint)
-int sweep_phase_public(itnbp_net* net, int nsweeps) {
+// nsweeps FP64 Jacobi sweeps over the default parallel schedule, messages
+// normalised, no convergence test (forward pass of the adjoint); with trace,
+// the messages after every trace_every-th sweep it go to
+// trace + (it / trace_every) trace_stride
+int sweep_phase_public(itnbp_net* net, int nsweeps, double* trace, long long trace_stride, int trace_every) {
   std::vector<int> seq;
   RETCHK(default_sequence(net, ITNBP_SCHED_PARALLEL, seq));
   if (seq.empty()) return ITNBP_OK;
@@ -448,7 +455,7 @@ int sweep_phase_public(itnbp_net* net, int nsweeps) {
   int it = 0, conv = 0;
   double md = 0, mr = 0;
   return sweep_phase(net, seq, &o, ph, nsweeps, -1.0, false, &it, &md, &mr, &conv,
-                     fast_path_available(net, 1, ITNBP_SCHED_PARALLEL));
+                     fast_path_available(net, 1, ITNBP_SCHED_PARALLEL), trace, trace_stride, trace_every);
 }
 
 extern "C" {

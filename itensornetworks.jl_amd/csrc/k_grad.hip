@@ -338,7 +338,8 @@ __global__ void k_pair_bar(const double* __restrict__ half, const long long* __r
 }  // namespace
 
 // forward sweeps helper (itnbp_api.hip)
-int sweep_phase_public(itnbp_net* net, int nsweeps);
+int sweep_phase_public(itnbp_net* net, int nsweeps, double* trace = nullptr, long long trace_stride = 0,
+                       int trace_every = 1);
 
 extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, const double* ops1,
                                  const double* ops2, double* energy, double* grad) {
@@ -396,12 +397,8 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
   // forward: identity messages, nsweeps Jacobi sweeps, checkpoints every c sweeps
   GR_CHK(itnbp_init_identity_messages(net));
   GR_HIP(hipMemcpyAsync(d_ck, net->d_msg[net->cur], MT * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  for (int t = 1; t <= nsweeps; ++t) {
-    GR_CHK(sweep_phase_public(net, 1));
-    if (t % c == 0)
-      GR_HIP(hipMemcpyAsync(d_ck + (long long)(t / c) * MT, net->d_msg[net->cur], MT * 8,
-                            hipMemcpyDeviceToDevice, ctx->stream));
-  }
+  // (one multi-sweep call; the checkpoint after sweep t, t % c == 0, to d_ck + (t / c) MT)
+  if (nsweeps > 0) GR_CHK(sweep_phase_public(net, nsweeps, d_ck, MT, c));
   // energy forward: rho_v (d x d) and halves A for every edge, rho_e
   std::vector<long long> r1off(nv), aoff_u(ne), aoff_v(ne), r2off(ne);
   std::vector<int> dims1(nv), dus(ne), dvs(ne), chis(ne);
@@ -560,11 +557,8 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
     // recompute M^(base+1..top) from the checkpoint into seg
     GR_HIP(hipMemcpyAsync(net->d_msg[net->cur], d_ck + (long long)s * MT, MT * 8, hipMemcpyDeviceToDevice,
                           ctx->stream));
-    for (int t = base + 1; t <= top; ++t) {
-      GR_CHK(sweep_phase_public(net, 1));
-      GR_HIP(hipMemcpyAsync(d_seg + (long long)(t - base) * MT, net->d_msg[net->cur], MT * 8,
-                            hipMemcpyDeviceToDevice, ctx->stream));
-    }
+    // M^(t) for t = base + 1 .. top to d_seg + (t - base) MT, in one call
+    if (top > base) GR_CHK(sweep_phase_public(net, top - base, d_seg, MT, 1));
     GR_HIP(hipMemcpyAsync(d_seg, d_ck + (long long)s * MT, MT * 8, hipMemcpyDeviceToDevice, ctx->stream));
     for (int tm1 = top; tm1 >= base; --tm1) {
       const double* prev = d_seg + (long long)(tm1 - base) * MT;
@@ -580,9 +574,7 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
   // leave the cache holding the final forward messages M^(T)
   GR_HIP(hipMemcpyAsync(net->d_msg[net->cur], d_ck + (long long)(nsweeps / c) * MT, MT * 8,
                         hipMemcpyDeviceToDevice, ctx->stream));
-  if (nsweeps % c != 0) {
-    for (int t = (nsweeps / c) * c + 1; t <= nsweeps; ++t) GR_CHK(sweep_phase_public(net, 1));
-  }
+  if (nsweeps % c != 0) GR_CHK(sweep_phase_public(net, nsweeps % c));
   GR_HIP(hipStreamSynchronize(ctx->stream));
   cleanup();
   return ITNBP_OK;
