@@ -299,3 +299,29 @@ def test_pair_rdm_eigenbasis_matches_oracle_and_generic(net34, monkeypatch):
     gen = itnbp.expect2_on_cache(bpc, zz)
     err = max(abs(fast[e] - gen[e]) for e in gen)
     assert err < 1e-12, err
+
+
+def test_density_matrix_routes_from_bf16_storage(net34, monkeypatch):
+    """The chi = 32 one- and two-site routes build their FP64 layouts from BF16 site
+    storage: <X>, <ZZ> and the vertex scalars agree with the generic kernels on the
+    same (BF16-rounded) tensors."""
+    g, psi = net34
+    bpc = itnbp.update(itnbp.initialize_cache(psi, storage="bf16"), maxiter=4, edge_sequence="parallel",
+                       precision="f64")
+    zz = O.two_site(O.Z, O.Z)
+    monkeypatch.setenv("ITNBP_RDM_FAST_MIN", "1")
+    bpc.ctx.profile(True)
+    fx = itnbp.expect_on_cache(bpc, "X")
+    fzz = itnbp.expect2_on_cache(bpc, zz)
+    fvs, _ = itnbp.region_scalars(bpc)
+    n1, _ = bpc.ctx.kernel_time("rdm1_eig")
+    n2, _ = bpc.ctx.kernel_time("rdm2_eig")
+    bpc.ctx.profile(False)
+    assert n1 == 2 and n2 == 1  # <X> and the scalars; <ZZ>
+    monkeypatch.setenv("ITNBP_DISABLE_FAST", "1")
+    gx = itnbp.expect_on_cache(bpc, "X")
+    gzz = itnbp.expect2_on_cache(bpc, zz)
+    gvs, _ = itnbp.region_scalars(bpc)
+    assert max(abs(fx[v] - gx[v]) for v in gx) < 1e-12
+    assert max(abs(fzz[e] - gzz[e]) for e in gzz) < 1e-12
+    np.testing.assert_allclose(fvs, gvs, rtol=1e-11)
