@@ -29,7 +29,7 @@ def test_forest_cover_sequence_matches_oracle():
         assert oracle_sequence(g, seq) == O.forest_cover_sequence(og)
 
 
-def test_config1_sequential_4x4_chi2_messages_and_sz():
+def test_sequential_4x4_chi2_messages_and_sz():
     """4x4 open PEPS, d=2, chi=2, N(0,1), normalised; sequential schedule, 30 sweeps,
     tol 1e-12; all messages and <Sz> at every site (parity 1e-10)."""
     g, psi = make((4, 4), 2, 1234)

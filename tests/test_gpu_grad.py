@@ -34,7 +34,7 @@ def test_gradient_matches_autograd(dims, chi, sweeps, ck):
     assert np.sqrt(num / den) < 1e-10
 
 
-def test_config5_gradient_vs_finite_differences():
+def test_gradient_8x8_chi4_40_sweeps_vs_finite_differences():
     """8x8 square PEPS, d=2, chi=4, N(0,1): gradient of the BP Ising energy through 40
     unrolled sweeps (checkpoint every 8) vs central finite differences, step 1e-6,
     along random directions (relative agreement 1e-6)."""
