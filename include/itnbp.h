@@ -149,6 +149,11 @@ int itnbp_updated_messages(itnbp_net* net, int n, const int* msg_ids, int normal
                            double* out);
 
 /* ---- environments / expectation values / scalars ---------------------- */
+/* With every message symmetric, degree-4 and degree-3 sites with d = 2 and every
+ * chi = 32 take the eigenbasis route (FP64 rotations of the site into the
+ * eigenbasis of its incoming messages, then weighted Grams) when a request holds
+ * at least ITNBP_RDM_FAST_MIN (default 16) such sites or half-pairs; the others
+ * use the generic open contraction.  Both are FP64 and agree to rounding. */
 /* one-site RDMs rho_v[s, s'] (d x d, unnormalised) of the BP environment
  * (environment(bpc, [operator vertex]), caches/beliefpropagationcache.jl:100-105;
  * expect.jl:5-19 contracts it with op and with delta) */
