@@ -226,23 +226,27 @@ int upload_plan(itnbp_net* net, const TaskPlan& plan, DevBuf& taskbuf) {
   return ITNBP_OK;
 }
 
-// launch groups [g0, g1) of an uploaded plan
+// launch groups [g0, g1) of an uploaded plan, on the context's stream or on sm
+// (then without the per-kernel timing marks, which live on the context's stream)
 int launch_plan(itnbp_net* net, const TaskPlan& plan, size_t g0, size_t g1, const OcTask* d_tasks,
-                const void* psi, int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats) {
+                const void* psi, int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats,
+                hipStream_t sm) {
   itnbp_ctx* ctx = net->ctx;
+  const bool main = sm == nullptr;
+  hipStream_t s = main ? ctx->stream : sm;
   for (size_t gi = g0; gi < g1; ++gi) {
     const LaunchGroup& lg = plan.groups[gi];
-    const int e0 = prof_mark(ctx);
+    const int e0 = main ? prof_mark(ctx) : -1;
     if (lg.kind == 0) {
       HIPCHK(ctx, launch_oc_small(psi_dtype, plan.acc_f64, net->d_vdesc, d_tasks + lg.begin, (int)lg.count,
-                                  psi, msg_in, out, d_stats, lg.lds, ctx->stream));
-      prof_add(ctx, "oc_small", e0, prof_mark(ctx));
+                                  psi, msg_in, out, d_stats, lg.lds, s));
+      if (main) prof_add(ctx, "oc_small", e0, prof_mark(ctx));
     } else {
       HIPCHK(ctx, launch_oc_large(psi_dtype, plan.acc_f64, net->d_vdesc, d_tasks + lg.begin,
                                   plan.tasks.data() + lg.begin, net->vdesc.data(), (int)lg.count, psi,
                                   msg_in, out, d_stats, net->scratch.p, lg.maxD, net->partials.p,
-                                  lg.nsplit, lg.part_stride, ctx->stream));
-      prof_add(ctx, "oc_large", e0, prof_mark(ctx));
+                                  lg.nsplit, lg.part_stride, s));
+      if (main) prof_add(ctx, "oc_large", e0, prof_mark(ctx));
     }
   }
   return ITNBP_OK;
@@ -357,9 +361,23 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
       void* nxt = ph.msg[1 - net->cur];
       if (partial)
         HIPCHK(ctx, hipMemcpyAsync(nxt, cur, net->msg_total * eb, hipMemcpyDeviceToDevice, ctx->stream));
+      // beside the chi = 32 pipeline, the generic tasks (e.g. the lattice corners)
+      // run on the side stream: they read cur and write disjoint messages and
+      // statistics slots of their own
+      const bool fork = use_fast && !plan.groups.empty();
+      if (fork) {
+        HIPCHK(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+        HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        RETCHK(launch_plan(net, plan, 0, plan.groups.size(), d_tasks, ph.psi, ph.psi_dtype, cur, nxt, d_stats,
+                           ctx->side));
+        HIPCHK(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+      }
       if (use_fast) RETCHK(fast_sweep(net, ph.acc_f64, cur, nxt, d_stats, o->normalize));
       if (use_small) RETCHK(small_sweep(net, ph.acc_f64, ph.psi, ph.psi_dtype, cur, nxt, d_stats, o->normalize));
-      RETCHK(launch_plan(net, plan, 0, plan.groups.size(), d_tasks, ph.psi, ph.psi_dtype, cur, nxt, d_stats));
+      if (fork)
+        HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+      else
+        RETCHK(launch_plan(net, plan, 0, plan.groups.size(), d_tasks, ph.psi, ph.psi_dtype, cur, nxt, d_stats));
       net->cur = 1 - net->cur;
     } else {
       void* cur = ph.msg[net->cur];
@@ -473,7 +491,10 @@ int itnbp_create(int device, itnbp_ctx** out) {
   }
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
-      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+      hipStreamCreate(&ctx->side) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
     delete ctx;
     return ITNBP_ERR_HIP;
   }
@@ -487,9 +508,13 @@ int itnbp_destroy(itnbp_ctx* ctx) {
   if (!ctx) return ITNBP_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return ITNBP_OK;

@@ -110,6 +110,10 @@ struct itnbp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // second stream (default priority) for work that overlaps the main stream
+  // inside one call, forked and joined through ev_fork / ev_join
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   std::string path;
   int num_cu = 256;

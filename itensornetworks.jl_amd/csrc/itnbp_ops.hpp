@@ -21,7 +21,8 @@ OcTask message_task(const itnbp_net* net, int m, int slot, bool with_old, int no
 int plan_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, TaskPlan& plan);
 int upload_plan(itnbp_net* net, const TaskPlan& plan, DevBuf& taskbuf);
 int launch_plan(itnbp_net* net, const TaskPlan& plan, size_t g0, size_t g1, const OcTask* d_tasks,
-                const void* psi, int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats);
+                const void* psi, int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats,
+                hipStream_t sm = nullptr);
 // run open-contraction tasks; msg_in/out in the accumulator precision
 int run_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, const void* psi,
               int psi_dtype, const void* msg_in, void* out, MsgStat* d_stats, DevBuf& taskbuf);
