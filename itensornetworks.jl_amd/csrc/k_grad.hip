@@ -175,7 +175,8 @@ __global__ __launch_bounds__(256) void k_vertex_vjp(const VDesc* __restrict__ vd
                                                     const long long* __restrict__ out_msg_off,
                                                     const long long* __restrict__ in_msg_off,
                                                     const double* __restrict__ cot,
-                                                    double* __restrict__ grad, int normalize) {
+                                                    double* __restrict__ grad, int normalize,
+                                                    double* __restrict__ part, long long pstride) {
   extern __shared__ __align__(16) double sm[];
   __shared__ double red[8];
   const GradVertex gv = gvs[blockIdx.x];
@@ -205,7 +206,12 @@ __global__ __launch_bounds__(256) void k_vertex_vjp(const VDesc* __restrict__ vd
   for (int i = tid; i < off; i += blockDim.x) Mb[i] = 0;
   __syncthreads();
   const int nterms = mode == 0 ? k : 1 + k;
-  for (int term = 0; term < nterms; ++term) {
+  // with part: workgroup (x, y) takes term y of vertex x only and writes its
+  // contributions (psibar, then Mbar of every bond) to part, zeros for a term that
+  // does not exist; k_vjp_reduce sums the terms
+  const int t0 = part ? (int)blockIdx.y : 0;
+  const int t1 = part ? min((int)blockIdx.y + 1, nterms) : nterms;
+  for (int term = t0; term < t1; ++term) {
     int j, s_open;
     if (mode == 0) {
       j = term + 1;
@@ -257,11 +263,47 @@ __global__ __launch_bounds__(256) void k_vertex_vjp(const VDesc* __restrict__ vd
     vjp_term(V, g, nabs, axes, Ms, Xs, Tb, psibar, Mbars, Xb0, Xb1);
   }
   // outputs
+  if (part) {
+    double* pp = part + ((long long)blockIdx.x * gridDim.y + blockIdx.y) * pstride;
+    for (int i = tid; i < D + off; i += blockDim.x) pp[i] = i < D ? psibar[i] : Mb[i - D];
+    return;
+  }
   for (int i = tid; i < D; i += blockDim.x) grad[V.psi_off + i] += psibar[i];
   for (int a = 1; a <= k; ++a) {
     const int n2 = V.dims[a] * V.dims[a];
     double* dst = mbar_out + in_msg_off[(long long)blockIdx.x * ITNBP_MAXK + a - 1];
     for (int i = tid; i < n2; i += blockDim.x) dst[i] = Mb[mb_off[a] + i];
+  }
+}
+
+// Sum of the per-term partials of k_vertex_vjp (terms in order): grad += psibar,
+// Mbar of every incoming message = the sum.  grid (vertices), 256 threads.
+__global__ __launch_bounds__(256) void k_vjp_reduce(const VDesc* __restrict__ vds,
+                                                    const GradVertex* __restrict__ gvs, int mode,
+                                                    const double* __restrict__ part, long long pstride,
+                                                    int nty, double* __restrict__ mbar_out,
+                                                    const long long* __restrict__ in_msg_off,
+                                                    double* __restrict__ grad) {
+  const GradVertex gv = gvs[blockIdx.x];
+  const VDesc V = vds[gv.u];
+  const int D = V.D, k = V.k;
+  const int nterms = mode == 0 ? k : 1 + k;
+  const double* pp = part + (long long)blockIdx.x * nty * pstride;
+  for (int i = threadIdx.x; i < D; i += blockDim.x) {
+    double acc = 0;
+    for (int t = 0; t < nterms; ++t) acc += pp[(long long)t * pstride + i];
+    grad[V.psi_off + i] += acc;
+  }
+  int off = 0;
+  for (int a = 1; a <= k; ++a) {
+    const int n2 = V.dims[a] * V.dims[a];
+    double* dst = mbar_out + in_msg_off[(long long)blockIdx.x * ITNBP_MAXK + a - 1];
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      double acc = 0;
+      for (int t = 0; t < nterms; ++t) acc += pp[(long long)t * pstride + D + off + i];
+      dst[i] = acc;
+    }
+    off += n2;
   }
 }
 
@@ -364,8 +406,9 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
   }
   const long long MT = net->msg_total;
   const int ncheck = nsweeps / c + 1;
-  DevBuf ck, seg, mbar, cot, tables, gradbuf;
+  DevBuf ck, seg, mbar, cot, tables, gradbuf, vpart;
   auto cleanup = [&]() {
+    free_buf(vpart);
     free_buf(ck);
     free_buf(seg);
     free_buf(mbar);
@@ -545,9 +588,25 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
   GR_HIP(hipMemsetAsync(d_grad, 0, net->psi_total * 8, ctx->stream));
   double* mb[2] = {(double*)mbar.p, (double*)mbar.p + MT};
   // energy terms at the final messages M^(T)
-  hipLaunchKernelGGL(k_vertex_vjp, dim3(nv), dim3(256), lds, ctx->stream, net->d_vdesc, d_gv, 1,
+  // one workgroup per (vertex, term), then the fixed-order sum of the terms
+  int maxk = 0;
+  long long pstride = 1;
+  for (int v = 0; v < nv; ++v) {
+    const VDesc& V = net->vdesc[v];
+    long long w = V.D;
+    for (int a = 1; a <= V.k; ++a) w += (long long)V.dims[a] * V.dims[a];
+    maxk = std::max(maxk, V.k);
+    pstride = std::max(pstride, w);
+  }
+  const int nty = maxk + 1;
+  GR_CHK(ensure_buf(ctx, vpart, (size_t)std::max(1, nv) * nty * pstride * 8));
+  double* d_part = (double*)vpart.p;
+  hipLaunchKernelGGL(k_vertex_vjp, dim3(nv, nty), dim3(256), lds, ctx->stream, net->d_vdesc, d_gv, 1,
                      (const double*)net->d_psi, (const double*)net->d_msg[net->cur], (const double*)nullptr, mb[0],
-                     d_out, d_in, (const double*)C, d_grad, 1);
+                     d_out, d_in, (const double*)C, d_grad, 1, d_part, pstride);
+  GR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_reduce, dim3(nv), dim3(256), 0, ctx->stream, net->d_vdesc, d_gv, 1,
+                     (const double*)d_part, pstride, nty, mb[0], d_in, d_grad);
   GR_HIP(hipGetLastError());
   int cb = 0;  // mb[cb] holds Mbar^(t)
   // segments in reverse: segment s covers t-1 in [s c, s c + c - 1]
@@ -562,9 +621,13 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
     GR_HIP(hipMemcpyAsync(d_seg, d_ck + (long long)s * MT, MT * 8, hipMemcpyDeviceToDevice, ctx->stream));
     for (int tm1 = top; tm1 >= base; --tm1) {
       const double* prev = d_seg + (long long)(tm1 - base) * MT;
-      hipLaunchKernelGGL(k_vertex_vjp, dim3(nv), dim3(256), lds, ctx->stream, net->d_vdesc, d_gv, 0,
+      if (maxk == 0) continue;  // no messages: nothing flows back through the sweeps
+      hipLaunchKernelGGL(k_vertex_vjp, dim3(nv, maxk), dim3(256), lds, ctx->stream, net->d_vdesc, d_gv, 0,
                          (const double*)net->d_psi, prev, (const double*)mb[cb], mb[1 - cb], d_out, d_in,
-                         (const double*)C, d_grad, 1);
+                         (const double*)C, d_grad, 1, d_part, pstride);
+      GR_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_vjp_reduce, dim3(nv), dim3(256), 0, ctx->stream, net->d_vdesc, d_gv, 0,
+                         (const double*)d_part, pstride, maxk, mb[1 - cb], d_in, d_grad);
       GR_HIP(hipGetLastError());
       cb = 1 - cb;
     }
