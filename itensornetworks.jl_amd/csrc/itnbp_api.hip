@@ -355,6 +355,7 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
   double red[4] = {0, 0, 0, 0};
   double best_res2 = INFINITY;
   int stall = 0;
+  const bool watch = stop_res2 >= 0 || (stop_on_tol && (o->tol >= 0 || o->res_tol >= 0)) || o->verbose;
   for (it = 1; it <= maxiter; ++it) {
     if (o->schedule == ITNBP_SCHED_PARALLEL) {
       void* cur = ph.msg[net->cur];
@@ -388,6 +389,10 @@ int sweep_phase(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_
     if (trace && it % trace_every == 0)
       HIPCHK(ctx, hipMemcpyAsync(trace + (long long)(it / trace_every) * trace_stride, ph.msg[net->cur],
                                  net->msg_total * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    // with no stopping or hand-over rule the statistics are read after the last
+    // sweep only (a non-finite message persists through later sweeps, so it is
+    // still caught there)
+    if (!watch && it < maxiter) continue;
     HIPCHK(ctx, launch_reduce_stats(d_stats, n, d_red, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(red, d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
