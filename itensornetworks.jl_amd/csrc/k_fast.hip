@@ -1968,13 +1968,6 @@ struct bf3 {
   bf16x8 h, m, l;
 };
 
-__device__ inline void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
-  h = (__bf16)x;
-  const float r = x - (float)h;
-  m = (__bf16)r;
-  l = (__bf16)(r - (float)m);
-}
-
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
@@ -1987,8 +1980,10 @@ __device__ inline f32x2 widen_bf16x2(bf16x2 v) {
   return r;
 }
 
-// split3 of eight values, two at a time: each rounding is one packed
-// conversion whose result is already the operand's register pair
+// Three-term split of eight values, x = h + m + l exactly to FP32 rounding:
+// h = bf16(x), m = bf16(x - h), l = bf16(x - h - m) (round to nearest even; the
+// differences are exact in FP32).  Two values at a time: each rounding is one
+// packed conversion whose result is already the operand's register pair.
 __device__ inline bf3 split8x3(const float (&x)[8]) {
   bf3 o;
 #pragma unroll
