@@ -47,23 +47,16 @@ WORKLOADS = {
     "peps32x32_chi32_bf16": dict(dims=(32, 32), chi=32, storage="bf16", scale="invsqrtchi"),
     "peps16x16_chi8_f64": dict(dims=(16, 16), chi=8, storage="f64", scale=1.0),
 }
-KERNEL_NAMES = ("eigh", "rot12_x3", "rot34_x3", "gram12_x3", "deg3_x3", "rot12_bf16", "rot34_bf16", "gram12_bf16",
+KERNEL_NAMES = ("eigh", "rot12_x3", "rot34_x3", "gram12_x3", "deg3_x3", "rot12_h2", "rot34_h2", "gram12_h2",
+                "deg3_h2", "rot12_bf16", "rot34_bf16", "gram12_bf16",
                 "deg3_bf16", "rot12_f64", "rot34_f64",
                 "gram12_f64", "deg3_f64", "absorb_bf16", "fused_bf16", "absorb_f64", "fused_f64", "finalize",
                 "oc_small", "oc_large")
-# Per 32x32 block of a bulk (degree-4, chi = 32) site: algorithmic HBM bytes and
-# 32^3 GEMM units (2 * 32^3 flops each) of every streaming kernel, and how many
-# blocks of a site one sweep pushes through it (2048 blocks per site and pass).
-#   rot12: X in, U1^T X U2 out;  rot34: R in, Y out + two Grams (Y and Y^T
-#   rotations: 3 units);  gram12: Q in, two Grams (+ the exact identity product
-#   for D^T);  absorb / fused: the absorbed-pair passes (two per sweep).
-BLOCK_INFO = {
-    "rot12_x3": (2048 + 4096, 2, 1), "rot34_x3": (4096 + 4096, 5, 1), "gram12_x3": (4096, 3, 1),
-    "rot12_bf16": (2048 + 2048, 2, 1), "rot34_bf16": (2048 + 2048, 5, 1), "gram12_bf16": (2048, 3, 1),
-    "rot12_f64": (8192 + 8192, 2, 1), "rot34_f64": (8192 + 8192, 5, 1), "gram12_f64": (8192, 2, 1),
-    "absorb_bf16": (2048 + 2048, 2, 2), "fused_bf16": (2048 + 2048, 4, 2),
-    "absorb_f64": (8192 + 8192, 2, 2), "fused_f64": (8192 + 8192, 4, 2),
-}
+# Streaming kernels that cover every bulk (degree-4, chi = 32) site of a sweep in
+# one launch: their algorithmic work per launch is that of the 900 bulk sites.
+BULK_KERNELS = ("rot12_x3", "rot34_x3", "gram12_x3", "rot12_h2", "rot34_h2", "gram12_h2", "rot12_bf16",
+                "rot34_bf16", "gram12_bf16",
+                "rot12_f64", "rot34_f64", "gram12_f64")
 # HBM bytes per bulk site and launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # (calibrated against a device-to-device copy of known size) of the kernels of
 # the 32x32 chi=32 workload: profiles/eig_pipeline_peps32x32_chi32.md
@@ -92,12 +85,38 @@ def log(*a):
 
 def message_flops(psi, g):
     """Algorithmic flops of one sweep: sum over messages u->v of 2 d prod(chi_u) sum(chi_u)."""
-    F = 0
-    for v in g.vertices:
+    return algorithmic_work(psi, g.vertices, 0, 0)[1]
+
+
+def algorithmic_work(psi, sites, w_psi, w_msg):
+    """Algorithmic work of recomputing every outgoing message of ``sites`` once:
+    bytes = each site tensor read once (w_psi bytes per element) plus every
+    incoming message read and every outgoing message written once (w_msg bytes);
+    flops = 2 d prod(chi_u) sum(chi_u) per outgoing message u -> v."""
+    nbytes, flops = 0, 0
+    for v in sites:
         t = psi[v]
-        k = t.ndim - 1
-        F += k * 2 * t.size * sum(t.shape[1:])
-    return F
+        chis = t.shape[1:]
+        k = len(chis)
+        nbytes += w_psi * t.size + w_msg * 2 * sum(c * c for c in chis)
+        flops += k * 2 * t.size * sum(chis)
+    return nbytes, flops
+
+
+def roofline_of(nbytes, flops, seconds, peak_tflops):
+    """Bound by arithmetic intensity against the ridge, achieved rate and fraction
+    of the bounding peak, plus both fractions."""
+    ai = flops / nbytes
+    ridge = peak_tflops * 1e12 / (PEAK_HBM_GBS * 1e9)
+    gbs = nbytes / seconds / 1e9
+    tfs = flops / seconds / 1e12
+    out = {"arith_intensity": ai, "ridge": ridge, "hbm_frac": gbs / PEAK_HBM_GBS, "mfma_frac": tfs / peak_tflops,
+           "achieved_gbs": gbs, "achieved_tflops": tfs}
+    if ai < ridge:
+        out.update(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s", frac=gbs / PEAK_HBM_GBS)
+    else:
+        out.update(bound="mfma", achieved=tfs, peak=peak_tflops, unit="TFLOP/s", frac=tfs / peak_tflops)
+    return out
 
 
 def cpu_baseline(psi, g, seconds):
@@ -154,6 +173,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="peps32x32_chi32_bf16", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--chosen", default="bf16_bf16x3", choices=("bf16_bf16x3", "bf16_f16x2"),
+                    help="progressive precision of the timed path (BF16 workload)")
+    ap.add_argument("--switch-tol", type=float, default=1e-2,
+                    help="residual that ends the one-term 16-bit phase of the timed path")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -179,7 +202,7 @@ def main():
     # progressive schedule: one-term FP16 sweeps while the residual > 1e-2, then
     # three-term BF16 sweeps.  The plain FP64 GPU path runs the same solves.
     STEP_RES_TOL = 1e-5
-    chosen_kw = dict(precision="bf16_bf16x3", switch_tol=1e-2) if low else dict(precision="f64")
+    chosen_kw = dict(precision=a.chosen, switch_tol=a.switch_tol) if low else dict(precision="f64")
 
     def solve_once(c, **kw):
         c.ctx.check(c.lib.itnbp_init_identity_messages(c.h))
@@ -267,45 +290,45 @@ def main():
     if low:
         sweep_rates["bf16x3"] = sweep_rate("bf16", "bf16x3", nsw)
         sweep_rates["fp16"] = sweep_rate("bf16", "f32", nsw)
+        sweep_rates["f16x2"] = sweep_rate("bf16", "f16x2", nsw)
     log(f"sweep rates { {k: round(v['ms_per_sweep'], 3) for k, v in sweep_rates.items()} }")
 
-    # ---- roofline of the dominant kernel of the chosen solves: algorithmic bytes
-    # (and GEMM flops) per launch from BLOCK_INFO x the blocks one launch covers,
-    # over the kernel's average launch time (HIP events on the library's stream)
-    dom = max((k for k in kernels if k in BLOCK_INFO), key=lambda k: kernels[k]["total_ms"], default=None)
-    nbulk = sum(1 for v in g.vertices if g.degree(v) == 4) if W["chi"] == 32 else 0
+    # ---- roofline of the dominant kernel of the chosen solves.  Its launch covers
+    # the bulk sites of one sweep; the algorithmic work of that launch is the
+    # work of recomputing every message out of those sites once (site tensors
+    # read once in the storage dtype, FP32 messages in and out; F flops per
+    # message), over the kernel's average launch time (HIP events on the
+    # library's stream).  `traffic` is the kernel's calibrated counter bytes.
+    bulk = [v for v in g.vertices if g.degree(v) == 4] if W["chi"] == 32 else []
+    dom = max((k for k in kernels if k in BULK_KERNELS), key=lambda k: kernels[k]["total_ms"], default=None)
+    w_psi = 2 if low else 8
     roof = None
-    if dom is not None and nbulk:
-        per_block, units, passes = BLOCK_INFO[dom]
-        if dom.startswith(("absorb_bf16", "fused_bf16")) and os.environ.get("ITNBP_BF16_PRECISE") == "1":
-            per_block += 2048   # P in FP32
-        # the eigenbasis kernels cover every bulk site in one launch per sweep; the
-        # absorbed-pair passes split a sweep into chunks of sites
-        eig_kernel = dom.startswith(("rot12", "rot34", "gram12"))
-        launches_per_sweep = 1.0 if eig_kernel else kernels[dom]["launches"] / sweeps
-        blocks = passes * nbulk * 2048 / launches_per_sweep
-        flops = blocks * units * 2 * 32 ** 3
-        nbytes = blocks * per_block
+    if dom is not None and bulk:
+        w_msg = 8 if dom.endswith("f64") else 4
+        nbytes, flops = algorithmic_work(psi, bulk, w_psi, w_msg)
         avg_s = kernels[dom]["avg_ms"] / 1e3
-        ai = flops / nbytes
-        is_bf16 = not dom.endswith("f64")
-        peak_f = PEAK_BF16_TFLOPS if is_bf16 else PEAK_F64_TFLOPS
-        ridge = peak_f * 1e12 / (PEAK_HBM_GBS * 1e9)
-        if ai < ridge:
-            ach = nbytes / avg_s / 1e9
-            roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": ach / PEAK_HBM_GBS, "traffic": None}
-        else:
-            ach = flops / avg_s / 1e12
-            roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak_f, "unit": "TFLOP/s",
-                    "frac": ach / peak_f, "traffic": None}
-        if dom in MEASURED_TRAFFIC_PER_SITE and eig_kernel:
-            roof["traffic"] = MEASURED_TRAFFIC_PER_SITE[dom] * nbulk
+        peak_f = PEAK_F64_TFLOPS if dom.endswith("f64") else PEAK_BF16_TFLOPS
+        roof = {"kernel": dom}
+        roof.update(roofline_of(nbytes, flops, avg_s, peak_f))
+        roof["traffic"] = None
+        if dom in MEASURED_TRAFFIC_PER_SITE:
+            roof["traffic"] = MEASURED_TRAFFIC_PER_SITE[dom] * len(bulk)
+            roof["traffic_over_algorithmic"] = roof["traffic"] / nbytes
             roof["traffic_source"] = TRAFFIC_SOURCE[dom]
-        roof.update({"algorithmic_bytes_per_launch": nbytes, "flops_per_launch": flops,
-                     "blocks_per_launch": blocks, "arith_intensity": ai,
-                     "avg_launch_ms": kernels[dom]["avg_ms"],
-                     "achieved_tflops": flops / avg_s / 1e12})
+        roof.update({"units_per_launch": f"{len(bulk)} bulk sites (degree 4, chi 32), {4 * len(bulk)} messages",
+                     "algorithmic_bytes_per_launch": nbytes, "flops_per_launch": flops,
+                     "avg_launch_ms": kernels[dom]["avg_ms"]})
+    # whole sweeps of each flavour: every site and message of the lattice
+    sweep_bytes = {k: algorithmic_work(psi, g.vertices, w_psi if k != "f64" else 8, 8 if k == "f64" else 4)[0]
+                   for k in sweep_rates}
+    whole = {}
+    for k, sr in sweep_rates.items():
+        peak_f = PEAK_F64_TFLOPS if k == "f64" else PEAK_BF16_TFLOPS
+        whole[k] = roofline_of(sweep_bytes[k], F, sr["ms_per_sweep"] / 1e3, peak_f)
+        whole[k]["ms_per_sweep"] = sr["ms_per_sweep"]
+    whole["chosen_solve"] = roofline_of(sweep_bytes.get("bf16x3", sweep_bytes["f64"]), F,
+                                        wall / sweeps, PEAK_BF16_TFLOPS if low else PEAK_F64_TFLOPS)
+    whole["chosen_solve"]["ms_per_sweep"] = wall / sweeps * 1e3
 
     # ---- full solves and accuracy of <X>, <ZZ> vs FP64 (one-time per-network setup --
     # block layouts, autotuning -- is done by a warm-up sweep before the clock starts)
@@ -389,14 +412,20 @@ def main():
             return None
 
         acc64 = to_accuracy("f64")
-        accprog = to_accuracy("bf16", precision="bf16_bf16x3", switch_tol=1e-2)
+        accprog = to_accuracy("bf16", **chosen_kw)
+        accalt = to_accuracy("bf16", precision="bf16_f16x2" if a.chosen == "bf16_bf16x3" else "bf16_bf16x3",
+                             switch_tol=a.switch_tol)
+        acch2 = to_accuracy("bf16", precision="f16x2")
         accx3 = to_accuracy("bf16", precision="bf16x3")
         accmix = to_accuracy("bf16", precision="bf16_f64", switch_tol=1e-4)
         solve = {
             "fp64": {"sweeps": it_ref, "seconds": t_ref, "updates_per_s": nmsg * it_ref / t_ref,
                      "max_residual": res_ref},
             "observables": observables,
-            "to_err_X_1e-6": {"fp64": acc64, "chosen": accprog, "bf16x3": accx3, "fp16_then_fp64": accmix,
+            "to_err_X_1e-6": {"fp64": acc64, "chosen": accprog, "other_progressive": accalt, "bf16x3": accx3,
+                              "f16x2": acch2, "fp16_then_fp64": accmix,
+                              "speedup_other_progressive": (acc64["seconds"] / accalt["seconds"])
+                              if acc64 and accalt else None,
                               "speedup": (acc64["seconds"] / accprog["seconds"]) if acc64 and accprog else None,
                               "speedup_bf16x3": (acc64["seconds"] / accx3["seconds"]) if acc64 and accx3 else None,
                               "speedup_fp16_then_fp64": (acc64["seconds"] / accmix["seconds"])
@@ -478,19 +507,9 @@ def main():
         del c2
         # 256 heterogeneous instances (L x L grids, L in 6..12, or heavy-hex), chi in
         # {4, 6, 8, 12, 16}, FP32 N(0, 1/chi), 50 Jacobi sweeps in shared launches
-        rng3 = np.random.default_rng(3)
-        caches, nmsg3 = [], 0
-        for _ in range(256):
-            if rng3.random() < 0.5:
-                L = int(rng3.integers(6, 13))
-                g3 = itnbp.named_grid((L, L))
-            else:
-                L = int(rng3.integers(3, 7))
-                g3 = itnbp.heavy_hex(L, L)
-            chi3 = int(rng3.choice([4, 6, 8, 12, 16]))
-            psi3 = itnbp.random_tensornetwork(rng3, np.float32, g3, link_space=chi3, scale=1 / np.sqrt(chi3))
-            caches.append(itnbp.initialize_cache(psi3))
-            nmsg3 += 2 * g3.ne
+        inst3 = itnbp.random_instance_set(np.random.default_rng(3), 256)
+        caches = [itnbp.initialize_cache(p3) for _, _, p3 in inst3]
+        nmsg3 = sum(2 * p3.graph.ne for _, _, p3 in inst3)
         ctx.check(caches[0].lib.itnbp_synchronize(ctx.h))
         t = time.perf_counter()
         res3 = itnbp.update_batched_(caches, maxiter=50, precision="f32")
@@ -537,11 +556,18 @@ def main():
         "algorithmic_tflops_per_sweep": F / 1e12,
         "kernels": kernels,
         "roofline": roof,
+        "roofline_whole_sweep": whole,
         "solve": solve,
         "extras": extras,
         "cpu_baseline": {"value": cpu_rate, "unit": "message-updates/s", "cores": cpu_threads,
                          "kind": "port",
-                         "sample": f"{cpu_n} FP64 bulk-vertex message updates (chi=32) by the NumPy oracle"},
+                         "sample": (f"{cpu_n} FP64 message updates out of 8 bulk sites (degree 4, chi = 32) from "
+                                    f"identity messages, {a.cpu_seconds:g} s budget"),
+                         "what": ("CPU restatement of the reference algorithm (NumPy FP64, BLAS threads as "
+                                  "counted in `cores`; every message contracted on its own, no partial products "
+                                  "shared); the reference itself is not runnable here (no Julia offline). It omits "
+                                  "the reference's deep copy of the whole cache on every message update, so it is "
+                                  "faster than the reference would be.")},
     }
     print(json.dumps(line), flush=True)
 

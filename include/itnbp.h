@@ -68,9 +68,16 @@ enum {
   ITNBP_PREC_BF16_BF16X3 = 5, /* progressive: the ITNBP_PREC_F32 kernels until the
                                  residual < switch_tol (or stops decreasing), then
                                  ITNBP_PREC_BF16X3 */
-  ITNBP_PREC_BF16_BF16X3_F64 = 6 /* the ITNBP_PREC_F32 kernels until the residual
+  ITNBP_PREC_BF16_BF16X3_F64 = 6, /* the ITNBP_PREC_F32 kernels until the residual
                                  < 1e-2, ITNBP_PREC_BF16X3 until < switch_tol (or
                                  stagnating), then FP64 refinement */
+  ITNBP_PREC_F16X2 = 7,       /* compensated FP16 MFMA on the chi = 32 eigenbasis
+                                 pipeline: every rounded operand split into two FP16
+                                 terms (about 22 significant bits), FP32 intermediates
+                                 and messages; elsewhere as ITNBP_PREC_BF16X3 */
+  ITNBP_PREC_BF16_F16X2 = 8   /* progressive: the ITNBP_PREC_F32 kernels until the
+                                 residual < switch_tol (or stops decreasing), then
+                                 ITNBP_PREC_F16X2 */
 };
 
 typedef struct itnbp_ctx itnbp_ctx;
@@ -116,6 +123,9 @@ int itnbp_net_destroy(itnbp_net* net);
  * caches/beliefpropagationcache.jl:43-47) */
 int itnbp_net_copy(itnbp_net* src, itnbp_net** out);
 int64_t itnbp_net_psi_numel(itnbp_net* net);
+/* current bond dimension of every edge (they change with itnbp_apply_gates);
+ * linkinds / dim of the reference network (src/abstractitensornetwork.jl) */
+int itnbp_get_bond_dims(itnbp_net* net, int* bond_dims);
 int64_t itnbp_net_msg_numel(itnbp_net* net);
 
 /* site tensors, all vertices, column-major, in the net's dtype
@@ -179,11 +189,18 @@ int itnbp_normalize(itnbp_net* net);
  * disjoint bonds, envs = the net's current messages (apply.jl:33-146).
  * gates: n (d_u d_v)^2 FP64 matrices. maxdim <= 0: no cap; cutoff < 0: none.
  * truncerr[n]; svals[n*max_keep] (zero padded) may be NULL; kept[n] may be NULL.
- * The bond keeps its dimension: if fewer values survive, the tensors are
- * zero padded (the network is unchanged). */
+ * Each gate's bond takes the kept dimension (apply.jl:81-93): it grows or
+ * shrinks on the device; the messages of a bond whose dimension changed become
+ * identities of the new dimension; itnbp_get_bond_dims reports the new
+ * dimensions. */
 int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
                       int maxdim, double cutoff, int normalize, double* truncerr,
                       double* svals, int max_keep, int* kept);
+/* one-site gates psi_v <- O psi_v on n distinct vertices (apply.jl:110-116):
+ * ops: n d x d matrices O[s', s]; normalize divides each updated tensor by its
+ * Frobenius norm (:113-115); norms[n] (may be NULL) receives ||O psi_v||. */
+int itnbp_apply_site_gates(itnbp_net* net, int n, const int* verts, const double* ops, int normalize,
+                           double* norms);
 
 /* ---- adjoint ---------------------------------------------------------- */
 /* BP energy E = sum_e <O2_e> + sum_v <O1_v> after nsweeps Jacobi sweeps from

@@ -46,15 +46,7 @@ int set_err(itnbp_ctx* ctx, int code, const std::string& msg) {
 
 template <class F>
 int guarded(itnbp_ctx* ctx, F&& f) {
-  try {
-    return f();
-  } catch (const std::bad_alloc&) {
-    return set_err(ctx, ITNBP_ERR_NOMEM, "host allocation failed");
-  } catch (const std::exception& e) {
-    return set_err(ctx, ITNBP_ERR_ARG, e.what());
-  } catch (...) {
-    return set_err(ctx, ITNBP_ERR_ARG, "unknown error");
-  }
+  return itnbp_guarded(ctx, static_cast<F&&>(f));
 }
 
 }  // namespace
@@ -428,11 +420,33 @@ namespace {
 
 int default_sequence(const itnbp_net* net, int schedule, std::vector<int>& seq);
 
+// reduced-precision modes (every ITNBP_PREC_* but F64)
+bool prec_known(int p) { return p >= ITNBP_PREC_F32 && p <= ITNBP_PREC_BF16_F16X2; }
+// compensated arithmetic of the (last) reduced-precision phase: 0 none, 1 three
+// BF16 terms, 2 two FP16 terms
+int prec_split(int p) {
+  if (p == ITNBP_PREC_F16X2 || p == ITNBP_PREC_BF16_F16X2) return 2;
+  if (p == ITNBP_PREC_BF16X3 || p == ITNBP_PREC_BF16X3_F64 || p == ITNBP_PREC_BF16_BF16X3 ||
+      p == ITNBP_PREC_BF16_BF16X3_F64)
+    return 1;
+  return 0;
+}
+// one-term 16-bit phase first, then the compensated one
+bool prec_progressive(int p) {
+  return p == ITNBP_PREC_BF16_BF16X3 || p == ITNBP_PREC_BF16_BF16X3_F64 || p == ITNBP_PREC_BF16_F16X2;
+}
+// no FP64 refinement phase
+bool prec_low_only(int p) {
+  return p == ITNBP_PREC_F32 || p == ITNBP_PREC_BF16X3 || p == ITNBP_PREC_BF16_BF16X3 || p == ITNBP_PREC_F16X2 ||
+         p == ITNBP_PREC_BF16_F16X2;
+}
+
 int check_sequence(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_opts* o) {
   std::vector<char> set(net->msg_set.begin(), net->msg_set.end());
-  const bool need_old = o->tol >= 0 || o->res_tol >= 0 || o->precision == ITNBP_PREC_BF16_F64 ||
-                        o->precision == ITNBP_PREC_BF16X3_F64 || o->precision == ITNBP_PREC_BF16_BF16X3 ||
-                        o->precision == ITNBP_PREC_BF16_BF16X3_F64;
+  // every phased mode measures the residual against the previous messages
+  const bool need_old = o->tol >= 0 || o->res_tol >= 0 ||
+                        (o->precision != ITNBP_PREC_F64 && o->precision != ITNBP_PREC_F32 &&
+                         o->precision != ITNBP_PREC_BF16X3 && o->precision != ITNBP_PREC_F16X2);
   for (int m : seq) {
     if (m < 0 || m >= 2 * net->ne) return set_err(net->ctx, ITNBP_ERR_ARG, "message id out of range in edge sequence");
     const int u = msg_src(net, m), v = msg_dst(net, m);
@@ -703,6 +717,152 @@ int itnbp_net_copy(itnbp_net* src, itnbp_net** out) {
   return ITNBP_OK;
 }
 
+}  // extern "C"
+
+// ---- bond resizing -------------------------------------------------------------
+namespace {
+
+struct RepackDesc {
+  long long src_off, dst_off, D;      // element offsets and new element count
+  int r;                              // rank (physical + bonds)
+  int old_dims[ITNBP_MAXR], new_dims[ITNBP_MAXR];
+};
+
+// dst tensor v = src tensor v cut or zero-padded along every axis to the new
+// dimensions (column-major); grid (chunks, vertices)
+template <typename T>
+__global__ void k_repack_psi(const T* __restrict__ src, T* __restrict__ dst, const RepackDesc* __restrict__ rd) {
+  const RepackDesc d = rd[blockIdx.y];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < d.D; i += (long long)gridDim.x * blockDim.x) {
+    long long rem = i, so = 0, stride = 1;
+    bool in = true;
+    for (int a = 0; a < d.r; ++a) {
+      const int x = (int)(rem % d.new_dims[a]);
+      rem /= d.new_dims[a];
+      in = in && x < d.old_dims[a];
+      so += stride * x;
+      stride *= d.old_dims[a];
+    }
+    T z;
+    memset(&z, 0, sizeof(T));
+    dst[d.dst_off + i] = in ? src[d.src_off + so] : z;
+  }
+}
+
+struct MsgRepack {
+  long long src_off, dst_off;
+  int chi, reset;  // new chi; reset: identity of the new dimension
+};
+
+__global__ void k_repack_msgs(const double* __restrict__ src, double* __restrict__ dst,
+                              const MsgRepack* __restrict__ mr) {
+  const MsgRepack m = mr[blockIdx.x];
+  for (int i = threadIdx.x; i < m.chi * m.chi; i += blockDim.x)
+    dst[m.dst_off + i] = m.reset ? ((i % m.chi) == (i / m.chi) ? 1.0 : 0.0) : src[m.src_off + i];
+}
+
+}  // namespace
+
+// Gives every edge the bond dimension chi_new[e]: site tensors are cut or
+// zero-padded along the changed bonds, the messages of a changed edge become
+// identities of the new dimension (set), the others are kept.  Every state
+// derived from the bond dimensions is released.
+int net_resize_bonds(itnbp_net* net, const std::vector<int>& chi_new) {
+  itnbp_ctx* ctx = net->ctx;
+  if ((int)chi_new.size() != net->ne) return set_err(ctx, ITNBP_ERR_ARG, "resize: one dimension per edge");
+  if (chi_new == net->chi) return ITNBP_OK;
+  itnbp_net* tmp = nullptr;
+  RETCHK(itnbp_net_create(ctx, net->nv, net->ne, net->edges.data(), net->inc_ptr.data(), net->inc_edge.data(),
+                          chi_new.data(), net->phys.data(), net->dtype, &tmp));
+  std::unique_ptr<itnbp_net, int (*)(itnbp_net*)> guard(tmp, itnbp_net_destroy);
+  std::vector<RepackDesc> rd(net->nv);
+  for (int v = 0; v < net->nv; ++v) {
+    const VDesc& A = net->vdesc[v];
+    const VDesc& B = tmp->vdesc[v];
+    rd[v].src_off = A.psi_off;
+    rd[v].dst_off = B.psi_off;
+    rd[v].D = B.D;
+    rd[v].r = A.k + 1;
+    for (int a = 0; a <= A.k; ++a) {
+      rd[v].old_dims[a] = A.dims[a];
+      rd[v].new_dims[a] = B.dims[a];
+    }
+  }
+  std::vector<MsgRepack> mr(2 * net->ne);
+  for (int m = 0; m < 2 * net->ne; ++m) {
+    const int e = m / 2;
+    mr[m].src_off = net->msg_off[m];
+    mr[m].dst_off = tmp->msg_off[m];
+    mr[m].chi = chi_new[e];
+    mr[m].reset = chi_new[e] != net->chi[e] ? 1 : 0;
+  }
+  DevBuf b1, b2;
+  RETCHK(upload(ctx, b1, rd.data(), rd.size() * sizeof(RepackDesc)));
+  if (!mr.empty()) RETCHK(upload(ctx, b2, mr.data(), mr.size() * sizeof(MsgRepack)));
+  long long maxD = 1;
+  for (auto& d : rd) maxD = std::max(maxD, d.D);
+  const dim3 grid((unsigned)std::min<long long>((maxD + 255) / 256, 1024), net->nv);
+  switch (net->dtype) {
+    case ITNBP_F64:
+      hipLaunchKernelGGL(k_repack_psi<double>, grid, dim3(256), 0, ctx->stream, (const double*)net->d_psi,
+                         (double*)tmp->d_psi, (const RepackDesc*)b1.p);
+      break;
+    case ITNBP_F32:
+      hipLaunchKernelGGL(k_repack_psi<float>, grid, dim3(256), 0, ctx->stream, (const float*)net->d_psi,
+                         (float*)tmp->d_psi, (const RepackDesc*)b1.p);
+      break;
+    default:
+      hipLaunchKernelGGL(k_repack_psi<bf16_t>, grid, dim3(256), 0, ctx->stream, (const bf16_t*)net->d_psi,
+                         (bf16_t*)tmp->d_psi, (const RepackDesc*)b1.p);
+      break;
+  }
+  if (!mr.empty())
+    hipLaunchKernelGGL(k_repack_msgs, dim3(2 * net->ne), dim3(256), 0, ctx->stream, (const double*)net->d_msg[net->cur],
+                       tmp->d_msg[0], (const MsgRepack*)b2.p);
+  hipError_t ek = hipGetLastError();
+  if (ek == hipSuccess) ek = hipStreamSynchronize(ctx->stream);
+  free_buf(b1);
+  free_buf(b2);
+  if (ek != hipSuccess) return set_err(ctx, ITNBP_ERR_HIP, std::string("resize: ") + hipGetErrorString(ek));
+  // hand tmp's description and buffers to net; net's old ones go with tmp
+  release_fast_state(net);
+  release_small_state(net);
+  for (int m = 0; m < 2 * net->ne; ++m)
+    if (mr[m].reset) net->msg_set[m] = 1;
+  std::swap(net->chi, tmp->chi);
+  std::swap(net->psi_off, tmp->psi_off);
+  std::swap(net->msg_off, tmp->msg_off);
+  std::swap(net->psi_total, tmp->psi_total);
+  std::swap(net->msg_total, tmp->msg_total);
+  std::swap(net->vdesc, tmp->vdesc);
+  std::swap(net->max_chi, tmp->max_chi);
+  std::swap(net->max_k, tmp->max_k);
+  std::swap(net->d_psi, tmp->d_psi);
+  std::swap(net->d_psi64, tmp->d_psi64);
+  for (int i = 0; i < 2; ++i) {
+    std::swap(net->d_msg[i], tmp->d_msg[i]);
+    std::swap(net->d_msg32[i], tmp->d_msg32[i]);
+  }
+  std::swap(net->d_vdesc, tmp->d_vdesc);
+  std::swap(net->scratch, tmp->scratch);
+  std::swap(net->tasks, tmp->tasks);
+  std::swap(net->stats, tmp->stats);
+  std::swap(net->partials, tmp->partials);
+  std::swap(net->red, tmp->red);
+  std::swap(net->aux, tmp->aux);
+  std::swap(net->mdesc, tmp->mdesc);
+  net->cur = 0;
+  return ITNBP_OK;
+}
+
+extern "C" {
+
+int itnbp_get_bond_dims(itnbp_net* net, int* bond_dims) {
+  if (!net || (net->ne > 0 && !bond_dims)) return ITNBP_ERR_ARG;
+  std::copy(net->chi.begin(), net->chi.end(), bond_dims);
+  return ITNBP_OK;
+}
+
 int64_t itnbp_net_psi_numel(itnbp_net* net) { return net ? net->psi_total : -1; }
 int64_t itnbp_net_msg_numel(itnbp_net* net) { return net ? net->msg_total : -1; }
 
@@ -877,18 +1037,14 @@ int itnbp_update(itnbp_net* net, const itnbp_update_opts* o, itnbp_update_result
       ctx->path = fast ? fast_path_name(net, 1) : "generic-f64";
       RETCHK(sweep_phase(net, seq, o, ph, o->maxiter, -1.0, true, &it, &md, &mr, &conv, fast));
       r.iterations = it;
-    } else if (o->precision == ITNBP_PREC_F32 || o->precision == ITNBP_PREC_BF16_F64 ||
-               o->precision == ITNBP_PREC_BF16X3 || o->precision == ITNBP_PREC_BF16X3_F64 ||
-               o->precision == ITNBP_PREC_BF16_BF16X3 || o->precision == ITNBP_PREC_BF16_BF16X3_F64) {
-      const bool x3 = o->precision == ITNBP_PREC_BF16X3 || o->precision == ITNBP_PREC_BF16X3_F64 ||
-                      o->precision == ITNBP_PREC_BF16_BF16X3 || o->precision == ITNBP_PREC_BF16_BF16X3_F64;
-      const bool progressive = o->precision == ITNBP_PREC_BF16_BF16X3 || o->precision == ITNBP_PREC_BF16_BF16X3_F64;
+    } else if (prec_known(o->precision)) {
+      const int split = prec_split(o->precision);
+      const bool progressive = prec_progressive(o->precision);
       for (int i = 0; i < 2; ++i)
         if (!net->d_msg32[i]) HIPCHK(ctx, hipMalloc((void**)&net->d_msg32[i], std::max<size_t>(16, net->msg_total * 4)));
       HIPCHK(ctx, launch_convert_msgs(net->d_msg[net->cur], net->d_msg32[net->cur], net->msg_total, ctx->stream));
       Phase ph{0, net->d_psi, net->dtype, {net->d_msg32[0], net->d_msg32[1]}};
-      const bool low_only = o->precision == ITNBP_PREC_F32 || o->precision == ITNBP_PREC_BF16X3 ||
-                            o->precision == ITNBP_PREC_BF16_BF16X3;
+      const bool low_only = prec_low_only(o->precision);
       const double sw2 = low_only ? -1.0 : o->switch_tol * o->switch_tol;
       // progressive: the one-term 16-bit kernels until the residual < switch_tol
       // (or stagnates), then the three-term kernels
@@ -901,13 +1057,13 @@ int itnbp_update(itnbp_net* net, const itnbp_update_opts* o, itnbp_update_result
         // BF16_BF16X3: switch_tol ends the one-term phase; BF16_BF16X3_F64: the
         // one-term phase ends at residual 1e-2 (two decades above its floor) and
         // switch_tol ends the three-term phase
-        const double pre = o->precision == ITNBP_PREC_BF16_BF16X3 ? o->switch_tol : 1e-2;
+        const double pre = o->precision == ITNBP_PREC_BF16_BF16X3_F64 ? 1e-2 : o->switch_tol;
         const double pre2 = pre > 0 ? pre * pre : 0.0;
         const int rc_pre = sweep_phase(net, seq, o, ph, o->maxiter, pre2, false, &it_pre, &md, &mr, &conv, fast0);
         RETCHK(rc_pre);
         conv = 0;
       }
-      net->low_x3 = x3 ? 1 : 0;
+      net->low_x3 = split;
       const bool fast = fast_path_available(net, 0, o->schedule);
       ctx->path += fast ? fast_path_name(net, 0) : "generic-f32";
       int rc_low = ITNBP_OK;

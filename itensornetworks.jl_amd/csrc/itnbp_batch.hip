@@ -12,16 +12,20 @@
 
 #include "itnbp_ops.hpp"
 
-extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_opts* o,
-                                    itnbp_update_result* res) {
-  if (n <= 0 || !nets || !o || !nets[0]) return ITNBP_ERR_ARG;
+namespace {
+
+int update_batched(int n, itnbp_net** nets, const itnbp_update_opts* o, itnbp_update_result* res) {
   itnbp_ctx* ctx = nets[0]->ctx;
   if (o->maxiter < 1) return itnbp_fail(ctx, ITNBP_ERR_ARG, "You need to specify a number of iterations for BP!");
   if (o->schedule != ITNBP_SCHED_PARALLEL || o->nseq != 0)
     return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "batched update runs the default parallel schedule");
-  // FP32 arithmetic serves ITNBP_PREC_F32 and ITNBP_PREC_BF16X3 (no 16-bit batched kernels)
-  if (o->precision == ITNBP_PREC_BF16_F64 || o->precision == ITNBP_PREC_BF16X3_F64)
-    return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "batched update supports FP64 and FP32 precision");
+  // FP64, or FP32 arithmetic for ITNBP_PREC_F32, ITNBP_PREC_BF16X3 and
+  // ITNBP_PREC_F16X2 (FP32-class results; there are no 16-bit batched kernels);
+  // every phased or refining mode is refused
+  if (o->precision != ITNBP_PREC_F64 && o->precision != ITNBP_PREC_F32 && o->precision != ITNBP_PREC_BF16X3 &&
+      o->precision != ITNBP_PREC_F16X2)
+    return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
+                      "batched update supports precision f64, and f32, bf16x3 and f16x2 (FP32 arithmetic) only");
   const int dtype = nets[0]->dtype;
   // merged description
   std::vector<int> edges, inc_ptr{0}, inc_edge, chi, phys;
@@ -172,4 +176,12 @@ extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_
   ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->path = small_first > 0 ? "batched-generic [vertex " + std::to_string(small_first) + "]" : "batched-generic";
   return ITNBP_OK;
+}
+
+}  // namespace
+
+extern "C" int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_opts* o,
+                                    itnbp_update_result* res) {
+  if (n <= 0 || !nets || !o || !nets[0]) return ITNBP_ERR_ARG;
+  return itnbp_guarded(nets[0]->ctx, [&]() -> int { return update_batched(n, nets, o, res); });
 }

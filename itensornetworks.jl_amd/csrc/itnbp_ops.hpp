@@ -1,6 +1,8 @@
 // Host-side helpers shared by the libitnbp translation units.
 #pragma once
+#include <exception>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -59,6 +61,26 @@ int small_task_count(itnbp_net* net);
 int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, void* msg_out,
                 MsgStat* stats, int normalize);
 void release_small_state(itnbp_net* net);
+
+// ---- bond dimensions (itnbp_api.hip) -----------------------------------------
+// give every edge e the bond dimension chi_new[e] (tensors cut or zero-padded,
+// messages of changed edges reset to identities); releases derived state
+int net_resize_bonds(itnbp_net* net, const std::vector<int>& chi_new);
+
+// Runs f() and turns any C++ exception into a status code, so that none
+// crosses the extern "C" boundary.
+template <class F>
+int itnbp_guarded(itnbp_ctx* ctx, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return itnbp_fail(ctx, ITNBP_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return itnbp_fail(ctx, ITNBP_ERR_ARG, e.what());
+  } catch (...) {
+    return itnbp_fail(ctx, ITNBP_ERR_ARG, "unknown error");
+  }
+}
 
 #define ITNBP_RETCHK(call)          \
   do {                              \

@@ -16,7 +16,9 @@
 // linear algebra: two-sided Jacobi in registers for the environment
 // eigenproblems (one wave each), Householder QR and one-sided (Hestenes) Jacobi
 // SVD in LDS.
+#include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "itnbp_ops.hpp"
@@ -27,6 +29,7 @@ constexpr int AP_XY = 8192;   // max E * n (external size x d chi_b) per side
 constexpr int AP_TH = 32;     // max rows/cols of Theta (r d)
 constexpr int AP_ENV = 8;     // max external bond dimension
 constexpr int AP_MAXEXT = 3;  // max external bonds per side (degree 4)
+constexpr int AP_SITE_D = 16; // max physical dimension of a one-site gate
 
 struct ApplyItem {
   int u, v;      // vertices
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
                                                      double* __restrict__ psi,
                                                      const double* __restrict__ msg,
                                                      const double* __restrict__ gates, ApplyParams prm,
-                                                     double* __restrict__ scratch,
+                                                     double* __restrict__ work,
                                                      double* __restrict__ truncerr,
                                                      double* __restrict__ svals, int* __restrict__ kept,
                                                      int* __restrict__ status) {
@@ -394,7 +397,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   const int tid = threadIdx.x;
   const SideGeom gu = side_geom(vds[it.u], it.ju);
   const SideGeom gv = side_geom(vds[it.v], it.jv);
-  double* scr_u = scratch + (long long)blockIdx.x * 2 * AP_XY;
+  double* scr_u = work + (long long)blockIdx.x * 2 * AP_XY;
   double* scr_v = scr_u + AP_XY;
 
   // 1. environment roots (sqrt and inverse sqrt), both sides, one environment per wave
@@ -509,10 +512,13 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
         svals[(long long)it.slot * prm.max_keep + i] = i < keep && i < N ? svec[order[i]] : 0.0;
     if (kept) kept[it.slot] = keep;
     status[it.slot] = st;
-    sh_keep = keep < chib ? keep : chib;
+    sh_keep = st ? -1 : keep;
   }
   __syncthreads();
   const int keep = sh_keep;
+  // a bond whose kept rank exceeds its dimension is left untouched (the host
+  // reports the error); every write to psi comes after this point
+  if (keep < 0) return;
   // left factor Lf (Mr x keep) = U sqrt(S) into Ru, right factor Rf (Nc x keep) = V sqrt(S) into Rv;
   // A = U diag(s) (M x N), V = VV (N x N); if transposed, Theta = VV S U^T
   for (int idx = tid; idx < Mr * keep; idx += blockDim.x) {
@@ -589,30 +595,89 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   }
 }
 
+// One-site gates (apply.jl:110-116): psi_v[s', b] = sum_s O[s', s] psi_v[s, b]
+// in place, then psi_v /= ||psi_v|| if normalize (:113-115).  One workgroup per
+// gate; the vertices of one launch are distinct.  ops: d x d column-major per
+// gate at op_off[i].
+__global__ __launch_bounds__(256) void k_apply_site(const int* __restrict__ verts, const VDesc* __restrict__ vds,
+                                                    double* __restrict__ psi, const double* __restrict__ ops,
+                                                    const long long* __restrict__ op_off, int normalize,
+                                                    double* __restrict__ norms) {
+  __shared__ double O[AP_SITE_D * AP_SITE_D];
+  __shared__ double red[4];
+  const VDesc V = vds[verts[blockIdx.x]];
+  const int d = V.dims[0];
+  const int cols = V.D / d;
+  double* t = psi + V.psi_off;
+  for (int i = threadIdx.x; i < d * d; i += blockDim.x) O[i] = ops[op_off[blockIdx.x] + i];
+  __syncthreads();
+  double nrm2 = 0;
+  for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+    double x[AP_SITE_D], y[AP_SITE_D];
+    for (int s = 0; s < d; ++s) x[s] = t[(long long)c * d + s];
+    for (int sp = 0; sp < d; ++sp) {
+      double acc = 0;
+      for (int s = 0; s < d; ++s) acc += O[sp + d * s] * x[s];
+      y[sp] = acc;
+      nrm2 += acc * acc;
+    }
+    for (int s = 0; s < d; ++s) t[(long long)c * d + s] = y[s];
+  }
+  const double tot = block_sum_d(nrm2, red);
+  if (threadIdx.x == 0) norms[blockIdx.x] = sqrt(tot);
+  if (!normalize || !(tot > 0)) return;
+  const double sc = 1.0 / sqrt(tot);
+  for (long long i = threadIdx.x; i < V.D; i += blockDim.x) t[i] *= sc;
+}
+
 size_t apply_lds_bytes() {
   return sizeof(double) * (2 * AP_XY + 2 * 1024 + 2 * AP_MAXEXT * 2 * AP_ENV * AP_ENV + 64 * 3 + 4 * 32 +
                            64 * 5 + 16);
 }
 
-}  // namespace
+// Checks the call and fills the launch items; goff receives the number of gate
+// matrix elements.
+// Rank bound of Theta for the bond e (apply.jl:74-81): each side's QR keeps
+// r = min(E, d chi_e) rows, and Theta is (r_u d_u) x (r_v d_v).
+long long theta_rank_bound(const itnbp_net* net, int e) {
+  long long rank_bound = 1;
+  for (int side = 0; side < 2; ++side) {
+    const int x = net->edges[2 * e + side];
+    long long E = 1;
+    for (int a = net->inc_ptr[x]; a < net->inc_ptr[x + 1]; ++a)
+      if (net->inc_edge[a] != e) E *= net->chi[net->inc_edge[a]];
+    const long long r = std::min(E, (long long)net->phys[x] * net->chi[e]) * net->phys[x];
+    rank_bound = side == 0 ? r : std::min(rank_bound, r);
+  }
+  return rank_bound;
+}
 
-extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
-                                 int maxdim, double cutoff, int normalize, double* truncerr,
-                                 double* svals, int max_keep, int* kept) {
-  if (!net || n < 0 || (n > 0 && (!edge_ids || !gates || !truncerr))) return ITNBP_ERR_ARG;
+// Checks the call against the kernel's limits with bond dimensions chi (the
+// network's, or those it will have after growing) and fills the launch items;
+// goff receives the number of gate matrix elements.  With fixed_bonds the
+// kept rank must fit the bond as it is.
+int prepare_apply(itnbp_net* net, int n, const int* edge_ids, int maxdim, double cutoff, double* svals,
+                  int max_keep, const std::vector<int>& chi, bool fixed_bonds, std::vector<ApplyItem>& items,
+                  long long& goff) {
   itnbp_ctx* ctx = net->ctx;
-  if (n == 0) return ITNBP_OK;
   if (net->dtype != ITNBP_F64) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply needs FP64 site tensors");
   if (svals && max_keep <= 0) return itnbp_fail(ctx, ITNBP_ERR_ARG, "max_keep must be > 0 with svals");
-  std::vector<ApplyItem> items(n);
+  items.assign(n, ApplyItem{});
   std::vector<char> used(net->nv, 0);
-  long long goff = 0;
+  goff = 0;
   for (int i = 0; i < n; ++i) {
     const int e = edge_ids[i];
     if (e < 0 || e >= net->ne) return itnbp_fail(ctx, ITNBP_ERR_ARG, "edge id out of range");
     const int u = net->edges[2 * e], v = net->edges[2 * e + 1];
     if (used[u] || used[v]) return itnbp_fail(ctx, ITNBP_ERR_ARG, "the bonds of one apply call must be disjoint");
     used[u] = used[v] = 1;
+    // without a cutoff the kept rank is min(rank bound of Theta, maxdim), known
+    // before the launch: a bond that cannot change must hold it
+    const long long rank_bound = theta_rank_bound(net, e);
+    const long long keep_bound = maxdim > 0 ? std::min<long long>(rank_bound, maxdim) : rank_bound;
+    if (fixed_bonds && cutoff < 0 && keep_bound > chi[e])
+      return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
+                        "apply: the truncation would keep more values than the bond dimension; pass maxdim <= chi");
     // product environment: every message into u but from v (and into v but from u) must be set
     for (int side = 0; side < 2; ++side) {
       const int x = side == 0 ? u : v;
@@ -624,10 +689,10 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
         if (f == e) continue;
         if (!net->msg_set[msg_id_into(net, f, x)])
           return itnbp_fail(ctx, ITNBP_ERR_STATE, "environment message not set");
-        if (net->chi[f] > AP_ENV) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply supports external bond dimensions <= 8");
-        E *= net->chi[f];
+        if (chi[f] > AP_ENV) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply supports external bond dimensions <= 8");
+        E *= chi[f];
       }
-      const long long nn = (long long)net->phys[x] * net->chi[e];
+      const long long nn = (long long)net->phys[x] * chi[e];
       if (E * nn > AP_XY) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: site tensor too large for the LDS kernel");
       if (std::min(E, nn) * net->phys[x] > AP_TH) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: Theta larger than 32x32");
       if (std::min(E, nn) * nn > 1024) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: R factor larger than 1024");
@@ -640,33 +705,61 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
     items[i].slot = i;
     goff += (long long)net->phys[u] * net->phys[v] * net->phys[u] * net->phys[v];
   }
-  DevBuf d_items, d_gates, d_scr, d_out;
-  auto cleanup = [&]() {
-    free_buf(d_items);
-    free_buf(d_gates);
-    free_buf(d_scr);
-    free_buf(d_out);
-  };
-  const int mk = svals ? max_keep : 1;
-  int rc = ensure_buf(ctx, d_items, n * sizeof(ApplyItem));
-  if (rc == ITNBP_OK) rc = ensure_buf(ctx, d_gates, goff * 8);
-  if (rc == ITNBP_OK) rc = ensure_buf(ctx, d_scr, (size_t)n * 2 * AP_XY * 8);
-  if (rc == ITNBP_OK) rc = ensure_buf(ctx, d_out, (size_t)n * (8 + 8 * mk + 4 + 4));
-  if (rc != ITNBP_OK) {
-    cleanup();
-    return rc;
-  }
-  char* ob = (char*)d_out.p;
-  double* d_te = (double*)ob;
-  double* d_sv = (double*)(ob + 8 * n);
-  int* d_kept = (int*)(ob + 8 * n + 8 * (size_t)n * mk);
-  int* d_st = d_kept + n;
-  hipError_t e1 = hipMemcpyAsync(d_items.p, items.data(), n * sizeof(ApplyItem), hipMemcpyHostToDevice, ctx->stream);
-  hipError_t e2 = hipMemcpyAsync(d_gates.p, gates, goff * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e1 != hipSuccess || e2 != hipSuccess) {
-    cleanup();
-    return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: upload failed");
-  }
+  return ITNBP_OK;
+}
+
+// Device memory of one apply call: launch items, gate matrices, the per-gate
+// workspace of k_apply_gates (2 AP_XY doubles per gate) and the per-gate outputs.
+struct ApplyBufs {
+  ApplyItem* items;
+  double* gates;
+  double* work;
+  double* te;
+  double* sv;
+  int* kept;
+  int* st;
+};
+
+// bytes of each part, in the order of ApplyBufs, every part 256-byte aligned
+void apply_buf_sizes(int n, long long goff, int mk, size_t (&sz)[7]) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  sz[0] = al(n * sizeof(ApplyItem));
+  sz[1] = al((size_t)goff * 8);
+  sz[2] = al((size_t)n * 2 * AP_XY * 8);
+  sz[3] = al((size_t)n * 8);
+  sz[4] = al((size_t)n * mk * 8);
+  sz[5] = al((size_t)n * 4);
+  sz[6] = al((size_t)n * 4);
+}
+
+ApplyBufs carve_apply_bufs(char* base, int n, long long goff, int mk) {
+  size_t sz[7];
+  apply_buf_sizes(n, goff, mk, sz);
+  ApplyBufs b;
+  b.items = (ApplyItem*)base;
+  base += sz[0];
+  b.gates = (double*)base;
+  base += sz[1];
+  b.work = (double*)base;
+  base += sz[2];
+  b.te = (double*)base;
+  base += sz[3];
+  b.sv = (double*)base;
+  base += sz[4];
+  b.kept = (int*)base;
+  base += sz[5];
+  b.st = (int*)base;
+  return b;
+}
+
+// upload, one launch, read-back; every device buffer comes from b
+int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const double* gates, long long goff,
+              int maxdim, double cutoff, int normalize, double* truncerr, double* svals, int mk, int* kept,
+              const ApplyBufs& b) {
+  itnbp_ctx* ctx = net->ctx;
+  hipError_t e1 = hipMemcpyAsync(b.items, items.data(), n * sizeof(ApplyItem), hipMemcpyHostToDevice, ctx->stream);
+  hipError_t e2 = hipMemcpyAsync(b.gates, gates, goff * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e1 != hipSuccess || e2 != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: upload failed");
   ApplyParams prm;
   prm.maxdim = maxdim;
   prm.cutoff = cutoff;
@@ -675,32 +768,132 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
   prm.max_keep = mk;
   const size_t lds = apply_lds_bytes();
   hipError_t ea = hipFuncSetAttribute((const void*)k_apply_gates, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (ea != hipSuccess) {
-    cleanup();
+  if (ea != hipSuccess)
     return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(ea));
-  }
   const int pm0 = prof_mark(ctx);
-  hipLaunchKernelGGL(k_apply_gates, dim3(n), dim3(256), lds, ctx->stream, (const ApplyItem*)d_items.p,
-                     net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)d_gates.p, prm,
-                     (double*)d_scr.p, d_te, svals ? d_sv : nullptr, d_kept, d_st);
+  hipLaunchKernelGGL(k_apply_gates, dim3(n), dim3(256), lds, ctx->stream, (const ApplyItem*)b.items,
+                     net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)b.gates, prm,
+                     b.work, b.te, svals ? b.sv : nullptr, b.kept, b.st);
   hipError_t ek = hipGetLastError();
   prof_add(ctx, "apply_gates", pm0, prof_mark(ctx));
   std::vector<int> st(n);
   std::vector<int> kp(n);
-  if (ek == hipSuccess) ek = hipMemcpyAsync(truncerr, d_te, n * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (ek == hipSuccess && svals) ek = hipMemcpyAsync(svals, d_sv, (size_t)n * mk * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (ek == hipSuccess) ek = hipMemcpyAsync(kp.data(), d_kept, n * 4, hipMemcpyDeviceToHost, ctx->stream);
-  if (ek == hipSuccess) ek = hipMemcpyAsync(st.data(), d_st, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess) ek = hipMemcpyAsync(truncerr, b.te, n * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess && svals) ek = hipMemcpyAsync(svals, b.sv, (size_t)n * mk * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess) ek = hipMemcpyAsync(kp.data(), b.kept, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (ek == hipSuccess) ek = hipMemcpyAsync(st.data(), b.st, n * 4, hipMemcpyDeviceToHost, ctx->stream);
   if (ek == hipSuccess) ek = hipStreamSynchronize(ctx->stream);
-  cleanup();
   if (ek != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: ") + hipGetErrorString(ek));
   if (kept) std::copy(kp.begin(), kp.end(), kept);
   invalidate_fast_state(net);
   for (int i = 0; i < n; ++i)
     if (st[i])
       return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
-                        "apply: the truncation keeps more values than the bond dimension; pass maxdim <= chi");
+                        "apply: the truncation keeps more values than the bond dimension (that bond's tensors "
+                        "are unchanged; the other gates of the call were applied); pass maxdim <= chi");
   for (int i = 0; i < n; ++i)
     if (!std::isfinite(truncerr[i])) return itnbp_fail(ctx, ITNBP_ERR_NONFINITE, "apply: non-finite result");
   return ITNBP_OK;
+}
+
+}  // namespace
+
+extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
+                                 int maxdim, double cutoff, int normalize, double* truncerr,
+                                 double* svals, int max_keep, int* kept) {
+  if (!net || n < 0 || (n > 0 && (!edge_ids || !gates || !truncerr))) return ITNBP_ERR_ARG;
+  if (n == 0) return ITNBP_OK;
+  itnbp_ctx* ctx = net->ctx;
+  return itnbp_guarded(ctx, [&]() -> int {
+    // The new bond has the kept dimension (apply.jl:81-93).  A bond whose kept
+    // rank may exceed its dimension first grows (zero-padded) to the largest
+    // rank the truncation can keep, min(rank bound of Theta, maxdim); the kernel
+    // writes the kept columns followed by zeros; then every bond of the call is
+    // cut to its kept rank, which drops only zeros.
+    for (int i = 0; i < n; ++i)
+      if (edge_ids[i] < 0 || edge_ids[i] >= net->ne) return itnbp_fail(ctx, ITNBP_ERR_ARG, "edge id out of range");
+    std::vector<int> target(n), chi_grow = net->chi;
+    for (int i = 0; i < n; ++i) {
+      const int e = edge_ids[i];
+      const long long rb = theta_rank_bound(net, e);
+      target[i] = (int)std::min<long long>(maxdim > 0 ? std::min<long long>(rb, maxdim) : rb, 64);
+      chi_grow[e] = std::max(chi_grow[e], target[i]);
+    }
+    std::vector<ApplyItem> items;
+    long long goff = 0;
+    ITNBP_RETCHK(prepare_apply(net, n, edge_ids, maxdim, cutoff, svals, max_keep, chi_grow, false, items, goff));
+    ITNBP_RETCHK(net_resize_bonds(net, chi_grow));
+    const int mk = svals ? max_keep : 1;
+    size_t sz[7];
+    apply_buf_sizes(n, goff, mk, sz);
+    size_t total = 0;
+    for (size_t b : sz) total += b;
+    DevBuf d;
+    ITNBP_RETCHK(ensure_buf(ctx, d, total));
+    std::vector<int> kp(n);
+    const int rc = run_apply(net, n, items, gates, goff, maxdim, cutoff, normalize, truncerr, svals, mk, kp.data(),
+                             carve_apply_bufs((char*)d.p, n, goff, mk));
+    free_buf(d);
+    if (rc != ITNBP_OK) return rc;
+    std::vector<int> chi_cut = net->chi;
+    for (int i = 0; i < n; ++i) {
+      kp[i] = std::max(1, std::min(kp[i], target[i]));
+      chi_cut[edge_ids[i]] = kp[i];
+      if (svals)
+        for (int j = kp[i]; j < mk; ++j) svals[(size_t)i * mk + j] = 0.0;
+    }
+    if (kept) std::copy(kp.begin(), kp.end(), kept);
+    return net_resize_bonds(net, chi_cut);
+  });
+}
+
+extern "C" int itnbp_apply_site_gates(itnbp_net* net, int n, const int* verts, const double* ops, int normalize,
+                                      double* norms) {
+  if (!net || n < 0 || (n > 0 && (!verts || !ops))) return ITNBP_ERR_ARG;
+  if (n == 0) return ITNBP_OK;
+  itnbp_ctx* ctx = net->ctx;
+  return itnbp_guarded(ctx, [&]() -> int {
+    if (net->dtype != ITNBP_F64) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply needs FP64 site tensors");
+    std::vector<char> used(net->nv, 0);
+    std::vector<long long> off(n);
+    long long tot = 0;
+    for (int i = 0; i < n; ++i) {
+      const int v = verts[i];
+      if (v < 0 || v >= net->nv) return itnbp_fail(ctx, ITNBP_ERR_ARG, "vertex out of range");
+      if (used[v]) return itnbp_fail(ctx, ITNBP_ERR_ARG, "the vertices of one site-gate call must be distinct");
+      used[v] = 1;
+      const int d = net->phys[v];
+      if (d > AP_SITE_D) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "one-site gates support d <= 16");
+      off[i] = tot;
+      tot += (long long)d * d;
+    }
+    DevBuf b;
+    const size_t ob = ((size_t)n * 4 + 255) & ~(size_t)255, oo = ((size_t)n * 8 + 255) & ~(size_t)255;
+    ITNBP_RETCHK(ensure_buf(ctx, b, ob + 2 * oo + (size_t)tot * 8));
+    char* p = (char*)b.p;
+    int* d_v = (int*)p;
+    long long* d_off = (long long*)(p + ob);
+    double* d_nrm = (double*)(p + ob + oo);
+    double* d_ops = (double*)(p + ob + 2 * oo);
+    std::vector<double> nrm(n);
+    hipError_t e = hipMemcpyAsync(d_v, verts, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ops, ops, (size_t)tot * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+      const int pm0 = prof_mark(ctx);
+      hipLaunchKernelGGL(k_apply_site, dim3(n), dim3(256), 0, ctx->stream, (const int*)d_v, net->d_vdesc,
+                         (double*)net->d_psi, (const double*)d_ops, (const long long*)d_off, normalize, d_nrm);
+      e = hipGetLastError();
+      prof_add(ctx, "apply_site", pm0, prof_mark(ctx));
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(nrm.data(), d_nrm, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    free_buf(b);
+    if (e != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply_site_gates: ") + hipGetErrorString(e));
+    invalidate_fast_state(net);
+    if (norms) std::copy(nrm.begin(), nrm.end(), norms);
+    for (int i = 0; i < n; ++i)
+      if (!std::isfinite(nrm[i])) return itnbp_fail(ctx, ITNBP_ERR_NONFINITE, "apply_site_gates: non-finite result");
+    return ITNBP_OK;
+  });
 }

@@ -426,6 +426,13 @@ __global__ __launch_bounds__(256) void k_fused_bf16(const FastItem* __restrict__
 // ---------------------------------------------------------------------------
 constexpr int ST_LD = 40;  // LDS row stride (bf16) of the staging/tiles: 80 bytes, 16-byte aligned
 
+// Work index of workgroup b of a 64-wide grid row.  Workgroups are dispatched
+// round-robin over the 8 XCDs (chiplets, each with its own L2), so XCD x
+// receives b = x, x + 8, ..., x + 56; they take the 8 consecutive work
+// indices 8x .. 8x + 7.  Work indices 2z + s and 2(z + 1) + s write the two
+// 64-byte halves of the same 128-byte lines, which then meet in one L2.
+__device__ inline int xcd_local64(int b) { return ((b & 7) << 3) | (b >> 3); }
+
 // TO: type of P and of the second product's operands (BF16 for the
 // absorbed-pair passes, FP16 for rot12 of the eigenbasis pipeline); the first
 // product multiplies the BF16 site tensor and stays in BF16.
@@ -437,7 +444,7 @@ __global__ __launch_bounds__(512) void k_absorb_bf16c(const FastItem* __restrict
   typedef typename Mma<TO>::v8 o8;
   __shared__ __attribute__((aligned(16))) TO S[FBLK * ST_LD];
   const FastItem it = items[blockIdx.y];
-  const int q = blockIdx.x;
+  const int q = xcd_local64(blockIdx.x);
   const int s = q & 1, z = q >> 1;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   bf16x8 bMbh[2], bMbl[2];
@@ -1956,20 +1963,49 @@ __global__ __launch_bounds__(256) void k_rot3_d3_bf16(const EigItem* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Compensated flavour of the eigenbasis pipeline: R and Q in FP32, every MFMA
-// operand that is not exact in BF16 split into three BF16 terms
-// (x = h + m + l, about 24 significant bits) and every product a b taken as
-// the six terms of order <= 2: hh + hm + mh + hl + mm + lh (the site tensors
-// are BF16 and enter unsplit; a product with one exact operand has three).
+// Compensated flavours of the eigenbasis pipeline: R and Q in FP32, and every
+// MFMA operand that is not exact in 16 bits carried as N terms of a 16-bit
+// type (x = t0 + t1 + ... to FP32 rounding: t0 the rounding of x, every later
+// term the rounding of what the earlier ones leave; the differences are exact
+// in FP32).  A product a b takes the term products a_i b_j with i + j <= N - 1,
+// summed in the FP32 MFMA accumulator; the site tensors enter unsplit.
+//   PrecX3: three BF16 terms (about 24 significant bits); six products, three
+//           with an exact operand.
+//   PrecH2: two FP16 terms (about 22 significant bits); three products, two
+//           with an exact operand.  FP16's 5-bit exponent suffices because the
+//           16-bit layouts hold every site scaled by a power of two that puts
+//           its largest entry in [1/2, 1) and the rotations are orthogonal;
+//           a remainder below the smallest normal FP16 keeps an absolute error
+//           of 2^-25 against operands of order one.  The BF16 site tensor
+//           entries convert to FP16 exactly down to 2^-17.
 // ---------------------------------------------------------------------------
 constexpr int SX_LD = 33;  // FP32 staging row stride (odd: conflict-free column writes)
 
-struct bf3 {
-  bf16x8 h, m, l;
-};
-
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+struct PrecX3 {
+  typedef __bf16 T;
+  typedef bf16x8 v8;
+  typedef bf16x2 v2;
+  static constexpr int N = 3;
+  static __device__ inline f32x16 mma(v8 a, v8 b, f32x16 c) { return MFMA_BF16(a, b, c); }
+};
+struct PrecH2 {
+  typedef _Float16 T;
+  typedef f16x8 v8;
+  typedef f16x2 v2;
+  static constexpr int N = 2;
+  static __device__ inline f32x16 mma(v8 a, v8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <class P>
+struct Terms {
+  typename P::v8 t[P::N];
+};
 
 // the two BF16 halves of a packed pair, widened to FP32 (exact)
 __device__ inline f32x2 widen_bf16x2(bf16x2 v) {
@@ -1980,76 +2016,114 @@ __device__ inline f32x2 widen_bf16x2(bf16x2 v) {
   return r;
 }
 
-// Three-term split of eight values, x = h + m + l exactly to FP32 rounding:
-// h = bf16(x), m = bf16(x - h), l = bf16(x - h - m) (round to nearest even; the
-// differences are exact in FP32).  Two values at a time: each rounding is one
-// packed conversion whose result is already the operand's register pair.
-__device__ inline bf3 split8x3(const float (&x)[8]) {
-  bf3 o;
+// N-term split of eight values, two at a time: every rounding is one packed
+// conversion (round to nearest even) whose result is already the operand's
+// register pair
+template <class P>
+__device__ inline Terms<P> split8(const float (&x)[8]) {
+  Terms<P> o;
 #pragma unroll
   for (int p = 0; p < 8; p += 2) {
-    f32x2 v;
-    v.x = x[p];
-    v.y = x[p + 1];
-    const bf16x2 h = __builtin_convertvector(v, bf16x2);
-    const f32x2 r = v - widen_bf16x2(h);
-    const bf16x2 m = __builtin_convertvector(r, bf16x2);
-    const bf16x2 lo = __builtin_convertvector(r - widen_bf16x2(m), bf16x2);
-    o.h[p] = h.x;
-    o.h[p + 1] = h.y;
-    o.m[p] = m.x;
-    o.m[p + 1] = m.y;
-    o.l[p] = lo.x;
-    o.l[p + 1] = lo.y;
+    f32x2 r;
+    r.x = x[p];
+    r.y = x[p + 1];
+#pragma unroll
+    for (int i = 0; i < P::N; ++i) {
+      const typename P::v2 h = __builtin_convertvector(r, typename P::v2);
+      o.t[i][p] = h.x;
+      o.t[i][p + 1] = h.y;
+      if (i + 1 < P::N) {
+        if constexpr (P::N == 3)
+          r = r - widen_bf16x2(h);
+        else
+          r = r - __builtin_convertvector(h, f32x2);
+      }
+    }
   }
   return o;
 }
 
-__device__ inline f32x16 mfma6(const bf3& a, const bf3& b, f32x16 acc) {
-  acc = MFMA_BF16(a.l, b.h, acc);
-  acc = MFMA_BF16(a.m, b.m, acc);
-  acc = MFMA_BF16(a.h, b.l, acc);
-  acc = MFMA_BF16(a.m, b.h, acc);
-  acc = MFMA_BF16(a.h, b.m, acc);
-  acc = MFMA_BF16(a.h, b.h, acc);
+// eight BF16 site tensor entries as an exact operand of the policy's MFMA
+template <class P>
+__device__ inline typename P::v8 exact8(bf16x8 v) {
+  if constexpr (P::N == 3) {
+    return v;
+  } else {
+    typename P::v8 o;
+#pragma unroll
+    for (int p = 0; p < 8; p += 2) {
+      bf16x2 b;
+      b.x = v[p];
+      b.y = v[p + 1];
+      const f16x2 h = __builtin_convertvector(widen_bf16x2(b), f16x2);
+      o[p] = h.x;
+      o[p + 1] = h.y;
+    }
+    return o;
+  }
+}
+
+// every term product of order <= N - 1, smallest first
+template <class P>
+__device__ inline f32x16 mmN(const Terms<P>& a, const Terms<P>& b, f32x16 acc) {
+  if constexpr (P::N == 3) {
+    acc = P::mma(a.t[2], b.t[0], acc);
+    acc = P::mma(a.t[1], b.t[1], acc);
+    acc = P::mma(a.t[0], b.t[2], acc);
+    acc = P::mma(a.t[1], b.t[0], acc);
+    acc = P::mma(a.t[0], b.t[1], acc);
+  } else {
+    acc = P::mma(a.t[1], b.t[0], acc);
+    acc = P::mma(a.t[0], b.t[1], acc);
+  }
+  return P::mma(a.t[0], b.t[0], acc);
+}
+
+// exact a times split b
+template <class P>
+__device__ inline f32x16 mm_exactN(typename P::v8 a, const Terms<P>& b, f32x16 acc) {
+#pragma unroll
+  for (int i = P::N - 1; i >= 0; --i) acc = P::mma(a, b.t[i], acc);
+  return acc;
+}
+
+// split a times exact b
+template <class P>
+__device__ inline f32x16 mm_exactN_r(const Terms<P>& a, typename P::v8 b, f32x16 acc) {
+#pragma unroll
+  for (int i = P::N - 1; i >= 0; --i) acc = P::mma(a.t[i], b, acc);
   return acc;
 }
 
 // v with the sign bits of mask flipped (exact negation of selected elements)
-__device__ inline bf16x8 flip_signs(bf16x8 v, uint4 m) {
+template <typename V>
+__device__ inline V flip_signs(V v, uint4 m) {
   uint4 u = __builtin_bit_cast(uint4, v);
   u.x ^= m.x;
   u.y ^= m.y;
   u.z ^= m.z;
   u.w ^= m.w;
-  return __builtin_bit_cast(bf16x8, u);
+  return __builtin_bit_cast(V, u);
 }
 
 // Weighted Gram term acc += sum_k w_k a[m][k] a[n][k] with one split: the
-// A operand is sqrt|w_k| a[.][k] (split into three terms) and the B operand the
+// A operand is sqrt|w_k| a[.][k] (split into N terms) and the B operand the
 // same terms with the sign of w_k (mask) applied, so b = sign(w) sqrt|w| a exactly
-__device__ inline f32x16 wgram6(const float (&aw)[8], uint4 mask, f32x16 acc) {
-  const bf3 a = split8x3(aw);
-  bf3 b;
-  b.h = flip_signs(a.h, mask);
-  b.m = flip_signs(a.m, mask);
-  b.l = flip_signs(a.l, mask);
-  return mfma6(a, b, acc);
-}
-
-// exact a (BF16) times split b
-__device__ inline f32x16 mfma3x(bf16x8 a, const bf3& b, f32x16 acc) {
-  acc = MFMA_BF16(a, b.l, acc);
-  acc = MFMA_BF16(a, b.m, acc);
-  acc = MFMA_BF16(a, b.h, acc);
-  return acc;
+template <class P>
+__device__ inline f32x16 wgramN(const float (&aw)[8], uint4 mask, f32x16 acc) {
+  const Terms<P> a = split8<P>(aw);
+  Terms<P> b;
+#pragma unroll
+  for (int i = 0; i < P::N; ++i) b.t[i] = flip_signs(a.t[i], mask);
+  return mmN<P>(a, b, acc);
 }
 
 // rot12: R block (s + 2 (x' + 32 y')) row z, columns w = (U1^T X U2)[x'][y'] of
 // the 32 layout-B blocks X = (s + 2 (z + 32 w)).  Items: the absorb kernel's
 // (psi_in = layout B, P = R offset, Ma = U1, Mb = U2).
 // grid (64, items), 512 threads: WG -> (s, z) = (q & 1, q >> 1), wave -> w = 4 wv + {0..3}
-__global__ __launch_bounds__(512) void k_rot12_x3(const FastItem* __restrict__ items,
+template <class P>
+__global__ __launch_bounds__(512) void k_rot12_s(const FastItem* __restrict__ items,
                                                   const __bf16* __restrict__ lay,
                                                   const float* __restrict__ U, float* __restrict__ R) {
   __shared__ float S[FBLK * SX_LD];  // row (y' + 32 x'), column w
@@ -2057,7 +2131,7 @@ __global__ __launch_bounds__(512) void k_rot12_x3(const FastItem* __restrict__ i
   const int q = blockIdx.x;
   const int s = q & 1, z = q >> 1;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
-  bf3 bU2[2], aU1[2];
+  Terms<P> bU2[2], aU1[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     float u2[8], u1[8];
@@ -2066,29 +2140,30 @@ __global__ __launch_bounds__(512) void k_rot12_x3(const FastItem* __restrict__ i
       u2[p] = U[it.Mb + (16 * t + 8 * g + p) + FC * c];  // U2, K = b2 natural
       u1[p] = U[it.Ma + kperm(t, g, p) + FC * c];        // U1^T, K = b1 in accumulator-row order
     }
-    bU2[t] = split8x3(u2);
-    aU1[t] = split8x3(u1);
+    bU2[t] = split8<P>(u2);
+    aU1[t] = split8<P>(u1);
   }
-  bf16x8 aXs[4][2];
+  typename P::v8 aXs[4][2];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const __bf16* X = lay + it.psi_in + (long long)(s + 2 * (z + FC * (4 * wv + i))) * FBLK;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) aXs[i][t] = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+    for (int t = 0; t < 2; ++t)
+      aXs[i][t] = exact8<P>(*reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g));
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int w = 4 * wv + i;
     f32x16 V = {};
 #pragma unroll
-    for (int t = 0; t < 2; ++t) V = mfma3x(aXs[i][t], bU2[t], V);
+    for (int t = 0; t < 2; ++t) V = mm_exactN<P>(aXs[i][t], bU2[t], V);
     f32x16 Y = {};
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       float v8[8];
 #pragma unroll
       for (int p = 0; p < 8; ++p) v8[p] = V[8 * t + p];
-      Y = mfma6(aU1[t], split8x3(v8), Y);
+      Y = mmN<P>(aU1[t], split8<P>(v8), Y);
     }
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
@@ -2115,7 +2190,8 @@ __global__ __launch_bounds__(512) void k_rot12_x3(const FastItem* __restrict__ i
 // tile at column w ^ r3 (a swizzle that keeps the column writes, the row-wise
 // copy-out and the transposed read-back conflict-free); the wave reads its
 // block back transposed for T'3 instead of forming Y^T on the MFMA.
-__global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ items, const float* __restrict__ R,
+template <class P>
+__global__ __launch_bounds__(512) void k_rot34_s(const EigItem* __restrict__ items, const float* __restrict__ R,
                                                   const float* __restrict__ U, const float* __restrict__ lam,
                                                   float* __restrict__ Q, float* __restrict__ part) {
   __shared__ float S[FBLK * SX_LD];  // row (r4 + 32 r3), column r2 ^ r3
@@ -2143,7 +2219,7 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
     }
     msk34[bd][t][gg] = make_uint4(mw[0], mw[1], mw[2], mw[3]);
   }
-  bf3 aU3[2], bU4[2];
+  Terms<P> aU3[2], bU4[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     float u3[8], u4[8];
@@ -2152,8 +2228,8 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
       u3[p] = U[itp->U[2] + kperm(t, g, p) + FC * c];
       u4[p] = U[itp->U[3] + (16 * t + 8 * g + p) + FC * c];
     }
-    aU3[t] = split8x3(u3);
-    bU4[t] = split8x3(u4);
+    aU3[t] = split8<P>(u3);
+    bU4[t] = split8<P>(u4);
   }
   __syncthreads();
   f32x16 T3 = {}, T4 = {};
@@ -2188,7 +2264,7 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
       for (int t = 0; t < 2; ++t) {
         const float x8[8] = {cur[i][2 * t].x, cur[i][2 * t].y, cur[i][2 * t].z, cur[i][2 * t].w,
                              cur[i][2 * t + 1].x, cur[i][2 * t + 1].y, cur[i][2 * t + 1].z, cur[i][2 * t + 1].w};
-        V = mfma6(split8x3(x8), bU4[t], V);
+        V = mmN<P>(split8<P>(x8), bU4[t], V);
       }
       f32x16 Y = {};
 #pragma unroll
@@ -2196,7 +2272,7 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
         float v8[8];
 #pragma unroll
         for (int p = 0; p < 8; ++p) v8[p] = V[8 * t + p];
-        Y = mfma6(aU3[t], split8x3(v8), Y);
+        Y = mmN<P>(aU3[t], split8<P>(v8), Y);
       }
       // Y[r3][r4] (lane r4, rows r3 = xb) to the tile
 #pragma unroll
@@ -2221,11 +2297,11 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
 #pragma unroll
         for (int p = 0; p < 8; ++p)
           a8[p] = sw12 * s4[p] * S[(kperm(t, g, p) + FC * c) * SX_LD + (w ^ c)];
-        T3 = wgram6(a8, m4, T3);
+        T3 = wgramN<P>(a8, m4, T3);
         // T'4: weights w12 l3 over the K index r3 of Y
 #pragma unroll
         for (int p = 0; p < 8; ++p) a8[p] = sw12 * s3[p] * Y[8 * t + p];
-        T4 = wgram6(a8, m3, T4);
+        T4 = wgramN<P>(a8, m3, T4);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -2274,8 +2350,8 @@ __global__ __launch_bounds__(512) void k_rot34_x3(const EigItem* __restrict__ it
 
 // gram12 (T'1, T'2) of the FP32 Q blocks; the layout of k_gram12_bf16, two
 // blocks of look-ahead per wave
-template <int NB>
-__global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ items, const float* __restrict__ Q,
+template <class P, int NB>
+__global__ __launch_bounds__(256) void k_gram12_s(const EigItem* __restrict__ items, const float* __restrict__ Q,
                                                    const float* __restrict__ lam, float* __restrict__ part) {
   __shared__ float Red[2 * FBLK];
   const EigItem* itp = items + blockIdx.y;
@@ -2286,7 +2362,7 @@ __global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ i
   // the sign masks of the single-split Gram
   float sl1v[16], lam2v[16];
   uint4 m1[2];
-  bf16x8 bI[2];
+  typename P::v8 bI[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     unsigned mw[4] = {0u, 0u, 0u, 0u};
@@ -2296,7 +2372,7 @@ __global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ i
       sl1v[8 * t + p] = sqrtf(fabsf(l1));
       if (l1 < 0.f) mw[p >> 1] |= (p & 1) ? 0x80000000u : 0x8000u;
       lam2v[8 * t + p] = lam[itp->L[1] + 16 * t + 8 * g + p];
-      bI[t][p] = (__bf16)((16 * t + 8 * g + p) == c ? 1.0f : 0.0f);
+      bI[t][p] = (typename P::T)((16 * t + 8 * g + p) == c ? 1.0f : 0.0f);
     }
     m1[t] = make_uint4(mw[0], mw[1], mw[2], mw[3]);
   }
@@ -2331,11 +2407,9 @@ __global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ i
         float b8[8];
 #pragma unroll
         for (int p = 0; p < 8; ++p) b8[p] = w34 * lam2v[8 * t + p] * d8[p];
-        const bf3 a = split8x3(d8);
-        T1 = mfma6(a, split8x3(b8), T1);
-        Dt = MFMA_BF16(a.l, bI[t], Dt);  // D I: the transpose, exact
-        Dt = MFMA_BF16(a.m, bI[t], Dt);
-        Dt = MFMA_BF16(a.h, bI[t], Dt);
+        const Terms<P> a = split8<P>(d8);
+        T1 = mmN<P>(a, split8<P>(b8), T1);
+        Dt = mm_exactN_r<P>(a, bI[t], Dt);  // D I: the transpose of the split D
       }
       const float sw34 = sqrtf(fabsf(w34));
       const unsigned nw = w34 < 0.f ? 0x80008000u : 0u;
@@ -2345,7 +2419,7 @@ __global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ i
 #pragma unroll
         for (int p = 0; p < 8; ++p) a8[p] = sw34 * sl1v[8 * t + p] * Dt[8 * t + p];
         const uint4 mk = make_uint4(m1[t].x ^ nw, m1[t].y ^ nw, m1[t].z ^ nw, m1[t].w ^ nw);
-        T2 = wgram6(a8, mk, T2);
+        T2 = wgramN<P>(a8, mk, T2);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -2380,12 +2454,13 @@ __global__ __launch_bounds__(256) void k_gram12_x3(const EigItem* __restrict__ i
 
 // degree-3 sites, compensated: rot12_d3 and rot3_d3 of the 16-bit kernels on
 // FP32 R3 / Q3 blocks
-__global__ __launch_bounds__(256) void k_rot12_d3_x3(const EigItem* __restrict__ items,
+template <class P>
+__global__ __launch_bounds__(256) void k_rot12_d3_s(const EigItem* __restrict__ items,
                                                      const __bf16* __restrict__ lay,
                                                      const float* __restrict__ U, float* __restrict__ R) {
   const EigItem* itp = items + blockIdx.y;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
-  bf3 aU1[2], bU2[2];
+  Terms<P> aU1[2], bU2[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     float u1[8], u2[8];
@@ -2394,28 +2469,29 @@ __global__ __launch_bounds__(256) void k_rot12_d3_x3(const EigItem* __restrict__
       u1[p] = U[itp->U[0] + kperm(t, g, p) + FC * c];
       u2[p] = U[itp->U[1] + (16 * t + 8 * g + p) + FC * c];
     }
-    aU1[t] = split8x3(u1);
-    bU2[t] = split8x3(u2);
+    aU1[t] = split8<P>(u1);
+    bU2[t] = split8<P>(u2);
   }
-  bf16x8 aX[4][2];
+  typename P::v8 aX[4][2];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const __bf16* X = lay + itp->X + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) aX[i][t] = *reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g);
+    for (int t = 0; t < 2; ++t)
+      aX[i][t] = exact8<P>(*reinterpret_cast<const bf16x8*>(X + c * FC + 16 * t + 8 * g));
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     f32x16 V = {};
 #pragma unroll
-    for (int t = 0; t < 2; ++t) V = mfma3x(aX[i][t], bU2[t], V);
+    for (int t = 0; t < 2; ++t) V = mm_exactN<P>(aX[i][t], bU2[t], V);
     f32x16 Y = {};
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       float v8[8];
 #pragma unroll
       for (int p = 0; p < 8; ++p) v8[p] = V[8 * t + p];
-      Y = mfma6(aU1[t], split8x3(v8), Y);
+      Y = mmN<P>(aU1[t], split8<P>(v8), Y);
     }
     float* dst = R + itp->R + (long long)(16 * blockIdx.x + 4 * wv + i) * FBLK;
 #pragma unroll
@@ -2423,7 +2499,8 @@ __global__ __launch_bounds__(256) void k_rot12_d3_x3(const EigItem* __restrict__
   }
 }
 
-__global__ __launch_bounds__(256) void k_rot3_d3_x3(const EigItem* __restrict__ items, const float* __restrict__ R,
+template <class P>
+__global__ __launch_bounds__(256) void k_rot3_d3_s(const EigItem* __restrict__ items, const float* __restrict__ R,
                                                     const float* __restrict__ U, const float* __restrict__ lam,
                                                     float* __restrict__ Q, float* __restrict__ part) {
   __shared__ float Red[FBLK];
@@ -2431,7 +2508,7 @@ __global__ __launch_bounds__(256) void k_rot3_d3_x3(const EigItem* __restrict__ 
   const int q = blockIdx.x;
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 5, c = l & 31;
   const int x = 4 * q + wv, s = x & 1, r1 = x >> 1;
-  bf3 aU3[2], bZ[2];
+  Terms<P> aU3[2], bZ[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     float u3[8], z8[8];
@@ -2440,14 +2517,14 @@ __global__ __launch_bounds__(256) void k_rot3_d3_x3(const EigItem* __restrict__ 
       u3[p] = U[itp->U[2] + (16 * t + 8 * g + p) + FC * c];  // U3^T, K = b3 natural
       z8[p] = R[itp->R + (long long)(s + 2 * (16 * t + 8 * g + p)) * FBLK + r1 * FC + c];  // lane r2, K = b3
     }
-    aU3[t] = split8x3(u3);
-    bZ[t] = split8x3(z8);
+    aU3[t] = split8<P>(u3);
+    bZ[t] = split8<P>(z8);
   }
   f32x16 Y = {}, YT = {};
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    Y = mfma6(aU3[t], bZ[t], Y);
-    YT = mfma6(bZ[t], aU3[t], YT);
+    Y = mmN<P>(aU3[t], bZ[t], Y);
+    YT = mmN<P>(bZ[t], aU3[t], YT);
   }
   const float l1 = lam[itp->L[0] + r1];
   f32x16 T3 = {};
@@ -2459,7 +2536,7 @@ __global__ __launch_bounds__(256) void k_rot3_d3_x3(const EigItem* __restrict__ 
       a8[p] = YT[8 * t + p];
       b8[p] = l1 * lam[itp->L[1] + kperm(t, g, p)] * a8[p];
     }
-    T3 = mfma6(split8x3(a8), split8x3(b8), T3);
+    T3 = mmN<P>(split8<P>(a8), split8<P>(b8), T3);
   }
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
@@ -3005,8 +3082,9 @@ struct FastState {
   DevBuf d_e3items, lay3_16, lay3_64, P3_16, P3_32, P3_64, offs3;
   bool lay3_16_ok = false, lay3_64_ok = false;
   // compensated 16-bit arithmetic (split operands, FP32 intermediates) in the
-  // current reduced-precision phase
-  bool x3 = false;
+  // current reduced-precision phase; h2: two FP16 terms instead of three BF16
+  // terms on the eigenbasis kernels
+  bool x3 = false, h2 = false;
   // power-of-two exponents of the 16-bit layouts: bulk site bi at bi, degree-3
   // site k (index into deg3) at bulk.size() + k
   DevBuf sexp, pmax;
@@ -3072,12 +3150,14 @@ bool fast_path_available(itnbp_net* net, int acc_f64, int schedule) {
 }
 
 // compensated reduced-precision arithmetic: requested by the update
-// (ITNBP_PREC_BF16X3*) or by ITNBP_BF16_PRECISE=1
-static bool precise_of(const itnbp_net* net) { return net->low_x3 || fast_precise(); }
+// (ITNBP_PREC_BF16X3*, ITNBP_PREC_*F16X2) or by ITNBP_BF16_PRECISE=1
+static bool precise_of(const itnbp_net* net) { return net->low_x3 != 0 || fast_precise(); }
 
 const char* fast_path_name(itnbp_net* net, int acc_f64) {
   if (use_eig(net))
-    return acc_f64 ? "mfma-f64-eig-chi32" : (precise_of(net) ? "mfma-bf16x3-eig-chi32" : "mfma-bf16-eig-chi32");
+    return acc_f64 ? "mfma-f64-eig-chi32"
+                   : (precise_of(net) ? (net->low_x3 == 2 ? "mfma-f16x2-eig-chi32" : "mfma-bf16x3-eig-chi32")
+                                      : "mfma-bf16-eig-chi32");
   return acc_f64 ? "mfma-f64-chi32" : (precise_of(net) ? "mfma-bf16x2-chi32" : "mfma-bf16c-chi32");
 }
 
@@ -3210,6 +3290,7 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
   // eigen slot 4 bi + (c - 1) holds the message into bulk vertex bi on bond c
   st->eig = use_eig(net);
   st->x3 = !acc_f64 && precise_of(net);
+  st->h2 = st->x3 && net->low_x3 == 2;
   st->eitems.clear();
   st->k1items.clear();
   st->k1eitems.clear();
@@ -3556,8 +3637,9 @@ static void eig16_stage(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3
   }
 }
 
-// the compensated flavour (FP32 R and Q, split BF16 operands)
-static void eigx3_stage(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3) {
+// the compensated flavours (FP32 R and Q, split 16-bit operands)
+template <class Prec>
+static void eigs_stage(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3) {
   const EigItem* ei = (const EigItem*)st->d_eitems.p;
   const EigItem* e3 = (const EigItem*)st->d_e3items.p;
   const float* U = (const float*)st->eigU32.p;
@@ -3566,24 +3648,25 @@ static void eigx3_stage(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3
   float* P3 = (float*)st->P3_32.p;
   float* part = (float*)st->part32.p;
   if (stage == 1 && nb > 0)
-    hipLaunchKernelGGL(k_rot12_x3, dim3(64, nb), dim3(512), 0, ctx->stream, (const FastItem*)st->d_k1items.p,
+    hipLaunchKernelGGL(k_rot12_s<Prec>, dim3(64, nb), dim3(512), 0, ctx->stream, (const FastItem*)st->d_k1items.p,
                        (const __bf16*)st->lay16.p, U, P);
   if (stage == 2 && nb > 0)
-    hipLaunchKernelGGL(k_rot34_x3, dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei, (const float*)P, U, L, P,
+    hipLaunchKernelGGL(k_rot34_s<Prec>, dim3(EIG_NPART, nb), dim3(512), 0, ctx->stream, ei, (const float*)P, U, L, P,
                        part);
   if (stage == 3 && nb > 0)
-    hipLaunchKernelGGL(k_gram12_x3<32>, dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei, (const float*)P, L, part);
+    hipLaunchKernelGGL((k_gram12_s<Prec, 32>), dim3(EIG_NPART, nb), dim3(256), 0, ctx->stream, ei, (const float*)P, L, part);
   if (stage == 4 && n3 > 0) {
-    hipLaunchKernelGGL(k_rot12_d3_x3, dim3(4, n3), dim3(256), 0, ctx->stream, e3, (const __bf16*)st->lay3_16.p, U,
+    hipLaunchKernelGGL(k_rot12_d3_s<Prec>, dim3(4, n3), dim3(256), 0, ctx->stream, e3, (const __bf16*)st->lay3_16.p, U,
                        P3);
-    hipLaunchKernelGGL(k_rot3_d3_x3, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const float*)P3, U, L, P3,
+    hipLaunchKernelGGL(k_rot3_d3_s<Prec>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const float*)P3, U, L, P3,
                        part);
-    hipLaunchKernelGGL(k_gram12_x3<1>, dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const float*)P3, L, part);
+    hipLaunchKernelGGL((k_gram12_s<Prec, 1>), dim3(EIG_NPART, n3), dim3(256), 0, ctx->stream, e3, (const float*)P3, L, part);
   }
 }
 
 static void eig16(itnbp_ctx* ctx, FastState* st, int stage, int nb, int n3) {
-  if (st->x3) eigx3_stage(ctx, st, stage, nb, n3);
+  if (st->x3 && st->h2) eigs_stage<PrecH2>(ctx, st, stage, nb, n3);
+  else if (st->x3) eigs_stage<PrecX3>(ctx, st, stage, nb, n3);
   else if (eig_bf16_intermediates()) eig16_stage<__bf16>(ctx, st, stage, nb, n3);
   else eig16_stage<_Float16>(ctx, st, stage, nb, n3);
 }
@@ -3679,10 +3762,16 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
   }
   const int e6 = prof_mark(ctx);
   prof_add(ctx, "eigh", e0, e1);
-  prof_add(ctx, acc_f64 ? "rot12_f64" : (st->x3 ? "rot12_x3" : "rot12_bf16"), e1, e2);
-  prof_add(ctx, acc_f64 ? "rot34_f64" : (st->x3 ? "rot34_x3" : "rot34_bf16"), e2, e3m);
-  prof_add(ctx, acc_f64 ? "gram12_f64" : (st->x3 ? "gram12_x3" : "gram12_bf16"), e3m, e4);
-  prof_add(ctx, acc_f64 ? "deg3_f64" : (st->x3 ? "deg3_x3" : "deg3_bf16"), e4, e5);
+  const char* fl = acc_f64 ? "f64" : (st->x3 ? (st->h2 ? "h2" : "x3") : "bf16");
+  static const char* const names[4][4] = {{"rot12_f64", "rot34_f64", "gram12_f64", "deg3_f64"},
+                                          {"rot12_x3", "rot34_x3", "gram12_x3", "deg3_x3"},
+                                          {"rot12_h2", "rot34_h2", "gram12_h2", "deg3_h2"},
+                                          {"rot12_bf16", "rot34_bf16", "gram12_bf16", "deg3_bf16"}};
+  const int fi = fl[0] == 'f' ? 0 : fl[0] == 'x' ? 1 : fl[0] == 'h' ? 2 : 3;
+  prof_add(ctx, names[fi][0], e1, e2);
+  prof_add(ctx, names[fi][1], e2, e3m);
+  prof_add(ctx, names[fi][2], e3m, e4);
+  prof_add(ctx, names[fi][3], e4, e5);
   prof_add(ctx, "finalize", e5, e6);
   ITNBP_HIPCHK(ctx, hipGetLastError());
   return ITNBP_OK;

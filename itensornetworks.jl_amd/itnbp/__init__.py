@@ -7,7 +7,7 @@ gfx950); there is no CPU fallback.
 """
 from ._lib import ITNBPError, ITNBPLibraryMissing, Context, default_context, load  # noqa: F401
 from .network import (ITensorNetwork, NamedGraph, comb_tree, heavy_hex, named_grid,  # noqa: F401
-                      path_graph, productstate, random_tensornetwork)
+                      path_graph, productstate, random_instance_set, random_tensornetwork)
 from .cache import (BeliefPropagationCache, QuadraticFormNetwork, OPS, default_edge_sequence,  # noqa: F401
                     edge_scalars, environment, expect, expect2_on_cache, expect_on_cache,
                     identity_messages, initialize_cache, logscalar, message_diff, norm_sqr_network,

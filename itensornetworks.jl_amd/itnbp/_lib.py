@@ -18,7 +18,8 @@ LIB_PATH = os.environ.get("ITNBP_LIB", os.path.join(HERE, "libitnbp.so"))
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_STATE = range(7)
 F64, F32, BF16 = 0, 1, 2
 SCHED_SEQUENTIAL, SCHED_PARALLEL = 0, 1
-PREC_F64, PREC_F32, PREC_BF16_F64, PREC_BF16X3, PREC_BF16X3_F64, PREC_BF16_BF16X3, PREC_BF16_BF16X3_F64 = range(7)
+(PREC_F64, PREC_F32, PREC_BF16_F64, PREC_BF16X3, PREC_BF16X3_F64, PREC_BF16_BF16X3, PREC_BF16_BF16X3_F64,
+ PREC_F16X2, PREC_BF16_F16X2) = range(9)
 
 _ERRNAMES = {ERR_ARG: "invalid argument", ERR_HIP: "HIP error", ERR_NOMEM: "out of memory",
              ERR_NONFINITE: "non-finite result", ERR_UNSUPPORTED: "unsupported",
@@ -63,6 +64,7 @@ SIGNATURES = {
     "itnbp_net_destroy": (_I, [_P]),
     "itnbp_net_copy": (_I, [_P, C.POINTER(_P)]),
     "itnbp_net_psi_numel": (C.c_int64, [_P]),
+    "itnbp_get_bond_dims": (_I, [_P, _IP]),
     "itnbp_net_msg_numel": (C.c_int64, [_P]),
     "itnbp_set_sites": (_I, [_P, _P]),
     "itnbp_get_sites": (_I, [_P, _P]),
@@ -84,6 +86,7 @@ SIGNATURES = {
     "itnbp_logscalar": (_I, [_P, _DP, _DP, _IP]),
     "itnbp_normalize": (_I, [_P]),
     "itnbp_apply_gates": (_I, [_P, _I, _IP, _DP, _I, _D, _I, _DP, _DP, _I, _IP]),
+    "itnbp_apply_site_gates": (_I, [_P, _I, _IP, _DP, _I, _DP]),
     "itnbp_energy_grad": (_I, [_P, _I, _I, _DP, _DP, _DP, _DP]),
     "itnbp_update_batched": (_I, [_I, C.POINTER(_P), C.POINTER(UpdateOpts), C.POINTER(UpdateResult)]),
     "itnbp_last_path": (C.c_char_p, [_P]),

@@ -31,7 +31,8 @@ def _reorder_two_site(m, du, dv):
 
 def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None, normalize=False,
                  callback=None):
-    """Apply two-site gates in place on the cache's site tensors, every gate with
+    """Apply one- and two-site gates in place on the cache's site tensors (one-site
+    gates on the device, apply.jl:110-116), every two-site gate with
     the cache's current messages as its (product) environment.  Gates are grouped
     greedily, in order, into launches of disjoint bonds.  Returns the list of
     truncation errors; ``callback(singular_values=..., truncation_error=...)`` is
@@ -43,9 +44,19 @@ def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None
     pending = list(enumerate(gates))
     errs = [None] * len(gates)
     while pending:
-        used, batch, rest = set(), [], []
+        used, batch, site_batch, rest = set(), [], [], []
         for i, gt in pending:
-            u, v = gt.sites
+            sites = tuple(gt.sites)
+            if len(sites) == 1:
+                if sites[0] in used:
+                    rest.append((i, gt))
+                else:
+                    used.add(sites[0])
+                    site_batch.append((i, gt))
+                continue
+            if len(sites) != 2:
+                raise ValueError("Gates with more than 2 sites is not supported yet.")
+            u, v = sites
             if not g.has_edge(u, v):
                 raise ValueError("Vertices where the gates are being applied must be neighbors for now.")
             if u in used or v in used:
@@ -54,6 +65,10 @@ def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None
                 used.update((u, v))
                 batch.append((i, gt))
         pending = rest
+        if site_batch:
+            _apply_site_gates(bpc, site_batch, normalize, errs)
+        if not batch:
+            continue
         ids, mats = [], []
         max_keep = 32
         for i, gt in batch:
@@ -72,17 +87,42 @@ def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None
         te = np.empty(len(batch), dtype=np.float64)
         sv = np.zeros(len(batch) * max_keep, dtype=np.float64)
         kept = np.empty(len(batch), dtype=np.int32)
-        bpc.ctx.check(bpc.lib.itnbp_apply_gates(
-            bpc.h, len(batch), L.iptr(ids), L.dptr(mats), int(maxdim or 0),
-            -1.0 if cutoff is None else float(cutoff), int(bool(normalize)), L.dptr(te), L.dptr(sv),
-            max_keep, L.iptr(kept)))
-        bpc._psi_dirty = True
+        try:
+            bpc.ctx.check(bpc.lib.itnbp_apply_gates(
+                bpc.h, len(batch), L.iptr(ids), L.dptr(mats), int(maxdim or 0),
+                -1.0 if cutoff is None else float(cutoff), int(bool(normalize)), L.dptr(te), L.dptr(sv),
+                max_keep, L.iptr(kept)))
+        finally:
+            # a failed call may still have changed the tensors of other bonds
+            bpc._psi_dirty = True
         for k, (i, gt) in enumerate(batch):
             errs[i] = float(te[k])
             if callback is not None:
                 callback(singular_values=sv[k * max_keep:k * max_keep + kept[k]].copy(),
                          truncation_error=float(te[k]))
     return errs
+
+
+def _apply_site_gates(bpc, site_batch, normalize, errs):
+    """One-site gates on distinct vertices, on the device (apply.jl:110-116); their
+    truncation error is 0."""
+    g = bpc.graph
+    verts = np.array([g.index[gt.sites[0]] for _, gt in site_batch], dtype=np.int32)
+    mats = []
+    for _, gt in site_batch:
+        d = bpc.qf.ket.phys_dim(gt.sites[0])
+        m = np.asarray(gt.matrix, dtype=np.float64)
+        if m.shape != (d, d):
+            raise ValueError("gate shape does not match the physical dimension")
+        mats.append(m.reshape(-1, order="F"))
+    mats = np.ascontiguousarray(np.concatenate(mats))
+    try:
+        bpc.ctx.check(bpc.lib.itnbp_apply_site_gates(bpc.h, len(verts), L.iptr(verts), L.dptr(mats),
+                                                     int(bool(normalize)), None))
+    finally:
+        bpc._psi_dirty = True
+    for i, _ in site_batch:
+        errs[i] = 0.0
 
 
 def _check_product_env(envs, u, v):
@@ -99,13 +139,10 @@ def apply(o: Gate, psi: ITensorNetwork, *, envs=None, maxdim=None, cutoff=None, 
     returns a new ITensorNetwork."""
     sites = tuple(o.sites)
     if len(sites) == 1:
-        v = sites[0]
-        out = psi.copy()
-        t = np.tensordot(np.asarray(o.matrix), psi[v], axes=([1], [0]))
-        if normalize:
-            t = t / np.linalg.norm(t)
-        out[v] = t
-        return out
+        ctx = envs.ctx if isinstance(envs, BeliefPropagationCache) else None
+        bpc = BeliefPropagationCache(QuadraticFormNetwork(psi), storage="f64", ctx=ctx)
+        apply_gates_(bpc, [o], normalize=normalize)
+        return bpc.tensornetwork()
     if len(sites) > 2:
         raise ValueError("Gates with more than 2 sites is not supported yet.")
     if len(sites) < 1:

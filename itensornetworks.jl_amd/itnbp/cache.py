@@ -39,7 +39,7 @@ OPS = {
 
 PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16_f64": L.PREC_BF16_F64, "bf16x3": L.PREC_BF16X3,
               "bf16x3_f64": L.PREC_BF16X3_F64, "bf16_bf16x3": L.PREC_BF16_BF16X3,
-              "bf16_bf16x3_f64": L.PREC_BF16_BF16X3_F64}
+              "bf16_bf16x3_f64": L.PREC_BF16_BF16X3_F64, "f16x2": L.PREC_F16X2, "bf16_f16x2": L.PREC_BF16_F16X2}
 
 
 def op_matrix(op, d=2):
@@ -98,13 +98,13 @@ def _flat_sites(psi: ITensorNetwork, storage):
     return bf16_round(flat)[1]
 
 
-def _unflat_sites(psi_like: ITensorNetwork, flat, storage):
+def _unflat_sites(psi_like: ITensorNetwork, flat, storage, shapes=None):
     if storage == L.BF16:
         flat = (flat.astype(np.uint32) << 16).view(np.float32)
     out = {}
     off = 0
     for v in psi_like.graph.vertices:
-        shape = psi_like[v].shape
+        shape = psi_like[v].shape if shapes is None else shapes[v]
         n = int(np.prod(shape))
         out[v] = np.array(flat[off:off + n]).reshape(shape, order="F")
         off += n
@@ -195,7 +195,13 @@ class BeliefPropagationCache:
             dt = {L.F64: np.float64, L.F32: np.float32, L.BF16: np.uint16}[self.storage]
             flat = np.empty(n, dtype=dt)
             self.ctx.check(self.lib.itnbp_get_sites(self.h, L.vptr(flat)))
-            self._host_psi = _unflat_sites(self._host_psi, flat, self.storage)
+            # bond dimensions as on the device (apply changes them)
+            g = self.graph
+            dims = np.empty(max(g.ne, 1), dtype=np.int32)
+            self.ctx.check(self.lib.itnbp_get_bond_dims(self.h, L.iptr(dims)))
+            shapes = {v: (self._host_psi.phys_dim(v),) + tuple(int(dims[g.edge_id(v, w)]) for w in g.incident(v))
+                      for v in g.vertices}
+            self._host_psi = _unflat_sites(self._host_psi, flat, self.storage, shapes)
             self.qf = QuadraticFormNetwork(self._host_psi)
             self._psi_dirty = False
         return self._host_psi
@@ -232,7 +238,7 @@ class BeliefPropagationCache:
     def message(self, e):
         """message(bpc, e) (abstractbeliefpropagationcache.jl:173-175): M[b_ket, b_bra]."""
         a, b = e
-        chi = self.qf.ket.bond_dim(a, b)
+        chi = self.tensornetwork().bond_dim(a, b)
         out = np.empty(chi * chi, dtype=np.float64)
         self.ctx.check(self.lib.itnbp_get_message(self.h, self._mid(e), L.dptr(out)))
         return out.reshape(chi, chi, order="F").astype(self.scalartype())
@@ -245,9 +251,10 @@ class BeliefPropagationCache:
         out = {}
         off = 0
         g = self.graph
+        ket = self.tensornetwork()
         for m in range(2 * g.ne):
             a, b = g.message_edge(m)
-            chi = self.qf.ket.bond_dim(a, b)
+            chi = ket.bond_dim(a, b)
             if self.lib.itnbp_message_is_set(self.h, m):
                 out[(a, b)] = flat[off:off + chi * chi].reshape(chi, chi, order="F").astype(self.scalartype())
             off += chi * chi
@@ -255,7 +262,7 @@ class BeliefPropagationCache:
 
     def set_message_(self, e, m):
         a, b = e
-        chi = self.qf.ket.bond_dim(a, b)
+        chi = self.tensornetwork().bond_dim(a, b)
         m = np.asarray(m, dtype=np.float64)
         if m.shape != (chi, chi):
             raise ValueError(f"message shape {m.shape} != {(chi, chi)}")
@@ -330,18 +337,84 @@ def default_edge_sequence(bpc):
     return [g.message_edge(int(m)) for m in out]
 
 
+def _directed_edge(g, x):
+    """``x`` as a directed edge (a, b) of ``g``, or None when it is not one (then
+    it is a group of edges)."""
+    try:
+        if len(x) == 2 and x[0] in g.index and x[1] in g.index and g.has_edge(x[0], x[1]):
+            return (x[0], x[1])
+    except TypeError:
+        pass
+    return None
+
+
+_GROUPED = "grouped"
+
+
 def _schedule(bpc, edge_sequence):
+    """(schedule, message ids) for the C ABI; (_GROUPED, groups) for edge groups
+    with more than one edge, which the host runs group by group."""
     g = bpc.graph
     if edge_sequence is None or edge_sequence == "forest_cover":
         return L.SCHED_SEQUENTIAL, None
     if edge_sequence == "parallel":
         return L.SCHED_PARALLEL, None
     seq = list(edge_sequence)
-    if seq and isinstance(seq[0], (list,)):
-        ids = [g.message_id(*e) for grp in seq for e in grp]
+    if seq and _directed_edge(g, seq[0]) is None:
+        groups = []
+        for grp in seq:
+            edges = [_directed_edge(g, e) for e in grp]
+            if not edges or any(e is None for e in edges):
+                raise ValueError(f"edge group {grp!r} is not a non-empty list of directed edges of the graph")
+            groups.append(edges)
+        if any(len(grp) > 1 for grp in groups):
+            return _GROUPED, groups
+        ids = [g.message_id(*grp[0]) for grp in groups]
         return L.SCHED_PARALLEL, np.array(ids, dtype=np.int32)
-    ids = [g.message_id(*e) for e in seq]
+    edges = [_directed_edge(g, e) for e in seq]
+    if any(e is None for e in edges):
+        raise ValueError("edge_sequence mixes edges and edge groups, or names an edge the graph does not have")
+    ids = [g.message_id(*e) for e in edges]
     return L.SCHED_SEQUENTIAL, np.array(ids, dtype=np.int32)
+
+
+def _update_grouped_(bpc, groups, maxiter, tol, normalize, precision):
+    """update_iteration over edge groups (abstractbeliefpropagationcache.jl:294-308):
+    every group is updated sequentially, starting from the messages the sweep
+    started from, and the groups' results are merged; the convergence measure is
+    the summed message_diff over the number of groups (:313-329)."""
+    g = bpc.graph
+    it = 0
+    mean = 0.0
+    for it in range(1, int(maxiter) + 1):
+        new, diff = {}, 0.0
+        for grp in groups:
+            work = bpc.copy()
+            o = L.UpdateOpts()
+            o.schedule = L.SCHED_SEQUENTIAL
+            o.maxiter = 1
+            o.tol = 0.0
+            o.res_tol = -1.0
+            o.precision = PRECISIONS[precision]
+            o.switch_tol = 0.0
+            o.normalize = int(normalize)
+            o.verbose = 0
+            ids = np.array([g.message_id(*e) for e in grp], dtype=np.int32)
+            o.nseq = len(ids)
+            o.seq = L.iptr(ids)
+            r = L.UpdateResult()
+            bpc.ctx.check(bpc.lib.itnbp_update(work.h, C.byref(o), C.byref(r)))
+            diff += r.mean_diff * len(grp)
+            for e in grp:
+                new[e] = work.message(e)
+            del work
+        bpc.set_messages_(new)
+        mean = diff / len(groups)
+        if tol is not None and mean <= tol:
+            break
+    bpc.last_update = dict(iterations=it, iterations_low=0, mean_diff=mean, max_residual=float("nan"),
+                           converged=tol is not None and mean <= tol, device_ms=0.0, path="grouped-host")
+    return bpc
 
 
 def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, verbose=False,
@@ -358,7 +431,9 @@ def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, v
     until the residual drops below ``switch_tol`` or stagnates, then FP64), or
     "bf16_bf16x3" (the "f32" sweeps until then, followed by "bf16x3"), or
     "bf16_bf16x3_f64" ("f32" sweeps to residual 1e-2, "bf16x3" to ``switch_tol``,
-    then FP64)."""
+    then FP64); "f16x2" is the compensated arithmetic with two FP16 terms per
+    operand (about 22 bits) on the chi = 32 eigenbasis pipeline, and
+    "bf16_f16x2" the "f32" sweeps until ``switch_tol`` followed by "f16x2"."""
     if alg != "bp":
         raise ValueError(f"unknown update algorithm {alg!r}")
     if isinstance(maxiter, str):
@@ -372,6 +447,8 @@ def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, v
             raise ValueError("only the 'contract' message update is available")
         normalize = bool(mua.get("normalize", True))
     sched, seq = _schedule(bpc, edge_sequence)
+    if sched == _GROUPED:
+        return _update_grouped_(bpc, seq, maxiter, tol, normalize, precision)
     o = L.UpdateOpts()
     o.schedule = sched
     o.maxiter = int(maxiter)
@@ -404,7 +481,7 @@ def update(bpc, alg="bp", **kwargs):
 def updated_message(bpc, e, *, normalize=True):
     """updated_message(bpc, e) (abstractbeliefpropagationcache.jl:225-239, 254-261)."""
     a, b = e
-    chi = bpc.qf.ket.bond_dim(a, b)
+    chi = bpc.tensornetwork().bond_dim(a, b)
     ids = np.array([bpc.graph.message_id(a, b)], dtype=np.int32)
     out = np.empty(chi * chi, dtype=np.float64)
     bpc.ctx.check(bpc.lib.itnbp_updated_messages(bpc.h, 1, L.iptr(ids), int(normalize), L.dptr(out)))

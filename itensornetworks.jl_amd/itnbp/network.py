@@ -228,6 +228,24 @@ def random_tensornetwork(rng, eltype, graph, *, link_space=1, phys_dim=2, scale=
     return ITensorNetwork(graph, tensors)
 
 
+def random_instance_set(rng, n=256, eltype=np.float32):
+    """``n`` independent networks for batched BP: each an L x L open grid (L uniform
+    in 6..12) or a heavy-hex lattice (3..6 hexagons a side), d = 2, one bond
+    dimension per instance uniform in {4, 6, 8, 12, 16}, entries N(0, 1/chi).
+    Returns a list of (kind, chi, ITensorNetwork)."""
+    out = []
+    for _ in range(n):
+        if rng.random() < 0.5:
+            L = int(rng.integers(6, 13))
+            g, kind = named_grid((L, L)), "grid"
+        else:
+            L = int(rng.integers(3, 7))
+            g, kind = heavy_hex(L, L), "heavy_hex"
+        chi = int(rng.choice([4, 6, 8, 12, 16]))
+        out.append((kind, chi, random_tensornetwork(rng, eltype, g, link_space=chi, scale=1 / np.sqrt(chi))))
+    return out
+
+
 def productstate(graph, states, eltype=np.float64, phys_dim=2):
     """Product state with bond dimension 1; ``states[v]`` is a basis index or a vector."""
     tensors = {}

@@ -355,6 +355,33 @@ def bp_update(g, psi, msgs, maxiter, tol=None, schedule="sequential", sequence=N
     return msgs, it, mean_diff, max_res
 
 
+def bp_update_groups(g, psi, msgs, groups, maxiter, tol=None, normalize=True):
+    """update over edge groups (abstractbeliefpropagationcache.jl:294-308, 313-329):
+    in every sweep each group is updated sequentially (:272-287) starting from the
+    messages the sweep started from, the groups' messages are merged, and the
+    summed message_diff is divided by the number of groups (``length`` of the
+    sequence, :321).  Returns (msgs, iterations, mean diff)."""
+    if maxiter is None:
+        raise ValueError("You need to specify a number of iterations for BP!")
+    msgs = dict(msgs)
+    it, mean = 0, None
+    for it in range(1, maxiter + 1):
+        merged, diff = {}, 0.0
+        for grp in groups:
+            work = dict(msgs)
+            for (u, v) in grp:
+                new = updated_message(g, psi, work, u, v, normalize)
+                diff += message_diff(new, work[(u, v)])
+                work[(u, v)] = new
+            for e in grp:
+                merged[e] = work[e]
+        msgs.update(merged)
+        mean = diff / len(groups)
+        if tol is not None and mean <= tol:
+            break
+    return msgs, it, mean
+
+
 # ----------------------------------------------------------------------------
 # environments, expectation values, scalars
 # ----------------------------------------------------------------------------

@@ -127,3 +127,78 @@ def test_tebd_with_bp_regauging_matches_oracle():
     for i, v in enumerate(g.vertices):
         np.testing.assert_allclose(new[v], opsi[i], atol=1e-9)
     assert oe[-1] < oe[0]
+
+
+def test_one_site_gates_run_on_the_device():
+    """One-site gates (apply.jl:110-116) mixed with a two-site gate in one call:
+    psi_v <- O psi_v (normalised) on the device, against the host contraction;
+    the two-site gate against the oracle."""
+    g = itnbp.named_grid((3, 3))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(12), np.float64, g, link_space=3)
+    bpc = converged_cache(psi)
+    og = oracle_graph(g)
+    msgs = to_index(g, bpc.messages())
+    rng = np.random.default_rng(13)
+    ops = {v: rng.standard_normal((2, 2)) for v in [(1, 1), (3, 3), (2, 1)]}
+    G = O.expm_sym(O.ising_bond_hamiltonian(2.5, 4, 3), -0.1)
+    gates = [itnbp.Gate(m, (v,)) for v, m in ops.items()] + [itnbp.Gate(G, ((2, 2), (2, 3)))]
+    errs = itnbp.apply_gates_(bpc, gates, maxdim=3, normalize=True)
+    assert errs[:3] == [0.0, 0.0, 0.0]
+    new = bpc.tensornetwork()
+    for v, m in ops.items():
+        t = np.tensordot(m, psi[v], axes=([1], [0]))
+        np.testing.assert_allclose(new[v], t / np.linalg.norm(t), atol=1e-14)
+    ref, _, err = O.apply_gate(G, og, oracle_psi(psi), msgs, g.index[(2, 2)], g.index[(2, 3)], maxdim=3,
+                               normalize=True)
+    for v in [(2, 2), (2, 3)]:
+        np.testing.assert_allclose(new[v], ref[g.index[v]], atol=1e-10)
+    assert errs[3] == pytest.approx(err, rel=1e-8)
+    # the functional form
+    out = itnbp.apply(itnbp.Gate(ops[(1, 1)], ((1, 1),)), psi)
+    t = np.tensordot(ops[(1, 1)], psi[(1, 1)], axes=([1], [0]))
+    np.testing.assert_allclose(out[(1, 1)], t, atol=1e-14)
+
+
+def test_bond_takes_the_kept_dimension():
+    """apply.jl:81-93: the new bond has the kept dimension.  Without maxdim or
+    cutoff a chi = 2 bond grows to the full rank of Theta (8); with a cutoff a
+    bond shrinks.  Tensors, kept ranks and truncation errors against the oracle,
+    which builds the new bond from the kept singular values; the messages of a
+    resized bond become identities."""
+    g = itnbp.named_grid((3, 3))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(21), np.float64, g, link_space=2)
+    bpc = converged_cache(psi)
+    og = oracle_graph(g)
+    msgs = to_index(g, bpc.messages())
+    G = O.expm_sym(O.two_site(O.X, O.Z) + 0.5 * O.ising_bond_hamiltonian(2.5, 4, 3), -0.4)
+    u, v = (2, 2), (2, 3)
+    kept = []
+    new = itnbp.apply(itnbp.Gate(G, (u, v)), psi, envs=bpc,
+                      callback=lambda singular_values, truncation_error: kept.append(len(singular_values)))
+    ref, s_ref, err_ref = O.apply_gate(G, og, oracle_psi(psi), msgs, g.index[u], g.index[v])
+    assert kept == [len(s_ref)] and len(s_ref) == 8
+    for x in (u, v):
+        assert new[x].shape == ref[g.index[x]].shape
+        np.testing.assert_allclose(new[x], ref[g.index[x]], atol=1e-10)
+    # in place on a cache, with a cutoff that shrinks the bond
+    c2 = bpc.copy()
+    errs = itnbp.apply_gates_(c2, [itnbp.Gate(G, ((1, 1), (2, 1)))], cutoff=1e-3)
+    ref2, s2, err2 = O.apply_gate(G, og, oracle_psi(psi), msgs, g.index[(1, 1)], g.index[(2, 1)], cutoff=1e-3)
+    t2 = c2.tensornetwork()
+    chi_new = t2.bond_dim((1, 1), (2, 1))
+    assert chi_new == len(s2)
+    for x in ((1, 1), (2, 1)):
+        np.testing.assert_allclose(t2[x], ref2[g.index[x]], atol=1e-10)
+    assert errs[0] == pytest.approx(err2, rel=1e-8)
+    if chi_new != 2:
+        np.testing.assert_array_equal(c2.message(((1, 1), (2, 1))), np.eye(chi_new))
+    # maxdim = 1 shrinks the bond to one; the resized cache still runs BP
+    c3 = bpc.copy()
+    errs3 = itnbp.apply_gates_(c3, [itnbp.Gate(G, ((1, 1), (2, 1)))], maxdim=1)
+    ref3, s3, err3 = O.apply_gate(G, og, oracle_psi(psi), msgs, g.index[(1, 1)], g.index[(2, 1)], maxdim=1)
+    t3 = c3.tensornetwork()
+    assert t3.bond_dim((1, 1), (2, 1)) == 1 == len(s3)
+    for x in ((1, 1), (2, 1)):
+        np.testing.assert_allclose(t3[x], ref3[g.index[x]], atol=1e-10)
+    assert errs3[0] == pytest.approx(err3, rel=1e-8) and errs3[0] > 0
+    itnbp.update_(c3, maxiter=5, edge_sequence="parallel")

@@ -325,3 +325,61 @@ def test_density_matrix_routes_from_bf16_storage(net34, monkeypatch):
     assert max(abs(fx[v] - gx[v]) for v in gx) < 1e-12
     assert max(abs(fzz[e] - gzz[e]) for e in gzz) < 1e-12
     np.testing.assert_allclose(fvs, gvs, rtol=1e-11)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_f16x2_flavour_matches_oracle(net34, symmetric):
+    """Precision "f16x2": symmetric messages run the eigenbasis kernels with every
+    rounded operand split into two FP16 terms (FP32 intermediates); a non-symmetric
+    message takes the absorbed-pair passes with two-term BF16 splits.  Two Jacobi
+    sweeps from random messages against the oracle on the same BF16 tensors:
+    within 1e-6 (a CPU rounding model of the two-FP16-term pipeline gives 2e-7 to
+    4e-7 per sweep) and 5e-6."""
+    g, psi = net34
+    rng = np.random.default_rng(27 if symmetric else 28)
+    msgs = _random_messages(g, psi, rng, symmetric)
+    bpc = itnbp.initialize_cache(psi, storage="bf16")
+    for e, m in msgs.items():
+        bpc.set_message_(e, m)
+    out = itnbp.update(bpc, maxiter=2, edge_sequence="parallel", precision="f16x2")
+    path = out.last_update["path"]
+    assert path.startswith("mfma-f16x2-eig-chi32" if symmetric else "mfma-bf16x2-chi32"), path
+    og, opsi = oracle_graph(g), oracle_psi(bpc.tensornetwork())
+    start = {(g.index[a], g.index[b]): m for (a, b), m in msgs.items()}
+    ref, _, _, _ = O.bp_update(og, opsi, start, maxiter=2, schedule="parallel")
+    got = to_index(g, out.messages())
+    err = max(np.abs(got[e] - ref[e]).max() for e in ref)
+    print(f"f16x2 symmetric={symmetric}: max message error {err:.3g}")
+    assert err < (1e-6 if symmetric else 5e-6), err
+
+
+def test_progressive_f16x2_reaches_its_floor(net34, oracle_ref):
+    """Precision "bf16_f16x2": one-term FP16 sweeps until the residual < 1e-2, then
+    two-FP16-term sweeps; after 40 sweeps within 2e-6 of the oracle's fixed point."""
+    g, psi = net34
+    m3, m40 = oracle_ref
+    bpc = itnbp.initialize_cache(psi, storage="bf16")
+    out = itnbp.update(bpc, maxiter=40, res_tol=1e-9, edge_sequence="parallel", precision="bf16_f16x2",
+                       switch_tol=1e-2)
+    path, lu = out.last_update["path"], out.last_update
+    assert path.startswith("mfma-bf16-eig-chi32+mfma-f16x2-eig-chi32"), path
+    got = to_index(g, out.messages())
+    err = max(np.abs(got[e] - m40[e]).max() for e in m40)
+    print(f"bf16_f16x2: {lu['iterations']} sweeps, residual {lu['max_residual']:.3g}, max message error {err:.3g}")
+    assert err < 2e-6, err
+
+
+def test_f16x2_degree3_only_lattice():
+    """Heavy-hex, chi = 32: the degree-3 kernels of the two-FP16-term flavour."""
+    g = itnbp.heavy_hex(4, 3)
+    psi = itnbp.random_tensornetwork(np.random.default_rng(33), np.float64, g, link_space=32,
+                                     scale=1 / np.sqrt(32))
+    bpc = itnbp.initialize_cache(psi, storage="bf16")
+    og, opsi = oracle_graph(g), oracle_psi(bpc.tensornetwork())
+    low = itnbp.update(bpc, maxiter=3, edge_sequence="parallel", precision="f16x2")
+    assert "f16x2-eig" in low.last_update["path"], low.last_update["path"]
+    ref3, _, _, _ = O.bp_update(og, opsi, O.identity_messages(og, opsi), maxiter=3, schedule="parallel")
+    got = to_index(g, low.messages())
+    err = max(np.abs(got[e] - ref3[e]).max() for e in ref3)
+    print(f"f16x2 degree-3 lattice: max message error after 3 sweeps {err:.3g}")
+    assert err < 1e-6, err

@@ -312,3 +312,41 @@ def test_grad_oracle_energy_matches_numpy_oracle_and_finite_differences():
         fd = (energy(eps) - energy(-eps)) / (2 * eps)
         an = sum(float(np.sum(gr * d)) for gr, d in zip(grads, dirs))
         assert an == pytest.approx(fd, rel=1e-6, abs=1e-8)
+
+
+def test_grouped_schedule_reduces_to_the_jacobi_and_sequential_sweeps(grid33):
+    """Edge groups (abstractbeliefpropagationcache.jl:294-308): singleton groups are
+    the Jacobi sweep, and one group holding the whole forest-cover order is the
+    sequential sweep.  A grouped update that did not restart every group from the
+    sweep's starting messages fails the first identity."""
+    g, psi = grid33
+    m0 = O.initialize_messages(g, psi)
+    seq = O.forest_cover_sequence(g)
+    mj, _, _, _ = O.bp_update(g, psi, m0, maxiter=3, schedule="parallel", sequence=seq)
+    mg, it, _ = O.bp_update_groups(g, psi, m0, [[e] for e in seq], maxiter=3)
+    assert it == 3
+    for e in seq:
+        np.testing.assert_allclose(mg[e], mj[e], rtol=0, atol=1e-14)
+    ms, _, _, _ = O.bp_update(g, psi, m0, maxiter=3, schedule="sequential", sequence=seq)
+    mg1, _, _ = O.bp_update_groups(g, psi, m0, [list(seq)], maxiter=3)
+    for e in seq:
+        np.testing.assert_allclose(mg1[e], ms[e], rtol=0, atol=1e-14)
+    # two-edge groups differ from both (a path the Jacobi-flattened schedule misses)
+    pairs = [seq[i:i + 2] for i in range(0, len(seq), 2)]
+    mp, _, _ = O.bp_update_groups(g, psi, m0, pairs, maxiter=1)
+    mj1, _, _, _ = O.bp_update(g, psi, m0, maxiter=1, schedule="parallel", sequence=seq)
+    assert max(np.abs(mp[e] - mj1[e]).max() for e in seq) > 1e-6
+
+
+def test_expm_sym_matches_scipy():
+    """The gate builder exp(t H) of a real symmetric H against scipy.linalg.expm
+    (Pade scaling and squaring, an independent method)."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(11)
+    for n in (2, 4, 8):
+        a = rng.standard_normal((n, n))
+        H = a + a.T
+        for t in (-0.05, 0.3):
+            np.testing.assert_allclose(O.expm_sym(H, t), expm(t * H), rtol=1e-12, atol=1e-13)
+    H = O.ising_bond_hamiltonian(2.5, 4, 3)
+    np.testing.assert_allclose(O.expm_sym(H, -0.05), expm(-0.05 * H), rtol=1e-12, atol=1e-13)
