@@ -192,7 +192,9 @@ int itnbp_normalize(itnbp_net* net);
  * Each gate's bond takes the kept dimension (apply.jl:81-93): it grows or
  * shrinks on the device; the messages of a bond whose dimension changed become
  * identities of the new dimension; itnbp_get_bond_dims reports the new
- * dimensions. */
+ * dimensions.  A gate whose two-site tensor is not finite leaves its bond
+ * untouched (kept = -1) and the call returns ITNBP_ERR_NONFINITE after the
+ * other gates were applied. */
 int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
                       int maxdim, double cutoff, int normalize, double* truncerr,
                       double* svals, int max_keep, int* kept);

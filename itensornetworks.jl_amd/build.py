@@ -2,7 +2,8 @@
 
     python itensornetworks.jl_amd/build.py [--jobs N]
 
-Compiles every ``csrc/*.hip`` to an object in ``csrc/_obj`` and links
+Compiles every ``csrc/*.hip`` (hipcc) and ``csrc/*.cpp`` (host-only, g++) to
+an object in ``csrc/_obj`` and links
 ``itnbp/libitnbp.so``.
 """
 import argparse
@@ -25,9 +26,18 @@ FLAGS = [
 ]
 
 
+# host-only sources (no HIP): built with the plain C++ compiler
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Werror"]
+
+
 def compile_one(src):
-    obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+    stem, ext = os.path.splitext(os.path.basename(src))
+    obj = os.path.join(OBJ, stem + (".cpp.o" if ext == ".cpp" else ".o"))
+    if ext == ".cpp":
+        cmd = [CXX] + CXXFLAGS + ["-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
@@ -36,7 +46,7 @@ def compile_one(src):
 
 def build(jobs=8, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, srcs))
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT] + objs

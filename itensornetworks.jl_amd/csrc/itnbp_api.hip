@@ -15,12 +15,12 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
-#include <deque>
 #include <exception>
 #include <new>
 #include <string>
 #include <vector>
 
+#include "itnbp_graph.hpp"
 #include "itnbp_internal.hpp"
 #include "itnbp_ops.hpp"
 
@@ -103,12 +103,12 @@ int upload(itnbp_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
   return ITNBP_OK;
 }
 
-int msg_id_into(const itnbp_net* net, int e, int to) {
-  // id 2e runs edges[2e] -> edges[2e+1]
-  return net->edges[2 * e + 1] == to ? 2 * e : 2 * e + 1;
+itnbp_graph::Topo topo_of(const itnbp_net* net) {
+  return itnbp_graph::Topo{net->nv, net->ne, net->edges.data(), net->inc_ptr.data(), net->inc_edge.data()};
 }
-int msg_src(const itnbp_net* net, int m) { return net->edges[2 * (m / 2) + (m & 1)]; }
-int msg_dst(const itnbp_net* net, int m) { return net->edges[2 * (m / 2) + 1 - (m & 1)]; }
+int msg_id_into(const itnbp_net* net, int e, int to) { return itnbp_graph::msg_id_into(topo_of(net), e, to); }
+int msg_src(const itnbp_net* net, int m) { return itnbp_graph::msg_src(topo_of(net), m); }
+int msg_dst(const itnbp_net* net, int m) { return itnbp_graph::msg_dst(topo_of(net), m); }
 
 int axis_of(const itnbp_net* net, int v, int e) {
   for (int a = net->inc_ptr[v]; a < net->inc_ptr[v + 1]; ++a)
@@ -165,40 +165,28 @@ int plan_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, Ta
     size_t budget = (size_t)6 << 30;
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) == hipSuccess) budget = std::min(budget, freeb / 3);
-    size_t i0 = 0;
-    while (i0 < large.size()) {
-      long long maxD = 0;
-      int maxP2 = 1, maxR = 1;
-      size_t i1 = i0;
-      for (; i1 < large.size(); ++i1) {
-        const VDesc& V = net->vdesc[large[i1].v];
-        const long long nD = std::max<long long>(maxD, V.D);
-        const size_t nneed = (i1 - i0 + 1) * (size_t)(2 * nD * eb);
-        if (i1 > i0 && nneed > budget) break;
-        maxD = nD;
-      }
-      for (size_t i = i0; i < i1; ++i) {
-        const VDesc& V = net->vdesc[large[i].v];
-        const OcTask& t = large[i];
-        const int ds = t.s_open ? V.dims[0] : 1;
-        const int nj = t.j > 0 ? V.dims[t.j] : 1;
-        const int P = ds * nj;
-        if (P > 64) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "open dimension > 64 is not supported");
-        maxP2 = std::max(maxP2, P * P);
-        maxR = std::max(maxR, V.D / P);
-      }
+    std::vector<long long> D(large.size());
+    std::vector<int> P(large.size());
+    for (size_t i = 0; i < large.size(); ++i) {
+      const VDesc& V = net->vdesc[large[i].v];
+      D[i] = V.D;
+      P[i] = (large[i].s_open ? V.dims[0] : 1) * (large[i].j > 0 ? V.dims[large[i].j] : 1);
+    }
+    std::vector<itnbp_graph::LargeBatch> batches;
+    std::string why;
+    if (const int rc = itnbp_graph::large_batches(D, P, budget, eb, batches, why)) return set_err(ctx, rc, why);
+    for (const auto& bt : batches) {
       LaunchGroup lg{};
       lg.kind = 1;
       lg.begin = plan.tasks.size();
-      lg.count = i1 - i0;
-      lg.maxD = maxD;
-      lg.nsplit = std::max(1, std::min(64, maxR / 1024));
-      lg.part_stride = (long long)lg.nsplit * maxP2;
-      plan.scratch_bytes = std::max(plan.scratch_bytes, lg.count * (size_t)(2 * maxD * eb));
+      lg.count = bt.count;
+      lg.maxD = bt.maxD;
+      lg.nsplit = std::max(1, std::min(64, bt.maxR / 1024));
+      lg.part_stride = (long long)lg.nsplit * bt.maxP2;
+      plan.scratch_bytes = std::max(plan.scratch_bytes, lg.count * (size_t)(2 * bt.maxD * eb));
       plan.partial_bytes = std::max(plan.partial_bytes, lg.count * (size_t)lg.part_stride * eb);
-      plan.tasks.insert(plan.tasks.end(), large.begin() + i0, large.begin() + i1);
+      plan.tasks.insert(plan.tasks.end(), large.begin() + bt.begin, large.begin() + bt.begin + bt.count);
       plan.groups.push_back(lg);
-      i0 = i1;
     }
   }
   plan.acc_f64 = acc_f64;
@@ -261,26 +249,8 @@ int run_tasks(itnbp_net* net, const std::vector<OcTask>& tasks, int acc_f64, con
 // ---------------------------------------------------------------------------
 namespace {
 
-bool reads(const itnbp_net* net, int reader, int m) {
-  // message reader = (a -> b) reads m = (u -> v) iff v == a and u != b
-  return msg_dst(net, m) == msg_src(net, reader) && msg_src(net, m) != msg_dst(net, reader);
-}
-
-// consecutive runs of the sequence in which no message reads another one of the run
 std::vector<std::pair<int, int>> sequential_runs(const itnbp_net* net, const std::vector<int>& seq) {
-  std::vector<std::pair<int, int>> runs;
-  int start = 0;
-  for (int i = 0; i < (int)seq.size(); ++i) {
-    bool conflict = false;
-    for (int j = start; j < i && !conflict; ++j)
-      conflict = seq[j] == seq[i] || reads(net, seq[i], seq[j]) || reads(net, seq[j], seq[i]);
-    if (conflict) {
-      runs.push_back({start, i});
-      start = i;
-    }
-  }
-  if (start < (int)seq.size()) runs.push_back({start, (int)seq.size()});
-  return runs;
+  return itnbp_graph::sequential_runs(topo_of(net), seq);
 }
 
 struct Phase {
@@ -442,30 +412,14 @@ bool prec_low_only(int p) {
 }
 
 int check_sequence(itnbp_net* net, const std::vector<int>& seq, const itnbp_update_opts* o) {
-  std::vector<char> set(net->msg_set.begin(), net->msg_set.end());
   // every phased mode measures the residual against the previous messages
   const bool need_old = o->tol >= 0 || o->res_tol >= 0 ||
                         (o->precision != ITNBP_PREC_F64 && o->precision != ITNBP_PREC_F32 &&
                          o->precision != ITNBP_PREC_BF16X3 && o->precision != ITNBP_PREC_F16X2);
-  for (int m : seq) {
-    if (m < 0 || m >= 2 * net->ne) return set_err(net->ctx, ITNBP_ERR_ARG, "message id out of range in edge sequence");
-    const int u = msg_src(net, m), v = msg_dst(net, m);
-    for (int a = net->inc_ptr[u]; a < net->inc_ptr[u + 1]; ++a) {
-      const int e = net->inc_edge[a];
-      const int in = msg_id_into(net, e, u);
-      if (msg_src(net, in) == v) continue;
-      if (!set[in])
-        return set_err(net->ctx, ITNBP_ERR_STATE,
-                       "message " + std::to_string(in) + " is read before it is set (messages have no default)");
-    }
-    if (need_old && !set[m])
-      return set_err(net->ctx, ITNBP_ERR_STATE,
-                     "message " + std::to_string(m) + " has no previous value to compare with (tol needs one)");
-    if (o->schedule == ITNBP_SCHED_SEQUENTIAL) set[m] = 1;
-  }
-  if (o->schedule == ITNBP_SCHED_PARALLEL)
-    for (int m : seq) set[m] = 1;
-  return ITNBP_OK;
+  std::string why;
+  const int rc = itnbp_graph::check_sequence(topo_of(net), net->msg_set, seq, o->schedule == ITNBP_SCHED_SEQUENTIAL,
+                                             need_old, why);
+  return rc ? set_err(net->ctx, rc, why) : ITNBP_OK;
 }
 
 }  // namespace
@@ -585,10 +539,13 @@ int itnbp_net_create(itnbp_ctx* ctx, int nv, int ne, const int* edges, const int
   if (!ctx || !out) return ITNBP_ERR_ARG;
   *out = nullptr;
   return guarded(ctx, [&]() -> int {
-    if (nv <= 0 || ne < 0 || (ne > 0 && (!edges || !bond_dims)) || !inc_ptr || (ne > 0 && !inc_edge) || !phys_dims)
-      return set_err(ctx, ITNBP_ERR_ARG, "null or empty network description");
     if (dtype != ITNBP_F64 && dtype != ITNBP_F32 && dtype != ITNBP_BF16)
       return set_err(ctx, ITNBP_ERR_ARG, "unknown dtype");
+    const itnbp_graph::Topo topo{nv, ne, edges, inc_ptr, inc_edge};
+    const itnbp_graph::Limits lim{ITNBP_MAXK, 64, 8, 1LL << 30};
+    itnbp_graph::Layout lay;
+    std::string why;
+    if (const int rc = itnbp_graph::describe(topo, bond_dims, phys_dims, lim, lay, why)) return set_err(ctx, rc, why);
     auto* net = new itnbp_net();
     std::unique_ptr<itnbp_net> guard(net);
     net->ctx = ctx;
@@ -597,65 +554,31 @@ int itnbp_net_create(itnbp_ctx* ctx, int nv, int ne, const int* edges, const int
     net->dtype = dtype;
     net->edges.assign(edges, edges + 2 * ne);
     net->inc_ptr.assign(inc_ptr, inc_ptr + nv + 1);
-    net->inc_edge.assign(inc_edge, inc_edge + (ne > 0 ? inc_ptr[nv] : 0));
+    net->inc_edge.assign(inc_edge, inc_edge + 2 * ne);
     net->chi.assign(bond_dims, bond_dims + ne);
     net->phys.assign(phys_dims, phys_dims + nv);
-    if (net->inc_ptr[0] != 0 || net->inc_ptr[nv] != 2 * ne)
-      return set_err(ctx, ITNBP_ERR_ARG, "inc_ptr must start at 0 and end at 2*ne");
-    std::vector<int> seen(ne, 0);
-    for (int e = 0; e < ne; ++e) {
-      const int a = net->edges[2 * e], b = net->edges[2 * e + 1];
-      if (a < 0 || a >= nv || b < 0 || b >= nv || a == b) return set_err(ctx, ITNBP_ERR_ARG, "bad edge");
-      if (net->chi[e] < 1 || net->chi[e] > 64) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "bond dimension must be in 1..64");
-    }
-    for (int v = 0; v < nv; ++v) {
-      const int k = net->inc_ptr[v + 1] - net->inc_ptr[v];
-      if (k < 0 || k > ITNBP_MAXK) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "vertex degree > 6 is not supported");
-      if (net->phys[v] < 1 || net->phys[v] > 8) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "physical dimension must be in 1..8");
-      for (int a = net->inc_ptr[v]; a < net->inc_ptr[v + 1]; ++a) {
-        const int e = net->inc_edge[a];
-        if (e < 0 || e >= ne) return set_err(ctx, ITNBP_ERR_ARG, "incident edge out of range");
-        if (net->edges[2 * e] != v && net->edges[2 * e + 1] != v)
-          return set_err(ctx, ITNBP_ERR_ARG, "incident edge does not touch its vertex");
-        seen[e]++;
-      }
-    }
-    for (int e = 0; e < ne; ++e)
-      if (seen[e] != 2) return set_err(ctx, ITNBP_ERR_ARG, "every edge must be incident to both endpoints once");
-    // messages
-    net->msg_off.resize(2 * ne);
-    long long mo = 0;
-    for (int m = 0; m < 2 * ne; ++m) {
-      net->msg_off[m] = mo;
-      mo += (long long)net->chi[m / 2] * net->chi[m / 2];
-      net->max_chi = std::max(net->max_chi, net->chi[m / 2]);
-    }
-    net->msg_total = mo;
+    net->msg_off = lay.msg_off;
+    net->msg_total = lay.msg_total;
+    net->max_chi = lay.max_chi;
+    net->max_k = lay.max_k;
     net->msg_set.assign(2 * ne, 0);
-    // vertices
-    net->psi_off.resize(nv);
+    net->psi_off = lay.psi_off;
+    net->psi_total = lay.psi_total;
     net->vdesc.resize(nv);
-    long long po = 0;
     for (int v = 0; v < nv; ++v) {
       VDesc& V = net->vdesc[v];
       memset(&V, 0, sizeof(V));
       V.k = net->inc_ptr[v + 1] - net->inc_ptr[v];
-      net->max_k = std::max(net->max_k, V.k);
       V.dims[0] = net->phys[v];
-      long long D = V.dims[0];
       for (int a = 1; a <= V.k; ++a) {
         const int e = net->inc_edge[net->inc_ptr[v] + a - 1];
         V.dims[a] = net->chi[e];
-        D *= V.dims[a];
         V.in_msg[a] = net->msg_off[msg_id_into(net, e, v)];
       }
-      if (D > (1LL << 30)) return set_err(ctx, ITNBP_ERR_UNSUPPORTED, "site tensor larger than 2^30 elements");
-      V.D = (int)D;
-      V.psi_off = po;
-      net->psi_off[v] = po;
-      po += D;
+      V.D = (int)lay.site_D[v];
+      V.psi_off = lay.psi_off[v];
     }
-    net->psi_total = po;
+    const long long po = lay.psi_total, mo = lay.msg_total;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     if (hipMalloc(&net->d_psi, std::max<size_t>(16, po * dtype_size(dtype))) != hipSuccess ||
         hipMalloc((void**)&net->d_msg[0], std::max<size_t>(16, mo * 8)) != hipSuccess ||
@@ -1146,80 +1069,7 @@ int itnbp_updated_messages(itnbp_net* net, int n, const int* ids, int normalize,
 namespace {
 
 int default_sequence(const itnbp_net* net, int schedule, std::vector<int>& seq) {
-  seq.clear();
-  const int ne = net->ne, nv = net->nv;
-  if (schedule == ITNBP_SCHED_PARALLEL) {
-    // vcat(edges(g), reverse.(edges(g)))  (edge_sequences.jl:49-51)
-    for (int e = 0; e < ne; ++e) seq.push_back(2 * e);
-    for (int e = 0; e < ne; ++e) seq.push_back(2 * e + 1);
-    return ITNBP_OK;
-  }
-  // forest cover: BFS spanning forests of the edges left over, each tree
-  // rooted at its lowest vertex; post-order DFS edges then their reverses
-  // (edge_sequences.jl:32-47)
-  std::vector<char> remaining(ne, 1);
-  int left = ne;
-  while (left > 0) {
-    std::vector<std::vector<std::pair<int, int>>> adj(nv);  // (neighbour, edge)
-    for (int e = 0; e < ne; ++e)
-      if (remaining[e]) {
-        const int a = net->edges[2 * e], b = net->edges[2 * e + 1];
-        adj[a].push_back({b, e});
-        adj[b].push_back({a, e});
-      }
-    for (auto& l : adj) std::sort(l.begin(), l.end());
-    std::vector<char> seen(nv, 0);
-    std::vector<std::vector<std::pair<int, int>>> children(nv);  // (child, edge)
-    std::vector<int> roots;
-    std::vector<char> in_forest(ne, 0);
-    for (int r = 0; r < nv; ++r) {
-      if (seen[r] || adj[r].empty()) continue;
-      seen[r] = 1;
-      roots.push_back(r);
-      std::deque<int> q{r};
-      while (!q.empty()) {
-        const int x = q.front();
-        q.pop_front();
-        for (auto& ye : adj[x]) {
-          if (!seen[ye.first]) {
-            seen[ye.first] = 1;
-            children[x].push_back(ye);
-            in_forest[ye.second] = 1;
-            q.push_back(ye.first);
-          }
-        }
-      }
-    }
-    for (int r : roots) {
-      std::vector<int> post;  // directed ids child -> parent
-      // iterative post-order DFS
-      std::vector<std::pair<int, size_t>> st{{r, 0}};
-      std::vector<int> parent_edge(nv, -1);
-      while (!st.empty()) {
-        auto& top = st.back();
-        const int x = top.first;
-        if (top.second < children[x].size()) {
-          auto ce = children[x][top.second++];
-          parent_edge[ce.first] = ce.second;
-          st.push_back({ce.first, 0});
-        } else {
-          st.pop_back();
-          if (parent_edge[x] >= 0) {
-            const int e = parent_edge[x];
-            const int parent = net->edges[2 * e] == x ? net->edges[2 * e + 1] : net->edges[2 * e];
-            post.push_back(msg_id_into(net, e, parent));
-          }
-        }
-      }
-      for (int m : post) seq.push_back(m);
-      for (int i = (int)post.size() - 1; i >= 0; --i) seq.push_back(post[i] ^ 1);
-    }
-    for (int e = 0; e < ne; ++e)
-      if (in_forest[e]) {
-        remaining[e] = 0;
-        --left;
-      }
-  }
+  itnbp_graph::default_sequence(topo_of(net), schedule == ITNBP_SCHED_PARALLEL, seq);
   return ITNBP_OK;
 }
 

@@ -58,6 +58,13 @@ bool small_path_enabled(const itnbp_net* net);
 // tensor within LDS, every outgoing message in seq and not yet handled)
 int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::vector<char>& handled);
 int small_task_count(itnbp_net* net);
+// batched update: the instance of every vertex and the device-side active
+// flags and sweep counter the per-vertex kernels read (null pointers: off)
+void small_set_batch(itnbp_net* net, const std::vector<int>& vinst, const int* active, const long long* sweep_ctr);
+// times the Gram tiles of the large-site kernel on scratch outputs and keeps
+// the fastest; small_big_tile reports it (0 without large sites)
+int small_autotune(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, int normalize);
+int small_big_tile(itnbp_net* net);
 int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, void* msg_out,
                 MsgStat* stats, int normalize);
 void release_small_state(itnbp_net* net);
@@ -66,6 +73,8 @@ void release_small_state(itnbp_net* net);
 // give every edge e the bond dimension chi_new[e] (tensors cut or zero-padded,
 // messages of changed edges reset to identities); releases derived state
 int net_resize_bonds(itnbp_net* net, const std::vector<int>& chi_new);
+
+// ---- apply (k_apply.hip) ----------------------------------------------------
 
 // Runs f() and turns any C++ exception into a status code, so that none
 // crosses the extern "C" boundary.

@@ -18,6 +18,7 @@
 // SVD in LDS.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -506,7 +507,8 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     }
     int st = 0;
     if (keep > chib) st = 1;  // the bond cannot grow on the device
-    truncerr[it.slot] = tot > 0 ? disc / tot : 0.0;
+    if (!isfinite(tot)) st = 2;  // non-finite Theta (a NaN or Inf in psi or the environment)
+    truncerr[it.slot] = tot > 0 ? disc / tot : (st == 2 ? tot : 0.0);
     if (svals)
       for (int i = 0; i < prm.max_keep; ++i)
         svals[(long long)it.slot * prm.max_keep + i] = i < keep && i < N ? svec[order[i]] : 0.0;
@@ -516,8 +518,9 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   }
   __syncthreads();
   const int keep = sh_keep;
-  // a bond whose kept rank exceeds its dimension is left untouched (the host
-  // reports the error); every write to psi comes after this point
+  // a bond whose kept rank exceeds its dimension, or whose Theta is not
+  // finite, is left untouched (the host reports the error); every write to psi
+  // comes after this point
   if (keep < 0) return;
   // left factor Lf (Mr x keep) = U sqrt(S) into Ru, right factor Rf (Nc x keep) = V sqrt(S) into Rv;
   // A = U diag(s) (M x N), V = VV (N x N); if transposed, Theta = VV S U^T
@@ -784,8 +787,16 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const 
   if (ek == hipSuccess) ek = hipMemcpyAsync(st.data(), b.st, n * 4, hipMemcpyDeviceToHost, ctx->stream);
   if (ek == hipSuccess) ek = hipStreamSynchronize(ctx->stream);
   if (ek != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: ") + hipGetErrorString(ek));
+  // a gate the kernel refused (its bond untouched) reports kept = -1
+  for (int i = 0; i < n; ++i)
+    if (st[i]) kp[i] = -1;
   if (kept) std::copy(kp.begin(), kp.end(), kept);
   invalidate_fast_state(net);
+  for (int i = 0; i < n; ++i)
+    if (st[i] == 2)
+      return itnbp_fail(ctx, ITNBP_ERR_NONFINITE,
+                        "apply: non-finite two-site tensor (that bond's tensors are unchanged; the other gates of "
+                        "the call were applied)");
   for (int i = 0; i < n; ++i)
     if (st[i])
       return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
@@ -812,6 +823,7 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
     // cut to its kept rank, which drops only zeros.
     for (int i = 0; i < n; ++i)
       if (edge_ids[i] < 0 || edge_ids[i] >= net->ne) return itnbp_fail(ctx, ITNBP_ERR_ARG, "edge id out of range");
+    const std::vector<int> chi0 = net->chi;
     std::vector<int> target(n), chi_grow = net->chi;
     for (int i = 0; i < n; ++i) {
       const int e = edge_ids[i];
@@ -830,12 +842,20 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
     for (size_t b : sz) total += b;
     DevBuf d;
     ITNBP_RETCHK(ensure_buf(ctx, d, total));
-    std::vector<int> kp(n);
+    std::vector<int> kp(n, -1);
     const int rc = run_apply(net, n, items, gates, goff, maxdim, cutoff, normalize, truncerr, svals, mk, kp.data(),
                              carve_apply_bufs((char*)d.p, n, goff, mk));
     free_buf(d);
-    if (rc != ITNBP_OK) return rc;
     std::vector<int> chi_cut = net->chi;
+    if (rc != ITNBP_OK) {
+      // the gates that ran keep their new bonds; a refused gate's bond (its
+      // tensors untouched, grown with zeros) returns to its old dimension
+      for (int i = 0; i < n; ++i)
+        chi_cut[edge_ids[i]] = kp[i] < 0 ? chi0[edge_ids[i]] : std::max(1, std::min(kp[i], target[i]));
+      if (kept) std::copy(kp.begin(), kp.end(), kept);
+      (void)net_resize_bonds(net, chi_cut);
+      return rc;
+    }
     for (int i = 0; i < n; ++i) {
       kp[i] = std::max(1, std::min(kp[i], target[i]));
       chi_cut[edge_ids[i]] = kp[i];

@@ -1179,7 +1179,8 @@ template <typename TM>
 __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ moff, int n,
                                                  const TM* __restrict__ msg, double* __restrict__ U64,
                                                  double* __restrict__ L64, float* __restrict__ U32,
-                                                 float* __restrict__ L32, int warm, double tol2) {
+                                                 float* __restrict__ L32, int* __restrict__ Lexp, int warm,
+                                                 double tol2) {
   constexpr int LD = 33;
   __shared__ double As[4][FC * LD];
   __shared__ double Vs[4][FC * LD];
@@ -1328,10 +1329,19 @@ __global__ __launch_bounds__(256) void k_msg_eigh(const long long* __restrict__ 
     u64[o] = v;  // U[b + 32 r]: column r is the eigenvector of eigenvalue r
     if (u32) u32[o] = (float)v;
   }
-  if (l < FC) {
-    const double lam = A[l * LD + l];
-    L64[(long long)e * FC + l] = lam;
-    if (L32) L32[(long long)e * FC + l] = (float)lam;
+  // the FP32 copy is scaled by 2^-x so that its largest |lam| lies in [1/2, 1):
+  // products of eigenvalues (the Gram weights of the 16-bit kernels) then stay
+  // of order one whatever the messages' scale, and the finaliser multiplies the
+  // Gram back by 2^(sum of the x of the bonds it weighs)
+  const double lam = l < FC ? A[l * LD + l] : 0.0;
+  if (l < FC) L64[(long long)e * FC + l] = lam;
+  if (L32) {
+    double m = fabs(lam);
+    for (int k = 32; k > 0; k >>= 1) m = fmax(m, __shfl_xor(m, k));
+    int x = 0;
+    if (m > 0 && isfinite(m)) frexp(m, &x);
+    if (l < FC) L32[(long long)e * FC + l] = (float)ldexp(lam, -x);
+    if (l == 0) Lexp[e] = x;
   }
 }
 
@@ -2683,7 +2693,8 @@ __global__ __launch_bounds__(256) void k_eig_finalize(const EigItem* __restrict_
                                                       const TP* __restrict__ part,
                                                       const double* __restrict__ U64, const TM* msg_in,
                                                       TM* msg_out, MsgStat* __restrict__ stats,
-                                                      int normalize, const int* __restrict__ sexp) {
+                                                      int normalize, const int* __restrict__ sexp,
+                                                      const int* __restrict__ lexp) {
   constexpr int LD = 33;
   __shared__ double Tp[FC * LD], Us[FC * LD], X[FC * LD];
   __shared__ double red[8];
@@ -2713,7 +2724,15 @@ __global__ __launch_bounds__(256) void k_eig_finalize(const EigItem* __restrict_
     Tp[b * LD + bb] = acc;  // T[b][b']
   }
   __syncthreads();
-  const double us = 0.5 * site_unscale2(sexp, itp->site);
+  double us = 0.5 * site_unscale2(sexp, itp->site);
+  if (lexp) {
+    // the eigenvalue scales of the bonds that weigh this Gram (L[3] of a
+    // degree-3 site is the table of ones, scale 0)
+    int x = 0;
+    for (int b = 0; b < 4; ++b)
+      if (b != cb) x += lexp[itp->L[b] / FC];
+    us = ldexp(us, x);
+  }
   double n2 = 0;
   for (int o = threadIdx.x; o < FBLK; o += blockDim.x) {
     const int i = o & 31, j = o >> 5;  // message layout M[i + 32 j]
@@ -3073,6 +3092,7 @@ struct FastState {
   std::vector<EigItem> k1eitems;    // rot12 descriptors for the rot34 kernels without Grams
   std::vector<long long> eig_moff;  // message offset of every eigendecomposed message
   DevBuf d_eitems, d_k1items, d_k1eitems, d_eig_moff, eigU64, eigL64, eigU32, eigL32;
+  DevBuf eigLexp;  // power-of-two exponents of the FP32 eigenvalues, one per message (+ 0 for the ones)
   bool eig_warm = false;            // eigU64 holds an orthogonal matrix per slot (seeds the next eigh)
   long long eig_count = 0;          // eigh launches since the last allocation; every 16th starts cold
   // degree-3 sites (eigenbasis pipeline only)
@@ -3181,6 +3201,7 @@ void release_fast_state(itnbp_net* net) {
   free_buf(st->eigL64);
   free_buf(st->eigU32);
   free_buf(st->eigL32);
+  free_buf(st->eigLexp);
   free_buf(st->d_e3items);
   free_buf(st->lay3_16);
   free_buf(st->lay3_64);
@@ -3395,6 +3416,9 @@ int fast_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std::
       ITNBP_RETCHK(ensure_buf(ctx, st->eigL32, (ne4 + 1) * FC * 4));
       ITNBP_HIPCHK(ctx, hipMemcpyAsync((float*)st->eigL32.p + ne4 * FC, ones32, sizeof(ones32),
                                        hipMemcpyHostToDevice, ctx->stream));
+      static const int zero_exp = 0;
+      ITNBP_RETCHK(ensure_buf(ctx, st->eigLexp, (ne4 + 1) * 4));
+      ITNBP_HIPCHK(ctx, hipMemcpyAsync((int*)st->eigLexp.p + ne4, &zero_exp, 4, hipMemcpyHostToDevice, ctx->stream));
     }
     // layout C of every degree-3 site and the block buffers of those in use
     const int n3all = (int)st->deg3.size();
@@ -3687,11 +3711,12 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
   if (acc_f64)
     hipLaunchKernelGGL((k_msg_eigh<double>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
                        (const long long*)st->d_eig_moff.p, ne4, (const double*)msg_in, (double*)st->eigU64.p,
-                       (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr, warm, 1e-28);
+                       (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr, (int*)nullptr, warm, 1e-28);
   else
     hipLaunchKernelGGL((k_msg_eigh<float>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
                        (const long long*)st->d_eig_moff.p, ne4, (const float*)msg_in, (double*)st->eigU64.p,
-                       (double*)st->eigL64.p, (float*)st->eigU32.p, (float*)st->eigL32.p, warm, 1e-14);
+                       (double*)st->eigL64.p, (float*)st->eigU32.p, (float*)st->eigL32.p, (int*)st->eigLexp.p,
+                       warm, 1e-14);
   st->eig_warm = true;
   ++st->eig_count;
   const int e1 = prof_mark(ctx);
@@ -3745,20 +3770,20 @@ static int eig_sweep(itnbp_ctx* ctx, FastState* st, int acc_f64, const void* msg
     if (nb > 0)
       hipLaunchKernelGGL((k_eig_finalize<double, double>), dim3(4, nb), dim3(256), 0, ctx->stream, ei,
                          (const double*)st->part64.p, (const double*)st->eigU64.p, (const double*)msg_in,
-                         (double*)msg_out, stats, normalize, (const int*)nullptr);
+                         (double*)msg_out, stats, normalize, (const int*)nullptr, (const int*)nullptr);
     if (n3 > 0)
       hipLaunchKernelGGL((k_eig_finalize<double, double>), dim3(3, n3), dim3(256), 0, ctx->stream, e3,
                          (const double*)st->part64.p, (const double*)st->eigU64.p, (const double*)msg_in,
-                         (double*)msg_out, stats, normalize, (const int*)nullptr);
+                         (double*)msg_out, stats, normalize, (const int*)nullptr, (const int*)nullptr);
   } else {
     if (nb > 0)
       hipLaunchKernelGGL((k_eig_finalize<float, float>), dim3(4, nb), dim3(256), 0, ctx->stream, ei,
                          (const float*)st->part32.p, (const double*)st->eigU64.p, (const float*)msg_in,
-                         (float*)msg_out, stats, normalize, (const int*)st->sexp.p);
+                         (float*)msg_out, stats, normalize, (const int*)st->sexp.p, (const int*)st->eigLexp.p);
     if (n3 > 0)
       hipLaunchKernelGGL((k_eig_finalize<float, float>), dim3(3, n3), dim3(256), 0, ctx->stream, e3,
                          (const float*)st->part32.p, (const double*)st->eigU64.p, (const float*)msg_in,
-                         (float*)msg_out, stats, normalize, (const int*)st->sexp.p);
+                         (float*)msg_out, stats, normalize, (const int*)st->sexp.p, (const int*)st->eigLexp.p);
   }
   const int e6 = prof_mark(ctx);
   prof_add(ctx, "eigh", e0, e1);
@@ -3877,7 +3902,8 @@ static int rotated_sites_setup(itnbp_net* net, const std::vector<int>& want, siz
   // cold start: the rotations must be orthogonal to rounding
   hipLaunchKernelGGL((k_msg_eigh<double>), dim3((ne4 + 3) / 4), dim3(256), 0, ctx->stream,
                      (const long long*)st->d_eig_moff.p, ne4, (const double*)net->d_msg[net->cur],
-                     (double*)st->eigU64.p, (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr, 0, 1e-28);
+                     (double*)st->eigU64.p, (double*)st->eigL64.p, (float*)nullptr, (float*)nullptr, (int*)nullptr, 0,
+                     1e-28);
   st->eig_warm = true;
   if (nbs > 0) {
     double* P = (double*)st->P64.p;

@@ -30,7 +30,7 @@ constexpr int SV_THREADS = 256;
 
 struct SmallTask {
   int v;
-  int pad;
+  int inst;                   // batched update: instance of the vertex (0 otherwise)
   long long out[ITNBP_MAXK];  // outgoing message on bond axis a + 1
   int slot[ITNBP_MAXK];
   long long yoff;             // k_vertex_big: offset of the site's scratch tensor
@@ -217,7 +217,8 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
                                                           const TS* __restrict__ psi, const TM* msg_in,
                                                           TM* msg_out, MsgStat* __restrict__ stats,
                                                           int normalize, int fmax, double* __restrict__ seeds,
-                                                          int warm) {
+                                                          int warm, const int* __restrict__ active,
+                                                          const long long* __restrict__ sweep_ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* Us = reinterpret_cast<double*>(smem);
   double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
@@ -236,6 +237,9 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
 
   __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
   const SmallTask* tk = tasks + blockIdx.x;
+  // batched update: a finished instance's messages were carried over already
+  if (active && !active[tk->inst]) return;
+  if (sweep_ctr) warm = (*sweep_ctr % 16) != 0;
   const VDesc* V = vdesc + tk->v;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
   const int k = V->k, D = V->D;
@@ -453,22 +457,31 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
 // e.g. chi = 12 or 16 at degree 4): psi is copied into a scratch tensor Y in
 // global memory (L2 / Infinity Cache resident while the workgroup works on it),
 // rotated there in place fibre by fibre, and every weighted Gram streams Y
-// through LDS in tiles of VB_FT fibres (base offsets and weights computed by the
-// loader).  Steps 2 and 5 are those of k_vertex_eig.
-constexpr int VB_FT = 192;  // fibres per Gram tile
+// through LDS in tiles of FT fibres (base offsets and weights computed by the
+// loader).  Steps 2 and 5 are those of k_vertex_eig.  The tile size is a
+// template parameter; the batched update's autotuner times the VB_TILES
+// variants and keeps the fastest, VB_FT otherwise.
+constexpr int VB_FT = 192;
+constexpr int VB_TILES[3] = {160, 192, 256};
 // fixed LDS of k_vertex_big: that of k_vertex_eig without the Jacobi work
 // areas, which alias the Gram tile (idle while the messages are decomposed), so
 // that four FP32 workgroups share a CU
 constexpr size_t VB_FIXED = SV_FIXED - 4 * SV_LD * SV_LD * 8;
-static_assert(VB_FT * SV_LD * sizeof(float) >= 4 * SV_LD * SV_LD * 8, "Jacobi work must fit in the tile area");
+static_assert(160 * SV_LD * sizeof(float) >= 4 * SV_LD * SV_LD * 8, "Jacobi work must fit in the tile area");
 
-template <typename TS, typename TA, typename TM>
+template <typename TA>
+constexpr size_t vb_lds(int ft) {
+  return VB_FIXED + (size_t)ft * 8 + (size_t)ft * SV_LD * sizeof(TA);
+}
+
+template <typename TS, typename TA, typename TM, int FT>
 __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restrict__ vdesc,
                                                           const SmallTask* __restrict__ tasks,
                                                           const TS* __restrict__ psi, const TM* msg_in,
                                                           TM* msg_out, MsgStat* __restrict__ stats, int normalize,
                                                           TA* __restrict__ scratch, double* __restrict__ seeds,
-                                                          int warm) {
+                                                          int warm, const int* __restrict__ active,
+                                                          const long long* __restrict__ sweep_ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* Us = reinterpret_cast<double*>(smem);
   double* lam = Us + ITNBP_MAXK * SV_LD * SV_LD;
@@ -480,12 +493,15 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
   double* Tp = gpart + SV_THREADS;
   double* X = Tp + SV_LD * SV_LD;
   double* red = X + SV_LD * SV_LD;
-  double* tw = reinterpret_cast<double*>(smem + VB_FIXED);      // tile weights [VB_FT]
-  TA* tv8 = reinterpret_cast<TA*>(tw + VB_FT);                 // tile values [VB_FT][SV_LD]
+  double* tw = reinterpret_cast<double*>(smem + VB_FIXED);      // tile weights [FT]
+  TA* tv8 = reinterpret_cast<TA*>(tw + FT);                    // tile values [FT][SV_LD]
   double* As = reinterpret_cast<double*>(tv8);                  // Jacobi work (step 2 only)
 
   __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
   const SmallTask* tk = tasks + blockIdx.x;
+  // batched update: a finished instance's messages were carried over already
+  if (active && !active[tk->inst]) return;
+  if (sweep_ctr) warm = (*sweep_ctr % 16) != 0;
   const VDesc* V = vdesc + tk->v;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
   const int k = V->k, D = V->D;
@@ -562,7 +578,7 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
     TA pv[SV_MAXCHI];
     double pw = 0;
     auto fetch = [&](int f0) {
-      if (tid < min(VB_FT, F - f0)) {
+      if (tid < min(FT, F - f0)) {
         const int f = f0 + tid;
         const int base = (f % st) + (f / st) * st * n;
         double w = 1.0;
@@ -574,8 +590,8 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
       }
     };
     fetch(0);
-    for (int f0 = 0; f0 < F; f0 += VB_FT) {
-      const int nf = min(VB_FT, F - f0);
+    for (int f0 = 0; f0 < F; f0 += FT) {
+      const int nf = min(FT, F - f0);
       if (tid < nf) {
         tw[tid] = pw;
 #pragma unroll
@@ -583,7 +599,7 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
           if (x < n) tv8[tid * SV_LD + x] = pv[x];
       }
       __syncthreads();
-      if (f0 + VB_FT < F) fetch(f0 + VB_FT);
+      if (f0 + FT < F) fetch(f0 + FT);
       if (rl < n) {
 #pragma unroll 2
         for (int j = grp; j < nf; j += 16) {
@@ -676,37 +692,109 @@ struct SmallState {
   // rounding in the accumulated rotations cannot build up
   DevBuf seeds;
   long long sweeps = 0;
+  // batched update: instance of every vertex (empty: 0), the per-instance
+  // active flags the kernels skip finished instances by, and the device sweep
+  // counter the warm start is taken from (both null outside the batched update)
+  std::vector<int> vinst;
+  const int* active = nullptr;
+  const long long* sweep_ctr = nullptr;
+  int ft = VB_FT;  // Gram tile of k_vertex_big
 };
 
 SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net->small); }
+
+template <typename TS, typename TA, typename TM, int FT>
+hipError_t launch_big(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
+                      MsgStat* stats, int normalize, double* seeds, int warm, hipStream_t stream) {
+  auto kern = k_vertex_big<TS, TA, TM, FT>;
+  const size_t lds = vb_lds<TA>(FT);
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((unsigned)st->big.size()), dim3(SV_THREADS), lds, stream, (const VDesc*)net->d_vdesc,
+                     (const SmallTask*)st->d_big.p, (const TS*)psi, (const TM*)msg_in, (TM*)msg_out, stats, normalize,
+                     (TA*)st->ybuf.p, seeds, warm, st->active, st->sweep_ctr);
+  return hipGetLastError();
+}
+
+template <typename TS, typename TA, typename TM>
+hipError_t launch_big_ft(int ft, itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
+                         MsgStat* stats, int normalize, double* seeds, int warm, hipStream_t stream) {
+  switch (ft) {
+    case 160: return launch_big<TS, TA, TM, 160>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
+    case 256: return launch_big<TS, TA, TM, 256>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
+    default: return launch_big<TS, TA, TM, 192>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
+  }
+}
 
 template <typename TS, typename TA, typename TM>
 hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
                        MsgStat* stats, int normalize) {
   const int warm = (st->sweeps % 16) != 0 ? 1 : 0;
   double* seeds = (double*)st->seeds.p;
+  // the big sites (the longest workgroups) first, so that the LDS-resident
+  // sites fill in behind them
+  if (!st->big.empty()) {
+    hipError_t e = launch_big_ft<TS, TA, TM>(st->ft, net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm,
+                                            net->ctx->stream);
+    if (e != hipSuccess) return e;
+  }
   if (!st->tasks.empty()) {
     auto kern = k_vertex_eig<TS, TA, TM>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3((unsigned)st->tasks.size()), dim3(SV_THREADS), st->lds, net->ctx->stream,
                        (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, (const TS*)psi,
-                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax, seeds, warm);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  if (!st->big.empty()) {
-    auto kern = k_vertex_big<TS, TA, TM>;
-    const size_t lds = VB_FIXED + (size_t)VB_FT * 8 + (size_t)VB_FT * SV_LD * sizeof(TA);
-    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3((unsigned)st->big.size()), dim3(SV_THREADS), lds, net->ctx->stream,
-                       (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_big.p, (const TS*)psi,
-                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, (TA*)st->ybuf.p, seeds, warm);
+                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax, seeds, warm, st->active,
+                       st->sweep_ctr);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+// Times one launch of k_vertex_big with every VB_TILES tile on scratch outputs
+// (messages, statistics and eigenvector seeds; the network's are untouched)
+// and keeps the fastest tile in st->ft.
+template <typename TS, typename TA, typename TM>
+int autotune_big(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, int normalize) {
+  itnbp_ctx* ctx = net->ctx;
+  DevBuf out, stats, seeds;
+  auto cleanup = [&]() {
+    free_buf(out);
+    free_buf(stats);
+    free_buf(seeds);
+  };
+  int rc = ensure_buf(ctx, out, (size_t)net->msg_total * sizeof(TM));
+  if (rc == ITNBP_OK) rc = ensure_buf(ctx, stats, sizeof(MsgStat) * (size_t)std::max(1, 2 * net->ne));
+  if (rc == ITNBP_OK) rc = ensure_buf(ctx, seeds, st->seeds.bytes);
+  if (rc != ITNBP_OK) {
+    cleanup();
+    return rc;
+  }
+  float best = INFINITY;
+  int best_ft = st->ft;
+  for (int rep = 0; rep < 2; ++rep)  // the first round warms the code and caches
+    for (int ft : VB_TILES) {
+      hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+      if (e == hipSuccess)
+        e = launch_big_ft<TS, TA, TM>(ft, net, st, psi, msg_in, out.p, (MsgStat*)stats.p, normalize,
+                                      (double*)seeds.p, 0, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+      if (e != hipSuccess) {
+        cleanup();
+        return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("vertex autotune: ") + hipGetErrorString(e));
+      }
+      if (rep == 1 && ms < best) {
+        best = ms;
+        best_ft = ft;
+      }
+    }
+  cleanup();
+  st->ft = best_ft;
+  return ITNBP_OK;
 }
 
 }  // namespace
@@ -752,6 +840,7 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
     const bool fits = lds + 256 <= 160 * 1024;  // + the static LDS of the kernel
     SmallTask t{};
     t.v = v;
+    t.inst = st->vinst.empty() ? 0 : st->vinst[v];
     for (int a = 1; a <= V.k && ok; ++a) {
       const int e = net->inc_edge[net->inc_ptr[v] + a - 1];
       const int w = net->edges[2 * e] == v ? net->edges[2 * e + 1] : net->edges[2 * e];
@@ -776,6 +865,18 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
       st->big.push_back(t);
     }
   }
+  // costliest sites first (rotations and Grams both scale as D x sum of the
+  // bond dimensions), so that the hardware dispatcher starts the long
+  // workgroups early and the short ones fill in behind them
+  auto cost = [&](const SmallTask& t) {
+    const VDesc& V = net->vdesc[t.v];
+    long long s = 0;
+    for (int a = 1; a <= V.k; ++a) s += V.dims[a];
+    return (long long)V.D * s;
+  };
+  auto by_cost = [&](const SmallTask& x, const SmallTask& y) { return cost(x) > cost(y); };
+  std::stable_sort(st->tasks.begin(), st->tasks.end(), by_cost);
+  std::stable_sort(st->big.begin(), st->big.end(), by_cost);
   if (!st->tasks.empty() || !st->big.empty())
     ITNBP_RETCHK(ensure_buf(ctx, st->seeds, (size_t)std::max(1, 2 * net->ne) * SEED_LD * SEED_LD * 8));
   if (!st->big.empty()) {
@@ -792,6 +893,39 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
     ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   }
   return ITNBP_OK;
+}
+
+void small_set_batch(itnbp_net* net, const std::vector<int>& vinst, const int* active, const long long* sweep_ctr) {
+  SmallState* st = small_of(net);
+  if (!st) {
+    st = new SmallState();
+    net->small = st;
+  }
+  st->vinst = vinst;
+  st->active = active;
+  st->sweep_ctr = sweep_ctr;
+}
+
+int small_autotune(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, int normalize) {
+  SmallState* st = small_of(net);
+  if (!st || st->big.empty()) return ITNBP_OK;
+  if (acc_f64) {
+    switch (psi_dtype) {
+      case ITNBP_F64: return autotune_big<double, double, double>(net, st, psi, msg_in, normalize);
+      case ITNBP_F32: return autotune_big<float, double, double>(net, st, psi, msg_in, normalize);
+      default: return autotune_big<bf16_t, double, double>(net, st, psi, msg_in, normalize);
+    }
+  }
+  switch (psi_dtype) {
+    case ITNBP_F64: return autotune_big<double, float, float>(net, st, psi, msg_in, normalize);
+    case ITNBP_F32: return autotune_big<float, float, float>(net, st, psi, msg_in, normalize);
+    default: return autotune_big<bf16_t, float, float>(net, st, psi, msg_in, normalize);
+  }
+}
+
+int small_big_tile(itnbp_net* net) {
+  SmallState* st = small_of(net);
+  return st && !st->big.empty() ? st->ft : 0;
 }
 
 int small_task_count(itnbp_net* net) {

@@ -215,7 +215,7 @@ def forest_cover_sequence(g):
 
             dfs(r, None)
             seq += post + [(b, a) for (a, b) in reversed(post)]
-        remaining = [e for e in remaining if e not in forest]
+        remaining = [(u, v) for (u, v) in remaining if (min(u, v), max(u, v)) not in forest]
     return seq
 
 

@@ -61,3 +61,39 @@ def test_batched_with_a_non_symmetric_message_uses_generic_kernels():
         ref, _, _, _ = O.bp_update(og, opsi, start, maxiter=r["iterations"], schedule="parallel")
         got = to_index(g, c.messages())
         assert max(np.abs(got[e] - ref[e]).max() for e in ref) < 1e-11
+
+
+def run_batched(inst, monkeypatch, graph, **kw):
+    monkeypatch.setenv("ITNBP_BATCH_GRAPH", "1" if graph else "0")
+    caches = [itnbp.BeliefPropagationCache(psi) for _, psi in inst]
+    for c, (g, psi) in zip(caches, inst):
+        c.set_messages_(itnbp.identity_messages(psi))
+    res = itnbp.update_batched_(caches, **kw)
+    return res, [to_index(g, c.messages()) for c, (g, _) in zip(caches, inst)]
+
+
+@pytest.mark.parametrize("kw", [dict(maxiter=25, res_tol=1e-9, precision="f64"),
+                                dict(maxiter=13, precision="f64"),
+                                dict(maxiter=30, tol=1e-10, precision="f32")])
+def test_graph_loop_matches_the_host_loop(monkeypatch, kw):
+    """The device-resident loop (eight sweeps captured in a HIP graph and
+    replayed, per-instance convergence and drop-out decided on the device, the
+    large-site Gram tile autotuned) stops every instance at the sweep the
+    host-driven loop does, with the same messages; without a tolerance every
+    instance runs maxiter sweeps, also when maxiter is not a multiple of eight."""
+    inst = instances(np.float32 if kw["precision"] == "f32" else np.float64)
+    rg, mg = run_batched(inst, monkeypatch, True, **kw)
+    rh, mh = run_batched(inst, monkeypatch, False, **kw)
+    assert rg[0]["path"].startswith("batched-graph [vertex "), rg[0]["path"]
+    assert rh[0]["path"].startswith("batched-generic"), rh[0]["path"]
+    assert [r["iterations"] for r in rg] == [r["iterations"] for r in rh]
+    assert [r["converged"] for r in rg] == [r["converged"] for r in rh]
+    if "tol" not in kw and "res_tol" not in kw:
+        assert all(r["iterations"] == kw["maxiter"] for r in rg)
+    f64 = kw["precision"] == "f64"
+    atol = 1e-11 if f64 else 1e-5
+    for a, b in zip(mg, mh):
+        assert max(np.abs(a[e] - b[e]).max() for e in a) < atol
+    for r_g, r_h in zip(rg, rh):
+        assert r_g["mean_diff"] == pytest.approx(r_h["mean_diff"], rel=1e-6 if f64 else 1e-2,
+                                                 abs=1e-14 if f64 else 1e-7)

@@ -1,0 +1,106 @@
+"""The host-side topology logic (csrc/itnbp_graph.cpp: network validation and
+layout, the default edge sequences, the conflict-free runs of a sequence, the
+set-before-read check, the batching of large tasks) built with g++ under the
+address and undefined-behaviour sanitizers and driven on its own (no GPU) over
+grids, heavy-hex lattices, trees, ragged random graphs and disconnected graphs;
+its forest cover also equals the oracle's."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import itnbp
+from helpers import ROOT, oracle_graph
+from oracle import bp_oracle as O
+
+DRIVER = os.path.join(ROOT, "tests", "cpp", "graph_driver.cpp")
+GRAPH_SRC = os.path.join(ROOT, "itensornetworks.jl_amd", "csrc", "itnbp_graph.cpp")
+SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
+ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0:halt_on_error=1",
+           UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("san") / "graph_driver")
+    cmd = ["g++", "-std=c++17", "-g", "-O1", "-Wall", "-Werror"] + SAN + [DRIVER, GRAPH_SRC, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def ragged_graph(rng, nv, ne_target, max_deg=6):
+    """Random simple graph with vertex degrees up to max_deg (some isolated
+    vertices, several components)."""
+    edges, deg = [], [0] * nv
+    seen = set()
+    for _ in range(20 * ne_target):
+        if len(edges) == ne_target:
+            break
+        a, b = (int(x) for x in rng.integers(0, nv, 2))
+        if a == b or (min(a, b), max(a, b)) in seen or deg[a] >= max_deg or deg[b] >= max_deg:
+            continue
+        seen.add((min(a, b), max(a, b)))
+        edges.append((a, b))
+        deg[a] += 1
+        deg[b] += 1
+    return itnbp.NamedGraph(list(range(nv)), edges)
+
+
+def random_tree(rng, nv):
+    return itnbp.NamedGraph(list(range(nv)), [(int(rng.integers(0, v)), v) for v in range(1, nv)])
+
+
+def graphs():
+    rng = np.random.default_rng(7)
+    yield "grid_5x5", itnbp.named_grid((5, 5))
+    yield "periodic_4x4", itnbp.named_grid((4, 4), periodic=True)
+    yield "heavy_hex_3x3", itnbp.heavy_hex(3, 3)
+    yield "path_7", itnbp.path_graph(7)
+    yield "comb_4x3", itnbp.comb_tree(4, 3)
+    yield "single_vertex", itnbp.NamedGraph([0], [])
+    yield "one_edge", itnbp.path_graph(2)
+    for i in range(3):
+        yield f"tree_{i}", random_tree(rng, 30 + 10 * i)
+    for i in range(4):
+        yield f"ragged_{i}", ragged_graph(rng, 40, 60 + 15 * i)
+
+
+def write_graph(path, g, rng):
+    edges, inc_ptr, inc_edge = g.csr()
+    chi = rng.integers(1, 5, g.ne)
+    phys = rng.integers(1, 4, g.nv)
+    with open(path, "wb") as f:
+        f.write(np.array([g.nv, g.ne], dtype=np.int32).tobytes())
+        for a in (edges, inc_ptr, inc_edge, chi, phys):
+            f.write(np.ascontiguousarray(a, dtype=np.int32).tobytes())
+
+
+@pytest.mark.parametrize("name,g", list(graphs()), ids=[n for n, _ in graphs()])
+def test_graph_logic_under_sanitizers(driver, tmp_path, name, g):
+    path = str(tmp_path / "g.bin")
+    write_graph(path, g, np.random.default_rng(len(name)))
+    r = subprocess.run([driver, path], capture_output=True, text=True, env=ENV, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr
+    out = json.loads(r.stdout)
+    assert out["failures"] == 0 and out["checks"] > 0
+    seq = [g.message_edge(m) for m in out["forest"]]
+    assert [(g.index[a], g.index[b]) for (a, b) in seq] == O.forest_cover_sequence(oracle_graph(g))
+
+
+def test_malformed_descriptions_under_sanitizers(driver):
+    r = subprocess.run([driver, "--invalid"], capture_output=True, text=True, env=ENV, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr
+    codes = json.loads(r.stdout)
+    L = itnbp._lib
+    assert codes == {
+        "valid": L.OK, "no_vertices": L.ERR_ARG, "self_loop": L.ERR_ARG, "vertex_out_of_range": L.ERR_ARG,
+        "inc_ptr_end": L.ERR_ARG, "inc_ptr_decreasing": L.ERR_ARG, "incident_out_of_range": L.ERR_ARG,
+        "incident_not_touching": L.ERR_ARG, "incident_twice": L.ERR_ARG, "chi_zero": L.ERR_UNSUPPORTED,
+        "chi_65": L.ERR_UNSUPPORTED, "phys_9": L.ERR_UNSUPPORTED, "site_too_large": L.ERR_UNSUPPORTED,
+        "degree_above_limit": L.ERR_UNSUPPORTED, "null_bond_dims": L.ERR_ARG, "open_dim_65": L.ERR_UNSUPPORTED,
+        "large_batches_invariants": 0}
