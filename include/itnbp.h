@@ -203,6 +203,18 @@ int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* 
  * Frobenius norm (:113-115); norms[n] (may be NULL) receives ||O psi_v||. */
 int itnbp_apply_site_gates(itnbp_net* net, int n, const int* verts, const double* ops, int normalize,
                            double* norms);
+/* itnbp_apply_gates without any device allocation: the launch items, gate
+ * matrices, per-gate workspace and per-gate outputs live in one file-scope
+ * device array of the library (16.4 MiB per GPU, held until the process
+ * ends).  No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops).  Limits:
+ * n <= 128 gates, (d_u d_v)^2 <= 256 elements per gate, max_keep <= 32;
+ * larger calls return ITNBP_ERR_UNSUPPORTED.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other threads, other
+ * contexts on the same device) race. */
+int itnbp_apply_gates_noalloc(itnbp_net* net, int n, const int* edge_ids, const double* gates,
+                              int maxdim, double cutoff, int normalize, double* truncerr,
+                              double* svals, int max_keep, int* kept);
 
 /* ---- adjoint ---------------------------------------------------------- */
 /* BP energy E = sum_e <O2_e> + sum_v <O1_v> after nsweeps Jacobi sweeps from

@@ -473,6 +473,9 @@ int itnbp_create(int device, itnbp_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+  // the workspace of itnbp_apply_gates_noalloc on this device (a failure here
+  // only disables that entry point, which then reports it)
+  (void)apply_noalloc_init(device);
   *out = ctx;
   return ITNBP_OK;
 }

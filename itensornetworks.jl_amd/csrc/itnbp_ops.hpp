@@ -75,6 +75,9 @@ void release_small_state(itnbp_net* net);
 int net_resize_bonds(itnbp_net* net, const std::vector<int>& chi_new);
 
 // ---- apply (k_apply.hip) ----------------------------------------------------
+// resolve the device address of the no-allocation workspace on `device` (the
+// current device), once per device
+int apply_noalloc_init(int device);
 
 // Runs f() and turns any C++ exception into a status code, so that none
 // crosses the extern "C" boundary.

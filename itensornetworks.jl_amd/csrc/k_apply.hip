@@ -807,7 +807,53 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const 
   return ITNBP_OK;
 }
 
+// ---- workspace of itnbp_apply_gates_noalloc --------------------------------
+// At most NOALLOC_MAX_GATES gates per call, gate matrices of at most
+// NOALLOC_MAX_GATE_ELEMS elements ((d_u d_v)^2 with d <= 4) and at most
+// NOALLOC_MAX_KEEP singular values per gate: 16.4 MiB in all, held by every
+// GPU the library runs on until the process ends.
+constexpr int NOALLOC_MAX_GATES = 128;
+constexpr int NOALLOC_MAX_GATE_ELEMS = 256;
+constexpr int NOALLOC_MAX_KEEP = 32;
+
+constexpr size_t noalloc_bytes() {
+  return ((NOALLOC_MAX_GATES * sizeof(ApplyItem) + 255) & ~(size_t)255) +
+         (size_t)NOALLOC_MAX_GATES * NOALLOC_MAX_GATE_ELEMS * 8 + (size_t)NOALLOC_MAX_GATES * 2 * AP_XY * 8 +
+         (size_t)NOALLOC_MAX_GATES * 8 + (size_t)NOALLOC_MAX_GATES * NOALLOC_MAX_KEEP * 8 +
+         2 * (size_t)NOALLOC_MAX_GATES * 4 + 8 * 256;
+}
+
+// The array's name is a host-side stand-in: host code uses only the device
+// address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[noalloc_bytes()];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+int g_noalloc_rc[NOALLOC_MAX_DEVICES] = {};
+bool g_noalloc_done[NOALLOC_MAX_DEVICES] = {};
+
 }  // namespace
+
+// Resolves the workspace address on `device` (current) once per device; called
+// by itnbp_create.  Returns ITNBP_OK or ITNBP_ERR_HIP (then the no-allocation
+// entry point refuses every call on that device).
+int apply_noalloc_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return ITNBP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_done[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      ptr = nullptr;
+    }
+    g_noalloc_ptr[device] = ptr;
+    g_noalloc_rc[device] = e == hipSuccess ? ITNBP_OK : ITNBP_ERR_HIP;
+    g_noalloc_done[device] = true;
+  }
+  return g_noalloc_rc[device];
+}
 
 extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
                                  int maxdim, double cutoff, int normalize, double* truncerr,
@@ -915,5 +961,46 @@ extern "C" int itnbp_apply_site_gates(itnbp_net* net, int n, const int* verts, c
     for (int i = 0; i < n; ++i)
       if (!std::isfinite(nrm[i])) return itnbp_fail(ctx, ITNBP_ERR_NONFINITE, "apply_site_gates: non-finite result");
     return ITNBP_OK;
+  });
+}
+
+extern "C" int itnbp_apply_gates_noalloc(itnbp_net* net, int n, const int* edge_ids, const double* gates,
+                                         int maxdim, double cutoff, int normalize, double* truncerr,
+                                         double* svals, int max_keep, int* kept) {
+  if (!net || n < 0 || (n > 0 && (!edge_ids || !gates || !truncerr))) return ITNBP_ERR_ARG;
+  if (n == 0) return ITNBP_OK;
+  itnbp_ctx* ctx = net->ctx;
+  return itnbp_guarded(ctx, [&]() -> int {
+    if (n > NOALLOC_MAX_GATES)
+      return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
+                        "apply_gates_noalloc: at most " + std::to_string(NOALLOC_MAX_GATES) + " gates per call");
+    const int mk = svals ? max_keep : 1;
+    if (mk > NOALLOC_MAX_KEEP)
+      return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
+                        "apply_gates_noalloc: max_keep at most " + std::to_string(NOALLOC_MAX_KEEP));
+    std::vector<ApplyItem> items;
+    long long goff = 0;
+    // no allocation, so no resizing: every bond keeps its dimension and fewer
+    // kept values are zero-padded
+    ITNBP_RETCHK(prepare_apply(net, n, edge_ids, maxdim, cutoff, svals, max_keep, net->chi, true, items, goff));
+    if (goff > (long long)n * NOALLOC_MAX_GATE_ELEMS)
+      return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
+                        "apply_gates_noalloc: gate matrices larger than (d_u d_v)^2 = 256 elements");
+    void* base = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_noalloc_mu);
+      if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES && g_noalloc_done[ctx->device])
+        base = g_noalloc_ptr[ctx->device];
+    }
+    if (!base)
+      return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply_gates_noalloc: the workspace address of this device is unknown");
+    size_t sz[7];
+    apply_buf_sizes(n, goff, mk, sz);
+    size_t total = 0;
+    for (size_t b : sz) total += b;
+    if (total > noalloc_bytes())
+      return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply_gates_noalloc: call exceeds the workspace");
+    return run_apply(net, n, items, gates, goff, maxdim, cutoff, normalize, truncerr, svals, mk, kept,
+                     carve_apply_bufs((char*)base, n, goff, mk));
   });
 }

@@ -87,6 +87,7 @@ SIGNATURES = {
     "itnbp_normalize": (_I, [_P]),
     "itnbp_apply_gates": (_I, [_P, _I, _IP, _DP, _I, _D, _I, _DP, _DP, _I, _IP]),
     "itnbp_apply_site_gates": (_I, [_P, _I, _IP, _DP, _I, _DP]),
+    "itnbp_apply_gates_noalloc": (_I, [_P, _I, _IP, _DP, _I, _D, _I, _DP, _DP, _I, _IP]),
     "itnbp_energy_grad": (_I, [_P, _I, _I, _DP, _DP, _DP, _DP]),
     "itnbp_update_batched": (_I, [_I, C.POINTER(_P), C.POINTER(UpdateOpts), C.POINTER(UpdateResult)]),
     "itnbp_last_path": (C.c_char_p, [_P]),
