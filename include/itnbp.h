@@ -233,6 +233,49 @@ int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, const double*
 int itnbp_update_batched(int n, itnbp_net** nets, const itnbp_update_opts* opts,
                          itnbp_update_result* res);
 
+/* ---- two-layer networks: bra != ket, real or complex ------------------ */
+/* BP on <phi|psi>, the LinearFormNetwork of inner(phi, psi; alg="bp")
+ * (inner.jl:128-160, formnetworks/linearformnetwork.jl:15-25), one {bra, ket}
+ * partition per site; with the operator folded into the ket, on <phi|A|psi>
+ * (inner(phi, A, psi; alg="bp"), formnetworks/bilinearformnetwork.jl:23-42);
+ * with phi = psi, the norm network of a complex state. FP64 or complex FP64
+ * (is_complex: every element an interleaved (re, im) pair of doubles).
+ * Site tensors column-major [s, bonds in incident order]; the bra layer holds
+ * phi itself (the conjugation is done by the kernels). Messages
+ * M[b_ket, b_bra] (chi_ket x chi_bra column-major); no default message
+ * (initialize_cache.jl:10-12 builds the cache without one). */
+typedef struct itnbp_lnet itnbp_lnet;
+int itnbp_lnet_create(itnbp_ctx* ctx, int nv, int ne, const int* edges, const int* inc_ptr,
+                      const int* inc_edge, const int* ket_bond_dims, const int* bra_bond_dims,
+                      const int* phys_dims, int is_complex, itnbp_lnet** out);
+int itnbp_lnet_destroy(itnbp_lnet* lnet);
+int64_t itnbp_lnet_msg_numel(itnbp_lnet* lnet);
+int64_t itnbp_lnet_ket_numel(itnbp_lnet* lnet);
+int64_t itnbp_lnet_bra_numel(itnbp_lnet* lnet);
+int itnbp_lnet_set_tensors(itnbp_lnet* lnet, const double* ket, const double* bra);
+/* ket_v <- A_v psi_v: ops are A_v[s', s, a_1 .. a_k] (s' the bra side), the
+ * merged bond index is b + chi_psi a, and the network's ket bond dimensions
+ * must be chi_psi * chi_op (bilinearformnetwork.jl:23-42 with the operator
+ * and ket link indices of every edge merged) */
+int itnbp_lnet_fold_operator(itnbp_lnet* lnet, const double* psi, const int* psi_bond_dims,
+                             const double* ops, const int* op_bond_dims);
+int itnbp_lnet_get_ket(itnbp_lnet* lnet, double* ket);
+int itnbp_lnet_set_message(itnbp_lnet* lnet, int msg_id, const double* data);
+int itnbp_lnet_get_message(itnbp_lnet* lnet, int msg_id, double* data);
+int itnbp_lnet_message_is_set(itnbp_lnet* lnet, int msg_id);
+/* delta(b_ket, b_bra) on every message (equal bond dimensions only) */
+int itnbp_lnet_init_identity_messages(itnbp_lnet* lnet);
+/* update(bpc; alg="bp") (abstractbeliefpropagationcache.jl:294-337); FP64 only */
+int itnbp_lnet_update(itnbp_lnet* lnet, const itnbp_update_opts* opts, itnbp_update_result* res);
+/* vertex scalars (nv (re, im) pairs) and edge scalars (ne pairs)
+ * (beliefpropagationcache.jl:107-119); logscalar: sum of the complex logs of
+ * the vertex scalars minus those of the edge scalars (:397-412) */
+int itnbp_lnet_region_scalars(itnbp_lnet* lnet, double* vertex_scalars, double* edge_scalars);
+int itnbp_lnet_logscalar(itnbp_lnet* lnet, double* re, double* im);
+/* R_v[s_ket, s_bra] = environment of v contracted with ket and conj(bra) over
+ * every bond (beliefpropagationcache.jl:100-105), unnormalised, d x d each */
+int itnbp_lnet_rdm1(itnbp_lnet* lnet, int n, const int* verts, double* out);
+
 /* ---- diagnostics ------------------------------------------------------ */
 /* name of the kernel path the last itnbp_update used */
 const char* itnbp_last_path(itnbp_ctx* ctx);

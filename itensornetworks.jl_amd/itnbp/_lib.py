@@ -90,6 +90,22 @@ SIGNATURES = {
     "itnbp_apply_gates_noalloc": (_I, [_P, _I, _IP, _DP, _I, _D, _I, _DP, _DP, _I, _IP]),
     "itnbp_energy_grad": (_I, [_P, _I, _I, _DP, _DP, _DP, _DP]),
     "itnbp_update_batched": (_I, [_I, C.POINTER(_P), C.POINTER(UpdateOpts), C.POINTER(UpdateResult)]),
+    "itnbp_lnet_create": (_I, [_P, _I, _I, _IP, _IP, _IP, _IP, _IP, _IP, _I, C.POINTER(_P)]),
+    "itnbp_lnet_destroy": (_I, [_P]),
+    "itnbp_lnet_msg_numel": (C.c_int64, [_P]),
+    "itnbp_lnet_ket_numel": (C.c_int64, [_P]),
+    "itnbp_lnet_bra_numel": (C.c_int64, [_P]),
+    "itnbp_lnet_set_tensors": (_I, [_P, _DP, _DP]),
+    "itnbp_lnet_fold_operator": (_I, [_P, _DP, _IP, _DP, _IP]),
+    "itnbp_lnet_get_ket": (_I, [_P, _DP]),
+    "itnbp_lnet_set_message": (_I, [_P, _I, _DP]),
+    "itnbp_lnet_get_message": (_I, [_P, _I, _DP]),
+    "itnbp_lnet_message_is_set": (_I, [_P, _I]),
+    "itnbp_lnet_init_identity_messages": (_I, [_P]),
+    "itnbp_lnet_update": (_I, [_P, C.POINTER(UpdateOpts), C.POINTER(UpdateResult)]),
+    "itnbp_lnet_region_scalars": (_I, [_P, _DP, _DP]),
+    "itnbp_lnet_logscalar": (_I, [_P, _DP, _DP]),
+    "itnbp_lnet_rdm1": (_I, [_P, _I, _IP, _DP]),
     "itnbp_last_path": (C.c_char_p, [_P]),
     "itnbp_profile_enable": (_I, [_P, _I]),
     "itnbp_profile_reset": (_I, [_P]),

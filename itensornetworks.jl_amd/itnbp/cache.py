@@ -128,7 +128,8 @@ class BeliefPropagationCache:
         if storage is None:
             storage = "f32" if psi.dtype == np.float32 else "f64"
         if np.iscomplexobj(np.zeros(0, dtype=psi.dtype)):
-            raise ValueError("complex site tensors are not supported by this build")
+            raise ValueError("complex site tensors run on the two-layer engine: build the cache with "
+                             "initialize_cache(psi) (an itnbp.LayeredBPCache)")
         self.storage = {"f64": L.F64, "f32": L.F32, "bf16": L.BF16}[storage]
         self.storage_name = storage
         self._host_psi = psi
@@ -320,6 +321,12 @@ def initialize_cache(tn, alg="bp", *, messages=None, storage=None, ctx=None, par
         bpc.partitioning = pn
         return bpc
     qf = tn if isinstance(tn, QuadraticFormNetwork) else QuadraticFormNetwork(tn)
+    if np.iscomplexobj(np.zeros(0, dtype=qf.ket.dtype)):
+        # complex states: BP on the norm network by the two-layer (bra, ket) kernels
+        from .layer import norm_cache
+        if storage not in (None, "f64"):
+            raise ValueError("complex site tensors are held in complex FP64")
+        return norm_cache(qf.ket, messages=messages, ctx=ctx)
     bpc = BeliefPropagationCache(qf, storage=storage, ctx=ctx)
     if messages is None:
         if not qf.is_tree():
@@ -436,6 +443,14 @@ def update_(bpc, alg="bp", *, maxiter="default", tol=None, edge_sequence=None, v
     "bf16_f16x2" the "f32" sweeps until ``switch_tol`` followed by "f16x2"."""
     if alg != "bp":
         raise ValueError(f"unknown update algorithm {alg!r}")
+    if getattr(bpc, "layered", False):
+        if precision != "f64" or message_update_alg is not None:
+            raise ValueError("the two-layer engine runs the FP64 'contract' update only")
+        if edge_sequence is not None and edge_sequence not in ("forest_cover", "parallel") and \
+                isinstance(edge_sequence[0], list):
+            raise ValueError("edge groups are not available on the two-layer engine")
+        return bpc.update_(maxiter=maxiter, tol=tol, res_tol=res_tol, edge_sequence=edge_sequence,
+                           verbose=verbose)
     if isinstance(maxiter, str):
         maxiter = bpc.default_bp_maxiter()
     if maxiter is None:
@@ -541,6 +556,8 @@ def expect_on_cache(bpc, op, vertices=None):
     """<O_v> = tr(O rho_v)/tr(rho_v) for every vertex (expect.jl:5-19).  On a
     cache over generalised partitions, rho_v is the partial trace of v's
     partition RDM and ``vertices`` are vertices of the original network."""
+    if getattr(bpc, "layered", False):
+        return bpc.expect(op, vertices)
     if bpc.partitioning is not None:
         pn = bpc.partitioning
         vertices = list(pn.psi.graph.vertices) if vertices is None else list(vertices)
@@ -618,6 +635,8 @@ def edge_scalars(bpc):
 
 def logscalar(bpc):
     """logscalar(bpc) (abstractbeliefpropagationcache.jl:397-408)."""
+    if getattr(bpc, "layered", False):
+        return bpc.logscalar()
     re, im, ninf = C.c_double(), C.c_double(), C.c_int()
     bpc.ctx.check(bpc.lib.itnbp_logscalar(bpc.h, C.byref(re), C.byref(im), C.byref(ninf)))
     if ninf.value:

@@ -258,3 +258,45 @@ def productstate(graph, states, eltype=np.float64, phys_dim=2):
             vec[:] = s
         tensors[v] = vec.reshape((phys_dim,) + (1,) * graph.degree(v))
     return ITensorNetwork(graph, tensors)
+
+
+def merge_link_indices(graph, tensors, link_dims):
+    """Site tensors whose edges carry several link indices, as the one-bond
+    network the library runs on.
+
+    ``link_dims[(u, w)]`` lists the dimensions of the edge's link indices in the
+    order they appear on both of its tensors; ``tensors[v]`` has the axes
+    ``(s, links of its first incident edge..., links of the next...)`` in the
+    graph's incident order.  The links of every edge merge into one bond of
+    dimension prod(link_dims) (first link fastest), so a message of the
+    returned network is the product-index form of the reference's per-edge
+    message vector (identity_messages pushes one delta per link index,
+    formnetworks/quadraticformnetwork.jl:115-119, whose product is the delta
+    of the merged bond); ``split_message`` undoes the merge."""
+    dims_of = {}
+    for (u, w), dims in link_dims.items():
+        dims_of[(u, w)] = tuple(int(x) for x in dims)
+        dims_of[(w, u)] = tuple(int(x) for x in dims)
+    out = {}
+    for v in graph.vertices:
+        t = np.asarray(tensors[v])
+        shape = [t.shape[0]]
+        a = 1
+        for w in graph.incident(v):
+            dims = dims_of[(v, w)]
+            if tuple(t.shape[a:a + len(dims)]) != dims:
+                raise ValueError(f"tensor of {v}: link axes toward {w} have shape {t.shape[a:a + len(dims)]}, "
+                                 f"expected {dims}")
+            shape.append(int(np.prod(dims)))
+            a += len(dims)
+        if a != t.ndim:
+            raise ValueError(f"tensor of {v} has {t.ndim - a} axes beyond its links")
+        out[v] = t.reshape(shape, order="F")
+    return ITensorNetwork(graph, out)
+
+
+def split_message(M, dims):
+    """A message of a merged bond as the tensor [k_1 .. k_n, k_1' .. k_n'] over
+    the edge's link indices (ket links, then bra links)."""
+    dims = tuple(int(x) for x in dims)
+    return np.asarray(M).reshape(dims + dims, order="F")
