@@ -37,7 +37,7 @@ struct ApplyItem {
   int ju, jv;    // bond axis at u, v
   long long gate;  // offset of the (d_u d_v)^2 gate
   int slot;
-  int pad;
+  int chib_in;     // bond dimension before the call (the layout may be grown past it)
 };
 
 struct ApplyParams {
@@ -396,8 +396,15 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
 
   const ApplyItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
-  const SideGeom gu = side_geom(vds[it.u], it.ju);
-  const SideGeom gv = side_geom(vds[it.v], it.jv);
+  // the factorisation runs on the bond as it was (columns b < chib_in); a bond
+  // grown for the kept rank only widens the layout the result is written to
+  SideGeom gu = side_geom(vds[it.u], it.ju);
+  SideGeom gv = side_geom(vds[it.v], it.jv);
+  for (SideGeom* g : {&gu, &gv}) {
+    g->chib = it.chib_in;
+    g->n = g->d * g->chib;
+    g->r = g->E < g->n ? g->E : g->n;
+  }
   double* scr_u = work + (long long)blockIdx.x * 2 * AP_XY;
   double* scr_v = scr_u + AP_XY;
 
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       }
     }
     int st = 0;
-    if (keep > chib) st = 1;  // the bond cannot grow on the device
+    if (keep > gu.dims[gu.j]) st = 1;  // more than the bond's layout holds
     if (!isfinite(tot)) st = 2;  // non-finite Theta (a NaN or Inf in psi or the environment)
     truncerr[it.slot] = tot > 0 ? disc / tot : (st == 2 ? tot : 0.0);
     if (svals)
@@ -588,7 +595,8 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     for (int i = tid; i < g.E * cols; i += blockDim.x) nrm2 += cur[i] * cur[i];
     nrm2 = block_sum_d(nrm2, red);
     const double sc = (prm.normalize && nrm2 > 0) ? 1.0 / sqrt(nrm2) : 1.0;
-    for (int idx = tid; idx < g.E * g.n; idx += blockDim.x) {
+    const int ncol = g.d * g.dims[g.j];  // columns of the bond's layout
+    for (int idx = tid; idx < g.E * ncol; idx += blockDim.x) {
       const int e = idx % g.E, c = idx / g.E;
       const int s = c % g.d, b = c / g.d;
       const double val = b < keep ? cur[e + (long long)g.E * (s + g.d * b)] * sc : 0.0;
@@ -695,8 +703,9 @@ int prepare_apply(itnbp_net* net, int n, const int* edge_ids, int maxdim, double
         if (chi[f] > AP_ENV) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply supports external bond dimensions <= 8");
         E *= chi[f];
       }
-      const long long nn = (long long)net->phys[x] * chi[e];
-      if (E * nn > AP_XY) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: site tensor too large for the LDS kernel");
+      const long long nn = (long long)net->phys[x] * net->chi[e];
+      if (E * nn > AP_XY || E * net->phys[x] * chi[e] > AP_XY)
+        return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: site tensor too large for the LDS kernel");
       if (std::min(E, nn) * net->phys[x] > AP_TH) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: Theta larger than 32x32");
       if (std::min(E, nn) * nn > 1024) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: R factor larger than 1024");
     }
@@ -706,6 +715,7 @@ int prepare_apply(itnbp_net* net, int n, const int* edge_ids, int maxdim, double
     items[i].jv = axis_of(net, v, e);
     items[i].gate = goff;
     items[i].slot = i;
+    items[i].chib_in = net->chi[e];
     goff += (long long)net->phys[u] * net->phys[v] * net->phys[u] * net->phys[v];
   }
   return ITNBP_OK;
@@ -1002,5 +1012,850 @@ extern "C" int itnbp_apply_gates_noalloc(itnbp_net* net, int n, const int* edge_
       return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply_gates_noalloc: call exceeds the workspace");
     return run_apply(net, n, items, gates, goff, maxdim, cutoff, normalize, truncerr, svals, mk, kept,
                      carve_apply_bufs((char*)base, n, goff, mk));
+  });
+}
+
+// ============================================================================
+// Adjoint of the gate application (vector-Jacobian product of
+// simple_update_bp, src/apply.jl:33-95, with the normalisation of :134-137).
+//
+// The kernel recomputes each bond's forward pass in a gauge of its own and
+// pulls the cotangents of psi_u', psi_v' back to psi_u, psi_v, the gate and
+// every environment message.  With X = psi (x) sqrt(env) as the E x n matrix of
+// a side (E external, n = d chi_b) the QR factor only enters through a gauge
+// that cancels between Theta and the rebuild, so:
+//   - a tall side (E >= n) uses R = C^{1/2}, C = X^T X, and the rebuild factor
+//     W = psi_P C^{-1/2}, psi_P = psi (x) P with P the projector on the kept
+//     environment eigenvectors (S Si = P);
+//   - a wide side (E < n) uses R = X and W = (x) Si.
+// Theta = G (R_u R_v), thin SVD U S V^T, F_u = U_k sqrt(S_k), F_v = V_k sqrt(S_k),
+// psi_u' = W_u F_u times the bond signs of the forward gauge, then normalised.
+// Pullbacks, in reverse: normalisation; W F products; sqrt(S); the thin SVD
+// (full-rank formula with the kept/discarded couplings, 1/(s_j^2 - s_i^2)
+// broadened by 1e-12 s_max^2 against degenerate values); the contraction with
+// the gate; R = C^{1/2} and C^{-1/2} through the eigenbasis of C with the
+// divided differences 1/(s_i + s_j) and -1/(s_i s_j (s_i + s_j)); X = psi (x) S;
+// sqrt and inverse sqrt of every environment through its eigenbasis (kept
+// eigenvalues; a dropped one contributes nothing).  P is taken as constant.
+// One workgroup per bond, FP64, intermediates in a global workspace per bond.
+// ============================================================================
+namespace {
+
+constexpr int VJ_SM = 1024;
+constexpr int VJ_ENV = 64 * 6 + 16;  // per external axis: V, S, Si, Sb, Sib, tmp (8x8 each) + w
+// per side: 9 matrices of up to AP_XY, 8 of up to 1024, 3 environments
+constexpr long long VJ_SIDE = 9LL * AP_XY + 8LL * VJ_SM + AP_MAXEXT * VJ_ENV + 64;
+// shared: 14 matrices of up to 1024 and small vectors
+constexpr long long VJ_SHARED = 14LL * VJ_SM + 6 * 64;
+constexpr long long VJ_GATE = 2 * VJ_SIDE + VJ_SHARED;
+
+struct VjpIo {
+  long long out_u, out_v;  // cotangents of psi_u', psi_v' (new bond dimension keep)
+  long long in_u, in_v;    // cotangents of psi_u, psi_v
+  long long dgate;         // cotangent of the gate
+  int keep, pad;
+};
+
+__device__ inline void bsync() {
+  __threadfence_block();
+  __syncthreads();
+}
+
+// C (m x n, ld ldc) = op(A) op(B) (+ C if acc), column-major; A m x k (or k x m
+// transposed), B k x n (or n x k transposed)
+__device__ void gmm(int m, int n, int k, const double* A, int lda, bool ta, const double* B, int ldb, bool tb,
+                    double* C, int ldc, bool acc = false) {
+  for (int idx = threadIdx.x; idx < m * n; idx += blockDim.x) {
+    const int i = idx % m, j = idx / m;
+    double s = 0;
+    for (int l = 0; l < k; ++l) {
+      const double a = ta ? A[l + (long long)lda * i] : A[i + (long long)lda * l];
+      const double b = tb ? B[j + (long long)ldb * l] : B[l + (long long)ldb * j];
+      s += a * b;
+    }
+    C[i + (long long)ldc * j] = acc ? C[i + (long long)ldc * j] + s : s;
+  }
+  bsync();
+}
+
+// Y = X with the symmetric chi x chi matrix Mt (row-major) on external axis pa
+__device__ void ext_mp_g(const SideGeom& g, int pa, const double* X, double* Y, int cols, const double* Mt) {
+  int stride = 1;
+  for (int i = 0; i < pa; ++i) stride *= g.dims[g.ext[i]];
+  const int chi = g.dims[g.ext[pa]];
+  for (int idx = threadIdx.x; idx < g.E * cols; idx += blockDim.x) {
+    const int e = idx % g.E, c = idx / g.E;
+    const int lo = e % stride, b2 = (e / stride) % chi, hi = e / (stride * chi);
+    double acc = 0;
+    for (int b = 0; b < chi; ++b) acc += X[(lo + stride * (b + chi * hi)) + (long long)g.E * c] * Mt[b * chi + b2];
+    Y[idx] = acc;
+  }
+  bsync();
+}
+
+// Symmetric eigendecomposition of one chi x chi message by one wave (the Jacobi
+// of env_roots_wave): V (row-major, V[r * chi + m] = component r of eigenvector
+// m), w[m], and the kept mask of map_eigvals' truncation (cutoff on |w|).
+__device__ unsigned env_eig_wave(const double* M, int chi, double cutoff, double* Vout, double* wout) {
+  const int l = threadIdx.x & 63;
+  const int ne = chi + (chi & 1);
+  const bool live = l < ne * ne;
+  const int i = live ? l / ne : 0, j = live ? l % ne : 0;
+  const bool in = live && i < chi && j < chi;
+  double a = in ? 0.5 * (M[i + chi * j] + M[j + chi * i]) : 0.0;
+  double v = (live && i == j) ? 1.0 : 0.0;
+  if (chi > 1) {
+    auto rot = [&](int p, int q, double& c, double& sn) {
+      const double app = __shfl(a, p * ne + p), aqq = __shfl(a, q * ne + q), apq = __shfl(a, p * ne + q);
+      c = 1.0;
+      sn = 0.0;
+      if (q < chi && apq != 0.0) {
+        const double zeta = (aqq - app) / (2 * apq);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+        c = 1 / sqrt(1 + t * t);
+        sn = c * t;
+      }
+    };
+    for (int sweep = 0; sweep < 40; ++sweep) {
+      double off = (in && i != j) ? a * a : 0.0, diag = (in && i == j) ? a * a : 0.0;
+      for (int m = 32; m > 0; m >>= 1) {
+        off += __shfl_xor(off, m);
+        diag += __shfl_xor(diag, m);
+      }
+      if (off <= 1e-30 * diag || off == 0.0) break;
+      for (int st = 0; st < ne - 1; ++st) {
+        auto partner = [&](int x) {
+          const int pos = x == 0 ? 0 : 1 + ((x - 1 - st) % (ne - 1) + (ne - 1)) % (ne - 1);
+          const int pp = ne - 1 - pos;
+          return pp == 0 ? 0 : 1 + (pp - 1 + st) % (ne - 1);
+        };
+        const int ri = partner(i), cj = partner(j);
+        double cr, sr, cc, sc;
+        rot(i < ri ? i : ri, i < ri ? ri : i, cr, sr);
+        rot(j < cj ? j : cj, j < cj ? cj : j, cc, sc);
+        const double ao = __shfl(a, ri * ne + j);
+        const double ar = i < ri ? cr * a - sr * ao : sr * ao + cr * a;
+        const double aro = __shfl(ar, i * ne + cj), vo = __shfl(v, i * ne + cj);
+        a = j < cj ? cc * ar - sc * aro : sc * aro + cc * ar;
+        v = j < cj ? cc * v - sc * vo : sc * vo + cc * v;
+      }
+    }
+  }
+  double w[AP_ENV];
+#pragma unroll
+  for (int m = 0; m < AP_ENV; ++m) w[m] = __shfl(a, (m < ne ? m : 0) * ne + (m < ne ? m : 0));
+  int order[AP_ENV];
+  for (int m = 0; m < chi; ++m) order[m] = m;
+  for (int x = 0; x < chi; ++x)
+    for (int y = x + 1; y < chi; ++y)
+      if (fabs(w[order[y]]) > fabs(w[order[x]])) {
+        const int t = order[x];
+        order[x] = order[y];
+        order[y] = t;
+      }
+  double tot = 0;
+  for (int m = 0; m < chi; ++m) tot += fabs(w[m]);
+  int keep = chi;
+  double disc = 0;
+  while (keep > 1 && tot > 0 && (disc + fabs(w[order[keep - 1]])) / tot <= cutoff) {
+    disc += fabs(w[order[keep - 1]]);
+    --keep;
+  }
+  unsigned kmask = 0;
+  for (int m = 0; m < keep; ++m) kmask |= 1u << order[m];
+  if (in) Vout[i * chi + j] = v;
+  if (l < chi) wout[l] = w[l];
+  return kmask;
+}
+
+// S = V f(w) V^T over the kept eigenvalues (row-major chi x chi), f = sqrt
+// (which = 0), 1/sqrt (1) or the indicator (2)
+__device__ void env_fun(const double* V, const double* w, unsigned kmask, int chi, int which, double* out) {
+  for (int idx = threadIdx.x & 63; idx < chi * chi; idx += 64) {
+    const int r = idx / chi, c = idx % chi;
+    double s = 0;
+    for (int m = 0; m < chi; ++m)
+      if ((kmask >> m) & 1u) {
+        const double f = which == 0 ? sqrt(w[m]) : which == 1 ? 1.0 / sqrt(w[m]) : 1.0;
+        s += V[r * chi + m] * V[c * chi + m] * f;
+      }
+    out[idx] = s;
+  }
+}
+
+// Mbar (row-major, chi x chi, symmetrised) = V [(V^T Sb V) o K1 + (V^T Sib V) o K2] V^T
+// with the divided differences of sqrt and 1/sqrt on the kept eigenvalues
+__device__ void env_pullback(const double* V, const double* w, unsigned kmask, int chi, const double* Sb,
+                             const double* Sib, double* tmp, double* Mb) {
+  const int l = threadIdx.x & 63;
+  // tmp = V^T (Sb_sym) V o K1 + V^T (Sib_sym) V o K2   (eigenbasis, row-major)
+  for (int idx = l; idx < chi * chi; idx += 64) {
+    const int p = idx / chi, q = idx % chi;
+    double a = 0, b = 0;
+    for (int r = 0; r < chi; ++r)
+      for (int c = 0; c < chi; ++c) {
+        const double vv = V[r * chi + p] * V[c * chi + q];
+        a += vv * 0.5 * (Sb[r * chi + c] + Sb[c * chi + r]);
+        b += vv * 0.5 * (Sib[r * chi + c] + Sib[c * chi + r]);
+      }
+    const bool kp = (kmask >> p) & 1u, kq = (kmask >> q) & 1u;
+    double k1 = 0, k2 = 0;
+    if (kp && kq) {
+      const double sp = sqrt(w[p]), sq = sqrt(w[q]);
+      k1 = 1.0 / (sp + sq);
+      k2 = -1.0 / (sp * sq * (sp + sq));
+    } else if (kp || kq) {
+      // one kept (f(x)), one dropped (f = 0): (f(w_p) - f(w_q)) / (w_p - w_q)
+      const double wk = kp ? w[p] : w[q], wd = kp ? w[q] : w[p];
+      const double den = wk - wd;
+      if (den != 0.0) {
+        k1 = sqrt(wk) / den;
+        k2 = 1.0 / sqrt(wk) / den;
+      }
+    }
+    tmp[idx] = a * k1 + b * k2;
+  }
+  __threadfence_block();
+  for (int idx = l; idx < chi * chi; idx += 64) {
+    const int r = idx / chi, c = idx % chi;
+    double s = 0, t = 0;
+    for (int p = 0; p < chi; ++p)
+      for (int q = 0; q < chi; ++q) {
+        s += V[r * chi + p] * tmp[p * chi + q] * V[c * chi + q];
+        t += V[c * chi + p] * tmp[p * chi + q] * V[r * chi + q];
+      }
+    Mb[idx] = 0.5 * (s + t);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_apply_vjp(const ApplyItem* __restrict__ items, const VjpIo* __restrict__ io,
+                                                   const VDesc* __restrict__ vds, const double* __restrict__ psi,
+                                                   const double* __restrict__ msg, const double* __restrict__ gates,
+                                                   ApplyParams prm, const double* __restrict__ dout,
+                                                   double* __restrict__ din, double* __restrict__ dgate,
+                                                   double* __restrict__ dmsg, double* __restrict__ work,
+                                                   int* __restrict__ status) {
+  __shared__ double red[8];
+  __shared__ int order[AP_TH];
+  __shared__ int sh_keep;
+  __shared__ unsigned kmask[2][AP_MAXEXT];
+  const ApplyItem it = items[blockIdx.x];
+  const VjpIo o = io[blockIdx.x];
+  const int tid = threadIdx.x;
+  SideGeom gs[2] = {side_geom(vds[it.u], it.ju), side_geom(vds[it.v], it.jv)};
+  const int vtx[2] = {it.u, it.v};
+  double* base = work + (long long)blockIdx.x * VJ_GATE;
+  // per side areas
+  double *X[2], *Pt[2], *A[2], *W[2], *Xb[2], *Pb[2], *Yb[2], *T1[2], *T2[2];
+  double *Z[2], *R[2], *B[2], *Rb[2], *Bb[2], *Cb[2], *Sg[2], *Wb[2], *env[2];
+  for (int s = 0; s < 2; ++s) {
+    double* p = base + s * VJ_SIDE;
+    X[s] = p; Pt[s] = p + AP_XY; A[s] = p + 2 * AP_XY; W[s] = p + 3 * AP_XY; Xb[s] = p + 4 * AP_XY;
+    Pb[s] = p + 5 * AP_XY; Yb[s] = p + 6 * AP_XY; T1[s] = p + 7 * AP_XY; T2[s] = p + 8 * AP_XY;
+    double* q = p + 9LL * AP_XY;
+    Z[s] = q; R[s] = q + VJ_SM; B[s] = q + 2 * VJ_SM; Rb[s] = q + 3 * VJ_SM; Bb[s] = q + 4 * VJ_SM;
+    Cb[s] = q + 5 * VJ_SM; Sg[s] = q + 6 * VJ_SM; Wb[s] = q + 7 * VJ_SM;
+    env[s] = q + 8 * VJ_SM;
+  }
+  double* sh = base + 2 * VJ_SIDE;
+  double *Th = sh, *Tt = sh + VJ_SM, *As = sh + 2 * VJ_SM, *Vs = sh + 3 * VJ_SM, *Fu = sh + 4 * VJ_SM,
+         *Fv = sh + 5 * VJ_SM, *Fub = sh + 6 * VJ_SM, *Fvb = sh + 7 * VJ_SM, *Thb = sh + 8 * VJ_SM,
+         *Ttb = sh + 9 * VJ_SM, *Um = sh + 10 * VJ_SM, *Vm = sh + 11 * VJ_SM, *M1 = sh + 12 * VJ_SM,
+         *Q2 = sh + 13 * VJ_SM;
+  double* svec = sh + 14 * VJ_SM;  // thin singular values (sorted)
+  double* sbar = svec + 64;
+  double* sgn = sbar + 64;
+  double* rot = sgn + 64;  // 128
+  auto env_p = [&](int s, int pa, int which) { return env[s] + pa * VJ_ENV + which * 64; };
+  // env slots: 0 V, 1 S, 2 Si, 3 Sbar, 4 Sibar, 5 tmp; w at 6 * 64
+
+  // ---- forward -------------------------------------------------------------
+  // F1. eigendecompositions of the environments, one wave each
+  {
+    const int wv = tid >> 6;
+    for (int q = wv; q < gs[0].next + gs[1].next; q += (int)blockDim.x / 64) {
+      const int s = q < gs[0].next ? 0 : 1;
+      const int pa = s == 0 ? q : q - gs[0].next;
+      const SideGeom& g = gs[s];
+      const int a = g.ext[pa];
+      const int chi = g.dims[a];
+      double* V = env_p(s, pa, 0);
+      double* w = env[s] + pa * VJ_ENV + 6 * 64;
+      const unsigned km = env_eig_wave(msg + vds[vtx[s]].in_msg[a], chi, prm.eig_cutoff, V, w);
+      __threadfence_block();
+      env_fun(V, w, km, chi, 0, env_p(s, pa, 1));
+      env_fun(V, w, km, chi, 1, env_p(s, pa, 2));
+      env_fun(V, w, km, chi, 2, env_p(s, pa, 5));  // P in the tmp slot until the pullback
+      for (int i = (tid & 63); i < 64; i += 64) {
+        env_p(s, pa, 3)[i] = 0.0;
+        env_p(s, pa, 4)[i] = 0.0;
+      }
+      if ((tid & 63) == 0) kmask[s][pa] = km;
+    }
+    bsync();
+  }
+  // F2-F3. X = psi (x) S, Pt = psi (x) P, R and W per side
+  for (int s = 0; s < 2; ++s) {
+    const SideGeom& g = gs[s];
+    for (int idx = tid; idx < g.E * g.n; idx += blockDim.x) {
+      const int e = idx % g.E, c = idx / g.E;
+      const double x = psi[g.psi_off + nat_off(g, c % g.d, e, c / g.d)];
+      X[s][idx] = x;
+      Pt[s][idx] = x;
+    }
+    bsync();
+    for (int pa = 0; pa < g.next; ++pa) {
+      ext_mp_g(g, pa, X[s], T1[s], g.n, env_p(s, pa, 1));
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) X[s][i] = T1[s][i];
+      ext_mp_g(g, pa, Pt[s], T1[s], g.n, env_p(s, pa, 5));
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) Pt[s][i] = T1[s][i];
+      bsync();
+    }
+    if (g.E >= g.n) {
+      // tall: singular values and right vectors of X give C = Z diag(sigma^2) Z^T
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) A[s][i] = X[s][i];
+      bsync();
+      jacobi_svd(A[s], g.E, g.n, Z[s], rot, red);
+      bsync();
+      for (int j = tid; j < g.n; j += blockDim.x) {
+        double t = 0;
+        for (int i = 0; i < g.E; ++i) t += A[s][i + (long long)g.E * j] * A[s][i + (long long)g.E * j];
+        Sg[s][j] = sqrt(t);
+      }
+      bsync();
+      for (int idx = tid; idx < g.n * g.n; idx += blockDim.x) {
+        const int i = idx % g.n, j = idx / g.n;
+        double r = 0, b = 0;
+        for (int m = 0; m < g.n; ++m) {
+          const double zz = Z[s][i + g.n * m] * Z[s][j + g.n * m];
+          r += zz * Sg[s][m];
+          b += Sg[s][m] > 0 ? zz / Sg[s][m] : 0.0;
+        }
+        R[s][idx] = r;  // C^{1/2}, r = n rows
+        B[s][idx] = b;  // C^{-1/2}
+      }
+      bsync();
+      gmm(g.E, g.n, g.n, Pt[s], g.E, false, B[s], g.n, false, W[s], g.E);  // W = Pt C^{-1/2}
+    } else {
+      // wide: R = X (r = E rows), W = (x) Si as an E x E matrix
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) R[s][i] = X[s][i];
+      for (int idx = tid; idx < g.E * g.E; idx += blockDim.x) {
+        const int e = idx % g.E, f = idx / g.E;
+        double w = 1.0;
+        int re = e, rf = f;
+        for (int pa = 0; pa < g.next; ++pa) {
+          const int chi = g.dims[g.ext[pa]];
+          w *= env_p(s, pa, 2)[(re % chi) * chi + (rf % chi)];
+          re /= chi;
+          rf /= chi;
+        }
+        W[s][idx] = w;
+      }
+      bsync();
+    }
+  }
+  const int du = gs[0].d, dv = gs[1].d;
+  const int ru = gs[0].E >= gs[0].n ? gs[0].n : gs[0].E, rv = gs[1].E >= gs[1].n ? gs[1].n : gs[1].E;
+  const int nu = gs[0].n, nv = gs[1].n, chib = gs[0].chib;
+  const int Mr = ru * du, Nc = rv * dv, dd = du * dv;
+  const double* G = gates + it.gate;
+  // F4. Tt[(ru + r su), (rv + r sv)] = sum_b R_u[ru, su + du b] R_v[rv, sv + dv b]; Theta = G Tt
+  for (int idx = tid; idx < Mr * Nc; idx += blockDim.x) {
+    const int row = idx % Mr, col = idx / Mr;
+    const int a = row % ru, su = row / ru, c = col % rv, sv = col / rv;
+    double t = 0;
+    for (int b = 0; b < chib; ++b) t += R[0][a + ru * (su + du * b)] * R[1][c + rv * (sv + dv * b)];
+    Tt[idx] = t;
+  }
+  bsync();
+  for (int idx = tid; idx < Mr * Nc; idx += blockDim.x) {
+    const int row = idx % Mr, col = idx / Mr;
+    const int a = row % ru, sup = row / ru, c = col % rv, svp = col / rv;
+    double t = 0;
+    for (int su = 0; su < du; ++su)
+      for (int sv = 0; sv < dv; ++sv)
+        t += G[(sup + du * svp) + dd * (su + du * sv)] * Tt[(a + ru * su) + Mr * (c + rv * sv)];
+    Th[idx] = t;
+  }
+  bsync();
+  // F5. thin SVD of Theta (one-sided Jacobi on the taller orientation)
+  const bool tr = Mr < Nc;
+  const int Ms = tr ? Nc : Mr, Ns = tr ? Mr : Nc;  // Ns = p
+  const int p = Ns;
+  for (int idx = tid; idx < Mr * Nc; idx += blockDim.x) {
+    const int row = idx % Mr, col = idx / Mr;
+    if (tr) As[col + Nc * row] = Th[idx];
+    else As[idx] = Th[idx];
+  }
+  bsync();
+  jacobi_svd(As, Ms, Ns, Vs, rot, red);
+  bsync();
+  for (int j = tid; j < Ns; j += blockDim.x) {
+    double t = 0;
+    for (int i = 0; i < Ms; ++i) t += As[i + Ms * j] * As[i + Ms * j];
+    sbar[j] = sqrt(t);  // unsorted singular values (sbar reused below)
+  }
+  bsync();
+  if (tid == 0) {
+    for (int i = 0; i < Ns; ++i) order[i] = i;
+    for (int i = 0; i < Ns; ++i)
+      for (int j = i + 1; j < Ns; ++j)
+        if (sbar[order[j]] > sbar[order[i]]) {
+          const int t = order[i];
+          order[i] = order[j];
+          order[j] = t;
+        }
+    double tot = 0;
+    for (int i = 0; i < Ns; ++i) tot += sbar[i] * sbar[i];
+    int keep = Ns;
+    if (prm.maxdim > 0 && keep > prm.maxdim) keep = prm.maxdim;
+    double disc = 0;
+    for (int i = keep; i < Ns; ++i) disc += sbar[order[i]] * sbar[order[i]];
+    if (prm.cutoff >= 0 && tot > 0)
+      while (keep > 1 && (disc + sbar[order[keep - 1]] * sbar[order[keep - 1]]) / tot <= prm.cutoff) {
+        disc += sbar[order[keep - 1]] * sbar[order[keep - 1]];
+        --keep;
+      }
+    int st = 0;
+    if (keep != o.keep) st = 1;
+    if (!isfinite(tot)) st = 2;
+    for (int i = 0; i < keep; ++i)
+      if (!(sbar[order[i]] > 0)) st = 3;
+    status[blockIdx.x] = st;
+    sh_keep = st ? -1 : keep;
+  }
+  __syncthreads();
+  const int keep = sh_keep;
+  if (keep < 0) return;
+  // sorted thin factors: Um (Mr x p), Vm (Nc x p), svec (p)
+  for (int idx = tid; idx < Mr * p; idx += blockDim.x) {
+    const int i = idx % Mr, j = idx / Mr, oj = order[j];
+    const double sj = sbar[oj];
+    Um[idx] = tr ? Vs[i + Ns * oj] : (sj > 0 ? As[i + Ms * oj] / sj : 0.0);
+  }
+  for (int idx = tid; idx < Nc * p; idx += blockDim.x) {
+    const int i = idx % Nc, j = idx / Nc, oj = order[j];
+    const double sj = sbar[oj];
+    Vm[idx] = tr ? (sj > 0 ? As[i + Ms * oj] / sj : 0.0) : Vs[i + Ns * oj];
+  }
+  bsync();
+  for (int j = tid; j < p; j += blockDim.x) svec[j] = sbar[order[j]];
+  bsync();
+  // F6. F_u = U_k sqrt(S), F_v = V_k sqrt(S)
+  for (int idx = tid; idx < Mr * keep; idx += blockDim.x) Fu[idx] = Um[idx] * sqrt(svec[idx / Mr]);
+  for (int idx = tid; idx < Nc * keep; idx += blockDim.x) Fv[idx] = Vm[idx] * sqrt(svec[idx / Nc]);
+  bsync();
+  // F7. pre-gauge outputs Y_s[e, s' + d j] = sum_r W[e, r] F[r + r_s s', j] into T2
+  const int rs[2] = {ru, rv};
+  const double* Fs[2] = {Fu, Fv};
+  for (int s = 0; s < 2; ++s) {
+    const SideGeom& g = gs[s];
+    const int r = rs[s];
+    for (int idx = tid; idx < g.E * g.d * keep; idx += blockDim.x) {
+      const int e = idx % g.E, c = idx / g.E, sp = c % g.d, j = c / g.d;
+      double t = 0;
+      for (int q = 0; q < r; ++q) t += W[s][e + (long long)g.E * q] * Fs[s][(q + r * sp) + (long long)rs[s] * g.d * j];
+      T2[s][idx] = t;
+    }
+    bsync();
+  }
+  // bond signs from psi_u' (largest |entry| of every slice, first in natural order)
+  if (tid < keep) {
+    const SideGeom& g = gs[0];
+    const int j = tid;
+    double best = -1, val = 0;
+    int best_off = 0x7fffffff;
+    for (int e = 0; e < g.E; ++e)
+      for (int s2 = 0; s2 < g.d; ++s2) {
+        const double x = T2[0][e + (long long)g.E * (s2 + g.d * j)];
+        const int off = nat_off(g, s2, e, j);
+        if (fabs(x) > best || (fabs(x) == best && off < best_off)) {
+          best = fabs(x);
+          best_off = off;
+          val = x;
+        }
+      }
+    sgn[j] = val < 0 ? -1.0 : 1.0;
+  }
+  bsync();
+
+  // ---- backward ------------------------------------------------------------
+  // B1-B3. cotangents of the pre-gauge outputs: read, undo the normalisation, the signs
+  for (int s = 0; s < 2; ++s) {
+    const SideGeom& g = gs[s];
+    SideGeom gk = g;
+    gk.dims[g.j] = keep;
+    const double* dsrc = dout + (s == 0 ? o.out_u : o.out_v);
+    const int cols = g.d * keep;
+    double n2 = 0, dot = 0;
+    for (int idx = tid; idx < g.E * cols; idx += blockDim.x) {
+      const int e = idx % g.E, c = idx / g.E;
+      const double y = T2[s][idx] * sgn[c / g.d];
+      const double yb = dsrc[nat_off(gk, c % g.d, e, c / g.d)];
+      Yb[s][idx] = yb;
+      n2 += y * y;
+      dot += y * yb;
+    }
+    n2 = block_sum_d(n2, red);
+    dot = block_sum_d(dot, red);
+    const double nrm = sqrt(n2);
+    const bool nz = prm.normalize && n2 > 0;
+    for (int idx = tid; idx < g.E * cols; idx += blockDim.x) {
+      const double y = T2[s][idx] * sgn[(idx / g.E) / g.d];
+      double yb = Yb[s][idx];
+      if (nz) yb = (yb - y * dot / n2) / nrm;  // pullback of y / ||y||
+      Yb[s][idx] = yb * sgn[(idx / g.E) / g.d];
+    }
+    bsync();
+  }
+  // B4. Wbar (E x r) and Fbar (r d x keep)
+  double* Fbs[2] = {Fub, Fvb};
+  for (int s = 0; s < 2; ++s) {
+    const SideGeom& g = gs[s];
+    const int r = rs[s];
+    for (int idx = tid; idx < g.E * r; idx += blockDim.x) {
+      const int e = idx % g.E, q = idx / g.E;
+      double t = 0;
+      for (int j = 0; j < keep; ++j)
+        for (int sp = 0; sp < g.d; ++sp)
+          t += Yb[s][e + (long long)g.E * (sp + g.d * j)] * Fs[s][(q + r * sp) + (long long)r * g.d * j];
+      T1[s][idx] = t;  // Wbar
+    }
+    for (int idx = tid; idx < r * g.d * keep; idx += blockDim.x) {
+      const int row = idx % (r * g.d), j = idx / (r * g.d);
+      const int q = row % r, sp = row / r;
+      double t = 0;
+      for (int e = 0; e < g.E; ++e) t += W[s][e + (long long)g.E * q] * Yb[s][e + (long long)g.E * (sp + g.d * j)];
+      Fbs[s][idx] = t;
+    }
+    bsync();
+  }
+  // B5. Ubar = Fbar_u sqrt(S), Vbar = Fbar_v sqrt(S) (zero beyond keep), sbar
+  for (int idx = tid; idx < Mr * p; idx += blockDim.x) {
+    const int j = idx / Mr;
+    M1[idx] = j < keep ? Fub[idx] * sqrt(svec[j]) : 0.0;  // Ubar
+  }
+  for (int idx = tid; idx < Nc * p; idx += blockDim.x) {
+    const int j = idx / Nc;
+    Q2[idx] = j < keep ? Fvb[idx] * sqrt(svec[j]) : 0.0;  // Vbar
+  }
+  for (int j = tid; j < p; j += blockDim.x) {
+    double t = 0;
+    if (j < keep) {
+      for (int i = 0; i < Mr; ++i) t += Fub[i + Mr * j] * Um[i + Mr * j];
+      for (int i = 0; i < Nc; ++i) t += Fvb[i + Nc * j] * Vm[i + Nc * j];
+      t /= 2 * sqrt(svec[j]);
+    }
+    sbar[j] = t;
+  }
+  bsync();
+  // B6. thin SVD pullback.  J = U^T Ubar, K = V^T Vbar (p x p) in Ttb, Thb
+  gmm(p, p, Mr, Um, Mr, true, M1, Mr, false, Ttb, p);
+  gmm(p, p, Nc, Vm, Nc, true, Q2, Nc, false, Thb, p);
+  {
+    const double smax = svec[0];
+    const double eps2 = 1e-24 * smax * smax * smax * smax;
+    // Inner (p x p) into Fu (free after B4), then M1' = U Inner + (Ubar - U J) S^-1
+    for (int idx = tid; idx < p * p; idx += blockDim.x) {
+      const int i = idx % p, j = idx / p;
+      double v = i == j ? sbar[i] : 0.0;
+      if (i != j) {
+        const double dfr = svec[j] * svec[j] - svec[i] * svec[i];
+        const double f = dfr / (dfr * dfr + eps2);
+        v += f * (Ttb[i + p * j] - Ttb[j + p * i]) * svec[j] + svec[i] * f * (Thb[i + p * j] - Thb[j + p * i]);
+      }
+      Fu[idx] = v;
+    }
+    bsync();
+  }
+  // Fv <- U Inner (Mr x p); As <- U J (Mr x p)
+  gmm(Mr, p, p, Um, Mr, false, Fu, p, false, Fv, Mr);
+  gmm(Mr, p, p, Um, Mr, false, Ttb, p, false, As, Mr);
+  for (int idx = tid; idx < Mr * p; idx += blockDim.x) {
+    const int j = idx / Mr;
+    const double inv = (j < keep && svec[j] > 0) ? 1.0 / svec[j] : 0.0;
+    Fv[idx] += (M1[idx] - As[idx]) * inv;  // U Inner + (I - U U^T) Ubar S^-1
+  }
+  // Vs <- S^-1 (Vbar^T - K^T V^T) (p x Nc)
+  bsync();
+  gmm(p, Nc, p, Thb, p, true, Vm, Nc, true, Vs, p);  // K^T V^T
+  for (int idx = tid; idx < p * Nc; idx += blockDim.x) {
+    const int j = idx % p, c = idx / p;
+    const double inv = (j < keep && svec[j] > 0) ? 1.0 / svec[j] : 0.0;
+    Vs[idx] = (Q2[c + Nc * j] - Vs[idx]) * inv;
+  }
+  bsync();
+  // Thetabar = Fv V^T + U Vs (Mr x Nc)
+  gmm(Mr, Nc, p, Fv, Mr, false, Vm, Nc, true, Thb, Mr);
+  gmm(Mr, Nc, p, Um, Mr, false, Vs, p, false, Thb, Mr, true);
+  // B7. Ttbar and Gbar
+  for (int idx = tid; idx < Mr * Nc; idx += blockDim.x) {
+    const int row = idx % Mr, col = idx / Mr;
+    const int a = row % ru, su = row / ru, c = col % rv, sv = col / rv;
+    double t = 0;
+    for (int sup = 0; sup < du; ++sup)
+      for (int svp = 0; svp < dv; ++svp)
+        t += G[(sup + du * svp) + dd * (su + du * sv)] * Thb[(a + ru * sup) + Mr * (c + rv * svp)];
+    Ttb[idx] = t;
+  }
+  for (int idx = tid; idx < dd * dd; idx += blockDim.x) {
+    const int i = idx % dd, jj = idx / dd;
+    const int sup = i % du, svp = i / du, su = jj % du, sv = jj / du;
+    double t = 0;
+    for (int a = 0; a < ru; ++a)
+      for (int c = 0; c < rv; ++c) t += Thb[(a + ru * sup) + Mr * (c + rv * svp)] * Tt[(a + ru * su) + Mr * (c + rv * sv)];
+    dgate[o.dgate + idx] = t;
+  }
+  bsync();
+  // B8. Rbar
+  for (int idx = tid; idx < ru * nu; idx += blockDim.x) {
+    const int a = idx % ru, cu = idx / ru, su = cu % du, b = cu / du;
+    double t = 0;
+    for (int c = 0; c < rv; ++c)
+      for (int sv = 0; sv < dv; ++sv) t += Ttb[(a + ru * su) + Mr * (c + rv * sv)] * R[1][c + rv * (sv + dv * b)];
+    Rb[0][idx] = t;
+  }
+  for (int idx = tid; idx < rv * nv; idx += blockDim.x) {
+    const int c = idx % rv, cv = idx / rv, sv = cv % dv, b = cv / dv;
+    double t = 0;
+    for (int a = 0; a < ru; ++a)
+      for (int su = 0; su < du; ++su) t += Ttb[(a + ru * su) + Mr * (c + rv * sv)] * R[0][a + ru * (su + du * b)];
+    Rb[1][idx] = t;
+  }
+  bsync();
+  // B9-B11 per side
+  for (int s = 0; s < 2; ++s) {
+    const SideGeom& g = gs[s];
+    const int n = g.n;
+    if (g.E >= n) {
+      // Bbar = Pt^T Wbar (n x n); Ptbar = Wbar B (E x n) into Pb
+      gmm(n, n, g.E, Pt[s], g.E, true, T1[s], g.E, false, Bb[s], n);
+      gmm(g.E, n, n, T1[s], g.E, false, B[s], n, false, Pb[s], g.E);
+      // Cbar = Z [(Z^T Rbar_s Z) o K1 + (Z^T Bbar_s Z) o K2] Z^T
+      for (int idx = tid; idx < n * n; idx += blockDim.x) {
+        const int i = idx % n, j = idx / n;
+        Cb[s][idx] = 0.5 * (Rb[s][i + n * j] + Rb[s][j + n * i]);
+        A[s][idx] = 0.5 * (Bb[s][i + n * j] + Bb[s][j + n * i]);
+      }
+      bsync();
+      gmm(n, n, n, Z[s], n, true, Cb[s], n, false, T2[s], n);
+      gmm(n, n, n, T2[s], n, false, Z[s], n, false, Cb[s], n);  // Z^T Rbar Z
+      gmm(n, n, n, Z[s], n, true, A[s], n, false, T2[s], n);
+      gmm(n, n, n, T2[s], n, false, Z[s], n, false, A[s], n);   // Z^T Bbar Z
+      for (int idx = tid; idx < n * n; idx += blockDim.x) {
+        const int i = idx % n, j = idx / n;
+        const double si = Sg[s][i], sj = Sg[s][j];
+        const double k1 = (si + sj) > 0 ? 1.0 / (si + sj) : 0.0;
+        const double k2 = (si > 0 && sj > 0) ? -1.0 / (si * sj * (si + sj)) : 0.0;
+        T2[s][idx] = Cb[s][idx] * k1 + A[s][idx] * k2;
+      }
+      bsync();
+      gmm(n, n, n, Z[s], n, false, T2[s], n, false, A[s], n);
+      gmm(n, n, n, A[s], n, false, Z[s], n, true, Cb[s], n);    // Cbar (symmetric)
+      // Xbar = 2 X Cbar
+      gmm(g.E, n, n, X[s], g.E, false, Cb[s], n, false, Xb[s], g.E);
+      for (int i = tid; i < g.E * n; i += blockDim.x) Xb[s][i] *= 2.0;
+      bsync();
+    } else {
+      for (int i = tid; i < g.E * n; i += blockDim.x) {
+        Xb[s][i] = Rb[s][i];
+        Pb[s][i] = 0.0;
+      }
+      bsync();
+      // Sibar_a from Wbar (E x E) with W = (x) Si
+      for (int pa = 0; pa < g.next; ++pa) {
+        const int chi = g.dims[g.ext[pa]];
+        double* Sib = env_p(s, pa, 4);
+        // every (e, f) pair with e_pa = x, f_pa = y
+        for (int idx = tid; idx < chi * chi; idx += blockDim.x) {
+          const int x = idx / chi, y = idx % chi;
+          double t = 0;
+          for (int e = 0; e < g.E; ++e)
+            for (int f = 0; f < g.E; ++f) {
+              int re = e, rf = f;
+              bool ok = true;
+              double w = 1.0;
+              for (int qa = 0; qa < g.next; ++qa) {
+                const int c2 = g.dims[g.ext[qa]];
+                const int ea = re % c2, fa = rf % c2;
+                re /= c2;
+                rf /= c2;
+                if (qa == pa) ok = ok && ea == x && fa == y;
+                else w *= env_p(s, qa, 2)[ea * c2 + fa];
+              }
+              if (ok) t += T1[s][e + (long long)g.E * f] * w;
+            }
+          Sib[x * chi + y] = t;
+        }
+        bsync();
+      }
+    }
+    // Sbar_a = sum Xbar[e (e_a = x), c] (psi (x)_{others} S)[e (e_a = y), c]
+    // (Xbar, Pb intact; T2, A, W, Pt are free)
+    for (int pa = 0; pa < g.next; ++pa) {
+      double* yc = T2[s];
+      double* yo = A[s];
+      for (int idx = tid; idx < g.E * n; idx += blockDim.x) {
+        const int e = idx % g.E, c = idx / g.E;
+        yc[idx] = psi[g.psi_off + nat_off(g, c % g.d, e, c / g.d)];
+      }
+      bsync();
+      for (int qa = 0; qa < g.next; ++qa) {
+        if (qa == pa) continue;
+        ext_mp_g(g, qa, yc, yo, n, env_p(s, qa, 1));
+        double* t = yc;
+        yc = yo;
+        yo = t;
+      }
+      int stride = 1;
+      for (int i = 0; i < pa; ++i) stride *= g.dims[g.ext[i]];
+      const int chi = g.dims[g.ext[pa]];
+      double* Sb = env_p(s, pa, 3);
+      for (int idx = tid; idx < chi * chi; idx += blockDim.x) {
+        const int x = idx / chi, y = idx % chi;
+        double t = 0;
+        for (int c = 0; c < n; ++c)
+          for (int rest = 0; rest < g.E / chi; ++rest) {
+            const int lo = rest % stride, hi = rest / stride;
+            t += Xb[s][(lo + stride * (x + chi * hi)) + (long long)g.E * c] *
+                 yc[(lo + stride * (y + chi * hi)) + (long long)g.E * c];
+          }
+        Sb[x * chi + y] = t;
+      }
+      bsync();
+    }
+    // psibar = Xbar (x) S (+ Ptbar (x) P on a tall side), to the natural layout
+    {
+      const double* src = Xb[s];
+      double* bufs2[2] = {T2[s], A[s]};
+      int bi = 0;
+      for (int pa = 0; pa < g.next; ++pa) {
+        ext_mp_g(g, pa, src, bufs2[bi], n, env_p(s, pa, 1));
+        src = bufs2[bi];
+        bi ^= 1;
+      }
+      const double* srcp = nullptr;
+      if (g.E >= n) {
+        srcp = Pb[s];
+        double* bufs3[2] = {W[s], Pt[s]};
+        int bj = 0;
+        for (int pa = 0; pa < g.next; ++pa) {
+          ext_mp_g(g, pa, srcp, bufs3[bj], n, env_p(s, pa, 5));
+          srcp = bufs3[bj];
+          bj ^= 1;
+        }
+      }
+      double* dst = din + (s == 0 ? o.in_u : o.in_v);
+      for (int idx = tid; idx < g.E * n; idx += blockDim.x) {
+        const int e = idx % g.E, c = idx / g.E;
+        dst[nat_off(g, c % g.d, e, c / g.d)] = src[idx] + (srcp ? srcp[idx] : 0.0);
+      }
+      bsync();
+    }
+    // environment pullbacks into dmsg (accumulated; every message of a call belongs to one bond)
+    {
+      const int wv = tid >> 6;
+      for (int pa = wv; pa < g.next; pa += (int)blockDim.x / 64) {
+        const int a = g.ext[pa];
+        const int chi = g.dims[a];
+        double* Mb = env_p(s, pa, 5);
+        env_pullback(env_p(s, pa, 0), env[s] + pa * VJ_ENV + 6 * 64, kmask[s][pa], chi, env_p(s, pa, 3),
+                     env_p(s, pa, 4), env_p(s, pa, 2), Mb);
+        __threadfence_block();
+        const long long off = vds[vtx[s]].in_msg[a];
+        for (int idx = tid & 63; idx < chi * chi; idx += 64) {
+          const int i = idx % chi, j = idx / chi;  // message layout M[i + chi j]
+          dmsg[off + idx] += Mb[i * chi + j];
+        }
+      }
+      bsync();
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int itnbp_apply_vjp(itnbp_net* net, int n, const int* edge_ids, const double* gates, int maxdim,
+                               double cutoff, int normalize, const int* kept, const double* dpsi_out,
+                               double* dpsi_in, double* dgates, double* dmsgs) {
+  if (!net || n < 0 || (n > 0 && (!edge_ids || !gates || !kept || !dpsi_out || !dpsi_in || !dgates || !dmsgs)))
+    return ITNBP_ERR_ARG;
+  if (n == 0) return ITNBP_OK;
+  itnbp_ctx* ctx = net->ctx;
+  return itnbp_guarded(ctx, [&]() -> int {
+    std::vector<ApplyItem> items;
+    long long goff = 0;
+    ITNBP_RETCHK(prepare_apply(net, n, edge_ids, maxdim, cutoff, nullptr, 1, net->chi, false, items, goff));
+    std::vector<VjpIo> io(n);
+    long long out_tot = 0, in_tot = 0;
+    for (int i = 0; i < n; ++i) {
+      const ApplyItem& it = items[i];
+      const int e = edge_ids[i];
+      const int chib = net->chi[e];
+      if (kept[i] < 1) return itnbp_fail(ctx, ITNBP_ERR_ARG, "apply_vjp: kept must be >= 1");
+      for (int side = 0; side < 2; ++side) {
+        const int x = side == 0 ? it.u : it.v;
+        const long long D = net->vdesc[x].D;
+        const long long E = D / ((long long)net->phys[x] * chib);
+        if (E * net->phys[x] * kept[i] > AP_XY)
+          return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply_vjp: the new site tensor is too large");
+        const long long Dn = D / chib * kept[i];
+        if (side == 0) {
+          io[i].out_u = out_tot;
+          io[i].in_u = in_tot;
+        } else {
+          io[i].out_v = out_tot;
+          io[i].in_v = in_tot;
+        }
+        out_tot += Dn;
+        in_tot += D;
+      }
+      io[i].dgate = it.gate;
+      io[i].keep = kept[i];
+      io[i].pad = 0;
+    }
+    DevBuf b_items, b_io, b_gates, b_out, b_in, b_dg, b_dm, b_work, b_st;
+    struct Guard {
+      DevBuf* b[9];
+      ~Guard() {
+        for (DevBuf* x : b) free_buf(*x);
+      }
+    } guard{{&b_items, &b_io, &b_gates, &b_out, &b_in, &b_dg, &b_dm, &b_work, &b_st}};
+    ITNBP_RETCHK(upload(ctx, b_items, items.data(), n * sizeof(ApplyItem)));
+    ITNBP_RETCHK(upload(ctx, b_io, io.data(), n * sizeof(VjpIo)));
+    ITNBP_RETCHK(upload(ctx, b_gates, gates, goff * 8));
+    ITNBP_RETCHK(upload(ctx, b_out, dpsi_out, out_tot * 8));
+    ITNBP_RETCHK(upload(ctx, b_dm, dmsgs, net->msg_total * 8));
+    ITNBP_RETCHK(ensure_buf(ctx, b_in, in_tot * 8));
+    ITNBP_RETCHK(ensure_buf(ctx, b_dg, goff * 8));
+    ITNBP_RETCHK(ensure_buf(ctx, b_work, (size_t)n * VJ_GATE * 8));
+    ITNBP_RETCHK(ensure_buf(ctx, b_st, n * sizeof(int)));
+    ApplyParams prm;
+    prm.maxdim = maxdim;
+    prm.cutoff = cutoff;
+    prm.normalize = normalize;
+    prm.eig_cutoff = 10 * 2.220446049250313e-16;
+    prm.max_keep = 1;
+    const int pm0 = prof_mark(ctx);
+    hipLaunchKernelGGL(k_apply_vjp, dim3(n), dim3(256), 0, ctx->stream, (const ApplyItem*)b_items.p,
+                       (const VjpIo*)b_io.p, net->d_vdesc, (const double*)net->d_psi, net->d_msg[net->cur],
+                       (const double*)b_gates.p, prm, (const double*)b_out.p, (double*)b_in.p, (double*)b_dg.p,
+                       (double*)b_dm.p, (double*)b_work.p, (int*)b_st.p);
+    ITNBP_HIPCHK(ctx, hipGetLastError());
+    prof_add(ctx, "apply_vjp", pm0, prof_mark(ctx));
+    std::vector<int> st(n);
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(st.data(), b_st.p, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(dpsi_in, b_in.p, in_tot * 8, hipMemcpyDeviceToHost, ctx->stream));
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(dgates, b_dg.p, goff * 8, hipMemcpyDeviceToHost, ctx->stream));
+    ITNBP_HIPCHK(ctx, hipMemcpyAsync(dmsgs, b_dm.p, net->msg_total * 8, hipMemcpyDeviceToHost, ctx->stream));
+    ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) {
+      if (st[i] == 1)
+        return itnbp_fail(ctx, ITNBP_ERR_ARG, "apply_vjp: kept differs from the rank the truncation keeps");
+      if (st[i] == 2) return itnbp_fail(ctx, ITNBP_ERR_NONFINITE, "apply_vjp: non-finite two-site tensor");
+      if (st[i] == 3)
+        return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply_vjp: a kept singular value is zero (no derivative)");
+    }
+    return ITNBP_OK;
   });
 }

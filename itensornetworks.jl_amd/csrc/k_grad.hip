@@ -383,8 +383,13 @@ __global__ void k_pair_bar(const double* __restrict__ half, const long long* __r
 int sweep_phase_public(itnbp_net* net, int nsweeps, double* trace = nullptr, long long trace_stride = 0,
                        int trace_every = 1);
 
-extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, const double* ops1,
-                                 const double* ops2, double* energy, double* grad) {
+namespace {
+
+// The gradient of the energy (ops1, ops2) after nsweeps sweeps, or with
+// mbar_seed the pullback of a given cotangent of the final messages M^(T)
+// through the sweeps (no energy terms).
+int energy_grad_impl(itnbp_net* net, int nsweeps, int checkpoint, const double* ops1, const double* ops2,
+                     const double* mbar_seed, double* energy, double* grad) {
   if (!net || nsweeps < 0 || !energy || !grad) return ITNBP_ERR_ARG;
   itnbp_ctx* ctx = net->ctx;
   if (net->dtype != ITNBP_F64) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "energy_grad needs FP64 site tensors");
@@ -608,6 +613,8 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
   hipLaunchKernelGGL(k_vjp_reduce, dim3(nv), dim3(256), 0, ctx->stream, net->d_vdesc, d_gv, 1,
                      (const double*)d_part, pstride, nty, mb[0], d_in, d_grad);
   GR_HIP(hipGetLastError());
+  // a given cotangent of M^(T) replaces the (empty) energy terms' one
+  if (mbar_seed) GR_HIP(hipMemcpyAsync(mb[0], mbar_seed, MT * 8, hipMemcpyHostToDevice, ctx->stream));
   int cb = 0;  // mb[cb] holds Mbar^(t)
   // segments in reverse: segment s covers t-1 in [s c, s c + c - 1]
   for (int s = (nsweeps - 1) / c; s >= 0 && nsweeps > 0; --s) {
@@ -643,4 +650,22 @@ extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, co
   return ITNBP_OK;
 #undef GR_CHK
 #undef GR_HIP
+}
+
+}  // namespace
+
+extern "C" int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, const double* ops1,
+                                 const double* ops2, double* energy, double* grad) {
+  if (!net) return ITNBP_ERR_ARG;
+  return itnbp_guarded(net->ctx, [&]() -> int {
+    return energy_grad_impl(net, nsweeps, checkpoint, ops1, ops2, nullptr, energy, grad);
+  });
+}
+
+extern "C" int itnbp_sweeps_vjp(itnbp_net* net, int nsweeps, int checkpoint, const double* mbar, double* grad) {
+  if (!net || !mbar) return ITNBP_ERR_ARG;
+  double unused = 0;
+  return itnbp_guarded(net->ctx, [&]() -> int {
+    return energy_grad_impl(net, nsweeps, checkpoint, nullptr, nullptr, mbar, &unused, grad);
+  });
 }

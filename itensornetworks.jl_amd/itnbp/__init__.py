@@ -13,7 +13,8 @@ from .cache import (BeliefPropagationCache, QuadraticFormNetwork, OPS, default_e
                     edge_scalars, environment, expect, expect2_on_cache, expect_on_cache,
                     identity_messages, initialize_cache, logscalar, message_diff, norm_sqr_network,
                     normalize, rdm, region_scalars, rescale, rescale_, scalar, update, update_,
-                    updated_message, vertex_scalars, energy_and_gradient, update_batched_)
-from .apply import Gate, apply, apply_gates_, edge_coloring, simple_update_round_, tebd_bp_  # noqa: F401,E402
+                    updated_message, vertex_scalars, energy_and_gradient, sweeps_vjp, update_batched_)
+from .apply import (Gate, apply, apply_gates_, apply_gates_vjp, edge_coloring, simple_update_round_,  # noqa: F401,E402
+                    tebd_bp_)
 from .io import load as load_network, save as save_network  # noqa: F401,E402
 from .layer import LayeredBPCache, inner, inner_network_cache, loginner, norm_cache  # noqa: F401,E402

@@ -215,3 +215,84 @@ def tebd_bp_(bpc: BeliefPropagationCache, gate_fn, nsteps, *, maxdim, cutoff=Non
             callback(step, errs)
     update_(bpc, maxiter=bp_maxiter, tol=bp_tol, edge_sequence=edge_sequence)
     return hist
+
+
+def apply_gates_vjp(bpc: BeliefPropagationCache, gates, kept, dpsi_out, *, maxdim=None, cutoff=None,
+                    normalize=False):
+    """Vector-Jacobian product of ``apply_gates_`` for two-site gates on disjoint
+    bonds (simple_update_bp, apply.jl:33-95, and the normalisation of :134-137).
+
+    Call it on the cache as it was before the gates (site tensors and messages
+    unchanged), with the arguments of the forward call, the ranks it kept and the
+    cotangents ``dpsi_out[i] = (dpsi_u', dpsi_v')`` of the updated tensors of gate
+    i (``gates[i].sites = (u, v)``; the new bond has dimension ``kept[i]``).
+    Returns ``(dpsi, dgates, dmsgs)``: per gate the cotangents of psi_u and psi_v
+    and of the gate matrix, and for every environment message used, its
+    cotangent (a dict by directed edge)."""
+    g = bpc.graph
+    ket = bpc.tensornetwork()
+    ids, mats, outs, flips = [], [], [], []
+    used = set()
+    for gt, (cu, cv) in zip(gates, dpsi_out):
+        u, v = gt.sites
+        if u in used or v in used:
+            raise ValueError("the bonds of one apply_gates_vjp call must be disjoint")
+        used.update((u, v))
+        e = g.edge_id(u, v)
+        m = np.asarray(gt.matrix, dtype=np.float64)
+        du, dv = ket.phys_dim(u), ket.phys_dim(v)
+        flip = g.edges[e] != (u, v)
+        if flip:
+            m = _reorder_two_site(m, du, dv)
+            cu, cv = cv, cu
+        ids.append(e)
+        mats.append(m.reshape(-1, order="F"))
+        outs += [np.asarray(cu, dtype=np.float64).reshape(-1, order="F"),
+                 np.asarray(cv, dtype=np.float64).reshape(-1, order="F")]
+        flips.append(flip)
+    n = len(ids)
+    ids = np.array(ids, dtype=np.int32)
+    mats = np.ascontiguousarray(np.concatenate(mats))
+    outs = np.ascontiguousarray(np.concatenate(outs))
+    kept = np.ascontiguousarray(kept, dtype=np.int32)
+    in_sizes = []
+    for e in ids:
+        a, b = g.edges[e]
+        in_sizes += [ket[a].size, ket[b].size]
+    din = np.zeros(sum(in_sizes))
+    dg = np.zeros(mats.size)
+    dm = np.zeros(int(bpc.lib.itnbp_net_msg_numel(bpc.h)))
+    bpc.ctx.check(bpc.lib.itnbp_apply_vjp(
+        bpc.h, n, L.iptr(ids), L.dptr(mats), int(maxdim or 0), -1.0 if cutoff is None else float(cutoff),
+        int(normalize), L.iptr(kept), L.dptr(outs), L.dptr(din), L.dptr(dg), L.dptr(dm)))
+    dpsi, dgates, off, goff = [], [], 0, 0
+    for i, e in enumerate(ids):
+        a, b = g.edges[e]
+        da = din[off:off + ket[a].size].reshape(ket[a].shape, order="F")
+        off += ket[a].size
+        db = din[off:off + ket[b].size].reshape(ket[b].shape, order="F")
+        off += ket[b].size
+        dd = ket.phys_dim(a) * ket.phys_dim(b)
+        gm = dg[goff:goff + dd * dd].reshape(dd, dd, order="F")
+        goff += dd * dd
+        if flips[i]:
+            gm = _reorder_two_site(gm, ket.phys_dim(a), ket.phys_dim(b))
+            da, db = db, da
+        dpsi.append((da, db))
+        dgates.append(gm)
+    dmsgs = {}
+    used_msgs = set()
+    for e in ids:
+        a, b = g.edges[e]
+        for x, y in ((a, b), (b, a)):
+            for w in g.incident(x):
+                if w != y:
+                    used_msgs.add((w, x))
+    off = 0
+    for mid in range(2 * g.ne):
+        a, b = g.message_edge(mid)
+        chi = ket.bond_dim(a, b)
+        if (a, b) in used_msgs:
+            dmsgs[(a, b)] = dm[off:off + chi * chi].reshape(chi, chi, order="F")
+        off += chi * chi
+    return dpsi, dgates, dmsgs

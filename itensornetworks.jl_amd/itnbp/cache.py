@@ -316,7 +316,7 @@ def initialize_cache(tn, alg="bp", *, messages=None, storage=None, ctx=None, par
     if partitioned_vertices is not None:
         from .partition import PartitionedNetwork
         psi = tn.ket if isinstance(tn, QuadraticFormNetwork) else tn
-        pn = PartitionedNetwork(psi, partitioned_vertices)
+        pn = PartitionedNetwork(psi, partitioned_vertices, ctx=ctx)
         bpc = initialize_cache(pn.network, messages=messages, storage=storage, ctx=ctx)
         bpc.partitioning = pn
         return bpc
@@ -739,4 +739,30 @@ def update_batched_(caches, *, maxiter, tol=None, res_tol=None, precision="f64",
                              max_residual=r.max_residual, converged=bool(r.converged), device_ms=r.device_ms,
                              path=ctx.last_path())
         out.append(c.last_update)
+    return out
+
+
+def sweeps_vjp(psi, nsweeps, mbar, *, checkpoint=1, ctx=None):
+    """Pullback through ``nsweeps`` Jacobi sweeps from identity messages: given the
+    cotangents ``mbar`` ({directed edge: matrix}; absent messages count as zero) of
+    the final messages, the gradient with respect to every site tensor
+    ({v: array}).  With ``apply_gates_vjp`` it gives the gradient of a TEBD step.
+    The reference has no autodiff."""
+    bpc = BeliefPropagationCache(QuadraticFormNetwork(psi), storage="f64", ctx=ctx)
+    g = psi.graph
+    mb = np.zeros(int(bpc.lib.itnbp_net_msg_numel(bpc.h)))
+    off = 0
+    for mid in range(2 * g.ne):
+        a, b = g.message_edge(mid)
+        chi = psi.bond_dim(a, b)
+        if (a, b) in mbar:
+            mb[off:off + chi * chi] = np.asarray(mbar[(a, b)], dtype=np.float64).reshape(-1, order="F")
+        off += chi * chi
+    grad = np.empty(int(bpc.lib.itnbp_net_psi_numel(bpc.h)), dtype=np.float64)
+    bpc.ctx.check(bpc.lib.itnbp_sweeps_vjp(bpc.h, int(nsweeps), int(checkpoint), L.dptr(mb), L.dptr(grad)))
+    out, off = {}, 0
+    for v in g.vertices:
+        n = psi[v].size
+        out[v] = grad[off:off + n].reshape(psi[v].shape, order="F")
+        off += n
     return out

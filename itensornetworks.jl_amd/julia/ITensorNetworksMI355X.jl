@@ -25,6 +25,9 @@
 #     cache's messages as the product environment; the vector form applies
 #     gates on disjoint bonds in one launch.
 #   * `ChainRulesCore.rrule(bp_energy, ...)` wraps the library's adjoint.
+#   * `inner(ϕ, [A,] ψ; alg = "bp")` and `loginner` (src/inner.jl:100-160) run
+#     BP on the LinearFormNetwork on the device's two-layer kernels (bra and
+#     ket of their own bond dimensions, real or complex).
 #
 # Marshalling: every site tensor is permuted to [siteind; link inds in
 # incident_edges order] and passed column-major; messages are chi x chi
@@ -612,5 +615,126 @@ function ChainRulesCore.rrule(::typeof(bp_energy), psi::ITensorNetwork, nsweeps:
     end
     return E, bp_energy_pullback
 end
+
+# -- inner products with alg = "bp" on the two-layer kernels (k_layer.hip) --------
+
+# flat column-major data of every site tensor of `tn` permuted to `axes(v)`
+# (real, or interleaved (re, im) pairs when `cplx`)
+function flat_layer(tn, verts, axes, cplx::Bool)
+    parts = [vec(array(permute(tn[v], axes(v)...))) for v in verts]
+    if cplx
+        z = reduce(vcat, [ComplexF64.(x) for x in parts]; init = ComplexF64[])
+        return collect(reinterpret(Float64, z))
+    end
+    return reduce(vcat, [Float64.(x) for x in parts]; init = Float64[])
+end
+
+"""
+    loginner_bp(ϕ, ψ; A = nothing, cache_update_kwargs = (;))
+
+log of BP's estimate of ⟨ϕ|ψ⟩ (or ⟨ϕ|A|ψ⟩) on the device: the LinearFormNetwork
+(BilinearFormNetwork with the operator's and the ket's link indices merged) with
+one {bra, ket} partition per site, no default message (initialize_cache.jl:10-12),
+`maxiter` 1 on a tree and required otherwise (abstractbeliefpropagationcache.jl:38,
+315-317); then logscalar (:397-412).  `A[v]` carries the site indices of ψ (ket
+side) and their primes (bra side).  `cache_update_kwargs` takes `maxiter`, `tol`
+and `edge_sequence` ("forest_cover" or "parallel"), as logscalar's do
+(contract.jl:41-58).
+"""
+function loginner_bp(phi::AbstractITensorNetwork, psi::AbstractITensorNetwork; A = nothing,
+        cache_update_kwargs = (;), ctx::Context = default_context())
+    maxiter = get(cache_update_kwargs, :maxiter, nothing)
+    tol = get(cache_update_kwargs, :tol, nothing)
+    edge_sequence = get(cache_update_kwargs, :edge_sequence, "forest_cover")
+    edge_sequence in ("forest_cover", "parallel") ||
+        error("loginner_bp: edge_sequence must be \"forest_cover\" or \"parallel\"")
+    L = Layout(psi)
+    verts = L.verts
+    cplx = any(v -> eltype(psi[v]) <: Complex || eltype(phi[v]) <: Complex ||
+                     (A !== nothing && eltype(A[v]) <: Complex), verts)
+    blinks = Dict{Any, Index}()
+    for (a, b) in L.edges
+        l = only(linkinds(phi, a => b))
+        blinks[(a, b)] = l
+        blinks[(b, a)] = l
+    end
+    bra_axes(v) = [only(siteinds(phi, v)); [blinks[(v, w)] for w in neighbors(psi, v)]]
+    ket_axes(v) = L.axes[v]
+    ne = length(L.edges)
+    edges_c = Cint[]
+    for (a, b) in L.edges
+        push!(edges_c, L.vindex[a], L.vindex[b])
+    end
+    inc_ptr = Cint[0]
+    inc_edge = Cint[]
+    for v in verts
+        for w in neighbors(psi, v)
+            push!(inc_edge, L.eindex[(v, w)])
+        end
+        push!(inc_ptr, length(inc_edge))
+    end
+    bchi = Cint[dim(blinks[(a, b)]) for (a, b) in L.edges]
+    pchi = Cint[dim(L.links[(a, b)]) for (a, b) in L.edges]
+    achi = A === nothing ? Cint[] : Cint[dim(only(linkinds(A, a => b))) for (a, b) in L.edges]
+    kchi = A === nothing ? pchi : pchi .* achi
+    phys = Cint[dim(only(siteinds(psi, v))) for v in verts]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ctx, ccall((:itnbp_lnet_create, libitnbp), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Cint,
+         Ref{Ptr{Cvoid}}),
+        ctx.h, length(verts), ne, edges_c, inc_ptr, inc_edge, kchi, bchi, phys, Cint(cplx), h))
+    lnet = h[]
+    try
+        bra = flat_layer(phi, verts, bra_axes, cplx)
+        if A === nothing
+            check(ctx, ccall((:itnbp_lnet_set_tensors, libitnbp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                lnet, flat_layer(psi, verts, ket_axes, cplx), bra))
+        else
+            nk = ccall((:itnbp_lnet_ket_numel, libitnbp), Int64, (Ptr{Cvoid},), lnet)
+            check(ctx, ccall((:itnbp_lnet_set_tensors, libitnbp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                lnet, zeros(Float64, (cplx ? 2 : 1) * nk), bra))
+            # A_v[s', s, links of A in the ket's incident order]
+            op_axes(v) = begin
+                s = only(siteinds(psi, v))
+                [ITensors.prime(s); s; [only(linkinds(A, v => w)) for w in neighbors(psi, v)]]
+            end
+            check(ctx, ccall((:itnbp_lnet_fold_operator, libitnbp), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Cint}, Ptr{Float64}, Ptr{Cint}),
+                lnet, flat_layer(psi, verts, ket_axes, cplx), pchi, flat_layer(A, verts, op_axes, cplx), achi))
+        end
+        if maxiter === nothing
+            maxiter = is_tree(psi) ? 1 : error("You need to specify a number of iterations for BP!")
+        end
+        parallel = edge_sequence == "parallel"
+        o = UpdateOpts(parallel ? ITNBP_SCHED_PARALLEL : ITNBP_SCHED_SEQUENTIAL, Cint(maxiter),
+            tol === nothing ? -1.0 : Float64(tol), -1.0, PRECISIONS["f64"], 0.0, Cint(1), Cint(0), C_NULL,
+            Cint(0))
+        r = Ref{UpdateResult}()
+        check(ctx, ccall((:itnbp_lnet_update, libitnbp), Cint, (Ptr{Cvoid}, Ref{UpdateOpts}, Ref{UpdateResult}),
+            lnet, o, r))
+        re, im = Ref(0.0), Ref(0.0)
+        check(ctx, ccall((:itnbp_lnet_logscalar, libitnbp), Cint, (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}),
+            lnet, re, im))
+        return (cplx || im[] != 0) ? complex(re[], im[]) : re[]
+    finally
+        ccall((:itnbp_lnet_destroy, libitnbp), Cint, (Ptr{Cvoid},), lnet)
+    end
+end
+
+# the reference's inner / loginner with alg = "bp" (src/inner.jl:112-160); the
+# inner network's link indices are never materialised, so dual_link_index_map has
+# no effect, and a caller's own cache (`cache!`) stays on the reference's path
+function bp_inner_kwargs(kw)
+    haskey(kw, :cache!) && error("loginner_bp: a caller-supplied cache! is not supported on the device path")
+    return (; cache_update_kwargs = get(kw, :cache_update_kwargs, (;)))
+end
+ITensorNetworks.loginner(::Algorithm"bp", phi::ITensorNetwork, psi::ITensorNetwork; kwargs...) =
+    loginner_bp(phi, psi; bp_inner_kwargs(kwargs)...)
+ITensorNetworks.loginner(::Algorithm"bp", phi::ITensorNetwork, A::ITensorNetwork, psi::ITensorNetwork; kwargs...) =
+    loginner_bp(phi, psi; A, bp_inner_kwargs(kwargs)...)
+ITensors.inner(::Algorithm"bp", phi::ITensorNetwork, psi::ITensorNetwork; kwargs...) =
+    exp(loginner_bp(phi, psi; bp_inner_kwargs(kwargs)...))
+ITensors.inner(::Algorithm"bp", phi::ITensorNetwork, A::ITensorNetwork, psi::ITensorNetwork; kwargs...) =
+    exp(loginner_bp(phi, psi; A, bp_inner_kwargs(kwargs)...))
 
 end # module

@@ -20,8 +20,10 @@ COMPAT = {
     "const char*": {"Cstring"},
     "itnbp_ctx*": {"Ptr{Cvoid}"},
     "itnbp_net*": {"Ptr{Cvoid}"},
+    "itnbp_lnet*": {"Ptr{Cvoid}"},
     "itnbp_ctx**": {"Ref{Ptr{Cvoid}}", "Ptr{Ptr{Cvoid}}"},
     "itnbp_net**": {"Ref{Ptr{Cvoid}}", "Ptr{Ptr{Cvoid}}"},
+    "itnbp_lnet**": {"Ref{Ptr{Cvoid}}", "Ptr{Ptr{Cvoid}}"},
     "const int*": {"Ptr{Cint}", "Ref{Cint}"},
     "int*": {"Ptr{Cint}", "Ref{Cint}"},
     "const double*": {"Ptr{Float64}", "Ref{Float64}", "Ptr{Cdouble}", "Ref{Cdouble}"},

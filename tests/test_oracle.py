@@ -350,3 +350,58 @@ def test_expm_sym_matches_scipy():
             np.testing.assert_allclose(O.expm_sym(H, t), expm(t * H), rtol=1e-12, atol=1e-13)
     H = O.ising_bond_hamiltonian(2.5, 4, 3)
     np.testing.assert_allclose(O.expm_sym(H, -0.05), expm(-0.05 * H), rtol=1e-12, atol=1e-13)
+
+
+def test_torch_simple_update_matches_the_oracle_and_finite_differences(grid33):
+    """The torch restatement of simple_update_bp (the checker of the gate
+    application's adjoint) equals the NumPy oracle, and its autograd VJP equals
+    central finite differences of the NumPy oracle along random directions of
+    psi_u, psi_v, the gate and every environment message."""
+    import torch
+    from oracle import grad_oracle as GO
+    g, psi = grid33
+    rng = np.random.default_rng(71)
+    msgs, _, _, _ = O.bp_update(g, psi, O.identity_messages(g, psi), maxiter=30, schedule="parallel")
+    T = lambda a: torch.tensor(np.asarray(a), dtype=torch.float64, requires_grad=True)
+    for (u, v), maxdim, norm in [((4, 5), None, False), ((0, 1), 2, True), ((1, 4), 2, False)]:
+        envs_u, envs_v = O.envs_for_bond(g, msgs, u, v)
+        G = O.expm_sym(O.ising_bond_hamiltonian(1.0, len(g.incident[u]), len(g.incident[v])), -0.3)
+        ax_u, ax_v = g.axis(u, v), g.axis(v, u)
+        pu, pv, _, _ = O.simple_update_bp(G, psi[u], psi[v], ax_u, ax_v, envs_u, envs_v, maxdim=maxdim,
+                                          normalize=norm)
+        cu, cv = rng.standard_normal(pu.shape), rng.standard_normal(pv.shape)
+        args = dict(G=T(G), psi_u=T(psi[u]), psi_v=T(psi[v]))
+        eu = {a: T(m) for a, m in envs_u.items()}
+        ev = {a: T(m) for a, m in envs_v.items()}
+        tu, tv = GO.simple_update_bp_torch(args["G"], args["psi_u"], args["psi_v"], ax_u, ax_v, eu, ev,
+                                           maxdim=maxdim, normalize=norm)
+        np.testing.assert_allclose(tu.detach().numpy(), pu, atol=1e-12)
+        L = (tu * torch.tensor(cu)).sum() + (tv * torch.tensor(cv)).sum()
+        L.backward()
+
+        def loss(G_, pu_, pv_, eu_, ev_):
+            a, b, _, _ = O.simple_update_bp(G_, pu_, pv_, ax_u, ax_v, eu_, ev_, maxdim=maxdim, normalize=norm)
+            return float(np.sum(a * cu) + np.sum(b * cv))
+
+        base = [G, psi[u], psi[v], dict(envs_u), dict(envs_v)]
+        grads = [args["G"].grad, args["psi_u"].grad, args["psi_v"].grad, {a: t.grad for a, t in eu.items()},
+                 {a: t.grad for a, t in ev.items()}]
+        h = 1e-6
+        for slot in range(5):
+            keys = list(base[slot].keys()) if slot >= 3 else [None]
+            for key in keys:
+                x0 = base[slot][key] if key is not None else base[slot]
+                gr = (grads[slot][key] if key is not None else grads[slot]).numpy()
+                dirn = rng.standard_normal(x0.shape)
+                if slot >= 3:
+                    dirn = dirn + dirn.T  # messages are symmetric
+                vals = []
+                for sgn in (1, -1):
+                    cur = [b if not isinstance(b, dict) else dict(b) for b in base]
+                    if key is None:
+                        cur[slot] = x0 + sgn * h * dirn
+                    else:
+                        cur[slot][key] = x0 + sgn * h * dirn
+                    vals.append(loss(*cur))
+                fd = (vals[0] - vals[1]) / (2 * h)
+                assert np.sum(gr * dirn) == pytest.approx(fd, rel=1e-5, abs=1e-7), ((u, v), slot, key)
