@@ -225,6 +225,26 @@ int itnbp_apply_gates_noalloc(itnbp_net* net, int n, const int* edge_ids, const 
  * ops1: nv d x d (may be NULL), ops2: ne (d_u d_v)^2 (may be NULL). */
 int itnbp_energy_grad(itnbp_net* net, int nsweeps, int checkpoint, const double* ops1,
                       const double* ops2, double* energy, double* grad);
+/* Pullback through nsweeps Jacobi sweeps from identity messages (the forward of
+ * itnbp_energy_grad): given the cotangent mbar of the final messages (all
+ * messages, itnbp_net_msg_numel doubles), the gradient with respect to every
+ * site tensor (itnbp_net_psi_numel doubles). With itnbp_apply_vjp it gives the
+ * gradient of a TEBD step (BP sweeps, then a gate round). No counterpart in
+ * the reference. Leaves the final forward messages in the network. */
+int itnbp_sweeps_vjp(itnbp_net* net, int nsweeps, int checkpoint, const double* mbar, double* grad);
+/* Vector-Jacobian product of itnbp_apply_gates (simple_update_bp, apply.jl:33-95,
+ * and the normalisation of :134-137) for n gates on disjoint bonds; no
+ * counterpart in the reference (it has no autodiff). Called on the network as
+ * it was before the gates, with the forward call's arguments and the ranks it
+ * kept. dpsi_out: per gate the cotangents of psi_u' then psi_v' (u, v = the
+ * edge's endpoints in the order of itnbp_net_create; column-major, the bond of
+ * dimension kept[i]). Writes dpsi_in (per gate psi_u then psi_v, the current
+ * shapes) and dgates (the gates' layout), and adds the cotangents of the
+ * environment messages into dmsgs (all messages, itnbp_net_msg_numel
+ * doubles). ITNBP_ERR_ARG if kept differs from the recomputed rank. */
+int itnbp_apply_vjp(itnbp_net* net, int n, const int* edge_ids, const double* gates, int maxdim,
+                    double cutoff, int normalize, const int* kept, const double* dpsi_out,
+                    double* dpsi_in, double* dgates, double* dmsgs);
 
 /* ---- batched heterogeneous BP ----------------------------------------- */
 /* n networks (any lattices, bond dims, same dtype) updated together: every
@@ -275,6 +295,26 @@ int itnbp_lnet_logscalar(itnbp_lnet* lnet, double* re, double* im);
 /* R_v[s_ket, s_bra] = environment of v contracted with ket and conj(bra) over
  * every bond (beliefpropagationcache.jl:100-105), unnormalised, d x d each */
 int itnbp_lnet_rdm1(itnbp_lnet* lnet, int n, const int* verts, double* out);
+
+/* ---- small tensor-network contractions ------------------------------- */
+/* Contraction of n real FP64 tensors on the device, for BP over generalised
+ * partitions: the tensor of a partition is the contraction of its sites'
+ * tensors over the bonds inside it (the factors the reference contracts with a
+ * partition's incoming messages, src/caches/abstractbeliefpropagationcache.jl:
+ * 225-239, for the partitioned cache of beliefpropagationcache.jl:20-35), and
+ * a member site's density matrix is a partial trace of its partition's.
+ * Tensor i has ranks[i] axes, column-major, labelled by the next ranks[i]
+ * entries of labels (ids in [0, nlabels), dimension label_dims[id]); data holds
+ * the tensors back to back.  A label repeated within one tensor is traced, one
+ * in no other tensor and not in the output is summed, one shared and not in
+ * the output is contracted.  out (column-major over out_labels, each at most
+ * once) receives the result.  Pairwise order: the smallest intermediate first
+ * (standing in for contraction_sequence, src/contraction_sequences.jl:9-20);
+ * every product runs as a batched GEMM on the FP64 matrix cores.  Limits: rank
+ * 16 per tensor, 2^28 elements per tensor and 2^30 per call
+ * (ITNBP_ERR_UNSUPPORTED beyond). */
+int itnbp_einsum(itnbp_ctx* ctx, int n, const int* ranks, const int* labels, const int* label_dims, int nlabels,
+                 const double* data, int out_rank, const int* out_labels, double* out);
 
 /* ---- diagnostics ------------------------------------------------------ */
 /* name of the kernel path the last itnbp_update used */

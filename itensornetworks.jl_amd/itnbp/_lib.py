@@ -79,6 +79,7 @@ SIGNATURES = {
     "itnbp_update": (_I, [_P, C.POINTER(UpdateOpts), C.POINTER(UpdateResult)]),
     "itnbp_updated_messages": (_I, [_P, _I, _IP, _I, _DP]),
     "itnbp_rdm1": (_I, [_P, _I, _IP, _DP]),
+    "itnbp_einsum": (_I, [_P, _I, _IP, _IP, _IP, _I, _DP, _I, _IP, _DP]),
     "itnbp_expect1": (_I, [_P, _I, _IP, _DP, _DP]),
     "itnbp_rdm2": (_I, [_P, _I, _IP, _DP]),
     "itnbp_expect2": (_I, [_P, _I, _IP, _DP, _DP]),
@@ -88,6 +89,8 @@ SIGNATURES = {
     "itnbp_apply_gates": (_I, [_P, _I, _IP, _DP, _I, _D, _I, _DP, _DP, _I, _IP]),
     "itnbp_apply_site_gates": (_I, [_P, _I, _IP, _DP, _I, _DP]),
     "itnbp_apply_gates_noalloc": (_I, [_P, _I, _IP, _DP, _I, _D, _I, _DP, _DP, _I, _IP]),
+    "itnbp_sweeps_vjp": (_I, [_P, _I, _I, _DP, _DP]),
+    "itnbp_apply_vjp": (_I, [_P, _I, _IP, _DP, _I, _D, _I, _IP, _DP, _DP, _DP, _DP]),
     "itnbp_energy_grad": (_I, [_P, _I, _I, _DP, _DP, _DP, _DP]),
     "itnbp_update_batched": (_I, [_I, C.POINTER(_P), C.POINTER(UpdateOpts), C.POINTER(UpdateResult)]),
     "itnbp_lnet_create": (_I, [_P, _I, _I, _IP, _IP, _IP, _IP, _IP, _IP, _I, C.POINTER(_P)]),

@@ -14,18 +14,59 @@ the partition network runs through the same device kernels as any network:
 * the bond between two partitions combines the lattice bonds that cross
   between them, ordered by edge id, first fastest -- the same order on both
   sides;
-* a member's one-site RDM is the partial trace of its partition's RDM.
+* a member's one-site RDM is the partial trace of its partition's RDM, also
+  taken on the device.
 
 Partition sizes are bounded by the device's generic kernels (combined bond
-dimension <= 64, degree <= 6) and by the host memory of the contraction.
+dimension <= 64, degree <= 6) and by the contraction's limits (2^28 elements
+per tensor, 2^30 per call).
 """
 from __future__ import annotations
 
 import numpy as np
 
+from . import _lib as L
 from .network import ITensorNetwork, NamedGraph
 
 _QF_TAGS = ("ket", "bra", "operator")
+
+
+def device_einsum(tensors, labels, out_labels, *, ctx=None):
+    """Contraction of real tensors on the device (itnbp_einsum): ``labels[i]``
+    lists hashable labels of the axes of ``tensors[i]``; a label repeated in
+    one tensor is traced, one shared by several and absent from
+    ``out_labels`` is contracted, one in a single tensor and not in the
+    output is summed.  Returns the array over ``out_labels``."""
+    ctx = ctx or L.default_context()
+    lib = ctx.lib
+    ids, dims = {}, []
+    flat, ranks, data = [], [], []
+    for t, lab in zip(tensors, labels):
+        t = np.asarray(t, dtype=np.float64)
+        if t.ndim != len(lab):
+            raise ValueError(f"a tensor of rank {t.ndim} has {len(lab)} labels")
+        for x, n in zip(lab, t.shape):
+            if x not in ids:
+                ids[x] = len(ids)
+                dims.append(int(n))
+            elif dims[ids[x]] != n:
+                raise ValueError(f"label {x!r} has dimensions {dims[ids[x]]} and {n}")
+            flat.append(ids[x])
+        ranks.append(t.ndim)
+        data.append(t.reshape(-1, order="F"))
+    missing = [x for x in out_labels if x not in ids]
+    if missing:
+        raise ValueError(f"output labels {missing} appear in no tensor")
+    outl = np.array([ids[x] for x in out_labels], dtype=np.int32)
+    shape = [dims[ids[x]] for x in out_labels]
+    out = np.empty(int(np.prod(shape)) if shape else 1, dtype=np.float64)
+    ranks = np.array(ranks, dtype=np.int32)
+    flat = np.array(flat if flat else [0], dtype=np.int32)
+    dims_a = np.array(dims if dims else [1], dtype=np.int32)
+    data = np.concatenate(data) if data else np.zeros(1)
+    ctx.check(lib.itnbp_einsum(ctx.h, len(tensors), L.iptr(ranks), L.iptr(flat), L.iptr(dims_a), len(dims),
+                               L.dptr(data), len(outl), L.iptr(outl), L.dptr(out)))
+    return out.reshape(shape, order="F") if shape else out.reshape(())
 
 
 def _original_vertex(x, vertices):
@@ -42,8 +83,9 @@ class PartitionedNetwork:
     list of vertex lists; quadratic-form vertices are accepted and mapped to
     their network vertex).  ``network`` is the partition network."""
 
-    def __init__(self, psi: ITensorNetwork, partitioned_vertices):
+    def __init__(self, psi: ITensorNetwork, partitioned_vertices, *, ctx=None):
         g = psi.graph
+        self._ctx = ctx
         vset = set(g.vertices)
         groups = partitioned_vertices
         if not isinstance(groups, dict):
@@ -83,6 +125,12 @@ class PartitionedNetwork:
         tensors = {key: self._contract(key) for key in keys}
         self.network = ITensorNetwork(qg, tensors)
 
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            self._ctx = L.default_context()
+        return self._ctx
+
     def _bundle(self, p, q):
         k = (p, q) if (p, q) in self.bundles else (q, p)
         return self.bundles[k]
@@ -90,27 +138,17 @@ class PartitionedNetwork:
     def _contract(self, key):
         psi, g = self.psi, self.psi.graph
         mem = self.members[key]
-        label, ops = {}, []
-
-        def lab(x):
-            if x not in label:
-                label[x] = len(label)
-            return label[x]
-
+        tensors, labels = [], []
         for v in mem:
-            t = np.asarray(psi[v])
-            idx = [lab(("s", v))] + [lab(("e", g.edge_id(v, w))) for w in g.incident(v)]
-            ops += [t, idx]
-        out = [label[("s", v)] for v in mem]
-        shape = [psi.phys_dim(v) for v in mem]
+            tensors.append(np.asarray(psi[v]))
+            labels.append([("s", v)] + [("e", g.edge_id(v, w)) for w in g.incident(v)])
+        out = [("s", v) for v in mem]
         bshape = []
         for q in self.graph.incident(key):
             es = self._bundle(key, q)
-            out += [label[("e", e)] for e in es]
-            dims = [psi.bond_dim(*g.edges[e]) for e in es]
-            shape += dims
-            bshape.append(int(np.prod(dims)))
-        t = np.einsum(*ops, out, optimize=True)
+            out += [("e", e) for e in es]
+            bshape.append(int(np.prod([psi.bond_dim(*g.edges[e]) for e in es])))
+        t = device_einsum(tensors, labels, out, ctx=self.ctx)
         # combine: physical indices (first member fastest), then each bundle
         return t.reshape([int(np.prod([psi.phys_dim(v) for v in mem]))] + bshape, order="F")
 
@@ -121,9 +159,6 @@ class PartitionedNetwork:
         m = len(mem)
         i = mem.index(v)
         r = np.asarray(rho_p).reshape(ds + ds, order="F")
-        # trace out every other member
-        for j in reversed(range(m)):
-            if j == i:
-                continue
-            r = np.trace(r, axis1=j, axis2=j + r.ndim // 2)
-        return r
+        # trace out every other member: its ket and bra axes share a label
+        lab = [("k", j) for j in range(m)] + [("k", j) if j != i else ("b", j) for j in range(m)]
+        return device_einsum([r], [lab], [("k", i), ("b", i)], ctx=self.ctx)

@@ -104,3 +104,21 @@ def test_malformed_descriptions_under_sanitizers(driver):
         "chi_65": L.ERR_UNSUPPORTED, "phys_9": L.ERR_UNSUPPORTED, "site_too_large": L.ERR_UNSUPPORTED,
         "degree_above_limit": L.ERR_UNSUPPORTED, "null_bond_dims": L.ERR_ARG, "open_dim_65": L.ERR_UNSUPPORTED,
         "large_batches_invariants": 0}
+
+
+def test_einsum_planner_under_sanitizers(tmp_path):
+    """The contraction planner of itnbp_einsum (csrc/itnbp_einsum.cpp) under
+    ASan/UBSan: 400 random label sets (traces, summed labels, hyperedges,
+    outer products, scalars) planned, executed by a CPU restatement of the
+    device steps and compared with a brute-force sum over every label
+    assignment; malformed descriptions are rejected."""
+    src = os.path.join(ROOT, "tests", "cpp", "einsum_driver.cpp")
+    plan_src = os.path.join(ROOT, "itensornetworks.jl_amd", "csrc", "itnbp_einsum.cpp")
+    exe = str(tmp_path / "einsum_driver")
+    r = subprocess.run(["g++", "-std=c++17", "-g", "-O1", "-Wall", "-Werror"] + SAN + [src, plan_src, "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, env=ENV, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout)
+    assert out["failures"] == 0 and out["cases"] == 400 and out["max_err"] < 1e-12

@@ -1,8 +1,8 @@
 """Generalised partitions (BeliefPropagationCache(tn; partitioned_vertices),
-test/test_expect.jl:22-39): the host contraction of each partition is checked
-against the exact state on the CPU; column-grouped BP on the device is exact
-on grids (the partition graph is a path) and singleton partitions reproduce
-plain BP."""
+test/test_expect.jl:22-39): the device contraction of each partition gives
+the exact state back; column-grouped BP on the device is exact on grids (the
+partition graph is a path) and singleton partitions reproduce plain BP; the
+device contraction itself (itnbp_einsum) against numpy's einsum."""
 import numpy as np
 import pytest
 
@@ -19,6 +19,7 @@ def _columns(g):
     return cols
 
 
+@pytest.mark.gpu
 def test_partition_network_has_the_same_state():
     g = itnbp.named_grid((3, 3))
     psi = itnbp.random_tensornetwork(np.random.default_rng(4), np.float64, g, link_space=2)
@@ -70,3 +71,38 @@ def test_singleton_partitions_reproduce_plain_bp():
                           cache_update_kwargs=dict(maxiter=40, edge_sequence="parallel"))
     for v in g.vertices:
         assert single[v] == pytest.approx(plain[v], abs=1e-12)
+
+
+def _einsum_cases():
+    rng = np.random.default_rng(21)
+    r = lambda *s: rng.standard_normal(s)
+    # (operands as (array, labels) pairs, output labels)
+    yield "matrix product", [(r(5, 7), "ab"), (r(7, 3), "bc")], "ac"
+    yield "chain with a batch label", [(r(4, 6, 5), "abz"), (r(6, 3, 5), "bcz"), (r(3, 2), "cd")], "adz"
+    yield "hyperedge", [(r(4, 5), "ab"), (r(5, 6), "bc"), (r(5, 2), "bd")], "acd"
+    yield "trace and sum", [(r(3, 4, 3, 2), "abac"), (r(4, 5), "bd")], "d"
+    yield "outer product", [(r(3, 2), "ab"), (r(4,), "c")], "cab"
+    yield "scalar", [(r(6, 5), "ab"), (r(6, 5), "ab")], ""
+    yield "permutation only", [(r(2, 3, 4, 5), "abcd")], "dbca"
+    yield "partial trace", [(r(2, 3, 2, 3), "abAb")], "aA"
+    yield "grid patch", [(r(2, 3, 4), "sab"), (r(2, 3, 5, 4), "tacd"), (r(2, 4, 6), "ubx"), (r(2, 5, 6, 3), "vcxy")], \
+        "stuvdy"
+    yield "large product", [(r(70, 130), "ab"), (r(130, 90), "bc")], "ca"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,ops,out", list(_einsum_cases()), ids=[c[0] for c in _einsum_cases()])
+def test_device_einsum_matches_numpy(name, ops, out):
+    from itnbp.partition import device_einsum
+    got = device_einsum([a for a, _ in ops], [list(lab) for _, lab in ops], list(out))
+    want = np.einsum(",".join(lab for _, lab in ops) + "->" + out, *[a for a, _ in ops])
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_device_einsum_errors():
+    from itnbp.partition import device_einsum
+    with pytest.raises(ValueError):
+        device_einsum([np.ones((2, 3))], [["a", "b"]], ["c"])      # output label in no tensor
+    with pytest.raises(itnbp.ITNBPError):
+        device_einsum([np.ones((2, 3))], [["a", "b"]], ["a", "a"])  # repeated output label
