@@ -65,6 +65,8 @@ void small_set_batch(itnbp_net* net, const std::vector<int>& vinst, const int* a
 // the fastest; small_big_tile reports it (0 without large sites)
 int small_autotune(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, int normalize);
 int small_big_tile(itnbp_net* net);
+// "mfma" or "valu" for the autotuned large-site kernel ("" before tuning)
+const char* small_big_kind(itnbp_net* net);
 int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, void* msg_out,
                 MsgStat* stats, int normalize);
 void release_small_state(itnbp_net* net);

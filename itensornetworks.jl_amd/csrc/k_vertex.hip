@@ -678,6 +678,206 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
   }
 }
 
+// FP32 flavour of k_vertex_big on the matrix cores (v_mfma_f32_16x16x4f32).
+// Step 3, the rotation of bond a, is per tile of 16 fibres the product
+//   Y'[f, r] = sum_j Y[f, j] U[j, r]       (A: 16 fibres x 4 of j, B: U, K = n <= 16)
+// and step 4, the Gram of bond a, is per group of 4 fibres
+//   T'[r, r'] += sum_f (w_f Y[f, r]) Y[f, r']   (K = 4 fibres; one load per lane
+//   feeds both operands, since A[r][k] and B[k][r'] hold the same fibre column).
+// Each wave takes its own tiles (NT in flight per iteration, their loads issued
+// together); the Gram accumulates in FP32 for 16 MFMAs (64 fibres) at a time and
+// is carried in FP64 between them, the four waves' partial Grams summed in FP64.
+// Lane layout (16x16x4 f32): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k,
+// D[4 (l / 16) + v][l % 16] in lane l, value v.
+#define F32_DROW(l, v) (4 * ((l) >> 4) + (v))
+#define F32_DCOL(l, v) ((l) & 15)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr size_t VBM_LDS = VB_FIXED + 4 * SV_LD * SV_LD * 8;  // fixed part + Jacobi work / wave partials
+constexpr int VBM_TILES[3] = {1, 2, 4};                         // MFMA tiles in flight per wave
+constexpr int VBM_FT = 2;
+static_assert(4 * SV_LD * SV_LD * 8 >= 4 * 256 * 8, "the wave partial Grams must fit the Jacobi work area");
+
+template <typename TS, typename TM, int NT>
+__global__ __launch_bounds__(SV_THREADS) void k_vertex_big_mfma(const VDesc* __restrict__ vdesc,
+                                                               const SmallTask* __restrict__ tasks,
+                                                               const TS* __restrict__ psi, const TM* msg_in,
+                                                               TM* msg_out, MsgStat* __restrict__ stats,
+                                                               int normalize, float* __restrict__ scratch,
+                                                               double* __restrict__ seeds, int warm,
+                                                               const int* __restrict__ active,
+                                                               const long long* __restrict__ sweep_ctr) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* Us = reinterpret_cast<double*>(smem);
+  double* lam = Us + ITNBP_MAXK * SV_LD * SV_LD;
+  double* rc = lam + ITNBP_MAXK * SV_MAXCHI;
+  double* rs = rc + 32;
+  int* rp = reinterpret_cast<int*>(rs + 32);
+  int* rq = rp + 32;
+  double* gpart = reinterpret_cast<double*>(rq + 32);
+  double* Tp = gpart + SV_THREADS;
+  double* X = Tp + SV_LD * SV_LD;
+  double* red = X + SV_LD * SV_LD;
+  double* As = reinterpret_cast<double*>(smem + VB_FIXED);  // Jacobi work (step 2), wave partials (step 4)
+
+  __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
+  const SmallTask* tk = tasks + blockIdx.x;
+  if (active && !active[tk->inst]) return;
+  if (sweep_ctr) warm = (*sweep_ctr % 16) != 0;
+  const VDesc* V = vdesc + tk->v;
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+  const int li = l & 15, lk = l >> 4;
+  const int k = V->k, D = V->D;
+  if (tid == 0) {
+    int s = 1;
+    for (int a = 0; a <= k; ++a) {
+      dims[a] = V->dims[a];
+      strides[a] = s;
+      s *= dims[a];
+    }
+  }
+  __syncthreads();
+  float* Y = scratch + tk->yoff;
+  const TS* src = psi + V->psi_off;
+  // 2. eigendecompositions, one bond per wave
+  for (int a = wv; a < k; a += 4)
+    bond_eigh(msg_in + V->in_msg[a + 1], dims[a + 1], As + wv * SV_LD * SV_LD, Us + a * SV_LD * SV_LD,
+              lam + a * SV_MAXCHI, seeds + (long long)tk->useed[a] * SEED_LD * SEED_LD, warm, rc + 8 * wv,
+              rs + 8 * wv, rp + 8 * wv, rq + 8 * wv);
+  __threadfence_block();
+  __syncthreads();
+  // 3. rotate every bond in place (bond 0 reads psi), 16 fibres per MFMA tile
+  for (int a = 0; a < k; ++a) {
+    const int n = dims[a + 1], st = strides[a + 1];
+    const int F = D / n;
+    const double* U = Us + a * SV_LD * SV_LD;
+    float bu[4];  // B operand: U[j][r], j = 4 kk + lk, r = li
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int j = 4 * kk + lk;
+      bu[kk] = (j < n && li < n) ? (float)U[j * SV_LD + li] : 0.f;
+    }
+    const int ntiles = (F + 15) / 16;
+    for (int t0 = wv * NT; t0 < ntiles; t0 += 4 * NT) {
+      float av[NT][4];
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        const int f = (t0 + q) * 16 + li;
+        const int base = f < F ? (f % st) + (f / st) * st * n : -1;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const int j = 4 * kk + lk;
+          av[q][kk] = (base >= 0 && j < n) ? (a == 0 ? load_as<float>(src, base + j * st) : Y[base + j * st]) : 0.f;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][kk], bu[kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int f = (t0 + q) * 16 + F32_DROW(l, v), r = F32_DCOL(l, v);
+          if (f < F && r < n) Y[(f % st) + (f / st) * st * n + r * st] = acc[v];
+        }
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+  // 4-5. per bond: weighted Gram, rotate back, normalise, statistics
+  for (int a = 0; a < k; ++a) {
+    const int n = dims[a + 1], st = strides[a + 1], nn = n * n;
+    const int F = D / n;
+    double g64[4] = {0, 0, 0, 0};
+    const int ngroups = (F + 3) / 4;  // 4 fibres per MFMA
+    // wave wv takes fibre groups wv, wv + 4, ...; NT groups per iteration, in
+    // runs of 16 MFMAs between FP64 flushes
+    for (int g0 = wv * NT; g0 < ngroups; g0 += 64 * NT) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int gg = g0; gg < min(ngroups, g0 + 64 * NT); gg += 4 * NT) {
+        float y[NT], wy[NT];
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+          const int f = (gg + q) * 4 + lk;
+          y[q] = 0.f;
+          wy[q] = 0.f;
+          if (gg + q < ngroups && f < F && li < n) {
+            const int base = (f % st) + (f / st) * st * n;
+            double w = 1.0;
+            for (int b = 0; b < k; ++b)
+              if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
+            y[q] = Y[base + li * st];
+            wy[q] = (float)w * y[q];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < NT; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wy[q], y[q], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) g64[v] += (double)acc[v];
+    }
+    // the four waves' partial Grams in LDS (the Jacobi work area), then T'
+    double* red4 = As;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red4[wv * 256 + F32_DROW(l, v) * 16 + F32_DCOL(l, v)] = g64[v];
+    __syncthreads();
+    if (tid < nn) {
+      const int rr = tid % n, r2 = tid / n;
+      Tp[rr * SV_LD + r2] = (red4[rr * 16 + r2] + red4[256 + rr * 16 + r2]) +
+                            (red4[512 + rr * 16 + r2] + red4[768 + rr * 16 + r2]);  // T'[r][r']
+    }
+    __syncthreads();
+    const double* U = Us + a * SV_LD * SV_LD;
+    if (tid < nn) {
+      const int b = tid % n, rr = tid / n;
+      double s = 0;
+      for (int q = 0; q < n; ++q) s += U[b * SV_LD + q] * Tp[q * SV_LD + rr];
+      X[b * SV_LD + rr] = s;  // (U T')[b][r']
+    }
+    __syncthreads();
+    double tval = 0;
+    if (tid < nn) {
+      const int i = tid % n, j = tid / n;  // message layout M[i + n j]
+      double sij = 0, sji = 0;
+      for (int q = 0; q < n; ++q) {
+        sij += X[i * SV_LD + q] * U[j * SV_LD + q];
+        sji += X[j * SV_LD + q] * U[i * SV_LD + q];
+      }
+      tval = 0.5 * (sij + sji);
+    }
+    const double n2 = block_sum(tval * tval, red);
+    const double nrm = sqrt(n2);
+    const double sc = (normalize && nrm != 0.0) ? 1.0 / nrm : 1.0;
+    const long long off = tk->out[a];
+    double dot = 0, o2 = 0, d2 = 0, bad = 0;
+    if (tid < nn) {
+      const double nv = tval * sc;
+      const double ov = (double)msg_in[off + tid];
+      if (!isfinite(nv)) bad = 1;
+      dot = ov * nv;
+      o2 = ov * ov;
+      d2 = (nv - ov) * (nv - ov);
+      msg_out[off + tid] = (TM)nv;
+    }
+    dot = block_sum(dot, red);
+    o2 = block_sum(o2, red);
+    d2 = block_sum(d2, red);
+    bad = block_sum(bad, red);
+    const int slot = tk->slot[a];
+    if (tid == 0 && slot >= 0) {
+      const double nn2 = n2 * sc * sc;
+      MsgStat stt;
+      stt.diff = 1.0 - (dot * dot) / (o2 * nn2);
+      stt.res2 = nn2 > 0 ? d2 / nn2 : 0.0;
+      stt.norm = nrm;
+      stt.bad = (bad > 0 || !isfinite(nrm)) ? 1.0 : 0.0;
+      stats[slot] = stt;
+    }
+    __syncthreads();
+  }
+}
+
 struct SmallState {
   std::vector<SmallTask> tasks;
   DevBuf d_tasks;
@@ -698,8 +898,27 @@ struct SmallState {
   std::vector<int> vinst;
   const int* active = nullptr;
   const long long* sweep_ctr = nullptr;
-  int ft = VB_FT;  // Gram tile of k_vertex_big
+  // k_vertex_big variant the autotuner chose (0: none yet) and for which flavour
+  int ft = 0;
+  bool ft_f32 = false;
 };
+
+// the k_vertex_big variant of a launch: ITNBP_VERTEX_BIG_TILE=<tile> forces
+// one (VB_TILES, or VBM_TILES for FP32), else the autotuner's choice for the
+// flavour, else its default
+template <typename TA>
+int big_variant(const SmallState* st) {
+  if (const char* e = getenv("ITNBP_VERTEX_BIG_TILE")) {
+    const int f = atoi(e);
+    for (int t : VB_TILES)
+      if (f == t) return f;
+    if (sizeof(TA) == 4)
+      for (int t : VBM_TILES)
+        if (f == t) return f;
+  }
+  if (st->ft > 0 && st->ft_f32 == (sizeof(TA) == 4)) return st->ft;
+  return sizeof(TA) == 4 ? VBM_FT : VB_FT;
+}
 
 SmallState* small_of(itnbp_net* net) { return reinterpret_cast<SmallState*>(net->small); }
 
@@ -716,9 +935,31 @@ hipError_t launch_big(itnbp_net* net, SmallState* st, const void* psi, const voi
   return hipGetLastError();
 }
 
+template <typename TS, typename TM, int NT>
+hipError_t launch_big_mfma(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
+                           MsgStat* stats, int normalize, double* seeds, int warm, hipStream_t stream) {
+  auto kern = k_vertex_big_mfma<TS, TM, NT>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)VBM_LDS);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((unsigned)st->big.size()), dim3(SV_THREADS), VBM_LDS, stream,
+                     (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_big.p, (const TS*)psi, (const TM*)msg_in,
+                     (TM*)msg_out, stats, normalize, (float*)st->ybuf.p, seeds, warm, st->active, st->sweep_ctr);
+  return hipGetLastError();
+}
+
+// ft: the VALU kernel's Gram tile (VB_TILES), or for the FP32 flavour the MFMA
+// kernel with ft tiles in flight per wave (VBM_TILES)
 template <typename TS, typename TA, typename TM>
 hipError_t launch_big_ft(int ft, itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
                          MsgStat* stats, int normalize, double* seeds, int warm, hipStream_t stream) {
+  if constexpr (sizeof(TA) == 4) {
+    switch (ft) {
+      case 1: return launch_big_mfma<TS, TM, 1>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
+      case 2: return launch_big_mfma<TS, TM, 2>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
+      case 4: return launch_big_mfma<TS, TM, 4>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
+      default: break;
+    }
+  }
   switch (ft) {
     case 160: return launch_big<TS, TA, TM, 160>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
     case 256: return launch_big<TS, TA, TM, 256>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm, stream);
@@ -734,8 +975,8 @@ hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const vo
   // the big sites (the longest workgroups) first, so that the LDS-resident
   // sites fill in behind them
   if (!st->big.empty()) {
-    hipError_t e = launch_big_ft<TS, TA, TM>(st->ft, net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm,
-                                            net->ctx->stream);
+    hipError_t e = launch_big_ft<TS, TA, TM>(big_variant<TA>(st), net, st, psi, msg_in, msg_out, stats, normalize,
+                                            seeds, warm, net->ctx->stream);
     if (e != hipSuccess) return e;
   }
   if (!st->tasks.empty()) {
@@ -752,9 +993,10 @@ hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const vo
   return hipSuccess;
 }
 
-// Times one launch of k_vertex_big with every VB_TILES tile on scratch outputs
-// (messages, statistics and eigenvector seeds; the network's are untouched)
-// and keeps the fastest tile in st->ft.
+// Times one launch of k_vertex_big with every VB_TILES tile (and, for FP32,
+// the MFMA kernel with every VBM_TILES count of tiles in flight) on scratch
+// outputs (messages, statistics and eigenvector seeds; the network's are
+// untouched) and keeps the fastest in st->ft.
 template <typename TS, typename TA, typename TM>
 int autotune_big(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, int normalize) {
   itnbp_ctx* ctx = net->ctx;
@@ -772,9 +1014,11 @@ int autotune_big(itnbp_net* net, SmallState* st, const void* psi, const void* ms
     return rc;
   }
   float best = INFINITY;
-  int best_ft = st->ft;
+  int best_ft = big_variant<TA>(st);
+  std::vector<int> cand(VB_TILES, VB_TILES + 3);
+  if (sizeof(TA) == 4) cand.insert(cand.end(), VBM_TILES, VBM_TILES + 3);
   for (int rep = 0; rep < 2; ++rep)  // the first round warms the code and caches
-    for (int ft : VB_TILES) {
+    for (int ft : cand) {
       hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
       if (e == hipSuccess)
         e = launch_big_ft<TS, TA, TM>(ft, net, st, psi, msg_in, out.p, (MsgStat*)stats.p, normalize,
@@ -794,6 +1038,7 @@ int autotune_big(itnbp_net* net, SmallState* st, const void* psi, const void* ms
     }
   cleanup();
   st->ft = best_ft;
+  st->ft_f32 = sizeof(TA) == 4;
   return ITNBP_OK;
 }
 
@@ -926,6 +1171,14 @@ int small_autotune(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, 
 int small_big_tile(itnbp_net* net) {
   SmallState* st = small_of(net);
   return st && !st->big.empty() ? st->ft : 0;
+}
+
+const char* small_big_kind(itnbp_net* net) {
+  SmallState* st = small_of(net);
+  if (!st || st->big.empty() || st->ft == 0) return "";
+  for (int t : VBM_TILES)
+    if (st->ft_f32 && st->ft == t) return "mfma";
+  return "valu";
 }
 
 int small_task_count(itnbp_net* net) {

@@ -97,3 +97,27 @@ def test_graph_loop_matches_the_host_loop(monkeypatch, kw):
     for r_g, r_h in zip(rg, rh):
         assert r_g["mean_diff"] == pytest.approx(r_h["mean_diff"], rel=1e-6 if f64 else 1e-2,
                                                  abs=1e-14 if f64 else 1e-7)
+
+
+@pytest.mark.parametrize("tile", ["1", "2", "4", "192"])
+def test_large_site_kernel_variants_match_the_oracle(monkeypatch, tile):
+    """Every variant of the large-site kernel (k_vertex_big: sites whose tensor
+    does not fit in LDS), forced through ITNBP_VERTEX_BIG_TILE: the FP32 MFMA
+    kernel with 1, 2 or 4 tiles in flight per wave and the VALU kernel with a
+    192-fibre Gram tile, on the chi = 16 degree-4 centre of a 3x3 grid and the
+    four chi = 12 degree-4 sites of a 4x4 grid, FP32, against the oracle."""
+    monkeypatch.setenv("ITNBP_VERTEX_BIG_TILE", tile)
+    rng = np.random.default_rng(17)
+    inst = []
+    for g, chi in [(itnbp.named_grid((3, 3)), 16), (itnbp.named_grid((4, 4)), 12)]:
+        inst.append((g, itnbp.random_tensornetwork(rng, np.float32, g, link_space=chi, scale=1 / np.sqrt(chi))))
+    res, got = run_batched(inst, monkeypatch, True, maxiter=12, precision="f32")
+    assert "[vertex " in res[0]["path"], res[0]["path"]
+    for r, m, (g, psi) in zip(res, got, inst):
+        og, opsi = oracle_graph(g), oracle_psi(psi)
+        ref, _, _, _ = O.bp_update(og, opsi, O.identity_messages(og, opsi), maxiter=r["iterations"],
+                                   schedule="parallel")
+        err = max(np.abs(m[e] - ref[e]).max() for e in ref)
+        print(f"tile {tile}: {g.nv} sites, {r['iterations']} sweeps, max abs err {err:.2e}, path {r['path']}",
+              flush=True)
+        assert err < 2e-5
