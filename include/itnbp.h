@@ -205,7 +205,7 @@ int itnbp_apply_site_gates(itnbp_net* net, int n, const int* verts, const double
                            double* norms);
 /* itnbp_apply_gates without any device allocation: the launch items, gate
  * matrices, per-gate workspace and per-gate outputs live in one file-scope
- * device array of the library (16.4 MiB per GPU, held until the process
+ * device array of the library (18.4 MiB per GPU, held until the process
  * ends).  No counterpart in the reference: for callers that cannot allocate
  * device memory (code under stream capture, tight serving loops).  Limits:
  * n <= 128 gates, (d_u d_v)^2 <= 256 elements per gate, max_keep <= 32;

@@ -18,6 +18,8 @@
 // SVD in LDS.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -27,6 +29,10 @@
 namespace {
 
 constexpr int AP_XY = 8192;   // max E * n (external size x d chi_b) per side
+// tall-skinny QR (E >= 4 n, n <= 16): the stacked R factors (4n x n) and the
+// heads and betas of their reflectors, per side, after the side's reflectors
+constexpr int AP_QRX = 4 * 16 * 16 + 4 * 16;
+constexpr int AP_SIDE = AP_XY + AP_QRX;  // global scratch per side of a gate
 constexpr int AP_TH = 32;     // max rows/cols of Theta (r d)
 constexpr int AP_ENV = 8;     // max external bond dimension
 constexpr int AP_MAXEXT = 3;  // max external bonds per side (degree 4)
@@ -46,7 +52,15 @@ struct ApplyParams {
   int normalize;
   double eig_cutoff;  // map_eigvals cutoff (10 eps)
   int max_keep;       // stride of the singular value output
+  // per-phase timestamps (wall clock; 16 slots per gate, slot 8 the SVD's
+  // sweep count), or null; ITNBP_APPLY_PHASES=1
+  unsigned long long* ts = nullptr;
 };
+
+// thread 0 of the workgroup records the end of phase ph (0: start)
+__device__ inline void phase_mark(const ApplyParams& prm, int ph) {
+  if (prm.ts && threadIdx.x == 0) prm.ts[blockIdx.x * 16 + ph] = wall_clock64();
+}
 
 __device__ inline double wave_sum_d(double v) {
   for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
@@ -67,7 +81,12 @@ __device__ double block_sum_d(double v, double* red) {
 // Round-robin pairing: step st of a cyclic sweep over n (even) indices.
 __device__ inline void rr_pair(int n, int st, int i, int& p, int& q) {
   // positions 0..n-1; position 0 fixed, others rotate
-  auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + st) % (n - 1); };
+  // (pos - 1 + st) < 2 (n - 1): one conditional subtraction instead of a modulo
+  auto at = [&](int pos) {
+    if (pos == 0) return 0;
+    const int x = pos - 1 + st;
+    return 1 + (x >= n - 1 ? x - (n - 1) : x);
+  };
   p = at(i);
   q = at(n - 1 - i);
   if (p > q) {
@@ -83,7 +102,7 @@ __device__ inline void rr_pair(int n, int st, int i, int& p, int& q) {
 // round-robin pairs (rr_pair's schedule), every lane deriving the rotations of
 // its row pair and column pair from the pre-step matrix through shuffles, so a
 // step needs no LDS and no barrier; stops when the off-diagonal mass is below
-// 1e-30 of the diagonal's (at most 40 sweeps).  Then the eigenvalue truncation
+// 1e-30 of the diagonal's or stops decreasing (at most 40 sweeps).  Then the eigenvalue truncation
 // of map_eigvals (cutoff = 10 eps, src/apply.jl:9-25), recomputed identically
 // by every lane.  S and Si row-major, ld chi.
 __device__ void env_roots_wave(const double* M, int chi, double cutoff, double* S, double* Si) {
@@ -106,19 +125,28 @@ __device__ void env_roots_wave(const double* M, int chi, double cutoff, double* 
         sn = c * t;
       }
     };
+    double prev = INFINITY;
     for (int sweep = 0; sweep < 40; ++sweep) {
       double off = (in && i != j) ? a * a : 0.0, diag = (in && i == j) ? a * a : 0.0;
       for (int m = 32; m > 0; m >>= 1) {
         off += __shfl_xor(off, m);
         diag += __shfl_xor(diag, m);
       }
-      if (off <= 1e-30 * diag || off == 0.0) break;
+      // converged, or the off-diagonal mass no longer decreases (rounding level)
+      if (off <= 1e-30 * diag || off == 0.0 || off >= prev) break;
+      prev = off;
       for (int st = 0; st < ne - 1; ++st) {
         // partner of index x in step st of the round-robin schedule (rr_pair)
+        // (x - 1 - st) lies in (-(ne - 1), ne - 1) and (pp - 1 + st) in [0, 2 (ne - 1)):
+        // conditional corrections instead of modulos
         auto partner = [&](int x) {
-          const int pos = x == 0 ? 0 : 1 + ((x - 1 - st) % (ne - 1) + (ne - 1)) % (ne - 1);
+          int t = x - 1 - st;
+          if (t < 0) t += ne - 1;
+          const int pos = x == 0 ? 0 : 1 + t;
           const int pp = ne - 1 - pos;
-          return pp == 0 ? 0 : 1 + (pp - 1 + st) % (ne - 1);
+          int u = pp - 1 + st;
+          if (u >= ne - 1) u -= ne - 1;
+          return pp == 0 ? 0 : 1 + u;
         };
         const int ri = partner(i), cj = partner(j);
         double cr, sr, cc, sc;
@@ -135,27 +163,39 @@ __device__ void env_roots_wave(const double* M, int chi, double cutoff, double* 
   double w[AP_ENV];
 #pragma unroll
   for (int m = 0; m < AP_ENV; ++m) w[m] = __shfl(a, (m < ne ? m : 0) * ne + (m < ne ? m : 0));
-  // truncation: sort |lambda| descending; drop from the small end while the
-  // dropped weight / total <= cutoff (keep >= 1)
-  int order[AP_ENV];
-  for (int m = 0; m < chi; ++m) order[m] = m;
-  for (int x = 0; x < chi; ++x)
-    for (int y = x + 1; y < chi; ++y)
-      if (fabs(w[order[y]]) > fabs(w[order[x]])) {
-        const int t = order[x];
-        order[x] = order[y];
-        order[y] = t;
-      }
+  // truncation: rank |lambda| descending (ties by index); drop from the small
+  // end while the dropped weight / total <= cutoff (keep >= 1).  Fully
+  // unrolled over AP_ENV, with no run-time index into w, so that w stays in
+  // registers.
+  int rank[AP_ENV];
   double tot = 0;
-  for (int m = 0; m < chi; ++m) tot += fabs(w[m]);
+#pragma unroll
+  for (int m = 0; m < AP_ENV; ++m) {
+    int r = 0;
+#pragma unroll
+    for (int m2 = 0; m2 < AP_ENV; ++m2)
+      if (m2 < chi && (fabs(w[m2]) > fabs(w[m]) || (fabs(w[m2]) == fabs(w[m]) && m2 < m))) ++r;
+    rank[m] = r;
+    if (m < chi) tot += fabs(w[m]);
+  }
+  // the value of rank r, and the dropped prefix from the small end
+  auto by_rank = [&](int r) {
+    double x = 0;
+#pragma unroll
+    for (int m = 0; m < AP_ENV; ++m)
+      if (m < chi && rank[m] == r) x = fabs(w[m]);
+    return x;
+  };
   int keep = chi;
   double disc = 0;
-  while (keep > 1 && tot > 0 && (disc + fabs(w[order[keep - 1]])) / tot <= cutoff) {
-    disc += fabs(w[order[keep - 1]]);
+  while (keep > 1 && tot > 0 && (disc + by_rank(keep - 1)) / tot <= cutoff) {
+    disc += by_rank(keep - 1);
     --keep;
   }
   unsigned kmask = 0;
-  for (int m = 0; m < keep; ++m) kmask |= 1u << order[m];
+#pragma unroll
+  for (int m = 0; m < AP_ENV; ++m)
+    if (m < chi && rank[m] < keep) kmask |= 1u << m;
   // S[r][c] = sum_m V[r][m] V[c][m] sqrt(l_m) over the kept m (lane (r, c) = (i, j))
   double sv = 0, siv = 0;
   for (int m = 0; m < AP_ENV; ++m) {
@@ -173,12 +213,28 @@ __device__ void env_roots_wave(const double* M, int chi, double cutoff, double* 
   }
 }
 
+// Exact x / d and x % d by a multiply-high with m = ceil(2^32 / d), valid for
+// x d < 2^32 (every index here is below 2^16, every divisor below 2^14): the
+// integer divisions of the index arithmetic otherwise cost ~30 instructions
+// each, and the flat element loops of this kernel run one wave per SIMD with
+// nothing to hide them behind.
+struct FastDiv {  // an aggregate, so that a SideGeom can live in LDS
+  unsigned d;
+  unsigned long long m;
+  __device__ unsigned div(unsigned x) const { return (unsigned)(((unsigned long long)x * m) >> 32); }
+  __device__ unsigned mod(unsigned x) const { return x - div(x) * d; }
+};
+__device__ inline FastDiv fast_div(unsigned d) { return FastDiv{d, ((1ull << 32) + d - 1) / d}; }
+
 struct SideGeom {
   int d, k, j, chib, E, n, r;
   int dims[ITNBP_MAXR];
   int ext[AP_MAXEXT];  // external axes in order
   int next;
   long long psi_off;
+  FastDiv fE, fd, fdim[ITNBP_MAXR];
+  // external axis pa: stride of its index in e, its dimension, and their product
+  FastDiv fst[AP_MAXEXT], fchi[AP_MAXEXT];
 };
 
 __device__ SideGeom side_geom(const VDesc& V, int j) {
@@ -198,8 +254,21 @@ __device__ SideGeom side_geom(const VDesc& V, int j) {
   g.n = g.d * g.chib;
   g.r = g.E < g.n ? g.E : g.n;
   g.psi_off = V.psi_off;
+  g.fE = fast_div(g.E);
+  g.fd = fast_div(g.d);
+  for (int a = 0; a < ITNBP_MAXR; ++a) g.fdim[a] = fast_div(a <= V.k ? V.dims[a] : 1);
+  int st = 1;
+  for (int pa = 0; pa < AP_MAXEXT; ++pa) {
+    const int chi = pa < g.next ? V.dims[g.ext[pa]] : 1;
+    g.fst[pa] = fast_div(st);
+    g.fchi[pa] = fast_div(chi);
+    st *= chi;
+  }
   return g;
 }
+
+// the tall-skinny QR applies: four row blocks of at least n rows, n <= 16
+__device__ inline bool tsqr_ok(const SideGeom& g) { return g.n <= 16 && g.E >= 4 * g.n; }
 
 // natural offset of psi element (s, external multi-index e, bond b)
 __device__ inline int nat_off(const SideGeom& g, int s, int e, int b) {
@@ -209,8 +278,9 @@ __device__ inline int nat_off(const SideGeom& g, int s, int e, int b) {
     if (a == g.j) {
       x = b;
     } else {
-      x = rem % g.dims[a];
-      rem /= g.dims[a];
+      const int q = (int)g.fdim[a].div(rem);
+      x = rem - q * g.dims[a];
+      rem = q;
     }
     off += stride * x;
     stride *= g.dims[a];
@@ -222,17 +292,19 @@ __device__ inline int nat_off(const SideGeom& g, int s, int e, int b) {
 // Y[e with b'] = sum_b X[e with b] * Mt[b][b'] (Mt row-major, ld chi)
 __device__ void ext_mode_product(const SideGeom& g, int pa, const double* X, double* Y, int cols,
                                  const double* Mt, bool transpose) {
-  int stride = 1;
-  for (int i = 0; i < pa; ++i) stride *= g.dims[g.ext[i]];
+  const int stride = (int)g.fst[pa].d;
   const int chi = g.dims[g.ext[pa]];
   for (int idx = threadIdx.x; idx < g.E * cols; idx += blockDim.x) {
-    const int e = idx % g.E, c = idx / g.E;
-    const int lo = e % stride, b2 = (e / stride) % chi, hi = e / (stride * chi);
+    const int c = (int)g.fE.div(idx), e = idx - c * g.E;
+    const int q = (int)g.fst[pa].div(e), lo = e - q * stride;
+    const int hi = (int)g.fchi[pa].div(q), b2 = q - hi * chi;
     double acc = 0;
-    for (int b = 0; b < chi; ++b) {
-      const double m = transpose ? Mt[b2 * chi + b] : Mt[b * chi + b2];
-      acc += X[(lo + stride * (b + chi * hi)) + (long long)g.E * c] * m;
-    }
+#pragma unroll
+    for (int b = 0; b < AP_ENV; ++b)
+      if (b < chi) {
+        const double m = transpose ? Mt[b2 * chi + b] : Mt[b * chi + b2];
+        acc += X[(lo + stride * (b + chi * hi)) + (long long)g.E * c] * m;
+      }
     Y[idx] = acc;
   }
   __syncthreads();
@@ -302,18 +374,156 @@ __device__ void apply_q(const double* X, int E, int r, const double* vh, const d
   }
 }
 
+// Tall-skinny QR of a side (E x n, E >= 4 n, n <= 16), replacing the
+// workgroup-wide Householder QR there: each wave factors its quarter of the
+// rows (rows [w E / 4, (w + 1) E / 4)) with no workgroup barrier, wave 0 then
+// factors the four stacked n x n R factors (4n x n), whose R is the side's.
+// Q = diag(Q_0 .. Q_3) P Q_S (P places stacked row w n + i at block row i of
+// block w), so X = Q [R; 0] as for one Householder QR, up to the signs of R's
+// rows, which the rebuild's Q absorbs.
+//
+// Householder QR of rows x n (n <= 16) at X (column stride ld) by one wave:
+// lane (rg, c) = (l / 16, l % 16) works on rows i = rg (mod 4) of column c,
+// partial sums joined across rg by two shuffles; reflector j is
+// [vh[j]; X[j+1 .., j]] with beta[j], R in the upper triangle.  Every element
+// is read and written by its own lane, so only the diagonal element (read by
+// every lane) needs the wave to be in step.
+__device__ void wave_qr16(double* X, int ld, int rows, int n, double* vh, double* beta) {
+  const int l = threadIdx.x & 63, rg = l >> 4, c = l & 15;
+  for (int j = 0; j < n; ++j) {
+    const int i0 = j + 1 + ((rg - ((j + 1) & 3)) & 3);  // first row > j with i = rg (mod 4)
+    // four rows per iteration with independent partial sums, so that the LDS
+    // loads of an iteration are in flight together (one wave per SIMD here:
+    // nothing else hides their latency)
+    double p = 0;
+    if (c < n) {
+      const double* xj = X + (long long)ld * j;
+      const double* xc = X + (long long)ld * c;
+      double p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+      int i = i0;
+      for (; i + 12 < rows; i += 16) {
+        p0 += xj[i] * xc[i];
+        p1 += xj[i + 4] * xc[i + 4];
+        p2 += xj[i + 8] * xc[i + 8];
+        p3 += xj[i + 12] * xc[i + 12];
+      }
+      for (; i < rows; i += 4) p0 += xj[i] * xc[i];
+      p = (p0 + p1) + (p2 + p3);
+    }
+    p += __shfl_xor(p, 16);
+    p += __shfl_xor(p, 32);
+    const double tail = __shfl(p, j);  // sum_{i > j} X[i][j]^2
+    const double xjj = X[j + (long long)ld * j];
+    const double s = xjj * xjj + tail;
+    const double nrm = sqrt(s);
+    const double alpha = xjj >= 0 ? -nrm : nrm;
+    const double v0 = xjj - alpha;
+    const double vv = tail + v0 * v0;
+    const double b = (nrm == 0.0 || vv == 0.0) ? 0.0 : 2.0 / vv;
+    if (c > j && c < n) {
+      const double dot = v0 * X[j + (long long)ld * c] + p;
+      const double f = b * dot;
+      if (rg == 0) X[j + (long long)ld * c] -= f * v0;
+      const double* xj = X + (long long)ld * j;
+      double* xc = X + (long long)ld * c;
+      int i = i0;
+      for (; i + 12 < rows; i += 16) {
+        const double a0 = xj[i], a1 = xj[i + 4], a2 = xj[i + 8], a3 = xj[i + 12];
+        const double c0 = xc[i], c1 = xc[i + 4], c2 = xc[i + 8], c3 = xc[i + 12];
+        xc[i] = c0 - f * a0;
+        xc[i + 4] = c1 - f * a1;
+        xc[i + 8] = c2 - f * a2;
+        xc[i + 12] = c3 - f * a3;
+      }
+      for (; i < rows; i += 4) xc[i] -= f * xj[i];
+    }
+    if (l == 0) {
+      vh[j] = v0;
+      beta[j] = b;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (l == 0) X[j + (long long)ld * j] = (b == 0.0) ? xjj : alpha;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Y[rows(i), c] <- H_0 .. H_{n-1} Y for c < cols (cols <= 16 per pass) by one
+// wave: reflector j is [vh[j]; V[j+1 .., j]] (column stride ldv) over the
+// logical rows i >= j, logical row i being Y row rowof(i); lane (rg, c) owns
+// logical rows i = rg (mod 4) of column c, and lane (0, c) the head row j.
+template <typename RowOf>
+__device__ void wave_apply_q16(const double* V, int ldv, int rows, int n, const double* vh, const double* beta,
+                               double* Y, int ldy, int cols, RowOf rowof) {
+  const int l = threadIdx.x & 63, rg = l >> 4, c16 = l & 15;
+  for (int c0 = 0; c0 < cols; c0 += 16) {
+    const int c = c0 + c16;
+    for (int j = n - 1; j >= 0; --j) {
+      const double b = beta[j];
+      if (b == 0.0) continue;
+      const int i0 = j + 1 + ((rg - ((j + 1) & 3)) & 3);
+      double p = 0;
+      const double* vj = V + (long long)ldv * j;
+      double* yc = Y + (long long)ldy * (c < cols ? c : 0);
+      if (c < cols) {
+        if (rg == 0) p = vh[j] * yc[rowof(j)];
+        double p1 = 0, p2 = 0, p3 = 0;
+        int i = i0;
+        for (; i + 12 < rows; i += 16) {
+          p += vj[i] * yc[rowof(i)];
+          p1 += vj[i + 4] * yc[rowof(i + 4)];
+          p2 += vj[i + 8] * yc[rowof(i + 8)];
+          p3 += vj[i + 12] * yc[rowof(i + 12)];
+        }
+        for (; i < rows; i += 4) p += vj[i] * yc[rowof(i)];
+        p = (p + p1) + (p2 + p3);
+      }
+      p += __shfl_xor(p, 16);
+      p += __shfl_xor(p, 32);
+      if (c < cols) {
+        const double f = b * p;
+        if (rg == 0) yc[rowof(j)] -= f * vh[j];
+        int i = i0;
+        for (; i + 12 < rows; i += 16) {
+          const double a0 = vj[i], a1 = vj[i + 4], a2 = vj[i + 8], a3 = vj[i + 12];
+          const double y0 = yc[rowof(i)], y1 = yc[rowof(i + 4)], y2 = yc[rowof(i + 8)], y3 = yc[rowof(i + 12)];
+          yc[rowof(i)] = y0 - f * a0;
+          yc[rowof(i + 4)] = y1 - f * a1;
+          yc[rowof(i + 8)] = y2 - f * a2;
+          yc[rowof(i + 12)] = y3 - f * a3;
+        }
+        for (; i < rows; i += 4) yc[rowof(i)] -= f * vj[i];
+      }
+      // row j - 1, the next head, belongs to another lane
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
 // One-sided Jacobi SVD of A (M x N, column-major, M >= N) in place:
-// A <- U diag(s), V (N x N column-major) accumulated.
-__device__ void jacobi_svd(double* A, int M, int N, double* V, double* rot, double* red) {
+// A <- U diag(s), V (N x N column-major) accumulated.  A pair is rotated while
+// |<a_p, a_q>| > tol |a_p| |a_q| with tol = 8 eps sqrt(M), the rounding error
+// of the inner product itself, and while both columns are above the rounding
+// level of A (|a|^2 > eps^2 ||A||_F^2): a column at that level is zero to
+// working precision, and rotating it against its rounding noise would keep
+// the sweeps going.  The pairs of a step are disjoint and each group of G
+// threads reads and writes only its own rows of its pair, so a step needs one
+// barrier (before the next step's pairing).  Returns the number of sweeps.
+__device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, double* red) {
   const int tid = threadIdx.x;
   for (int i = tid; i < N * N; i += blockDim.x) V[i] = (i % N == i / N) ? 1.0 : 0.0;
-  __syncthreads();
-  if (N == 1) return;
+  double fro = 0;
+  for (int i = tid; i < M * N; i += blockDim.x) fro += A[i] * A[i];
+  fro = block_sum_d(fro, red);
+  if (N == 1) return 0;
   const int ne = N + (N & 1);
-  __shared__ int changed;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    if (tid == 0) changed = 0;
-    __syncthreads();
+  const double tol = 8 * 2.220446049250313e-16 * sqrt((double)M);
+  const double floor2 = 2.220446049250313e-16 * 2.220446049250313e-16 * fro;
+  int sweep = 0;
+  for (; sweep < 60; ++sweep) {
+    int rotated = 0;
     for (int st = 0; st < ne - 1; ++st) {
       // each pair handled by a group of G threads
       const int npair = ne / 2;
@@ -339,14 +549,13 @@ __device__ void jacobi_svd(double* A, int M, int N, double* V, double* rot, doub
         gamma += __shfl_xor(gamma, m);
       }
       double c = 1, s = 0;
-      if (active && q < N && fabs(gamma) > 1e-15 * sqrt(alpha * beta) && gamma != 0.0) {
+      if (active && q < N && alpha > floor2 && beta > floor2 && fabs(gamma) > tol * sqrt(alpha * beta)) {
         const double zeta = (beta - alpha) / (2 * gamma);
         const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
         c = 1 / sqrt(1 + t * t);
         s = c * t;
-        if (gl == 0) changed = 1;
+        rotated = 1;
       }
-      __syncthreads();
       if (active && q < N && s != 0.0) {
         for (int i = gl; i < M; i += G) {
           const double ap = A[i + (long long)M * p], aq = A[i + (long long)M * q];
@@ -361,8 +570,11 @@ __device__ void jacobi_svd(double* A, int M, int N, double* V, double* rot, doub
       }
       __syncthreads();
     }
-    if (!changed) break;
+    // a barrier that also tells every thread whether any pair rotated (no
+    // shared flag to reset, so no thread can see the next sweep's reset)
+    if (!__syncthreads_or(rotated)) break;
   }
+  return sweep + 1;
 }
 
 __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict__ items,
@@ -398,15 +610,22 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   const int tid = threadIdx.x;
   // the factorisation runs on the bond as it was (columns b < chib_in); a bond
   // grown for the kept rank only widens the layout the result is written to
-  SideGeom gu = side_geom(vds[it.u], it.ju);
-  SideGeom gv = side_geom(vds[it.v], it.jv);
-  for (SideGeom* g : {&gu, &gv}) {
-    g->chib = it.chib_in;
-    g->n = g->d * g->chib;
-    g->r = g->E < g->n ? g->E : g->n;
+  // both sides' geometry in LDS: its arrays are indexed at run time, which
+  // would put a per-thread copy in scratch memory
+  __shared__ SideGeom sgeo[2];
+  if (tid < 2) {
+    SideGeom g = side_geom(vds[tid == 0 ? it.u : it.v], tid == 0 ? it.ju : it.jv);
+    g.chib = it.chib_in;
+    g.n = g.d * g.chib;
+    g.r = g.E < g.n ? g.E : g.n;
+    sgeo[tid] = g;
   }
-  double* scr_u = work + (long long)blockIdx.x * 2 * AP_XY;
-  double* scr_v = scr_u + AP_XY;
+  __syncthreads();
+  const SideGeom& gu = sgeo[0];
+  const SideGeom& gv = sgeo[1];
+  double* scr_u = work + (long long)blockIdx.x * 2 * AP_SIDE;
+  double* scr_v = scr_u + AP_SIDE;
+  phase_mark(prm, 0);
 
   // 1. environment roots (sqrt and inverse sqrt), both sides, one environment per wave
   {
@@ -422,15 +641,17 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     }
     __syncthreads();
   }
+  phase_mark(prm, 1);
   // 2. X = psi with sqrt(env) on the external bonds, QR; reflectors to scratch
   for (int side = 0; side < 2; ++side) {
     const SideGeom& g = side == 0 ? gu : gv;
     for (int idx = tid; idx < g.E * g.n; idx += blockDim.x) {
-      const int e = idx % g.E, c = idx / g.E;
-      const int s = c % g.d, b = c / g.d;
+      const int c = (int)g.fE.div(idx), e = idx - c * g.E;
+      const int b = (int)g.fd.div(c), s = c - b * g.d;
       X[idx] = psi[g.psi_off + nat_off(g, s, e, b)];
     }
     __syncthreads();
+    if (side == 0) phase_mark(prm, 10);
     double* cur = X;
     double* oth = Y;
     for (int pa = 0; pa < g.next; ++pa) {
@@ -446,16 +667,43 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     }
     double* vhs = side == 0 ? vh : vhv;
     double* bts = side == 0 ? beta : betav;
-    householder_qr(X, g.E, g.n, vhs, bts, red);
     double* R = side == 0 ? Ru : Rv;
-    for (int idx = tid; idx < g.r * g.n; idx += blockDim.x) {
-      const int i = idx % g.r, c = idx / g.r;
-      R[i + g.r * c] = (i <= c) ? X[i + (long long)g.E * c] : 0.0;  // R[i][c], column-major (ld r)
-    }
     double* scr = side == 0 ? scr_u : scr_v;
+    if (side == 0) phase_mark(prm, 11);
+    if (tsqr_ok(g)) {
+      // the stacked R factors S (4n x n) in Y, which is free after the copy to X
+      const int n4 = 4 * g.n, wv = tid >> 6;
+      double* S = Y;
+      if (wv < 4) {
+        const int r0 = wv * g.E / 4, r1 = (wv + 1) * g.E / 4;
+        wave_qr16(X + r0, g.E, r1 - r0, g.n, vhs + wv * g.n, bts + wv * g.n);
+        for (int idx = (tid & 63); idx < g.n * g.n; idx += 64) {
+          const int i = idx % g.n, c = idx / g.n;
+          S[wv * g.n + i + n4 * c] = (i <= c) ? X[r0 + i + (long long)g.E * c] : 0.0;
+        }
+      }
+      __syncthreads();
+      double* qx = scr + AP_XY;  // S's reflectors, then their heads and betas
+      if (wv == 0) wave_qr16(S, n4, n4, g.n, qx + n4 * g.n, qx + n4 * g.n + 16);
+      __syncthreads();
+      for (int idx = tid; idx < g.r * g.n; idx += blockDim.x) {
+        const int i = idx % g.r, c = idx / g.r;
+        R[i + g.r * c] = (i <= c) ? S[i + n4 * c] : 0.0;
+      }
+      for (int i = tid; i < n4 * g.n; i += blockDim.x) qx[i] = S[i];
+    } else {
+      householder_qr(X, g.E, g.n, vhs, bts, red);
+      for (int idx = tid; idx < g.r * g.n; idx += blockDim.x) {
+        const int i = idx % g.r, c = idx / g.r;
+        R[i + g.r * c] = (i <= c) ? X[i + (long long)g.E * c] : 0.0;  // R[i][c], column-major (ld r)
+      }
+    }
+    if (side == 0) phase_mark(prm, 12);
     for (int i = tid; i < g.E * g.n; i += blockDim.x) scr[i] = X[i];
     __syncthreads();
+    if (side == 0) phase_mark(prm, 13);
   }
+  phase_mark(prm, 2);
   // 3. Theta[(ru, su'), (rv, sv')] = sum G[(su',sv'),(su,sv)] Ru[ru, su + du b] Rv[rv, sv + dv b],
   //    stored directly in the taller orientation (X: M x N column-major), V in X + 1024
   const int du = gu.d, dv = gv.d, chib = gu.chib;
@@ -483,8 +731,13 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     else A[idx] = acc;
   }
   __syncthreads();
+  phase_mark(prm, 3);
   // 4. SVD
-  jacobi_svd(A, M, N, VV, rot, red);
+  const int nsweeps = jacobi_svd(A, M, N, VV, rot, red);
+  if (prm.ts && tid == 0) {
+    prm.ts[blockIdx.x * 16 + 8] = (unsigned long long)nsweeps;
+    prm.ts[blockIdx.x * 16 + 9] = (unsigned long long)(tsqr_ok(gu) + tsqr_ok(gv));
+  }
   for (int j = tid; j < N; j += blockDim.x) {
     double s = 0;
     for (int i = 0; i < M; ++i) s += A[i + M * j] * A[i + M * j];
@@ -524,6 +777,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     sh_keep = st ? -1 : keep;
   }
   __syncthreads();
+  phase_mark(prm, 4);
   const int keep = sh_keep;
   // a bond whose kept rank exceeds its dimension, or whose Theta is not
   // finite, is left untouched (the host reports the error); every write to psi
@@ -544,6 +798,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     Rv[idx] = !tr ? VV[i + N * o] * sqrt(sv) : (sv > 0 ? A[i + M * o] / sqrt(sv) : 0.0);
   }
   __syncthreads();
+  phase_mark(prm, 5);
   // 5. rebuild both sides: Y = [F as r x (d keep); 0], apply Q, then M^{-1/2};
   //    the largest-magnitude entry of every new-bond slice of psi_u' (first in
   //    column-major order) is made positive and the same sign applied to psi_v'
@@ -555,12 +810,34 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = scr[i];
     const int cols = g.d * keep;  // column c' = s + d b'
     for (int idx = tid; idx < g.E * cols; idx += blockDim.x) {
-      const int e = idx % g.E, c = idx / g.E;
-      const int s = c % g.d, b = c / g.d;
+      const int c = (int)g.fE.div(idx), e = idx - c * g.E;
+      const int b = (int)g.fd.div(c), s = c - b * g.d;
       Y[idx] = e < g.r ? F[(e + g.r * s) + (long long)(g.r * g.d) * b] : 0.0;
     }
     __syncthreads();
-    apply_q(X, g.E, g.r, side == 0 ? vh : vhv, side == 0 ? beta : betav, Y, cols);
+    if (tsqr_ok(g)) {
+      // Q [F; 0] = diag(Q_w) P Q_S [F; 0]: Q_S on the stacked rows (block w's
+      // first n rows) by wave 0, then each block's reflectors by its wave
+      const int n4 = 4 * g.n, wv = tid >> 6, E = g.E, nn = g.n;
+      const double* qx = scr + AP_XY;
+      // stacked row i -> block i / nn, row i % nn (i / nn by a multiply-high: i nn < 2^16)
+      const unsigned inv = (65536u + nn - 1) / nn;
+      if (wv == 0)
+        wave_apply_q16(qx, n4, n4, nn, qx + n4 * nn, qx + n4 * nn + 16, Y, E, cols, [=](int i) {
+          const int w = (int)(((unsigned)i * inv) >> 16);
+          return w * E / 4 + (i - w * nn);
+        });
+      __syncthreads();
+      if (wv < 4) {
+        const int r0 = wv * E / 4, r1 = (wv + 1) * E / 4;
+        const double* vb = side == 0 ? vh : vhv;
+        const double* bb = side == 0 ? beta : betav;
+        wave_apply_q16(X + r0, E, r1 - r0, nn, vb + wv * nn, bb + wv * nn, Y + r0, E, cols, [](int i) { return i; });
+      }
+      __syncthreads();
+    } else {
+      apply_q(X, g.E, g.r, side == 0 ? vh : vhv, side == 0 ? beta : betav, Y, cols);
+    }
     // M^{-1/2}: psi'[.. b ..] = sum_b' Y[.. b' ..] Si[b][b']
     double* cur = Y;
     double* oth = X;
@@ -571,13 +848,16 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       cur = oth;
       oth = t;
     }
-    if (side == 0 && tid < keep) {
-      const int j = tid;
-      double best = -1;
-      int best_off = 0x7fffffff;
-      double val = 0;
-      for (int e = 0; e < g.E; ++e)
-        for (int s2 = 0; s2 < g.d; ++s2) {
+    if (side == 0) {
+      // one wave per new-bond column: the largest |entry| (ties: the smallest
+      // natural offset, so the choice does not depend on the scan order)
+      const int wv = tid >> 6, l = tid & 63, nw = (int)(blockDim.x >> 6);
+      for (int j = wv; j < keep; j += nw) {
+        double best = -1;
+        int best_off = 0x7fffffff;
+        double val = 0;
+        for (int idx = l; idx < g.E * g.d; idx += 64) {
+          const int s2 = (int)g.fE.div(idx), e = idx - s2 * g.E;
           const double x = cur[e + (long long)g.E * (s2 + g.d * j)];
           const int off = nat_off(g, s2, e, j);
           if (fabs(x) > best || (fabs(x) == best && off < best_off)) {
@@ -586,10 +866,20 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
             val = x;
           }
         }
-      bond_sign[j] = val < 0 ? -1.0 : 1.0;
+        for (int m = 32; m > 0; m >>= 1) {
+          const double ob = __shfl_xor(best, m), ov = __shfl_xor(val, m);
+          const int oo = __shfl_xor(best_off, m);
+          if (ob > best || (ob == best && oo < best_off)) {
+            best = ob;
+            best_off = oo;
+            val = ov;
+          }
+        }
+        if (l == 0) bond_sign[j] = val < 0 ? -1.0 : 1.0;
+      }
     }
     __syncthreads();
-    for (int idx = tid; idx < g.E * cols; idx += blockDim.x) cur[idx] *= bond_sign[(idx / g.E) / g.d];
+    for (int idx = tid; idx < g.E * cols; idx += blockDim.x) cur[idx] *= bond_sign[g.fd.div(g.fE.div(idx))];
     __syncthreads();
     double nrm2 = 0;
     for (int i = tid; i < g.E * cols; i += blockDim.x) nrm2 += cur[i] * cur[i];
@@ -597,12 +887,13 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     const double sc = (prm.normalize && nrm2 > 0) ? 1.0 / sqrt(nrm2) : 1.0;
     const int ncol = g.d * g.dims[g.j];  // columns of the bond's layout
     for (int idx = tid; idx < g.E * ncol; idx += blockDim.x) {
-      const int e = idx % g.E, c = idx / g.E;
-      const int s = c % g.d, b = c / g.d;
+      const int c = (int)g.fE.div(idx), e = idx - c * g.E;
+      const int b = (int)g.fd.div(c), s = c - b * g.d;
       const double val = b < keep ? cur[e + (long long)g.E * (s + g.d * b)] * sc : 0.0;
       psi[g.psi_off + nat_off(g, s, e, b)] = val;
     }
     __syncthreads();
+    phase_mark(prm, 6 + side);
   }
 }
 
@@ -722,7 +1013,7 @@ int prepare_apply(itnbp_net* net, int n, const int* edge_ids, int maxdim, double
 }
 
 // Device memory of one apply call: launch items, gate matrices, the per-gate
-// workspace of k_apply_gates (2 AP_XY doubles per gate) and the per-gate outputs.
+// workspace of k_apply_gates (2 AP_SIDE doubles per gate) and the per-gate outputs.
 struct ApplyBufs {
   ApplyItem* items;
   double* gates;
@@ -738,7 +1029,7 @@ void apply_buf_sizes(int n, long long goff, int mk, size_t (&sz)[7]) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   sz[0] = al(n * sizeof(ApplyItem));
   sz[1] = al((size_t)goff * 8);
-  sz[2] = al((size_t)n * 2 * AP_XY * 8);
+  sz[2] = al((size_t)n * 2 * AP_SIDE * 8);
   sz[3] = al((size_t)n * 8);
   sz[4] = al((size_t)n * mk * 8);
   sz[5] = al((size_t)n * 4);
@@ -783,12 +1074,54 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const 
   hipError_t ea = hipFuncSetAttribute((const void*)k_apply_gates, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (ea != hipSuccess)
     return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(ea));
+  DevBuf tsb;
+  std::unique_ptr<DevBuf, void (*)(DevBuf*)> ts_guard(&tsb, [](DevBuf* x) { free_buf(*x); });
+  if (getenv("ITNBP_APPLY_PHASES")) {
+    ITNBP_RETCHK(ensure_buf(ctx, tsb, (size_t)n * 16 * sizeof(unsigned long long)));
+    ITNBP_HIPCHK(ctx, hipMemsetAsync(tsb.p, 0, (size_t)n * 16 * sizeof(unsigned long long), ctx->stream));
+    prm.ts = (unsigned long long*)tsb.p;
+  }
   const int pm0 = prof_mark(ctx);
   hipLaunchKernelGGL(k_apply_gates, dim3(n), dim3(256), lds, ctx->stream, (const ApplyItem*)b.items,
                      net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)b.gates, prm,
                      b.work, b.te, svals ? b.sv : nullptr, b.kept, b.st);
   hipError_t ek = hipGetLastError();
   prof_add(ctx, "apply_gates", pm0, prof_mark(ctx));
+  if (prm.ts && ek == hipSuccess) {
+    // mean duration of every phase over the gates that ran to the end, in us
+    std::vector<unsigned long long> ts((size_t)n * 16);
+    int khz = 0;
+    ek = hipMemcpyAsync(ts.data(), prm.ts, ts.size() * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (ek == hipSuccess) ek = hipStreamSynchronize(ctx->stream);
+    if (ek == hipSuccess) ek = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device);
+    if (ek == hipSuccess && khz > 0) {
+      double mean[7] = {0, 0, 0, 0, 0, 0, 0}, sweeps = 0, tsq = 0;
+      int cnt = 0;
+      for (int i = 0; i < n; ++i) {
+        const unsigned long long* t = &ts[(size_t)i * 16];
+        if (!t[7]) continue;
+        ++cnt;
+        for (int p = 0; p < 7; ++p) mean[p] += (double)(t[p + 1] - t[p]) * 1e3 / khz;
+        sweeps += (double)t[8];
+        tsq += (double)t[9];
+      }
+      const double c = std::max(cnt, 1);
+      double sub[3] = {0, 0, 0}, load = 0;
+      for (int i = 0; i < n; ++i) {
+        const unsigned long long* t = &ts[(size_t)i * 16];
+        if (!t[7] || !t[13]) continue;
+        load += (double)(t[10] - t[1]) * 1e3 / khz;
+        for (int p = 0; p < 3; ++p) sub[p] += (double)(t[11 + p] - t[10 + p]) * 1e3 / khz;
+      }
+      fprintf(stderr, "  side u of qr (us): psi load %.1f, mode products %.1f, factorisation %.1f, save %.1f\n",
+              load / c, sub[0] / c, sub[1] / c, sub[2] / c);
+      fprintf(stderr,
+              "apply phases (us, mean over %d gates): roots %.1f, qr %.1f (%.2f tall-skinny sides), theta %.1f, "
+              "svd %.1f (%.1f sweeps), factors %.1f, rebuild_u %.1f, rebuild_v %.1f\n",
+              cnt, mean[0] / c, mean[1] / c, tsq / c, mean[2] / c, mean[3] / c, sweeps / c, mean[4] / c,
+              mean[5] / c, mean[6] / c);
+    }
+  }
   std::vector<int> st(n);
   std::vector<int> kp(n);
   if (ek == hipSuccess) ek = hipMemcpyAsync(truncerr, b.te, n * 8, hipMemcpyDeviceToHost, ctx->stream);
@@ -820,7 +1153,7 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const 
 // ---- workspace of itnbp_apply_gates_noalloc --------------------------------
 // At most NOALLOC_MAX_GATES gates per call, gate matrices of at most
 // NOALLOC_MAX_GATE_ELEMS elements ((d_u d_v)^2 with d <= 4) and at most
-// NOALLOC_MAX_KEEP singular values per gate: 16.4 MiB in all, held by every
+// NOALLOC_MAX_KEEP singular values per gate: 18.4 MiB in all, held by every
 // GPU the library runs on until the process ends.
 constexpr int NOALLOC_MAX_GATES = 128;
 constexpr int NOALLOC_MAX_GATE_ELEMS = 256;
@@ -828,7 +1161,7 @@ constexpr int NOALLOC_MAX_KEEP = 32;
 
 constexpr size_t noalloc_bytes() {
   return ((NOALLOC_MAX_GATES * sizeof(ApplyItem) + 255) & ~(size_t)255) +
-         (size_t)NOALLOC_MAX_GATES * NOALLOC_MAX_GATE_ELEMS * 8 + (size_t)NOALLOC_MAX_GATES * 2 * AP_XY * 8 +
+         (size_t)NOALLOC_MAX_GATES * NOALLOC_MAX_GATE_ELEMS * 8 + (size_t)NOALLOC_MAX_GATES * 2 * AP_SIDE * 8 +
          (size_t)NOALLOC_MAX_GATES * 8 + (size_t)NOALLOC_MAX_GATES * NOALLOC_MAX_KEEP * 8 +
          2 * (size_t)NOALLOC_MAX_GATES * 4 + 8 * 256;
 }
@@ -1080,12 +1413,12 @@ __device__ void gmm(int m, int n, int k, const double* A, int lda, bool ta, cons
 
 // Y = X with the symmetric chi x chi matrix Mt (row-major) on external axis pa
 __device__ void ext_mp_g(const SideGeom& g, int pa, const double* X, double* Y, int cols, const double* Mt) {
-  int stride = 1;
-  for (int i = 0; i < pa; ++i) stride *= g.dims[g.ext[i]];
+  const int stride = (int)g.fst[pa].d;
   const int chi = g.dims[g.ext[pa]];
   for (int idx = threadIdx.x; idx < g.E * cols; idx += blockDim.x) {
-    const int e = idx % g.E, c = idx / g.E;
-    const int lo = e % stride, b2 = (e / stride) % chi, hi = e / (stride * chi);
+    const int c = (int)g.fE.div(idx), e = idx - c * g.E;
+    const int q = (int)g.fst[pa].div(e), lo = e - q * stride;
+    const int hi = (int)g.fchi[pa].div(q), b2 = q - hi * chi;
     double acc = 0;
     for (int b = 0; b < chi; ++b) acc += X[(lo + stride * (b + chi * hi)) + (long long)g.E * c] * Mt[b * chi + b2];
     Y[idx] = acc;
@@ -1116,18 +1449,27 @@ __device__ unsigned env_eig_wave(const double* M, int chi, double cutoff, double
         sn = c * t;
       }
     };
+    double prev = INFINITY;
     for (int sweep = 0; sweep < 40; ++sweep) {
       double off = (in && i != j) ? a * a : 0.0, diag = (in && i == j) ? a * a : 0.0;
       for (int m = 32; m > 0; m >>= 1) {
         off += __shfl_xor(off, m);
         diag += __shfl_xor(diag, m);
       }
-      if (off <= 1e-30 * diag || off == 0.0) break;
+      // converged, or the off-diagonal mass no longer decreases (rounding level)
+      if (off <= 1e-30 * diag || off == 0.0 || off >= prev) break;
+      prev = off;
       for (int st = 0; st < ne - 1; ++st) {
+        // (x - 1 - st) lies in (-(ne - 1), ne - 1) and (pp - 1 + st) in [0, 2 (ne - 1)):
+        // conditional corrections instead of modulos
         auto partner = [&](int x) {
-          const int pos = x == 0 ? 0 : 1 + ((x - 1 - st) % (ne - 1) + (ne - 1)) % (ne - 1);
+          int t = x - 1 - st;
+          if (t < 0) t += ne - 1;
+          const int pos = x == 0 ? 0 : 1 + t;
           const int pp = ne - 1 - pos;
-          return pp == 0 ? 0 : 1 + (pp - 1 + st) % (ne - 1);
+          int u = pp - 1 + st;
+          if (u >= ne - 1) u -= ne - 1;
+          return pp == 0 ? 0 : 1 + u;
         };
         const int ri = partner(i), cj = partner(j);
         double cr, sr, cc, sc;

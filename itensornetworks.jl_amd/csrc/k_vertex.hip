@@ -693,7 +693,9 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big(const VDesc* __restri
 #define F32_DCOL(l, v) ((l) & 15)
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr size_t VBM_LDS = VB_FIXED + 4 * SV_LD * SV_LD * 8;  // fixed part + Jacobi work / wave partials
+constexpr int VBM_CHUNK = 1024;  // fibres whose base offset and weight are tabulated at once
+// fixed part + Jacobi work / wave partials + the chunk tables (base offsets, weights)
+constexpr size_t VBM_LDS = VB_FIXED + 4 * SV_LD * SV_LD * 8 + (size_t)VBM_CHUNK * 8;
 constexpr int VBM_TILES[3] = {1, 2, 4};                         // MFMA tiles in flight per wave
 constexpr int VBM_FT = 2;
 static_assert(4 * SV_LD * SV_LD * 8 >= 4 * 256 * 8, "the wave partial Grams must fit the Jacobi work area");
@@ -719,6 +721,8 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big_mfma(const VDesc* __r
   double* X = Tp + SV_LD * SV_LD;
   double* red = X + SV_LD * SV_LD;
   double* As = reinterpret_cast<double*>(smem + VB_FIXED);  // Jacobi work (step 2), wave partials (step 4)
+  int* cbase = reinterpret_cast<int*>(smem + VB_FIXED + 4 * SV_LD * SV_LD * 8);  // [VBM_CHUNK]
+  float* cw = reinterpret_cast<float*>(cbase + VBM_CHUNK);                         // [VBM_CHUNK]
 
   __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
   const SmallTask* tk = tasks + blockIdx.x;
@@ -790,32 +794,44 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_big_mfma(const VDesc* __r
     const int n = dims[a + 1], st = strides[a + 1], nn = n * n;
     const int F = D / n;
     double g64[4] = {0, 0, 0, 0};
-    const int ngroups = (F + 3) / 4;  // 4 fibres per MFMA
-    // wave wv takes fibre groups wv, wv + 4, ...; NT groups per iteration, in
-    // runs of 16 MFMAs between FP64 flushes
-    for (int g0 = wv * NT; g0 < ngroups; g0 += 64 * NT) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int gg = g0; gg < min(ngroups, g0 + 64 * NT); gg += 4 * NT) {
-        float y[NT], wy[NT];
+    // fibres in chunks: their base offsets and weights tabulated once per fibre
+    // (not per lane), then wave wv takes the chunk's fibre groups (4 fibres per
+    // MFMA) wv, wv + 4, ...; NT groups per iteration, in runs of 16 MFMAs
+    // between FP64 flushes
+    for (int c0 = 0; c0 < F; c0 += VBM_CHUNK) {
+      const int nc = min(VBM_CHUNK, F - c0);
+      for (int i = tid; i < nc; i += SV_THREADS) {
+        const int f = c0 + i;
+        const int base = (f % st) + (f / st) * st * n;
+        double w = 1.0;
+        for (int b = 0; b < k; ++b)
+          if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
+        cbase[i] = base;
+        cw[i] = (float)w;
+      }
+      __syncthreads();
+      const int ngroups = (nc + 3) / 4;
+      for (int g0 = wv * NT; g0 < ngroups; g0 += 64 * NT) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int gg = g0; gg < min(ngroups, g0 + 64 * NT); gg += 4 * NT) {
+          float y[NT], wy[NT];
 #pragma unroll
-        for (int q = 0; q < NT; ++q) {
-          const int f = (gg + q) * 4 + lk;
-          y[q] = 0.f;
-          wy[q] = 0.f;
-          if (gg + q < ngroups && f < F && li < n) {
-            const int base = (f % st) + (f / st) * st * n;
-            double w = 1.0;
-            for (int b = 0; b < k; ++b)
-              if (b != a) w *= lam[b * SV_MAXCHI + (base / strides[b + 1]) % dims[b + 1]];
-            y[q] = Y[base + li * st];
-            wy[q] = (float)w * y[q];
+          for (int q = 0; q < NT; ++q) {
+            const int fl = (gg + q) * 4 + lk;
+            y[q] = 0.f;
+            wy[q] = 0.f;
+            if (gg + q < ngroups && fl < nc && li < n) {
+              y[q] = Y[cbase[fl] + li * st];
+              wy[q] = cw[fl] * y[q];
+            }
           }
+#pragma unroll
+          for (int q = 0; q < NT; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wy[q], y[q], acc, 0, 0, 0);
         }
 #pragma unroll
-        for (int q = 0; q < NT; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wy[q], y[q], acc, 0, 0, 0);
+        for (int v = 0; v < 4; ++v) g64[v] += (double)acc[v];
       }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) g64[v] += (double)acc[v];
+      __syncthreads();
     }
     // the four waves' partial Grams in LDS (the Jacobi work area), then T'
     double* red4 = As;
