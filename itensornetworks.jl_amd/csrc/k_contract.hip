@@ -121,15 +121,16 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_oc_small(const VDesc* __restric
                                                           MsgStat* __restrict__ stats) {
   extern __shared__ __align__(16) unsigned char smem[];
   const OcTask t = tasks[blockIdx.x];
-  const VDesc V = vds[t.v];
+  const VDesc& V = vds[t.v];  // by reference: a copy indexed at run time lands in scratch
   const OcGeom g = oc_geom(V, t);
   const int D = V.D;
   const int nmax = max_absorbed_dim(V, t.absorb_mask);
   double* red = reinterpret_cast<double*>(smem);  // 16 doubles
-  TA* bufs[2];
-  bufs[0] = reinterpret_cast<TA*>(smem + 128);
-  bufs[1] = bufs[0] + D;
-  TA* Mst = bufs[1] + D;
+  // the two ping-pong buffers, selected by arithmetic rather than a run-time
+  // indexed pointer array (which would live in scratch)
+  TA* const buf0 = reinterpret_cast<TA*>(smem + 128);
+  auto bufp = [&](int i) { return buf0 + (size_t)i * D; };
+  TA* Mst = bufp(2);
   TA* Tb = Mst + nmax * nmax;
   const TP* ps = psi + V.psi_off;
   const int tid = threadIdx.x;
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_oc_small(const VDesc* __restric
     const TM* M = msg_in + V.in_msg[a];
     for (int i = tid; i < n * n; i += blockDim.x) Mst[i] = (TA)M[i];
     __syncthreads();
-    TA* Y = bufs[(cur + 1) & 1];
+    TA* Y = bufp((cur + 1) & 1);
     const int An = A * n;
     for (int i = tid; i < D; i += blockDim.x) {
       const int lo = i % A;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_oc_small(const VDesc* __restric
       if (cur < 0) {
         for (int x = 0; x < n; ++x) acc += load_as<TA>(ps, base + A * x) * Mc[x];
       } else {
-        const TA* X = bufs[cur];
+        const TA* X = bufp(cur);
         for (int x = 0; x < n; ++x) acc += X[base + A * x] * Mc[x];
       }
       Y[i] = acc;
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_oc_small(const VDesc* __restric
   int G = 1;
   while (G * 2 * P2 <= (int)blockDim.x && G < 64) G *= 2;
   const int per = blockDim.x / G;
-  const TA* Phi = cur < 0 ? nullptr : bufs[cur];
+  const TA* Phi = cur < 0 ? nullptr : bufp(cur);
   const int strideHi = g.Aj * g.nj;
   for (int ob = 0; ob < P2; ob += per) {
     const int o = ob + tid / G;
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_mp_level(const VDesc* __restric
                                                           long long stride) {
   __shared__ TA Mst[64 * 64];
   const OcTask t = tasks[blockIdx.y];
-  const VDesc V = vds[t.v];
+  const VDesc& V = vds[t.v];  // by reference: a copy indexed at run time lands in scratch
   const int a = nth_absorbed_axis(V, t.absorb_mask, level);
   if (a == 0) return;
   int A = 1;
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_mp_tiled(const VDesc* __restric
   __shared__ TA Xt[MP_TILE];
   __shared__ TA Mst[64 * 64];
   const OcTask t = tasks[blockIdx.y];
-  const VDesc V = vds[t.v];
+  const VDesc& V = vds[t.v];  // by reference: a copy indexed at run time lands in scratch
   const int a = nth_absorbed_axis(V, t.absorb_mask, level);
   if (a == 0) return;
   int A = 1;
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_oc_partial(const VDesc* __restr
   __shared__ TA tPhi[64 * OC_RC];
   __shared__ TA tPsi[64 * OC_RC];
   const OcTask t = tasks[blockIdx.y];
-  const VDesc V = vds[t.v];
+  const VDesc& V = vds[t.v];  // by reference: a copy indexed at run time lands in scratch
   const OcGeom g = oc_geom(V, t);
   const int nabs = popcount_bonds(t.absorb_mask);
   const TA* S0 = scratch + (long long)blockIdx.y * 2 * stride;
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(ITNBP_BLOCK) void k_oc_finalize(const VDesc* __rest
   __shared__ TA Tb[64 * 64];
   __shared__ double red[8];
   const OcTask t = tasks[blockIdx.x];
-  const VDesc V = vds[t.v];
+  const VDesc& V = vds[t.v];  // by reference: a copy indexed at run time lands in scratch
   const OcGeom g = oc_geom(V, t);
   const int P2 = g.P * g.P;
   const TA* pp = partials + (long long)blockIdx.x * part_stride;

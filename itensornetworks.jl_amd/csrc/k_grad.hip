@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_vertex_vjp(const VDesc* __restrict__ vd
   __shared__ double red[8];
   const GradVertex gv = gvs[blockIdx.x];
   const int u = gv.u;
-  const VDesc V = vds[u];
+  const VDesc& V = vds[u];  // by reference: a copy indexed at run time lands in scratch
   const int D = V.D, k = V.k;
   const int tid = threadIdx.x;
   // LDS: Xs[0..k] (k+1)D, Xb0, Xb1, psibar: 3D, Tb (P^2 <= 4096), Mbar k x chi^2, T
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_vjp_reduce(const VDesc* __restrict__ vd
                                                     const long long* __restrict__ in_msg_off,
                                                     double* __restrict__ grad) {
   const GradVertex gv = gvs[blockIdx.x];
-  const VDesc V = vds[gv.u];
+  const VDesc& V = vds[gv.u];  // by reference: a copy indexed at run time lands in scratch
   const int D = V.D, k = V.k;
   const int nterms = mode == 0 ? k : 1 + k;
   const double* pp = part + (long long)blockIdx.x * nty * pstride;
