@@ -393,7 +393,8 @@ int update_batched(int n, itnbp_net** nets, const itnbp_update_opts* o, itnbp_up
   ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (graph_mode)
     ctx->path = std::string(graphed ? "batched-graph" : "batched-device") + " [vertex " + std::to_string(small_first) +
-                (tile ? std::string(", ") + small_big_kind(M) + " tile " + std::to_string(tile) : "") + "]";
+                (tile ? std::string(", ") + small_big_kind(M) + " tile " + std::to_string(tile) : "") +
+                (small_queue_used(M) ? ", persistent" : "") + "]";
   else
     ctx->path = small_first > 0 ? "batched-generic [vertex " + std::to_string(small_first) + "]" : "batched-generic";
   return ITNBP_OK;

@@ -67,6 +67,8 @@ int small_autotune(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, 
 int small_big_tile(itnbp_net* net);
 // "mfma" or "valu" for the autotuned large-site kernel ("" before tuning)
 const char* small_big_kind(itnbp_net* net);
+// the LDS-resident sites run as the persistent queue kernel
+bool small_queue_used(itnbp_net* net);
 int small_sweep(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, void* msg_out,
                 MsgStat* stats, int normalize);
 void release_small_state(itnbp_net* net);

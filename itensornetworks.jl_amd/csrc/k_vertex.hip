@@ -211,14 +211,16 @@ constexpr size_t SV_FIXED = (size_t)ITNBP_MAXK * SV_LD * SV_LD * 8   // U_c
                             + 2 * SV_LD * SV_LD * 8                   // T', U T'
                             + 64;                                     // block reduction
 
+// One site's update (the task tk) by the whole workgroup; k_vertex_eig runs
+// one task per workgroup, k_vertex_eig_queue a persistent grid that takes
+// tasks from a queue.
 template <typename TS, typename TA, typename TM>
-__global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restrict__ vdesc,
-                                                          const SmallTask* __restrict__ tasks,
-                                                          const TS* __restrict__ psi, const TM* msg_in,
-                                                          TM* msg_out, MsgStat* __restrict__ stats,
-                                                          int normalize, int fmax, double* __restrict__ seeds,
-                                                          int warm, const int* __restrict__ active,
-                                                          const long long* __restrict__ sweep_ctr) {
+__device__ __forceinline__ void vertex_eig_task(const VDesc* __restrict__ vdesc, const SmallTask* __restrict__ tk,
+                                                const TS* __restrict__ psi, const TM* msg_in, TM* msg_out,
+                                                MsgStat* __restrict__ stats, int normalize, int fmax,
+                                                double* __restrict__ seeds, int warm,
+                                                const int* __restrict__ active,
+                                                const long long* __restrict__ sweep_ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* Us = reinterpret_cast<double*>(smem);
   double* As = Us + ITNBP_MAXK * SV_LD * SV_LD;
@@ -236,7 +238,6 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
   TA* Y = reinterpret_cast<TA*>(smem + SV_FIXED + (((size_t)fmax * 12 + 15) & ~(size_t)15));
 
   __shared__ int dims[ITNBP_MAXR], strides[ITNBP_MAXR];
-  const SmallTask* tk = tasks + blockIdx.x;
   // batched update: a finished instance's messages were carried over already
   if (active && !active[tk->inst]) return;
   if (sweep_ctr) warm = (*sweep_ctr % 16) != 0;
@@ -451,6 +452,50 @@ __global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restri
     }
     __syncthreads();
   }
+}
+
+template <typename TS, typename TA, typename TM>
+__global__ __launch_bounds__(SV_THREADS) void k_vertex_eig(const VDesc* __restrict__ vdesc,
+                                                          const SmallTask* __restrict__ tasks,
+                                                          const TS* __restrict__ psi, const TM* msg_in,
+                                                          TM* msg_out, MsgStat* __restrict__ stats,
+                                                          int normalize, int fmax, double* __restrict__ seeds,
+                                                          int warm, const int* __restrict__ active,
+                                                          const long long* __restrict__ sweep_ctr) {
+  vertex_eig_task<TS, TA, TM>(vdesc, tasks + blockIdx.x, psi, msg_in, msg_out, stats, normalize, fmax, seeds, warm,
+                              active, sweep_ctr);
+}
+
+// Persistent form: a grid of resident workgroups takes the (cost-sorted) tasks
+// in order from a device counter that k_queue_reset clears before the launch;
+// every workgroup leaves once the counter has passed ntasks, so the grid
+// drains without any wait on another workgroup.
+template <typename TS, typename TA, typename TM>
+__global__ __launch_bounds__(SV_THREADS) void k_vertex_eig_queue(const VDesc* __restrict__ vdesc,
+                                                                const SmallTask* __restrict__ tasks, int ntasks,
+                                                                int* __restrict__ queue, const TS* __restrict__ psi,
+                                                                const TM* msg_in, TM* msg_out,
+                                                                MsgStat* __restrict__ stats, int normalize, int fmax,
+                                                                double* __restrict__ seeds, int warm,
+                                                                const int* __restrict__ active,
+                                                                const long long* __restrict__ sweep_ctr) {
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = next;
+    __syncthreads();  // every thread has read next before thread 0 takes the following task
+    if (t >= ntasks) return;
+    vertex_eig_task<TS, TA, TM>(vdesc, tasks + t, psi, msg_in, msg_out, stats, normalize, fmax, seeds, warm, active,
+                                sweep_ctr);
+    __syncthreads();  // the task's last LDS reads come before the next task's writes
+  }
+}
+
+// clears the task counter of k_vertex_eig_queue (a kernel, so that it can sit
+// in a captured graph; one lane writes)
+__global__ void k_queue_reset(int* __restrict__ queue) {
+  if (threadIdx.x == 0) queue[0] = 0;
 }
 
 // The same update for sites whose tensor does not fit in LDS (every chi <= 16,
@@ -917,7 +962,20 @@ struct SmallState {
   // k_vertex_big variant the autotuner chose (0: none yet) and for which flavour
   int ft = 0;
   bool ft_f32 = false;
+  // persistent small-site launch (k_vertex_eig_queue): its task counter, its
+  // grid of resident workgroups (0: not yet sized), and the autotuner's choice
+  // between it and one workgroup per task (-1: untuned, one per task)
+  DevBuf queue;
+  int queue_grid = 0;
+  int eig_queue = -1;
 };
+
+// ITNBP_VERTEX_QUEUE=0/1 forces the small-site launch form, else the
+// autotuner's choice, else one workgroup per task
+bool use_queue(const SmallState* st) {
+  if (const char* e = getenv("ITNBP_VERTEX_QUEUE")) return atoi(e) != 0;
+  return st->eig_queue == 1;
+}
 
 // the k_vertex_big variant of a launch: ITNBP_VERTEX_BIG_TILE=<tile> forces
 // one (VB_TILES, or VBM_TILES for FP32), else the autotuner's choice for the
@@ -983,6 +1041,38 @@ hipError_t launch_big_ft(int ft, itnbp_net* net, SmallState* st, const void* psi
   }
 }
 
+// the LDS-resident sites: one workgroup per task, or the persistent queue form
+template <typename TS, typename TA, typename TM>
+hipError_t launch_small(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
+                        MsgStat* stats, int normalize, double* seeds, int warm, bool queue, hipStream_t stream) {
+  const int ntasks = (int)st->tasks.size();
+  if (!queue || !st->queue.p) {
+    auto kern = k_vertex_eig<TS, TA, TM>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((unsigned)ntasks), dim3(SV_THREADS), st->lds, stream, (const VDesc*)net->d_vdesc,
+                       (const SmallTask*)st->d_tasks.p, (const TS*)psi, (const TM*)msg_in, (TM*)msg_out, stats,
+                       normalize, st->fmax, seeds, warm, st->active, st->sweep_ctr);
+    return hipGetLastError();
+  }
+  auto kern = k_vertex_eig_queue<TS, TA, TM>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
+  if (e != hipSuccess) return e;
+  if (st->queue_grid == 0) {
+    // every workgroup of the grid resident at once
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, SV_THREADS, st->lds);
+    if (e != hipSuccess) return e;
+    st->queue_grid = std::max(1, std::min(ntasks, std::max(1, per_cu) * net->ctx->num_cu));
+  }
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, stream, (int*)st->queue.p);
+  hipLaunchKernelGGL(kern, dim3((unsigned)st->queue_grid), dim3(SV_THREADS), st->lds, stream,
+                     (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, ntasks, (int*)st->queue.p,
+                     (const TS*)psi, (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax, seeds, warm,
+                     st->active, st->sweep_ctr);
+  return hipGetLastError();
+}
+
 template <typename TS, typename TA, typename TM>
 hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, void* msg_out,
                        MsgStat* stats, int normalize) {
@@ -996,14 +1086,8 @@ hipError_t launch_impl(itnbp_net* net, SmallState* st, const void* psi, const vo
     if (e != hipSuccess) return e;
   }
   if (!st->tasks.empty()) {
-    auto kern = k_vertex_eig<TS, TA, TM>;
-    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)st->lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3((unsigned)st->tasks.size()), dim3(SV_THREADS), st->lds, net->ctx->stream,
-                       (const VDesc*)net->d_vdesc, (const SmallTask*)st->d_tasks.p, (const TS*)psi,
-                       (const TM*)msg_in, (TM*)msg_out, stats, normalize, st->fmax, seeds, warm, st->active,
-                       st->sweep_ctr);
-    e = hipGetLastError();
+    hipError_t e = launch_small<TS, TA, TM>(net, st, psi, msg_in, msg_out, stats, normalize, seeds, warm,
+                                            use_queue(st), net->ctx->stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -1148,6 +1232,8 @@ int small_prepare(itnbp_net* net, const std::vector<int>& seq, int acc_f64, std:
     ITNBP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   }
   if (!st->tasks.empty()) {
+    ITNBP_RETCHK(ensure_buf(ctx, st->queue, 256));
+    st->queue_grid = 0;
     ITNBP_RETCHK(ensure_buf(ctx, st->d_tasks, st->tasks.size() * sizeof(SmallTask)));
     ITNBP_HIPCHK(ctx, hipMemcpyAsync(st->d_tasks.p, st->tasks.data(), st->tasks.size() * sizeof(SmallTask),
                                      hipMemcpyHostToDevice, ctx->stream));
@@ -1167,8 +1253,69 @@ void small_set_batch(itnbp_net* net, const std::vector<int>& vinst, const int* a
   st->sweep_ctr = sweep_ctr;
 }
 
+// Times one launch of the LDS-resident sites in each form (one workgroup per
+// task; the persistent queue) on scratch outputs and keeps the faster.
+template <typename TS, typename TA, typename TM>
+int autotune_small(itnbp_net* net, SmallState* st, const void* psi, const void* msg_in, int normalize) {
+  itnbp_ctx* ctx = net->ctx;
+  DevBuf out, stats, seeds;
+  auto cleanup = [&]() {
+    free_buf(out);
+    free_buf(stats);
+    free_buf(seeds);
+  };
+  int rc = ensure_buf(ctx, out, (size_t)net->msg_total * sizeof(TM));
+  if (rc == ITNBP_OK) rc = ensure_buf(ctx, stats, sizeof(MsgStat) * (size_t)std::max(1, 2 * net->ne));
+  if (rc == ITNBP_OK) rc = ensure_buf(ctx, seeds, st->seeds.bytes);
+  if (rc != ITNBP_OK) {
+    cleanup();
+    return rc;
+  }
+  float best = INFINITY;
+  int best_q = 0;
+  for (int rep = 0; rep < 2; ++rep)
+    for (int q = 0; q < 2; ++q) {
+      hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+      if (e == hipSuccess)
+        e = launch_small<TS, TA, TM>(net, st, psi, msg_in, out.p, (MsgStat*)stats.p, normalize, (double*)seeds.p, 0,
+                                     q == 1, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+      if (e != hipSuccess) {
+        cleanup();
+        return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("vertex autotune: ") + hipGetErrorString(e));
+      }
+      if (rep == 1 && ms < best) {
+        best = ms;
+        best_q = q;
+      }
+    }
+  cleanup();
+  st->eig_queue = best_q;
+  return ITNBP_OK;
+}
+
 int small_autotune(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, const void* msg_in, int normalize) {
   SmallState* st = small_of(net);
+  if (st && !st->tasks.empty() && st->queue.p) {
+    int rc;
+    if (acc_f64) {
+      switch (psi_dtype) {
+        case ITNBP_F64: rc = autotune_small<double, double, double>(net, st, psi, msg_in, normalize); break;
+        case ITNBP_F32: rc = autotune_small<float, double, double>(net, st, psi, msg_in, normalize); break;
+        default: rc = autotune_small<bf16_t, double, double>(net, st, psi, msg_in, normalize); break;
+      }
+    } else {
+      switch (psi_dtype) {
+        case ITNBP_F64: rc = autotune_small<double, float, float>(net, st, psi, msg_in, normalize); break;
+        case ITNBP_F32: rc = autotune_small<float, float, float>(net, st, psi, msg_in, normalize); break;
+        default: rc = autotune_small<bf16_t, float, float>(net, st, psi, msg_in, normalize); break;
+      }
+    }
+    if (rc != ITNBP_OK) return rc;
+  }
   if (!st || st->big.empty()) return ITNBP_OK;
   if (acc_f64) {
     switch (psi_dtype) {
@@ -1187,6 +1334,11 @@ int small_autotune(itnbp_net* net, int acc_f64, const void* psi, int psi_dtype, 
 int small_big_tile(itnbp_net* net) {
   SmallState* st = small_of(net);
   return st && !st->big.empty() ? st->ft : 0;
+}
+
+bool small_queue_used(itnbp_net* net) {
+  SmallState* st = small_of(net);
+  return st && !st->tasks.empty() && st->queue.p && use_queue(st);
 }
 
 const char* small_big_kind(itnbp_net* net) {
@@ -1232,6 +1384,7 @@ void release_small_state(itnbp_net* net) {
   SmallState* st = small_of(net);
   if (!st) return;
   free_buf(st->d_tasks);
+  free_buf(st->queue);
   free_buf(st->d_big);
   free_buf(st->ybuf);
   free_buf(st->seeds);

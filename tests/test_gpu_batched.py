@@ -121,3 +121,23 @@ def test_large_site_kernel_variants_match_the_oracle(monkeypatch, tile):
         print(f"tile {tile}: {g.nv} sites, {r['iterations']} sweeps, max abs err {err:.2e}, path {r['path']}",
               flush=True)
         assert err < 2e-5
+
+
+@pytest.mark.parametrize("queue", ["0", "1"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_persistent_small_site_launch_matches_the_oracle(monkeypatch, queue, graph):
+    """The LDS-resident sites as one workgroup per task and as the persistent
+    queue kernel (a resident grid taking the cost-sorted tasks from a device
+    counter), forced through ITNBP_VERTEX_QUEUE, in the graph loop and the host
+    loop, FP64 against the oracle."""
+    monkeypatch.setenv("ITNBP_VERTEX_QUEUE", queue)
+    inst = instances(np.float64, seed=5)
+    res, got = run_batched(inst, monkeypatch, graph, maxiter=15, precision="f64")
+    assert "[vertex " in res[0]["path"], res[0]["path"]
+    if graph:
+        assert ("persistent" in res[0]["path"]) == (queue == "1"), res[0]["path"]
+    for r, m, (g, psi) in zip(res, got, inst):
+        og, opsi = oracle_graph(g), oracle_psi(psi)
+        ref, _, _, _ = O.bp_update(og, opsi, O.identity_messages(og, opsi), maxiter=r["iterations"],
+                                   schedule="parallel")
+        assert max(np.abs(m[e] - ref[e]).max() for e in ref) < 1e-11
