@@ -194,7 +194,10 @@ int itnbp_normalize(itnbp_net* net);
  * identities of the new dimension; itnbp_get_bond_dims reports the new
  * dimensions.  A gate whose two-site tensor is not finite leaves its bond
  * untouched (kept = -1) and the call returns ITNBP_ERR_NONFINITE after the
- * other gates were applied. */
+ * other gates were applied.  Limits: degree <= 4, external bonds <= 32,
+ * d chi <= 64 on the gate's bond, Theta <= 128 x 128, kept rank <= 64
+ * (bonds with external bonds <= 8 and Theta <= 32 x 32 run on the LDS
+ * kernel, the others on its global-memory variant). */
 int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, const double* gates,
                       int maxdim, double cutoff, int normalize, double* truncerr,
                       double* svals, int max_keep, int* kept);

@@ -37,6 +37,13 @@ constexpr int AP_TH = 32;     // max rows/cols of Theta (r d)
 constexpr int AP_ENV = 8;     // max external bond dimension
 constexpr int AP_MAXEXT = 3;  // max external bonds per side (degree 4)
 constexpr int AP_SITE_D = 16; // max physical dimension of a one-site gate
+// the global-memory variant (k_apply_gates<true>) for bonds past the LDS limits:
+// every per-bond matrix in a global workspace, one workgroup per bond
+constexpr int AP_ENV_BIG = 32;    // max external bond dimension
+constexpr int AP_N_BIG = 64;      // max d chi_b
+constexpr int AP_TH_BIG = 128;    // max rows/cols of Theta
+constexpr int AP_KEEP_BIG = 64;   // max kept rank (the growth cap of itnbp_apply_gates)
+constexpr int AP_F_BIG = AP_TH_BIG * AP_KEEP_BIG;  // R factor, then the left / right factor
 
 struct ApplyItem {
   int u, v;      // vertices
@@ -55,6 +62,9 @@ struct ApplyParams {
   // per-phase timestamps (wall clock; 16 slots per gate, slot 8 the SVD's
   // sweep count), or null; ITNBP_APPLY_PHASES=1
   unsigned long long* ts = nullptr;
+  // global-memory variant: per-gate workspace stride and the sizes of its
+  // side matrices (QR of each side: E n; work: E max(n, d keep)), in doubles
+  long long big_stride = 0, big_q = 0, big_x = 0;
 };
 
 // thread 0 of the workgroup records the end of phase ph (0: start)
@@ -213,18 +223,155 @@ __device__ void env_roots_wave(const double* M, int chi, double cutoff, double* 
   }
 }
 
-// Exact x / d and x % d by a multiply-high with m = ceil(2^32 / d), valid for
-// x d < 2^32 (every index here is below 2^16, every divisor below 2^14): the
-// integer divisions of the index arithmetic otherwise cost ~30 instructions
-// each, and the flat element loops of this kernel run one wave per SIMD with
-// nothing to hide them behind.
+__device__ inline void wave_sync_ap() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// sqrt(M) and M^{-1/2} of a symmetrised chi x chi message for chi <= AP_ENV_BIG
+// (the global-memory variant), by one wave with the matrices in LDS (wk:
+// ENVB_WORK doubles per wave): cyclic two-sided Jacobi with the round-robin
+// pairs of rr_pair, the rotations of a step derived by the first ne / 2 lanes
+// and then applied to the rows, the columns and V; the stopping rule of
+// env_roots_wave (off-diagonal mass below 1e-30 of the diagonal's, or no
+// longer decreasing, at most 40 sweeps) and its eigenvalue truncation
+// (map_eigvals, src/apply.jl:9-25).  S and Si row-major, ld chi.
+constexpr int ENVB_LD = AP_ENV_BIG + 1;
+constexpr int ENVB_WORK = 2 * AP_ENV_BIG * ENVB_LD + 4 * AP_ENV_BIG;
+__device__ void env_roots_big(const double* M, int chi, double cutoff, double* S, double* Si, double* wk) {
+  const int l = threadIdx.x & 63;
+  double* A = wk;
+  double* V = A + AP_ENV_BIG * ENVB_LD;
+  double* rc = V + AP_ENV_BIG * ENVB_LD;  // per pair of a step: c, s, p, q
+  double* rs = rc + AP_ENV_BIG / 2;
+  int* rp = reinterpret_cast<int*>(rs + AP_ENV_BIG / 2);
+  int* rq = rp + AP_ENV_BIG / 2;
+  int* rank = rq + AP_ENV_BIG / 2;        // rank of each eigenvalue by |lambda|
+  int* byr = rank + AP_ENV_BIG;           // the eigenvalue of each rank
+  for (int o = l; o < chi * chi; o += 64) {
+    const int i = o / chi, j = o - i * chi;
+    A[i * ENVB_LD + j] = 0.5 * (M[i + chi * j] + M[j + chi * i]);
+    V[i * ENVB_LD + j] = i == j ? 1.0 : 0.0;
+  }
+  wave_sync_ap();
+  const int ne = chi + (chi & 1), np = ne / 2;
+  double prev = INFINITY;
+  for (int sweep = 0; sweep < 40 && chi > 1; ++sweep) {
+    double off = 0, dg = 0;
+    for (int o = l; o < chi * chi; o += 64) {
+      const int i = o / chi, j = o - i * chi;
+      const double a = A[i * ENVB_LD + j];
+      if (i != j)
+        off += a * a;
+      else
+        dg += a * a;
+    }
+    off = wave_sum_d(off);
+    dg = wave_sum_d(dg);
+    if (off <= 1e-30 * dg || off == 0.0 || off >= prev) break;
+    prev = off;
+    for (int st = 0; st < ne - 1; ++st) {
+      if (l < np) {
+        int p, q;
+        rr_pair(ne, st, l, p, q);
+        double c = 1, sn = 0;
+        if (q < chi) {
+          const double apq = A[p * ENVB_LD + q];
+          if (apq != 0.0) {
+            const double zeta = (A[q * ENVB_LD + q] - A[p * ENVB_LD + p]) / (2 * apq);
+            const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+            c = 1 / sqrt(1 + t * t);
+            sn = c * t;
+          }
+        }
+        rc[l] = c;
+        rs[l] = sn;
+        rp[l] = p;
+        rq[l] = q;
+      }
+      wave_sync_ap();
+      for (int o = l; o < np * chi; o += 64) {  // rows: A <- J^T A
+        const int i = o / chi, col = o - i * chi;
+        const int p = rp[i], q = rq[i];
+        if (q >= chi) continue;
+        const double c = rc[i], sn = rs[i];
+        const double ap = A[p * ENVB_LD + col], aq = A[q * ENVB_LD + col];
+        A[p * ENVB_LD + col] = c * ap - sn * aq;
+        A[q * ENVB_LD + col] = sn * ap + c * aq;
+      }
+      wave_sync_ap();
+      for (int o = l; o < np * chi; o += 64) {  // columns: A <- A J, V <- V J
+        const int i = o / chi, row = o - i * chi;
+        const int p = rp[i], q = rq[i];
+        if (q >= chi) continue;
+        const double c = rc[i], sn = rs[i];
+        const double ap = A[row * ENVB_LD + p], aq = A[row * ENVB_LD + q];
+        A[row * ENVB_LD + p] = c * ap - sn * aq;
+        A[row * ENVB_LD + q] = sn * ap + c * aq;
+        const double vp = V[row * ENVB_LD + p], vq = V[row * ENVB_LD + q];
+        V[row * ENVB_LD + p] = c * vp - sn * vq;
+        V[row * ENVB_LD + q] = sn * vp + c * vq;
+      }
+      wave_sync_ap();
+    }
+  }
+  // truncation: rank |lambda| descending (ties by index); drop from the small
+  // end while the dropped weight / total <= cutoff (keep >= 1)
+  if (l < chi) {
+    const double wl = fabs(A[l * ENVB_LD + l]);
+    int r = 0;
+    for (int m = 0; m < chi; ++m) {
+      const double wm = fabs(A[m * ENVB_LD + m]);
+      if (wm > wl || (wm == wl && m < l)) ++r;
+    }
+    rank[l] = r;
+    byr[r] = l;
+  }
+  wave_sync_ap();
+  double tot = 0;
+  for (int m = 0; m < chi; ++m) tot += fabs(A[m * ENVB_LD + m]);
+  int keep = chi;
+  double disc = 0;
+  while (keep > 1 && tot > 0) {
+    const int m = byr[keep - 1];
+    const double x = fabs(A[m * ENVB_LD + m]);
+    if ((disc + x) / tot > cutoff) break;
+    disc += x;
+    --keep;
+  }
+  // S[i][j] = sum_m V[i][m] V[j][m] sqrt(l_m) over the kept m, Si with 1 / sqrt
+  for (int o = l; o < chi * chi; o += 64) {
+    const int i = o / chi, j = o - i * chi;
+    double sv = 0, siv = 0;
+    for (int m = 0; m < chi; ++m) {
+      if (rank[m] >= keep) continue;
+      const double sq = sqrt(A[m * ENVB_LD + m]);  // NaN for a kept negative eigenvalue
+      const double vv = V[i * ENVB_LD + m] * V[j * ENVB_LD + m];
+      sv += vv * sq;
+      siv += vv / sq;
+    }
+    S[i * chi + j] = sv;
+    Si[i * chi + j] = siv;
+  }
+  wave_sync_ap();
+}
+
+// Exact x / d and x % d by a multiply-high: m = ceil(2^sh / d) with
+// sh = 32 + ceil(log2 d), q = (x m) >> sh, exact for every x < 2^31 (x d <
+// 2^sh, and x m fits 64 bits); the integer divisions of the index arithmetic
+// otherwise cost ~30 instructions each, and the flat element loops of this
+// kernel run one wave per SIMD with nothing to hide them behind.
 struct FastDiv {  // an aggregate, so that a SideGeom can live in LDS
-  unsigned d;
+  unsigned d, sh;
   unsigned long long m;
-  __device__ unsigned div(unsigned x) const { return (unsigned)(((unsigned long long)x * m) >> 32); }
+  __device__ unsigned div(unsigned x) const { return (unsigned)(((unsigned long long)x * m) >> sh); }
   __device__ unsigned mod(unsigned x) const { return x - div(x) * d; }
 };
-__device__ inline FastDiv fast_div(unsigned d) { return FastDiv{d, ((1ull << 32) + d - 1) / d}; }
+__device__ inline FastDiv fast_div(unsigned d) {
+  unsigned sh = 32;
+  while ((1ull << (sh - 32)) < d) ++sh;
+  return FastDiv{d, sh, ((1ull << sh) + d - 1) / d};
+}
 
 struct SideGeom {
   int d, k, j, chib, E, n, r;
@@ -289,7 +436,8 @@ __device__ inline int nat_off(const SideGeom& g, int s, int e, int b) {
 }
 
 // mode product on external axis index pa of X (E x cols, column-major rows e):
-// Y[e with b'] = sum_b X[e with b] * Mt[b][b'] (Mt row-major, ld chi)
+// Y[e with b'] = sum_b X[e with b] * Mt[b][b'] (Mt row-major, ld chi), chi <= CH
+template <int CH>
 __device__ void ext_mode_product(const SideGeom& g, int pa, const double* X, double* Y, int cols,
                                  const double* Mt, bool transpose) {
   const int stride = (int)g.fst[pa].d;
@@ -300,7 +448,7 @@ __device__ void ext_mode_product(const SideGeom& g, int pa, const double* X, dou
     const int hi = (int)g.fchi[pa].div(q), b2 = q - hi * chi;
     double acc = 0;
 #pragma unroll
-    for (int b = 0; b < AP_ENV; ++b)
+    for (int b = 0; b < CH; ++b)
       if (b < chi) {
         const double m = transpose ? Mt[b2 * chi + b] : Mt[b * chi + b2];
         acc += X[(lo + stride * (b + chi * hi)) + (long long)g.E * c] * m;
@@ -526,8 +674,11 @@ __device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, doubl
     int rotated = 0;
     for (int st = 0; st < ne - 1; ++st) {
       // each pair handled by a group of G threads
+      // (the largest power of two <= 64 with G npair <= blockDim.x: every pair
+      // has a group for N <= 2 blockDim.x)
       const int npair = ne / 2;
-      const int G = (int)blockDim.x / npair >= 64 ? 64 : ((int)blockDim.x / npair >= 32 ? 32 : ((int)blockDim.x / npair >= 16 ? 16 : 8));
+      int G = 64;
+      while (G > 1 && G * npair > (int)blockDim.x) G >>= 1;
       const int pi = tid / G, gl = tid % G;
       double alpha = 0, beta = 0, gamma = 0;
       int p = 0, q = 0;
@@ -577,6 +728,14 @@ __device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, doubl
   return sweep + 1;
 }
 
+// BIG = false: the side matrices (E n <= AP_XY) and every small matrix in LDS,
+// the Householder reflectors of each side saved to `work` (2 AP_SIDE doubles
+// per gate).  BIG = true: the same steps with every matrix in a global
+// workspace of prm.big_stride doubles per gate at `work` (layout below), for
+// external bonds up to AP_ENV_BIG, d chi_b <= AP_N_BIG and Theta up to
+// AP_TH_BIG square; LDS holds the geometry, the vectors and the environment
+// eigenproblems.
+template <bool BIG>
 __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict__ items,
                                                      const VDesc* __restrict__ vds,
                                                      double* __restrict__ psi,
@@ -586,25 +745,47 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
                                                      double* __restrict__ truncerr,
                                                      double* __restrict__ svals, int* __restrict__ kept,
                                                      int* __restrict__ status) {
+  constexpr int TH = BIG ? AP_TH_BIG : AP_TH;           // rows/cols of Theta
+  constexpr int ENV = BIG ? AP_ENV_BIG : AP_ENV;        // external bond dimension
+  constexpr int RS = ENV * ENV;                         // one environment root
   extern __shared__ __align__(16) double sm[];
-  double* X = sm;                       // AP_XY: QR, then SVD workspace
-  double* Y = X + AP_XY;                // AP_XY: mode-product partner, rebuild
-  double* Ru = Y + AP_XY;               // 1024: R_u (r x n), then the left factor
-  double* Rv = Ru + 1024;               // 1024: R_v, then the right factor
-  double* roots = Rv + 1024;            // 2 sides x AP_MAXEXT x (sqrt, inv sqrt) x AP_ENV^2
-  double* small = roots + 2 * AP_MAXEXT * 2 * AP_ENV * AP_ENV;
-  double* ea = small;
-  double* ev = ea + 64;
-  double* ew = ev + 64;
-  double* rot = ew + 64;
+  double *X, *Y, *Ru, *Rv, *roots, *small, *A, *VV, *scr_u, *scr_v;
+  if constexpr (BIG) {
+    // per gate: QR of side u (E_u n_u), of side v, two work matrices
+    // (E max(n, d keep)), the R / factor matrices, Theta and V, the roots
+    double* w = work + (long long)blockIdx.x * prm.big_stride;
+    scr_u = w;
+    scr_v = scr_u + prm.big_q;
+    X = scr_v + prm.big_q;
+    Y = X + prm.big_x;
+    Ru = Y + prm.big_x;
+    Rv = Ru + AP_F_BIG;
+    A = Rv + AP_F_BIG;
+    VV = A + AP_TH_BIG * AP_TH_BIG;
+    roots = VV + AP_TH_BIG * AP_TH_BIG;
+    small = sm;
+  } else {
+    X = sm;                       // AP_XY: QR, then SVD workspace
+    Y = X + AP_XY;                // AP_XY: mode-product partner, rebuild
+    Ru = Y + AP_XY;               // 1024: R_u (r x n), then the left factor
+    Rv = Ru + 1024;               // 1024: R_v, then the right factor
+    roots = Rv + 1024;            // 2 sides x AP_MAXEXT x (sqrt, inv sqrt) x AP_ENV^2
+    small = roots + 2 * AP_MAXEXT * 2 * RS;
+    A = X;                        // Theta, then U diag(s)
+    VV = X + 1024;                // V of the SVD
+    scr_u = work + (long long)blockIdx.x * 2 * AP_SIDE;
+    scr_v = scr_u + AP_SIDE;
+  }
+  double* rot = small;
   double* vh = rot + 4 * 32;
-  double* beta = vh + 64;
-  double* vhv = beta + 64;
-  double* betav = vhv + 64;
-  double* svec = betav + 64;
-  double* red = svec + 64;
+  double* beta = vh + AP_N_BIG;
+  double* vhv = beta + AP_N_BIG;
+  double* betav = vhv + AP_N_BIG;
+  double* svec = betav + AP_N_BIG;
+  double* red = svec + TH;
+  double* envw = red + 16;  // BIG: ENVB_WORK doubles per wave
   __shared__ int sh_keep;
-  __shared__ int order[AP_TH];
+  __shared__ int order[TH];
 
   const ApplyItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
@@ -623,8 +804,6 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   __syncthreads();
   const SideGeom& gu = sgeo[0];
   const SideGeom& gv = sgeo[1];
-  double* scr_u = work + (long long)blockIdx.x * 2 * AP_SIDE;
-  double* scr_v = scr_u + AP_SIDE;
   phase_mark(prm, 0);
 
   // 1. environment roots (sqrt and inverse sqrt), both sides, one environment per wave
@@ -636,8 +815,11 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       const SideGeom& g = side == 0 ? gu : gv;
       const VDesc& V = vds[side == 0 ? it.u : it.v];
       const int a = g.ext[pa];
-      double* S = roots + ((side * AP_MAXEXT + pa) * 2) * AP_ENV * AP_ENV;
-      env_roots_wave(msg + V.in_msg[a], g.dims[a], prm.eig_cutoff, S, S + AP_ENV * AP_ENV);
+      double* S = roots + ((side * AP_MAXEXT + pa) * 2) * RS;
+      if constexpr (BIG)
+        env_roots_big(msg + V.in_msg[a], g.dims[a], prm.eig_cutoff, S, S + RS, envw + wv * ENVB_WORK);
+      else
+        env_roots_wave(msg + V.in_msg[a], g.dims[a], prm.eig_cutoff, S, S + RS);
     }
     __syncthreads();
   }
@@ -655,8 +837,8 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     double* cur = X;
     double* oth = Y;
     for (int pa = 0; pa < g.next; ++pa) {
-      const double* S = roots + ((side * AP_MAXEXT + pa) * 2) * AP_ENV * AP_ENV;
-      ext_mode_product(g, pa, cur, oth, g.n, S, false);
+      const double* S = roots + ((side * AP_MAXEXT + pa) * 2) * RS;
+      ext_mode_product<ENV>(g, pa, cur, oth, g.n, S, false);
       double* t = cur;
       cur = oth;
       oth = t;
@@ -670,7 +852,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     double* R = side == 0 ? Ru : Rv;
     double* scr = side == 0 ? scr_u : scr_v;
     if (side == 0) phase_mark(prm, 11);
-    if (tsqr_ok(g)) {
+    if (!BIG && tsqr_ok(g)) {
       // the stacked R factors S (4n x n) in Y, which is free after the copy to X
       const int n4 = 4 * g.n, wv = tid >> 6;
       double* S = Y;
@@ -712,8 +894,6 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   const double* G = gates + it.gate;
   const bool tr = Mr < Nc;
   const int M = tr ? Nc : Mr, N = tr ? Mr : Nc;
-  double* A = X;
-  double* VV = X + 1024;
   for (int idx = tid; idx < Mr * Nc; idx += blockDim.x) {
     const int row = idx % Mr, col = idx / Mr;
     const int ru = row % gu.r, sup = row / gu.r;
@@ -802,12 +982,16 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   // 5. rebuild both sides: Y = [F as r x (d keep); 0], apply Q, then M^{-1/2};
   //    the largest-magnitude entry of every new-bond slice of psi_u' (first in
   //    column-major order) is made positive and the same sign applied to psi_v'
-  __shared__ double bond_sign[AP_TH];
+  __shared__ double bond_sign[BIG ? AP_KEEP_BIG : AP_TH];
   for (int side = 0; side < 2; ++side) {
     const SideGeom& g = side == 0 ? gu : gv;
     const double* F = side == 0 ? Ru : Rv;  // rows (r_i, s), r fastest
     const double* scr = side == 0 ? scr_u : scr_v;
-    for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = scr[i];
+    // the side's reflectors: read in place from the global workspace (BIG),
+    // else copied back into LDS
+    const double* QX = BIG ? scr : X;
+    if (!BIG)
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = scr[i];
     const int cols = g.d * keep;  // column c' = s + d b'
     for (int idx = tid; idx < g.E * cols; idx += blockDim.x) {
       const int c = (int)g.fE.div(idx), e = idx - c * g.E;
@@ -815,7 +999,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       Y[idx] = e < g.r ? F[(e + g.r * s) + (long long)(g.r * g.d) * b] : 0.0;
     }
     __syncthreads();
-    if (tsqr_ok(g)) {
+    if (!BIG && tsqr_ok(g)) {
       // Q [F; 0] = diag(Q_w) P Q_S [F; 0]: Q_S on the stacked rows (block w's
       // first n rows) by wave 0, then each block's reflectors by its wave
       const int n4 = 4 * g.n, wv = tid >> 6, E = g.E, nn = g.n;
@@ -836,14 +1020,14 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       }
       __syncthreads();
     } else {
-      apply_q(X, g.E, g.r, side == 0 ? vh : vhv, side == 0 ? beta : betav, Y, cols);
+      apply_q(QX, g.E, g.r, side == 0 ? vh : vhv, side == 0 ? beta : betav, Y, cols);
     }
     // M^{-1/2}: psi'[.. b ..] = sum_b' Y[.. b' ..] Si[b][b']
     double* cur = Y;
     double* oth = X;
     for (int pa = 0; pa < g.next; ++pa) {
-      const double* Si = roots + ((side * AP_MAXEXT + pa) * 2 + 1) * AP_ENV * AP_ENV;
-      ext_mode_product(g, pa, cur, oth, cols, Si, true);
+      const double* Si = roots + ((side * AP_MAXEXT + pa) * 2 + 1) * RS;
+      ext_mode_product<ENV>(g, pa, cur, oth, cols, Si, true);
       double* t = cur;
       cur = oth;
       oth = t;
@@ -932,9 +1116,33 @@ __global__ __launch_bounds__(256) void k_apply_site(const int* __restrict__ vert
   for (long long i = threadIdx.x; i < V.D; i += blockDim.x) t[i] *= sc;
 }
 
+// dynamic LDS of k_apply_gates<false> / <true>: the matrices (LDS variant
+// only), then rot, four reflector-head vectors, the singular values, the
+// block reduction and (global-memory variant) one environment work area per wave
 size_t apply_lds_bytes() {
-  return sizeof(double) * (2 * AP_XY + 2 * 1024 + 2 * AP_MAXEXT * 2 * AP_ENV * AP_ENV + 64 * 3 + 4 * 32 +
-                           64 * 5 + 16);
+  return sizeof(double) * (2 * AP_XY + 2 * 1024 + 2 * AP_MAXEXT * 2 * AP_ENV * AP_ENV + 4 * 32 + 4 * AP_N_BIG +
+                           AP_TH + 16);
+}
+size_t apply_big_lds_bytes() {
+  return sizeof(double) * (4 * 32 + 4 * AP_N_BIG + AP_TH_BIG + 16 + 4 * ENVB_WORK);
+}
+
+// per-side sizes of a bond for the kernel limits: E (external size), n = d chi_b
+// (the bond as it is), cols = d chi (the bond's layout during the call)
+struct SideSize {
+  long long E, n, cols;
+};
+SideSize side_size(const itnbp_net* net, int x, int e, const std::vector<int>& chi) {
+  SideSize z{1, (long long)net->phys[x] * net->chi[e], (long long)net->phys[x] * chi[e]};
+  for (int a = net->inc_ptr[x]; a < net->inc_ptr[x + 1]; ++a)
+    if (net->inc_edge[a] != e) z.E *= chi[net->inc_edge[a]];
+  return z;
+}
+
+// global workspace of k_apply_gates<true> per gate, in doubles, for side
+// matrices of q = E n and x = E max(n, cols) doubles
+long long big_stride(long long q, long long x) {
+  return 2 * q + 2 * x + 2LL * AP_F_BIG + 2LL * AP_TH_BIG * AP_TH_BIG + 2LL * AP_MAXEXT * 2 * AP_ENV_BIG * AP_ENV_BIG;
 }
 
 // Checks the call and fills the launch items; goff receives the number of gate
@@ -958,13 +1166,17 @@ long long theta_rank_bound(const itnbp_net* net, int e) {
 // network's, or those it will have after growing) and fills the launch items;
 // goff receives the number of gate matrix elements.  With fixed_bonds the
 // kept rank must fit the bond as it is.
+//
+// With big non-null, a bond past the LDS kernel's limits that fits those of
+// the global-memory variant is marked (*big)[i] = 1 instead of refused.
 int prepare_apply(itnbp_net* net, int n, const int* edge_ids, int maxdim, double cutoff, double* svals,
                   int max_keep, const std::vector<int>& chi, bool fixed_bonds, std::vector<ApplyItem>& items,
-                  long long& goff) {
+                  long long& goff, std::vector<char>* big = nullptr) {
   itnbp_ctx* ctx = net->ctx;
   if (net->dtype != ITNBP_F64) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply needs FP64 site tensors");
   if (svals && max_keep <= 0) return itnbp_fail(ctx, ITNBP_ERR_ARG, "max_keep must be > 0 with svals");
   items.assign(n, ApplyItem{});
+  if (big) big->assign(n, 0);
   std::vector<char> used(net->nv, 0);
   goff = 0;
   for (int i = 0; i < n; ++i) {
@@ -981,24 +1193,45 @@ int prepare_apply(itnbp_net* net, int n, const int* edge_ids, int maxdim, double
       return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
                         "apply: the truncation would keep more values than the bond dimension; pass maxdim <= chi");
     // product environment: every message into u but from v (and into v but from u) must be set
+    const char* lds_why = nullptr;  // the first LDS limit the bond exceeds
     for (int side = 0; side < 2; ++side) {
       const int x = side == 0 ? u : v;
       const VDesc& V = net->vdesc[x];
       if (V.k > AP_MAXEXT + 1) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply supports degree <= 4");
-      long long E = 1;
+      int chi_ext = 1;
       for (int a = net->inc_ptr[x]; a < net->inc_ptr[x + 1]; ++a) {
         const int f = net->inc_edge[a];
         if (f == e) continue;
         if (!net->msg_set[msg_id_into(net, f, x)])
           return itnbp_fail(ctx, ITNBP_ERR_STATE, "environment message not set");
-        if (chi[f] > AP_ENV) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply supports external bond dimensions <= 8");
-        E *= chi[f];
+        chi_ext = std::max(chi_ext, chi[f]);
       }
-      const long long nn = (long long)net->phys[x] * net->chi[e];
-      if (E * nn > AP_XY || E * net->phys[x] * chi[e] > AP_XY)
-        return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: site tensor too large for the LDS kernel");
-      if (std::min(E, nn) * net->phys[x] > AP_TH) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: Theta larger than 32x32");
-      if (std::min(E, nn) * nn > 1024) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, "apply: R factor larger than 1024");
+      const SideSize z = side_size(net, x, e, chi);
+      const long long d = net->phys[x], r = std::min(z.E, z.n);
+      if (lds_why) continue;
+      if (chi_ext > AP_ENV) lds_why = "apply supports external bond dimensions <= 8";
+      else if (z.E * z.n > AP_XY || z.E * z.cols > AP_XY) lds_why = "apply: site tensor too large for the LDS kernel";
+      else if (r * d > AP_TH) lds_why = "apply: Theta larger than 32x32";
+      else if (r * z.n > 1024) lds_why = "apply: R factor larger than 1024";
+    }
+    if (lds_why) {
+      // the global-memory variant: external bonds <= 32, d chi_b <= 64, Theta <= 128 x 128
+      bool fits = big != nullptr;
+      for (int side = 0; side < 2 && fits; ++side) {
+        const int x = side == 0 ? u : v;
+        const SideSize z = side_size(net, x, e, chi);
+        for (int a = net->inc_ptr[x]; a < net->inc_ptr[x + 1]; ++a)
+          if (net->inc_edge[a] != e && chi[net->inc_edge[a]] > AP_ENV_BIG) fits = false;
+        fits = fits && z.n <= AP_N_BIG && std::min(z.E, z.n) * net->phys[x] <= AP_TH_BIG &&
+               chi[e] <= AP_KEEP_BIG && z.E * std::max(z.n, z.cols) < (1LL << 30);
+      }
+      if (!fits) {
+        if (!big) return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED, lds_why);
+        return itnbp_fail(ctx, ITNBP_ERR_UNSUPPORTED,
+                          "apply supports external bond dimensions <= 32, d chi <= 64 on the bond and Theta <= "
+                          "128 x 128");
+      }
+      (*big)[i] = 1;
     }
     items[i].u = u;
     items[i].v = v;
@@ -1057,10 +1290,24 @@ ApplyBufs carve_apply_bufs(char* base, int n, long long goff, int mk) {
 }
 
 // upload, one launch, read-back; every device buffer comes from b
-int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const double* gates, long long goff,
+// upload, the launches, read-back; every device buffer but the global-memory
+// variant's workspace comes from b.  Gates marked in big run on
+// k_apply_gates<true> after the others, in chunks whose workspace fits the
+// device's free memory (at most 16 GiB).
+int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, const double* gates, long long goff,
               int maxdim, double cutoff, int normalize, double* truncerr, double* svals, int mk, int* kept,
-              const ApplyBufs& b) {
+              const ApplyBufs& b, const std::vector<char>* big = nullptr) {
   itnbp_ctx* ctx = net->ctx;
+  // the LDS kernel's gates first, then the global-memory variant's (every
+  // output is indexed by the gate's slot, so the order is free)
+  std::vector<ApplyItem> items;
+  items.reserve(n);
+  for (int i = 0; i < n; ++i)
+    if (!big || !(*big)[i]) items.push_back(items_in[i]);
+  const int nl = (int)items.size();
+  for (int i = 0; i < n; ++i)
+    if (big && (*big)[i]) items.push_back(items_in[i]);
+  const int nb = n - nl;
   hipError_t e1 = hipMemcpyAsync(b.items, items.data(), n * sizeof(ApplyItem), hipMemcpyHostToDevice, ctx->stream);
   hipError_t e2 = hipMemcpyAsync(b.gates, gates, goff * 8, hipMemcpyHostToDevice, ctx->stream);
   if (e1 != hipSuccess || e2 != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: upload failed");
@@ -1071,7 +1318,8 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const 
   prm.eig_cutoff = 10 * 2.220446049250313e-16;
   prm.max_keep = mk;
   const size_t lds = apply_lds_bytes();
-  hipError_t ea = hipFuncSetAttribute((const void*)k_apply_gates, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t ea = hipFuncSetAttribute((const void*)k_apply_gates<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds);
   if (ea != hipSuccess)
     return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(ea));
   DevBuf tsb;
@@ -1082,11 +1330,54 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items, const 
     prm.ts = (unsigned long long*)tsb.p;
   }
   const int pm0 = prof_mark(ctx);
-  hipLaunchKernelGGL(k_apply_gates, dim3(n), dim3(256), lds, ctx->stream, (const ApplyItem*)b.items,
-                     net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)b.gates, prm,
-                     b.work, b.te, svals ? b.sv : nullptr, b.kept, b.st);
+  if (nl > 0)
+    hipLaunchKernelGGL(k_apply_gates<false>, dim3(nl), dim3(256), lds, ctx->stream, (const ApplyItem*)b.items,
+                       net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)b.gates, prm,
+                       b.work, b.te, svals ? b.sv : nullptr, b.kept, b.st);
   hipError_t ek = hipGetLastError();
   prof_add(ctx, "apply_gates", pm0, prof_mark(ctx));
+  DevBuf bw;
+  std::unique_ptr<DevBuf, void (*)(DevBuf*)> bw_guard(&bw, [](DevBuf* x) { free_buf(*x); });
+  if (ek == hipSuccess && nb > 0) {
+    // side matrices of the largest gate: E n for the QR, E max(n, d chi) for the work
+    long long q = 1, xs = 1;
+    for (int i = nl; i < n; ++i)
+      for (int side = 0; side < 2; ++side) {
+        const VDesc& V = net->vdesc[side == 0 ? items[i].u : items[i].v];
+        const int j = side == 0 ? items[i].ju : items[i].jv;
+        long long E = 1;
+        for (int a = 1; a <= V.k; ++a)
+          if (a != j) E *= V.dims[a];
+        const long long nn = (long long)V.dims[0] * items[i].chib_in, cols = (long long)V.dims[0] * V.dims[j];
+        q = std::max(q, E * nn);
+        xs = std::max(xs, E * std::max(nn, cols));
+      }
+    ApplyParams pb = prm;
+    pb.ts = nullptr;
+    pb.big_q = q;
+    pb.big_x = xs;
+    pb.big_stride = (big_stride(q, xs) + 31) & ~31LL;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+    const size_t budget = std::min<size_t>(fr / 2, 16ULL << 30);
+    const long long per = pb.big_stride * 8;
+    const int chunk = (int)std::max<long long>(1, std::min<long long>(nb, (long long)(budget / (size_t)per)));
+    ITNBP_RETCHK(ensure_buf(ctx, bw, (size_t)chunk * per));
+    const size_t lds_big = apply_big_lds_bytes();
+    ea = hipFuncSetAttribute((const void*)k_apply_gates<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_big);
+    if (ea != hipSuccess)
+      return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(ea));
+    const int pm1 = prof_mark(ctx);
+    for (int c0 = nl; c0 < n && ek == hipSuccess; c0 += chunk) {
+      const int cn = std::min(chunk, n - c0);
+      hipLaunchKernelGGL(k_apply_gates<true>, dim3(cn), dim3(256), lds_big, ctx->stream,
+                         (const ApplyItem*)b.items + c0, net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur],
+                         (const double*)b.gates, pb, (double*)bw.p, b.te, svals ? b.sv : nullptr, b.kept, b.st);
+      ek = hipGetLastError();
+    }
+    prof_add(ctx, "apply_gates_global", pm1, prof_mark(ctx));
+  }
   if (prm.ts && ek == hipSuccess) {
     // mean duration of every phase over the gates that ran to the end, in us
     std::vector<unsigned long long> ts((size_t)n * 16);
@@ -1221,8 +1512,10 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
       chi_grow[e] = std::max(chi_grow[e], target[i]);
     }
     std::vector<ApplyItem> items;
+    std::vector<char> big;
     long long goff = 0;
-    ITNBP_RETCHK(prepare_apply(net, n, edge_ids, maxdim, cutoff, svals, max_keep, chi_grow, false, items, goff));
+    ITNBP_RETCHK(
+        prepare_apply(net, n, edge_ids, maxdim, cutoff, svals, max_keep, chi_grow, false, items, goff, &big));
     ITNBP_RETCHK(net_resize_bonds(net, chi_grow));
     const int mk = svals ? max_keep : 1;
     size_t sz[7];
@@ -1233,7 +1526,7 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
     ITNBP_RETCHK(ensure_buf(ctx, d, total));
     std::vector<int> kp(n, -1);
     const int rc = run_apply(net, n, items, gates, goff, maxdim, cutoff, normalize, truncerr, svals, mk, kp.data(),
-                             carve_apply_bufs((char*)d.p, n, goff, mk));
+                             carve_apply_bufs((char*)d.p, n, goff, mk), &big);
     free_buf(d);
     std::vector<int> chi_cut = net->chi;
     if (rc != ITNBP_OK) {

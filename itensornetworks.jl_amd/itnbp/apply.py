@@ -70,7 +70,7 @@ def apply_gates_(bpc: BeliefPropagationCache, gates, *, maxdim=None, cutoff=None
         if not batch:
             continue
         ids, mats = [], []
-        max_keep = 32
+        max_keep = 64  # the largest kept rank (bonds grow to at most 64)
         for i, gt in batch:
             u, v = gt.sites
             e = g.edge_id(u, v)

@@ -202,3 +202,35 @@ def test_bond_takes_the_kept_dimension():
         np.testing.assert_allclose(t3[x], ref3[g.index[x]], atol=1e-10)
     assert errs3[0] == pytest.approx(err3, rel=1e-8) and errs3[0] > 0
     itnbp.update_(c3, maxiter=5, edge_sequence="parallel")
+
+
+@pytest.mark.parametrize("chi", [16, 32])
+def test_large_bonds_run_on_the_global_memory_kernel(chi):
+    """Bonds past the LDS kernel's limits (external bonds above 8, Theta up to
+    128 x 128) run on its global-memory variant: on a 3x3 chi = 16 or 32 PEPS a
+    bulk bond (the centre's side has E = chi^3 external rows against n = 2 chi,
+    the tall QR; at chi = 32 Theta is 128 x 128) and a corner bond (E = chi < n,
+    the wide QR), truncated to maxdim = chi, against the oracle (apply.jl:33-95):
+    tensors, truncation errors and singular values."""
+    g = itnbp.named_grid((3, 3))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(31), np.float64, g, link_space=chi)
+    bpc = converged_cache(psi)
+    og = oracle_graph(g)
+    opsi = oracle_psi(psi)
+    msgs = to_index(g, bpc.messages())
+    deg = {v: g.degree(v) for v in g.vertices}
+    bonds = [((2, 2), (2, 3)), ((1, 1), (1, 2))]
+    gates = [O.expm_sym(O.ising_bond_hamiltonian(2.5, deg[u], deg[v]), -0.1) for u, v in bonds]
+    svals = []
+    errs = itnbp.apply_gates_(bpc, [itnbp.Gate(G, b) for G, b in zip(gates, bonds)], maxdim=chi, normalize=True,
+                              callback=lambda singular_values, truncation_error: svals.append(singular_values))
+    new = bpc.tensornetwork()
+    for i, ((u, v), G) in enumerate(zip(bonds, gates)):
+        ref, s_ref, err_ref = O.apply_gate(G, og, opsi, msgs, g.index[u], g.index[v], maxdim=chi, normalize=True)
+        assert new.bond_dim(u, v) == len(s_ref)
+        for x in (u, v):
+            r = ref[g.index[x]]
+            assert new[x].shape == r.shape
+            assert np.linalg.norm(new[x] - r) <= 1e-10 * np.linalg.norm(r), x
+        np.testing.assert_allclose(svals[i], s_ref, rtol=1e-10, atol=1e-13 * s_ref[0])
+        assert errs[i] == pytest.approx(err_ref, rel=1e-8, abs=1e-15)
