@@ -44,6 +44,9 @@ constexpr int AP_N_BIG = 64;      // max d chi_b
 constexpr int AP_TH_BIG = 128;    // max rows/cols of Theta
 constexpr int AP_KEEP_BIG = 64;   // max kept rank (the growth cap of itnbp_apply_gates)
 constexpr int AP_F_BIG = AP_TH_BIG * AP_KEEP_BIG;  // R factor, then the left / right factor
+// LDS of the small vectors (rot, four reflector-head vectors, the singular
+// values, the block reduction), rounded up to 256 bytes
+constexpr int AP_BIG_SMALL = (4 * 32 + 4 * AP_N_BIG + AP_TH_BIG + 16 + 31) & ~31;
 
 struct ApplyItem {
   int u, v;      // vertices
@@ -458,6 +461,43 @@ __device__ void ext_mode_product(const SideGeom& g, int pa, const double* X, dou
   __syncthreads();
 }
 
+// The same mode product for the global-memory variant, one fibre per thread:
+// a thread loads the chi entries of one (lo, hi, column) fibre once and writes
+// all chi outputs, with Mt (transposed if asked) staged in LDS at ms (CH x CH;
+// every lane of a wave reads the same entry).  The per-output form above
+// reloads each fibre chi times, which from global memory costs more than the
+// multiply-adds.
+template <int CH>
+__device__ void ext_mode_product_fibre(const SideGeom& g, int pa, const double* X, double* Y, int cols,
+                                       const double* Mt, bool transpose, double* ms) {
+  const int stride = (int)g.fst[pa].d;
+  const int chi = g.dims[g.ext[pa]];
+  for (int i = threadIdx.x; i < chi * chi; i += blockDim.x) {
+    const int b = i / chi, b2 = i - b * chi;
+    ms[b * CH + b2] = transpose ? Mt[b2 * chi + b] : Mt[b * chi + b2];
+  }
+  __syncthreads();
+  const int per = g.E / chi;  // fibres per column
+  const FastDiv fper = fast_div((unsigned)per);
+  const int nfib = per * cols;
+  for (int f = threadIdx.x; f < nfib; f += blockDim.x) {
+    const int c = (int)fper.div(f), r = f - c * per;
+    const int hi = (int)g.fst[pa].div(r), lo = r - hi * stride;
+    const long long base = (long long)g.E * c + lo + (long long)stride * chi * hi;
+    double x[CH];
+#pragma unroll
+    for (int b = 0; b < CH; ++b) x[b] = b < chi ? X[base + (long long)stride * b] : 0.0;
+#pragma unroll 4
+    for (int b2 = 0; b2 < chi; ++b2) {
+      double acc = 0;
+#pragma unroll
+      for (int b = 0; b < CH; ++b) acc += x[b] * ms[b * CH + b2];
+      Y[base + (long long)stride * b2] = acc;
+    }
+  }
+  __syncthreads();
+}
+
 // Householder QR in place on X (E x n, column-major); R in the upper triangle,
 // reflector tails below, heads in vh[], betas in beta[]
 __device__ void householder_qr(double* X, int E, int n, double* vh, double* beta, double* red) {
@@ -728,6 +768,161 @@ __device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, doubl
   return sweep + 1;
 }
 
+// ---- Gram-based QR of a side, global-memory variant ----------------------
+// The QR of a tall side enters simple_update_bp only through R (in Theta) and
+// Q (in the rebuild), and any factorisation X = Q R with orthonormal Q gives
+// the same psi' (R changes by an orthogonal factor that Theta's SVD absorbs).
+// Here: G = X^T X = V diag(l) V^T (one-sided Jacobi of the symmetric G),
+// directions with l <= n eps l_max dropped (rank r); X1 = X T with
+// T = V_r diag(l_r)^{-1/2}; G1 = X1^T X1 = R1^T R1 (Cholesky, G1 ~ I); then
+// R = R1 diag(l_r)^{1/2} V_r^T (r x n) and Q = X Tq, Tq = T R1^{-1}: the
+// second Gram removes the first one's loss of orthogonality (the scheme of
+// CholeskyQR2).  Directions of X below ~sqrt(n eps) of its largest singular
+// value are dropped, as rank-deficient ones.  Every step streams X through
+// the workgroup from global memory a fixed number of times, where the
+// column-by-column Householder QR reads it once per column.
+
+// G = X^T X (n x n, ld n, in LDS) of X (E x n column-major, global), the rows
+// staged through `tile` (tile_cap doubles of LDS) in chunks; thread (bi, bj)
+// accumulates the 4 x 4 block of G at (4 bi, 4 bj) (n <= 64)
+__device__ void gram_rows(const double* X, long long E, int n, double* G, double* tile, int tile_cap) {
+  const int tid = threadIdx.x, bi = tid & 15, bj = tid >> 4;
+  const int ld = tile_cap / n;  // rows of a chunk + 1 (odd pitch against bank conflicts)
+  const int cr = (ld & 1) ? ld - 1 : ld - 2;
+  const int pitch = cr + 1;
+  double acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+  for (long long e0 = 0; e0 < E; e0 += cr) {
+    const int rows = (int)((E - e0) < cr ? (E - e0) : cr);
+    __syncthreads();  // the previous chunk is consumed
+    for (int idx = tid; idx < rows * n; idx += blockDim.x) {
+      const int c = idx / rows, rr = idx - c * rows;
+      tile[c * pitch + rr] = X[e0 + rr + E * c];
+    }
+    __syncthreads();
+    if (4 * bi < n && 4 * bj < n)
+      for (int rr = 0; rr < rows; ++rr) {
+        double a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[q] = 4 * bi + q < n ? tile[(4 * bi + q) * pitch + rr] : 0.0;
+          b[q] = 4 * bj + q < n ? tile[(4 * bj + q) * pitch + rr] : 0.0;
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (4 * bi + p < n && 4 * bj + q < n) G[(4 * bi + p) + n * (4 * bj + q)] = acc[p][q];
+  __syncthreads();
+}
+
+// out[e + E j] = sum_c X[e + E c] W[c + n j] for j < m: one row per thread,
+// 16 outputs at a time, W (n x m, ld n) in LDS (every lane reads the same
+// entry); the row's entries are re-read (from cache) per group of 16 outputs
+__device__ void rows_times(const double* X, long long E, int n, const double* W, int m, double* out) {
+  for (long long e = threadIdx.x; e < E; e += blockDim.x)
+    for (int j0 = 0; j0 < m; j0 += 16) {
+      double acc[16];
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) acc[jj] = 0.0;
+#pragma unroll 4
+      for (int c = 0; c < n; ++c) {
+        const double x = X[e + E * c];
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) acc[jj] += x * (j0 + jj < m ? W[c + n * (j0 + jj)] : 0.0);
+      }
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj)
+        if (j0 + jj < m) out[e + E * (j0 + jj)] = acc[jj];
+    }
+  __syncthreads();
+}
+
+// The factorisation above for one side.  X (E x n, global) is left unchanged;
+// Xw (E x n, global) is work; lds: 4 n^2 + 4 n doubles of LDS (>= 4 AP_N_BIG^2
+// + ...); R (r x n, ld r) and Tq (n x r, ld n) to global, lam: n doubles of
+// LDS.  Returns r.
+__device__ int gram_qr(const double* X, long long E, int n, double* Xw, double* lds, int lds_cap, double* lam,
+                       int* keep_idx, double* R, double* Tq, double* rot, double* red) {
+  const int tid = threadIdx.x;
+  double* G = lds;             // n x n: G, then U diag(l); then G1 / R1 (r x r)
+  double* V = G + n * n;       // n x n
+  double* T = V + n * n;       // n x r
+  double* work = T + n * n;    // Gram staging
+  const int wcap = lds_cap - 3 * n * n;
+  __shared__ int s_rank;
+  gram_rows(X, E, n, G, T, lds_cap - 2 * n * n);
+  jacobi_svd(G, n, n, V, rot, red);
+  // eigenvalues of the PSD G: the column norms of U diag(l)
+  for (int j = tid; j < n; j += blockDim.x) {
+    double t = 0;
+    for (int i = 0; i < n; ++i) t += G[i + n * j] * G[i + n * j];
+    lam[j] = sqrt(t);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double lmax = 0;
+    for (int j = 0; j < n; ++j) lmax = lam[j] > lmax ? lam[j] : lmax;
+    int r = 0;
+    for (int j = 0; j < n; ++j)
+      if (lam[j] > n * 2.220446049250313e-16 * lmax) keep_idx[r++] = j;
+    if (r == 0) keep_idx[r++] = 0;  // X = 0: one zero direction, R = 0
+    s_rank = r;
+  }
+  __syncthreads();
+  const int r = s_rank;
+  for (int idx = tid; idx < n * r; idx += blockDim.x) {
+    const int c = idx % n, k = idx / n;
+    const double l = lam[keep_idx[k]];
+    T[idx] = l > 0 ? V[c + n * keep_idx[k]] / sqrt(l) : 0.0;
+  }
+  __syncthreads();
+  rows_times(X, E, n, T, r, Xw);  // X1 = X T
+  gram_rows(Xw, E, r, G, work, wcap);  // G1 = X1^T X1 (r x r, ld r)
+  // Cholesky G1 = R1^T R1 in place (upper triangle, ld r); a non-positive pivot
+  // (a zero column of X1) gives a unit diagonal and a zero row
+  for (int k = 0; k < r; ++k) {
+    const double pkk = G[k + r * k];
+    const double piv = pkk > 0 ? sqrt(pkk) : 0.0;
+    __syncthreads();
+    for (int j = k + tid; j < r; j += blockDim.x)
+      G[k + r * j] = j == k ? (piv > 0 ? piv : 1.0) : (piv > 0 ? G[k + r * j] / piv : 0.0);
+    __syncthreads();
+    const int m = r - k - 1;
+    for (int idx = tid; idx < m * m; idx += blockDim.x) {
+      const int i = k + 1 + idx % m, j = k + 1 + idx / m;
+      if (i <= j) G[i + r * j] -= G[k + r * i] * G[k + r * j];
+    }
+    __syncthreads();
+  }
+  // R = R1 diag(l_r)^{1/2} V_r^T
+  for (int idx = tid; idx < r * n; idx += blockDim.x) {
+    const int i = idx % r, c = idx / r;
+    double t = 0;
+    for (int k = i; k < r; ++k) t += G[i + r * k] * sqrt(lam[keep_idx[k]]) * V[c + n * keep_idx[k]];
+    R[i + r * c] = t;
+  }
+  // Tq = T R1^{-1}: forward substitution along each row of T
+  for (int c = tid; c < n; c += blockDim.x)
+    for (int j = 0; j < r; ++j) {
+      double t = T[c + n * j];
+      for (int k = 0; k < j; ++k) t -= Tq[c + n * k] * G[k + r * j];
+      Tq[c + n * j] = t / G[j + r * j];
+    }
+  __syncthreads();
+  return r;
+}
+
 // BIG = false: the side matrices (E n <= AP_XY) and every small matrix in LDS,
 // the Householder reflectors of each side saved to `work` (2 AP_SIDE doubles
 // per gate).  BIG = true: the same steps with every matrix in a global
@@ -748,11 +943,15 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   constexpr int TH = BIG ? AP_TH_BIG : AP_TH;           // rows/cols of Theta
   constexpr int ENV = BIG ? AP_ENV_BIG : AP_ENV;        // external bond dimension
   constexpr int RS = ENV * ENV;                         // one environment root
+  static_assert(4 * ENVB_WORK <= AP_TH_BIG * AP_TH_BIG, "environment work must fit the Theta area");
   extern __shared__ __align__(16) double sm[];
   double *X, *Y, *Ru, *Rv, *roots, *small, *A, *VV, *scr_u, *scr_v;
+  double *tq_u = nullptr, *tq_v = nullptr;  // BIG: Q = X Tq of each side
   if constexpr (BIG) {
-    // per gate: QR of side u (E_u n_u), of side v, two work matrices
-    // (E max(n, d keep)), the R / factor matrices, Theta and V, the roots
+    // per gate in global memory: QR of side u (E_u n_u), of side v, two work
+    // matrices (E max(n, d keep)), the R / factor matrices, V of the SVD
+    // (when it does not fit in LDS beside Theta), the roots; in LDS the small
+    // vectors, the matrix of a mode product and Theta / U diag(s)
     double* w = work + (long long)blockIdx.x * prm.big_stride;
     scr_u = w;
     scr_v = scr_u + prm.big_q;
@@ -760,10 +959,12 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     Y = X + prm.big_x;
     Ru = Y + prm.big_x;
     Rv = Ru + AP_F_BIG;
-    A = Rv + AP_F_BIG;
-    VV = A + AP_TH_BIG * AP_TH_BIG;
+    VV = Rv + AP_F_BIG;
     roots = VV + AP_TH_BIG * AP_TH_BIG;
+    tq_u = roots + 2 * AP_MAXEXT * 2 * RS;
+    tq_v = tq_u + AP_N_BIG * AP_N_BIG;
     small = sm;
+    A = sm + AP_BIG_SMALL + AP_ENV_BIG * AP_ENV_BIG;
   } else {
     X = sm;                       // AP_XY: QR, then SVD workspace
     Y = X + AP_XY;                // AP_XY: mode-product partner, rebuild
@@ -783,9 +984,11 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   double* betav = vhv + AP_N_BIG;
   double* svec = betav + AP_N_BIG;
   double* red = svec + TH;
-  double* envw = red + 16;  // BIG: ENVB_WORK doubles per wave
+  double* ms = small + AP_BIG_SMALL;  // BIG: the matrix of a mode product
+  double* envw = A;                   // BIG: ENVB_WORK doubles per wave, phase 1 only
   __shared__ int sh_keep;
   __shared__ int order[TH];
+  __shared__ int gkeep[BIG ? AP_N_BIG : 1];  // BIG: the directions the Gram QR keeps
 
   const ApplyItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
@@ -838,7 +1041,10 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     double* oth = Y;
     for (int pa = 0; pa < g.next; ++pa) {
       const double* S = roots + ((side * AP_MAXEXT + pa) * 2) * RS;
-      ext_mode_product<ENV>(g, pa, cur, oth, g.n, S, false);
+      if constexpr (BIG)
+        ext_mode_product_fibre<ENV>(g, pa, cur, oth, g.n, S, false, ms);
+      else
+        ext_mode_product<ENV>(g, pa, cur, oth, g.n, S, false);
       double* t = cur;
       cur = oth;
       oth = t;
@@ -874,10 +1080,18 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       }
       for (int i = tid; i < n4 * g.n; i += blockDim.x) qx[i] = S[i];
     } else {
-      householder_qr(X, g.E, g.n, vhs, bts, red);
-      for (int idx = tid; idx < g.r * g.n; idx += blockDim.x) {
-        const int i = idx % g.r, c = idx / g.r;
-        R[i + g.r * c] = (i <= c) ? X[i + (long long)g.E * c] : 0.0;  // R[i][c], column-major (ld r)
+      if constexpr (BIG) {
+        // the Gram QR: X unchanged, Y as work, the rank r replaces min(E, n)
+        const int r = gram_qr(X, g.E, g.n, Y, A, AP_TH_BIG * AP_TH_BIG, svec, gkeep, R, side == 0 ? tq_u : tq_v,
+                              rot, red);
+        if (tid == 0) sgeo[side].r = r;
+        __syncthreads();
+      } else {
+        householder_qr(X, g.E, g.n, vhs, bts, red);
+        for (int idx = tid; idx < g.r * g.n; idx += blockDim.x) {
+          const int i = idx % g.r, c = idx / g.r;
+          R[i + g.r * c] = (i <= c) ? X[i + (long long)g.E * c] : 0.0;  // R[i][c], column-major (ld r)
+        }
       }
     }
     if (side == 0) phase_mark(prm, 12);
@@ -894,6 +1108,8 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   const double* G = gates + it.gate;
   const bool tr = Mr < Nc;
   const int M = tr ? Nc : Mr, N = tr ? Mr : Nc;
+  if constexpr (BIG)
+    if (M * N + N * N <= AP_TH_BIG * AP_TH_BIG) VV = A + M * N;  // else in the global workspace
   for (int idx = tid; idx < Mr * Nc; idx += blockDim.x) {
     const int row = idx % Mr, col = idx / Mr;
     const int ru = row % gu.r, sup = row / gu.r;
@@ -916,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   const int nsweeps = jacobi_svd(A, M, N, VV, rot, red);
   if (prm.ts && tid == 0) {
     prm.ts[blockIdx.x * 16 + 8] = (unsigned long long)nsweeps;
-    prm.ts[blockIdx.x * 16 + 9] = (unsigned long long)(tsqr_ok(gu) + tsqr_ok(gv));
+    prm.ts[blockIdx.x * 16 + 9] = (unsigned long long)((!BIG && tsqr_ok(gu)) + (!BIG && tsqr_ok(gv)));
   }
   for (int j = tid; j < N; j += blockDim.x) {
     double s = 0;
@@ -987,47 +1203,62 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     const SideGeom& g = side == 0 ? gu : gv;
     const double* F = side == 0 ? Ru : Rv;  // rows (r_i, s), r fastest
     const double* scr = side == 0 ? scr_u : scr_v;
-    // the side's reflectors: read in place from the global workspace (BIG),
-    // else copied back into LDS
-    const double* QX = BIG ? scr : X;
-    if (!BIG)
-      for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = scr[i];
     const int cols = g.d * keep;  // column c' = s + d b'
-    for (int idx = tid; idx < g.E * cols; idx += blockDim.x) {
-      const int c = (int)g.fE.div(idx), e = idx - c * g.E;
-      const int b = (int)g.fd.div(c), s = c - b * g.d;
-      Y[idx] = e < g.r ? F[(e + g.r * s) + (long long)(g.r * g.d) * b] : 0.0;
-    }
-    __syncthreads();
-    if (!BIG && tsqr_ok(g)) {
-      // Q [F; 0] = diag(Q_w) P Q_S [F; 0]: Q_S on the stacked rows (block w's
-      // first n rows) by wave 0, then each block's reflectors by its wave
-      const int n4 = 4 * g.n, wv = tid >> 6, E = g.E, nn = g.n;
-      const double* qx = scr + AP_XY;
-      // stacked row i -> block i / nn, row i % nn (i / nn by a multiply-high: i nn < 2^16)
-      const unsigned inv = (65536u + nn - 1) / nn;
-      if (wv == 0)
-        wave_apply_q16(qx, n4, n4, nn, qx + n4 * nn, qx + n4 * nn + 16, Y, E, cols, [=](int i) {
-          const int w = (int)(((unsigned)i * inv) >> 16);
-          return w * E / 4 + (i - w * nn);
-        });
-      __syncthreads();
-      if (wv < 4) {
-        const int r0 = wv * E / 4, r1 = (wv + 1) * E / 4;
-        const double* vb = side == 0 ? vh : vhv;
-        const double* bb = side == 0 ? beta : betav;
-        wave_apply_q16(X + r0, E, r1 - r0, nn, vb + wv * nn, bb + wv * nn, Y + r0, E, cols, [](int i) { return i; });
+    if constexpr (BIG) {
+      // Q [F] = X Tq F: W = Tq F (n x cols) in the Theta area (free once the
+      // factors are out), then Y = X W with X (psi with sqrt(env)) from scr
+      const double* Tq = side == 0 ? tq_u : tq_v;
+      for (int idx = tid; idx < g.n * cols; idx += blockDim.x) {
+        const int c = idx % g.n, cc = idx / g.n;
+        const int b = cc / g.d, s = cc - b * g.d;
+        double t = 0;
+        for (int k = 0; k < g.r; ++k) t += Tq[c + g.n * k] * F[(k + g.r * s) + (long long)(g.r * g.d) * b];
+        A[idx] = t;
       }
       __syncthreads();
+      rows_times(scr, g.E, g.n, A, cols, Y);
     } else {
-      apply_q(QX, g.E, g.r, side == 0 ? vh : vhv, side == 0 ? beta : betav, Y, cols);
+      // the side's reflectors, copied back into LDS
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = scr[i];
+      for (int idx = tid; idx < g.E * cols; idx += blockDim.x) {
+        const int c = (int)g.fE.div(idx), e = idx - c * g.E;
+        const int b = (int)g.fd.div(c), s = c - b * g.d;
+        Y[idx] = e < g.r ? F[(e + g.r * s) + (long long)(g.r * g.d) * b] : 0.0;
+      }
+      __syncthreads();
+      if (tsqr_ok(g)) {
+        // Q [F; 0] = diag(Q_w) P Q_S [F; 0]: Q_S on the stacked rows (block w's
+        // first n rows) by wave 0, then each block's reflectors by its wave
+        const int n4 = 4 * g.n, wv = tid >> 6, E = g.E, nn = g.n;
+        const double* qx = scr + AP_XY;
+        // stacked row i -> block i / nn, row i % nn (i / nn by a multiply-high: i nn < 2^16)
+        const unsigned inv = (65536u + nn - 1) / nn;
+        if (wv == 0)
+          wave_apply_q16(qx, n4, n4, nn, qx + n4 * nn, qx + n4 * nn + 16, Y, E, cols, [=](int i) {
+            const int w = (int)(((unsigned)i * inv) >> 16);
+            return w * E / 4 + (i - w * nn);
+          });
+        __syncthreads();
+        if (wv < 4) {
+          const int r0 = wv * E / 4, r1 = (wv + 1) * E / 4;
+          const double* vb = side == 0 ? vh : vhv;
+          const double* bb = side == 0 ? beta : betav;
+          wave_apply_q16(X + r0, E, r1 - r0, nn, vb + wv * nn, bb + wv * nn, Y + r0, E, cols, [](int i) { return i; });
+        }
+        __syncthreads();
+      } else {
+        apply_q(X, g.E, g.r, side == 0 ? vh : vhv, side == 0 ? beta : betav, Y, cols);
+      }
     }
     // M^{-1/2}: psi'[.. b ..] = sum_b' Y[.. b' ..] Si[b][b']
     double* cur = Y;
     double* oth = X;
     for (int pa = 0; pa < g.next; ++pa) {
       const double* Si = roots + ((side * AP_MAXEXT + pa) * 2 + 1) * RS;
-      ext_mode_product<ENV>(g, pa, cur, oth, cols, Si, true);
+      if constexpr (BIG)
+        ext_mode_product_fibre<ENV>(g, pa, cur, oth, cols, Si, true, ms);
+      else
+        ext_mode_product<ENV>(g, pa, cur, oth, cols, Si, true);
       double* t = cur;
       cur = oth;
       oth = t;
@@ -1124,7 +1355,7 @@ size_t apply_lds_bytes() {
                            AP_TH + 16);
 }
 size_t apply_big_lds_bytes() {
-  return sizeof(double) * (4 * 32 + 4 * AP_N_BIG + AP_TH_BIG + 16 + 4 * ENVB_WORK);
+  return sizeof(double) * (AP_BIG_SMALL + AP_ENV_BIG * AP_ENV_BIG + AP_TH_BIG * AP_TH_BIG);
 }
 
 // per-side sizes of a bond for the kernel limits: E (external size), n = d chi_b
@@ -1142,7 +1373,8 @@ SideSize side_size(const itnbp_net* net, int x, int e, const std::vector<int>& c
 // global workspace of k_apply_gates<true> per gate, in doubles, for side
 // matrices of q = E n and x = E max(n, cols) doubles
 long long big_stride(long long q, long long x) {
-  return 2 * q + 2 * x + 2LL * AP_F_BIG + 2LL * AP_TH_BIG * AP_TH_BIG + 2LL * AP_MAXEXT * 2 * AP_ENV_BIG * AP_ENV_BIG;
+  return 2 * q + 2 * x + 2LL * AP_F_BIG + (long long)AP_TH_BIG * AP_TH_BIG +
+         2LL * AP_MAXEXT * 2 * AP_ENV_BIG * AP_ENV_BIG + 2LL * AP_N_BIG * AP_N_BIG;
 }
 
 // Checks the call and fills the launch items; goff receives the number of gate
@@ -1353,7 +1585,6 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
         xs = std::max(xs, E * std::max(nn, cols));
       }
     ApplyParams pb = prm;
-    pb.ts = nullptr;
     pb.big_q = q;
     pb.big_x = xs;
     pb.big_stride = (big_stride(q, xs) + 31) & ~31LL;
@@ -1371,6 +1602,7 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
     const int pm1 = prof_mark(ctx);
     for (int c0 = nl; c0 < n && ek == hipSuccess; c0 += chunk) {
       const int cn = std::min(chunk, n - c0);
+      pb.ts = prm.ts ? prm.ts + (size_t)c0 * 16 : nullptr;
       hipLaunchKernelGGL(k_apply_gates<true>, dim3(cn), dim3(256), lds_big, ctx->stream,
                          (const ApplyItem*)b.items + c0, net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur],
                          (const double*)b.gates, pb, (double*)bw.p, b.te, svals ? b.sv : nullptr, b.kept, b.st);
@@ -1379,16 +1611,18 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
     prof_add(ctx, "apply_gates_global", pm1, prof_mark(ctx));
   }
   if (prm.ts && ek == hipSuccess) {
-    // mean duration of every phase over the gates that ran to the end, in us
+    // mean duration of every phase over the gates that ran to the end, in us,
+    // for the LDS kernel's gates [0, nl) and the global-memory variant's [nl, n)
     std::vector<unsigned long long> ts((size_t)n * 16);
     int khz = 0;
     ek = hipMemcpyAsync(ts.data(), prm.ts, ts.size() * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (ek == hipSuccess) ek = hipStreamSynchronize(ctx->stream);
     if (ek == hipSuccess) ek = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device);
-    if (ek == hipSuccess && khz > 0) {
+    for (int part = 0; part < 2 && ek == hipSuccess && khz > 0; ++part) {
+      const int i0 = part == 0 ? 0 : nl, i1 = part == 0 ? nl : n;
       double mean[7] = {0, 0, 0, 0, 0, 0, 0}, sweeps = 0, tsq = 0;
       int cnt = 0;
-      for (int i = 0; i < n; ++i) {
+      for (int i = i0; i < i1; ++i) {
         const unsigned long long* t = &ts[(size_t)i * 16];
         if (!t[7]) continue;
         ++cnt;
@@ -1396,20 +1630,22 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
         sweeps += (double)t[8];
         tsq += (double)t[9];
       }
-      const double c = std::max(cnt, 1);
+      if (cnt == 0) continue;
+      const double c = cnt;
       double sub[3] = {0, 0, 0}, load = 0;
-      for (int i = 0; i < n; ++i) {
+      for (int i = i0; i < i1; ++i) {
         const unsigned long long* t = &ts[(size_t)i * 16];
         if (!t[7] || !t[13]) continue;
         load += (double)(t[10] - t[1]) * 1e3 / khz;
         for (int p = 0; p < 3; ++p) sub[p] += (double)(t[11 + p] - t[10 + p]) * 1e3 / khz;
       }
-      fprintf(stderr, "  side u of qr (us): psi load %.1f, mode products %.1f, factorisation %.1f, save %.1f\n",
-              load / c, sub[0] / c, sub[1] / c, sub[2] / c);
+      const char* what = part == 0 ? "LDS kernel" : "global-memory variant";
+      fprintf(stderr, "  %s, side u of qr (us): psi load %.1f, mode products %.1f, factorisation %.1f, save %.1f\n",
+              what, load / c, sub[0] / c, sub[1] / c, sub[2] / c);
       fprintf(stderr,
-              "apply phases (us, mean over %d gates): roots %.1f, qr %.1f (%.2f tall-skinny sides), theta %.1f, "
+              "apply phases, %s (us, mean over %d gates): roots %.1f, qr %.1f (%.2f tall-skinny sides), theta %.1f, "
               "svd %.1f (%.1f sweeps), factors %.1f, rebuild_u %.1f, rebuild_v %.1f\n",
-              cnt, mean[0] / c, mean[1] / c, tsq / c, mean[2] / c, mean[3] / c, sweeps / c, mean[4] / c,
+              what, cnt, mean[0] / c, mean[1] / c, tsq / c, mean[2] / c, mean[3] / c, sweeps / c, mean[4] / c,
               mean[5] / c, mean[6] / c);
     }
   }
