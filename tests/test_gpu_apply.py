@@ -234,3 +234,42 @@ def test_large_bonds_run_on_the_global_memory_kernel(chi):
             assert np.linalg.norm(new[x] - r) <= 1e-10 * np.linalg.norm(r), x
         np.testing.assert_allclose(svals[i], s_ref, rtol=1e-10, atol=1e-13 * s_ref[0])
         assert errs[i] == pytest.approx(err_ref, rel=1e-8, abs=1e-15)
+
+
+def test_mixed_small_and_large_bonds_and_a_rank_deficient_side():
+    """One call with gates on both kernels on a 4x4 PEPS whose bonds at (2,2)
+    have chi = 12 and all others 4: the gates next to (2,2) run on the
+    global-memory variant, the far ones in LDS.  One large side is
+    rank-deficient (the slices b >= 6 of (2,2)'s gate bond vanish, so X has rank
+    n / 2 with E >> n: the Gram QR drops the null directions where the oracle's
+    Householder QR keeps zero rows).  Tensors, singular values and truncation
+    errors against the oracle."""
+    g = itnbp.named_grid((4, 4))
+    big_v = (2, 2)
+    link = [12 if big_v in (a, b) else 4 for (a, b) in g.edges]
+    psi = itnbp.random_tensornetwork(np.random.default_rng(41), np.float64, g, link_space=link)
+    t = psi[big_v].copy()
+    idx = [slice(None)] * t.ndim
+    idx[g.axis(big_v, (2, 3))] = slice(6, None)
+    t[tuple(idx)] = 0.0
+    psi[big_v] = t
+    bpc = converged_cache(psi)
+    og = oracle_graph(g)
+    opsi = oracle_psi(psi)
+    msgs = to_index(g, bpc.messages())
+    deg = {v: g.degree(v) for v in g.vertices}
+    # global-memory variant: the first two; LDS kernel: the last two
+    bonds = [((2, 2), (2, 3)), ((1, 1), (1, 2)), ((3, 4), (4, 4)), ((4, 1), (4, 2))]
+    gates = [O.expm_sym(O.two_site(O.X, O.Z) + O.ising_bond_hamiltonian(2.5, deg[u], deg[v]), -0.3) for u, v in bonds]
+    svals = []
+    errs = itnbp.apply_gates_(bpc, [itnbp.Gate(G, b) for G, b in zip(gates, bonds)], maxdim=6, normalize=True,
+                              callback=lambda singular_values, truncation_error: svals.append(singular_values))
+    new = bpc.tensornetwork()
+    for i, ((u, v), G) in enumerate(zip(bonds, gates)):
+        ref, s_ref, err_ref = O.apply_gate(G, og, opsi, msgs, g.index[u], g.index[v], maxdim=6, normalize=True)
+        assert new.bond_dim(u, v) == len(s_ref)
+        for x in (u, v):
+            r = ref[g.index[x]]
+            assert np.linalg.norm(new[x] - r) <= 1e-10 * np.linalg.norm(r), (u, v, x)
+        np.testing.assert_allclose(svals[i], s_ref, rtol=1e-10, atol=1e-13 * s_ref[0])
+        assert errs[i] == pytest.approx(err_ref, rel=1e-8, abs=1e-15)
