@@ -492,7 +492,7 @@ def main():
         t_bp = time.perf_counter() - t
         deg = {v: g2.degree(v) for v in g2.vertices}
 
-        def gate(u, v):
+        def gate(u, v, deg=deg):
             H = np.kron(np.diag([1.0, -1.0]), np.diag([1.0, -1.0])) + 2.5 * (
                 np.kron(np.eye(2), X) / deg[u] + np.kron(X, np.eye(2)) / deg[v])
             w, U = np.linalg.eigh(H)
@@ -513,6 +513,20 @@ def main():
                                             "max_truncerr": max(errs.values()),
                                             "gate_build_seconds_untimed": t_gates}
         del c2
+        # a round at chi = 16 (8x8, 112 bonds): the global-memory gate kernel
+        g16 = itnbp.named_grid((8, 8))
+        p16 = itnbp.random_tensornetwork(np.random.default_rng(16), np.float64, g16, link_space=16)
+        c16 = itnbp.update(itnbp.initialize_cache(p16), maxiter=60, tol=1e-10, edge_sequence="parallel")
+        deg16 = {v: g16.degree(v) for v in g16.vertices}
+        gates16 = {(u, v): gate(u, v, deg16) for (u, v) in g16.edges}
+        ctx.check(c16.lib.itnbp_synchronize(ctx.h))
+        t = time.perf_counter()
+        errs16 = itnbp.simple_update_round_(c16, lambda u, v: gates16[(u, v)], maxdim=16, normalize=True)
+        ctx.check(c16.lib.itnbp_synchronize(ctx.h))
+        t16 = time.perf_counter() - t
+        extras["apply_round_8x8_chi16"] = {"bonds": len(errs16), "seconds": t16, "svd_per_s": len(errs16) / t16,
+                                           "max_truncerr": max(errs16.values())}
+        del c16
         # 256 heterogeneous instances (L x L grids, L in 6..12, or heavy-hex), chi in
         # {4, 6, 8, 12, 16}, FP32 N(0, 1/chi), 50 Jacobi sweeps in shared launches
         inst3 = itnbp.random_instance_set(np.random.default_rng(3), 256)

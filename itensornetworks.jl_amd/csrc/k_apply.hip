@@ -768,6 +768,92 @@ __device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, doubl
   return sweep + 1;
 }
 
+// One-sided Jacobi SVD of A (M x N column-major in LDS, N <= M <= 32) by one
+// wave with both matrices in registers: lane (c, h) = (l % 32, l / 32) holds
+// rows [16 h, 16 h + 16) of column c of A and of V.  A step fetches the
+// partner column of the round-robin schedule (rr_pair's) with lane shuffles,
+// joins the two halves' sums with one cross-half shuffle, and both lanes of a
+// pair derive the same rotation; the rotation threshold, the rounding floor and
+// the stopping rule are those of jacobi_svd.  A step needs no LDS and no
+// barrier (jacobi_svd's steps are LDS round trips with a workgroup barrier
+// each, ~1.3 us).  Writes A <- U diag(s) and V (N x N) back; returns the number
+// of sweeps.  Every lane of the wave must call it.
+__device__ int wave_jacobi_svd32(double* A, int M, int N, double* V) {
+  const int l = threadIdx.x & 63, c = l & 31, h = l >> 5;
+  double a[16], v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int row = 16 * h + i;
+    a[i] = (c < N && row < M) ? A[row + M * c] : 0.0;
+    v[i] = (c < N && row == c) ? 1.0 : 0.0;
+  }
+  double fro = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) fro += a[i] * a[i];
+  fro = wave_sum_d(fro);
+  const int ne = N + (N & 1);
+  const double tol = 8 * 2.220446049250313e-16 * sqrt((double)M);
+  const double floor2 = 2.220446049250313e-16 * 2.220446049250313e-16 * fro;
+  int sweep = 0;
+  for (; sweep < 60 && N > 1; ++sweep) {
+    bool rotated = false;
+    for (int st = 0; st < ne - 1; ++st) {
+      // partner of column c in step st (the inverse of rr_pair's placement)
+      int q = c;
+      if (c < ne) {
+        int t = c - 1 - st;
+        if (t < 0) t += ne - 1;
+        const int pos = c == 0 ? 0 : 1 + t;
+        const int pp = ne - 1 - pos;
+        int u = pp - 1 + st;
+        if (u >= ne - 1) u -= ne - 1;
+        q = pp == 0 ? 0 : 1 + u;
+      }
+      const int src = q + 32 * h;
+      double b[16];
+      double own = 0, oth = 0, cr = 0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        b[i] = __shfl(a[i], src);
+        own += a[i] * a[i];
+        oth += b[i] * b[i];
+        cr += a[i] * b[i];
+      }
+      own += __shfl_xor(own, 32);
+      oth += __shfl_xor(oth, 32);
+      cr += __shfl_xor(cr, 32);
+      const bool isp = c < q;  // this lane holds the pair's first column
+      const double alpha = isp ? own : oth, beta = isp ? oth : own;
+      // first column: c a_p - s a_q; second: s a_p + c a_q (the partner is the
+      // other); a lane that does not rotate keeps c = 1, s = 0 (exact)
+      double cs = 1.0, sn = 0.0;
+      if (c < N && q < N && c != q && alpha > floor2 && beta > floor2 && fabs(cr) > tol * sqrt(alpha * beta)) {
+        const double zeta = (beta - alpha) / (2 * cr);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+        cs = 1 / sqrt(1 + t * t);
+        sn = cs * t;
+        rotated = true;
+      }
+      const double sp = isp ? -sn : sn;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) a[i] = cs * a[i] + sp * b[i];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const double wi = __shfl(v[i], src);
+        v[i] = cs * v[i] + sp * wi;
+      }
+    }
+    if (__ballot(rotated) == 0) break;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int row = 16 * h + i;
+    if (c < N && row < M) A[row + M * c] = a[i];
+    if (c < N && row < N) V[row + N * c] = v[i];
+  }
+  return sweep + 1;
+}
+
 // ---- Gram-based QR of a side, global-memory variant ----------------------
 // The QR of a tall side enters simple_update_bp only through R (in Theta) and
 // Q (in the rebuild), and any factorisation X = Q R with orthonormal Q gives
@@ -1129,7 +1215,15 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   __syncthreads();
   phase_mark(prm, 3);
   // 4. SVD
-  const int nsweeps = jacobi_svd(A, M, N, VV, rot, red);
+  // Theta up to 32 x 32: one wave with the matrices in registers; larger ones
+  // (global-memory variant) by the workgroup
+  int nsweeps = 0;
+  if (M <= 32) {
+    if (tid < 64) nsweeps = wave_jacobi_svd32(A, M, N, VV);
+    __syncthreads();
+  } else {
+    nsweeps = jacobi_svd(A, M, N, VV, rot, red);
+  }
   if (prm.ts && tid == 0) {
     prm.ts[blockIdx.x * 16 + 8] = (unsigned long long)nsweeps;
     prm.ts[blockIdx.x * 16 + 9] = (unsigned long long)((!BIG && tsqr_ok(gu)) + (!BIG && tsqr_ok(gv)));
