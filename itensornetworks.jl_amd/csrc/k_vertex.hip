@@ -24,6 +24,8 @@
 
 namespace {
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 constexpr int SV_MAXCHI = 16;
 constexpr int SV_LD = SV_MAXCHI + 1;
 constexpr int SV_THREADS = 256;
@@ -370,37 +372,56 @@ __device__ __forceinline__ void vertex_eig_task(const VDesc* __restrict__ vdesc,
       }
       __syncthreads();
     }
-    const int G = SV_THREADS / nn;  // fibre groups (>= 1 since n <= 16)
-    const int g = tid / nn, e = tid % nn;
-    double acc = 0;
-    if (n > 8 && g < G) {
-      const int r = e % n, r2 = e / n;
-      // four fibres per step, each with its own accumulator, so that their LDS
-      // loads are in flight together
-      double a1 = 0, a2 = 0, a3 = 0;
-      int f = g;
-      for (; f + 3 * G < F; f += 4 * G) {
-        const int b0 = fb[f], b1 = fb[f + G], b2 = fb[f + 2 * G], b3 = fb[f + 3 * G];
-        const double w0 = fw[f], w1 = fw[f + G], w2 = fw[f + 2 * G], w3 = fw[f + 3 * G];
-        const double p0 = (double)Y[b0 + r * st] * (double)Y[b0 + r2 * st];
-        const double p1 = (double)Y[b1 + r * st] * (double)Y[b1 + r2 * st];
-        const double p2 = (double)Y[b2 + r * st] * (double)Y[b2 + r2 * st];
-        const double p3 = (double)Y[b3 + r * st] * (double)Y[b3 + r2 * st];
-        acc += w0 * p0;
-        a1 += w1 * p1;
-        a2 += w2 * p2;
-        a3 += w3 * p3;
-      }
-      for (; f < F; f += G) acc += fw[f] * (double)Y[fb[f] + r * st] * (double)Y[fb[f] + r2 * st];
-      acc += (a1 + a2) + a3;
-    }
+    // T' = sum_f w_f y_f y_f^T on the FP64 matrix cores: v_mfma_f64_16x16x4f64
+    // with A[i][k] = w_f Y[f, i] and B[k][j] = Y[f, j] for the four fibres
+    // f = 4 b + k of a fibre block b (lane i + 16 k loads Y[f, i] once; rows and
+    // columns >= n are zero); wave wv takes the blocks b = wv (mod 4), two
+    // accumulators in turn so that consecutive MFMAs do not wait on each other.
+    // D[4 v + l / 16][l % 16] is value v of lane l; the four waves' partial
+    // Grams meet in the Jacobi work area (idle after step 2) and are summed in
+    // a fixed order.  (For n <= 8 the VALU form above is faster: three
+    // quarters of the 16 x 16 tile would be padding.)
     if (n > 8) {
-      gpart[tid] = acc;
+      const int li = l & 15, lk = l >> 4;
+      f64x4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+      const int nblk = (F + 3) >> 2;
+      int b4 = wv;
+      for (; b4 + 4 < nblk; b4 += 8) {
+        const int f0 = 4 * b4 + lk, f1 = f0 + 16;
+        double y0 = 0, y1 = 0, w0 = 0, w1 = 0;
+        if (li < n) {
+          if (f0 < F) {
+            y0 = (double)Y[fb[f0] + li * st];
+            w0 = fw[f0];
+          }
+          if (f1 < F) {
+            y1 = (double)Y[fb[f1] + li * st];
+            w1 = fw[f1];
+          }
+        }
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w0 * y0, y0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w1 * y1, y1, acc1, 0, 0, 0);
+      }
+      for (; b4 < nblk; b4 += 4) {
+        const int f0 = 4 * b4 + lk;
+        double y0 = 0, w0 = 0;
+        if (li < n && f0 < F) {
+          y0 = (double)Y[fb[f0] + li * st];
+          w0 = fw[f0];
+        }
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w0 * y0, y0, acc0, 0, 0, 0);
+      }
+      double* part = As;  // 4 x 256 doubles
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 4 * v + lk;
+        if (row < n && li < n) part[wv * 256 + row * 16 + li] = acc0[v] + acc1[v];
+      }
       __syncthreads();
       if (tid < nn) {
-        double t = 0;
-        for (int gg = 0; gg < G; ++gg) t += gpart[gg * nn + tid];
-        Tp[(tid % n) * SV_LD + tid / n] = t;  // T'[r][r']
+        const int rr = tid % n, r2 = tid / n;
+        Tp[rr * SV_LD + r2] = (part[rr * 16 + r2] + part[256 + rr * 16 + r2]) +
+                              (part[512 + rr * 16 + r2] + part[768 + rr * 16 + r2]);
       }
       __syncthreads();
     }
