@@ -334,7 +334,7 @@ def main():
         peak_f = PEAK_F64_TFLOPS if k == "f64" else PEAK_BF16_TFLOPS
         whole[k] = roofline_of(sweep_bytes[k], F, sr["ms_per_sweep"] / 1e3, peak_f)
         whole[k]["ms_per_sweep"] = sr["ms_per_sweep"]
-    whole["chosen_solve"] = roofline_of(sweep_bytes.get("bf16x3", sweep_bytes["f64"]), F,
+    whole["chosen_solve"] = roofline_of(sweep_bytes.get("f16x2", sweep_bytes.get("bf16x3", sweep_bytes["f64"])), F,
                                         wall / sweeps, PEAK_BF16_TFLOPS if low else PEAK_F64_TFLOPS)
     whole["chosen_solve"]["ms_per_sweep"] = wall / sweeps * 1e3
 
@@ -561,9 +561,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3" if low else "f64",
-        "dtype_detail": ("BF16 MFMA operands with FP32 accumulation: FP16 intermediates while the residual "
-                         "> 1e-2, then every operand in three BF16 terms with FP32 intermediates") if low else "f64",
+        "dtype": ("f16" if a.chosen == "bf16_f16x2" else "bf16") if low else "f64",
+        "dtype_detail": (
+            ("16-bit MFMA operands with FP32 accumulation: BF16 site tensors, one-term FP16 intermediates while "
+             f"the residual > {a.switch_tol:g}, then every operand in two FP16 terms (hi + lo) with FP32 "
+             "intermediates; FP64 partial sums and back-rotation")
+            if a.chosen == "bf16_f16x2" else
+            ("BF16 MFMA operands with FP32 accumulation: one-term FP16 intermediates while the residual > "
+             f"{a.switch_tol:g}, then every operand in three BF16 terms with FP32 intermediates; FP64 partial "
+             "sums and back-rotation")) if low else "f64",
         "data": "synthetic: seeded N(0, 1/chi) site tensors rounded to BF16",
         "config": {"workload": a.workload, "lattice": list(W["dims"]), "chi": W["chi"], "d": 2,
                    "schedule": "jacobi", "messages_per_sweep": nmsg,
