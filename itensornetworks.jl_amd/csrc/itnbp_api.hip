@@ -620,6 +620,7 @@ int itnbp_net_destroy(itnbp_net* net) {
   free_buf(net->red);
   free_buf(net->aux);
   free_buf(net->mdesc);
+  free_buf(net->apply_ws);
   release_fast_state(net);
   release_small_state(net);
   delete net;

@@ -151,6 +151,7 @@ struct itnbp_net {
   VDesc* d_vdesc = nullptr;
   DevBuf scratch, tasks, stats, partials, red, aux;
   DevBuf mdesc;          // per message: offset (long long) and chi (int), for device-side initialisation
+  DevBuf apply_ws;       // itnbp_apply_gates: launch items, gates, LDS-kernel work, outputs (grows, kept)
   void* fast = nullptr;  // state of the specialised pipeline (k_fast.hip)
   void* small = nullptr; // state of the per-vertex small-chi kernel (k_vertex.hip)
 };

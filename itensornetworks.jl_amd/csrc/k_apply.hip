@@ -776,8 +776,9 @@ __device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, doubl
 // pair derive the same rotation; the rotation threshold, the rounding floor and
 // the stopping rule are those of jacobi_svd.  A step needs no LDS and no
 // barrier (jacobi_svd's steps are LDS round trips with a workgroup barrier
-// each, ~1.3 us).  Writes A <- U diag(s) and V (N x N) back; returns the number
-// of sweeps.  Every lane of the wave must call it.
+// each, ~1.3 us).  Writes A <- U diag(s) and, with WANT_V, V (N x N) back;
+// returns the number of sweeps.  Every lane of the wave must call it.
+template <bool WANT_V>
 __device__ int wave_jacobi_svd32(double* A, int M, int N, double* V) {
   const int l = threadIdx.x & 63, c = l & 31, h = l >> 5;
   double a[16], v[16];
@@ -837,10 +838,12 @@ __device__ int wave_jacobi_svd32(double* A, int M, int N, double* V) {
       const double sp = isp ? -sn : sn;
 #pragma unroll
       for (int i = 0; i < 16; ++i) a[i] = cs * a[i] + sp * b[i];
+      if (WANT_V) {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const double wi = __shfl(v[i], src);
-        v[i] = cs * v[i] + sp * wi;
+        for (int i = 0; i < 16; ++i) {
+          const double wi = __shfl(v[i], src);
+          v[i] = cs * v[i] + sp * wi;
+        }
       }
     }
     if (__ballot(rotated) == 0) break;
@@ -849,7 +852,7 @@ __device__ int wave_jacobi_svd32(double* A, int M, int N, double* V) {
   for (int i = 0; i < 16; ++i) {
     const int row = 16 * h + i;
     if (c < N && row < M) A[row + M * c] = a[i];
-    if (c < N && row < N) V[row + N * c] = v[i];
+    if (WANT_V && c < N && row < N) V[row + N * c] = v[i];
   }
   return sweep + 1;
 }
@@ -1215,11 +1218,15 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   __syncthreads();
   phase_mark(prm, 3);
   // 4. SVD
-  // Theta up to 32 x 32: one wave with the matrices in registers; larger ones
-  // (global-memory variant) by the workgroup
+  // Theta up to 32 x 32: one wave with A in registers, V of the kept columns
+  // afterwards from a copy of Theta (V_o = Theta^T A_o / s_o^2); larger ones
+  // (global-memory variant) by the workgroup, V accumulated
   int nsweeps = 0;
+  double* Th0 = A + 2048;  // M <= 32: beside A and V in the Theta area
   if (M <= 32) {
-    if (tid < 64) nsweeps = wave_jacobi_svd32(A, M, N, VV);
+    for (int i = tid; i < M * N; i += blockDim.x) Th0[i] = A[i];
+    __syncthreads();
+    if (tid < 64) nsweeps = wave_jacobi_svd32<false>(A, M, N, VV);
     __syncthreads();
   } else {
     nsweeps = jacobi_svd(A, M, N, VV, rot, red);
@@ -1273,6 +1280,19 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   // finite, is left untouched (the host reports the error); every write to psi
   // comes after this point
   if (keep < 0) return;
+  if (M <= 32) {
+    // the kept right singular vectors: V_o = Theta^T U_o / s_o = Theta^T A_o / s_o^2
+    // (a zero singular value gets a zero column, which its sqrt(s) = 0 factor
+    // multiplies anyway)
+    for (int idx = tid; idx < N * keep; idx += blockDim.x) {
+      const int i = idx % N, o = order[idx / N];
+      const double s2 = svec[o] * svec[o];
+      double t = 0;
+      for (int r = 0; r < M; ++r) t += Th0[r + M * i] * A[r + M * o];
+      VV[i + N * o] = s2 > 0 ? t / s2 : 0.0;
+    }
+    __syncthreads();
+  }
   // left factor Lf (Mr x keep) = U sqrt(S) into Ru, right factor Rf (Nc x keep) = V sqrt(S) into Rv;
   // A = U diag(s) (M x N), V = VV (N x N); if transposed, Theta = VV S U^T
   for (int idx = tid; idx < Mr * keep; idx += blockDim.x) {
@@ -1852,12 +1872,11 @@ extern "C" int itnbp_apply_gates(itnbp_net* net, int n, const int* edge_ids, con
     apply_buf_sizes(n, goff, mk, sz);
     size_t total = 0;
     for (size_t b : sz) total += b;
-    DevBuf d;
-    ITNBP_RETCHK(ensure_buf(ctx, d, total));
+    // the net keeps the workspace between calls (a gate round is several calls)
+    ITNBP_RETCHK(ensure_buf(ctx, net->apply_ws, total));
     std::vector<int> kp(n, -1);
     const int rc = run_apply(net, n, items, gates, goff, maxdim, cutoff, normalize, truncerr, svals, mk, kp.data(),
-                             carve_apply_bufs((char*)d.p, n, goff, mk), &big);
-    free_buf(d);
+                             carve_apply_bufs((char*)net->apply_ws.p, n, goff, mk), &big);
     std::vector<int> chi_cut = net->chi;
     if (rc != ITNBP_OK) {
       // the gates that ran keep their new bonds; a refused gate's bond (its
