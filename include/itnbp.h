@@ -244,7 +244,9 @@ int itnbp_sweeps_vjp(itnbp_net* net, int nsweeps, int checkpoint, const double* 
  * dimension kept[i]). Writes dpsi_in (per gate psi_u then psi_v, the current
  * shapes) and dgates (the gates' layout), and adds the cotangents of the
  * environment messages into dmsgs (all messages, itnbp_net_msg_numel
- * doubles). ITNBP_ERR_ARG if kept differs from the recomputed rank. */
+ * doubles). ITNBP_ERR_ARG if kept differs from the recomputed rank.
+ * Limits: those of itnbp_apply_gates' LDS kernel (external bonds <= 8, Theta
+ * <= 32 x 32); other bonds return ITNBP_ERR_UNSUPPORTED. */
 int itnbp_apply_vjp(itnbp_net* net, int n, const int* edge_ids, const double* gates, int maxdim,
                     double cutoff, int normalize, const int* kept, const double* dpsi_out,
                     double* dpsi_in, double* dgates, double* dmsgs);
