@@ -502,6 +502,9 @@ def main():
         t = time.perf_counter()
         gates2 = {(u, v): gate(u, v) for (u, v) in g2.edges}
         t_gates = time.perf_counter() - t
+        # one untimed round on a copy first (workspaces, clocks)
+        itnbp.simple_update_round_(c2.copy(), lambda u, v: gates2[(u, v)], maxdim=8, normalize=True)
+        ctx.check(c2.lib.itnbp_synchronize(ctx.h))
         t = time.perf_counter()
         errs = itnbp.simple_update_round_(c2, lambda u, v: gates2[(u, v)], maxdim=8, normalize=True)
         ctx.check(c2.lib.itnbp_synchronize(ctx.h))
@@ -519,6 +522,7 @@ def main():
         c16 = itnbp.update(itnbp.initialize_cache(p16), maxiter=60, tol=1e-10, edge_sequence="parallel")
         deg16 = {v: g16.degree(v) for v in g16.vertices}
         gates16 = {(u, v): gate(u, v, deg16) for (u, v) in g16.edges}
+        itnbp.simple_update_round_(c16.copy(), lambda u, v: gates16[(u, v)], maxdim=16, normalize=True)
         ctx.check(c16.lib.itnbp_synchronize(ctx.h))
         t = time.perf_counter()
         errs16 = itnbp.simple_update_round_(c16, lambda u, v: gates16[(u, v)], maxdim=16, normalize=True)

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1753,9 +1754,17 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
   for (int i = 0; i < n; ++i)
     if (big && (*big)[i]) items.push_back(items_in[i]);
   const int nb = n - nl;
-  hipError_t e1 = hipMemcpyAsync(b.items, items.data(), n * sizeof(ApplyItem), hipMemcpyHostToDevice, ctx->stream);
-  hipError_t e2 = hipMemcpyAsync(b.gates, gates, goff * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e1 != hipSuccess || e2 != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: upload failed");
+  // one upload of the launch items and the gates, laid out as in b (the gates
+  // follow the items at their aligned offset)
+  size_t sz[7];
+  apply_buf_sizes(n, goff, mk, sz);
+  std::vector<char> up(sz[0] + (size_t)goff * 8);
+  std::memcpy(up.data(), items.data(), n * sizeof(ApplyItem));
+  std::memcpy(up.data() + sz[0], gates, (size_t)goff * 8);
+  if ((char*)b.gates != (char*)b.items + sz[0])
+    return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: workspace layout");
+  if (hipMemcpyAsync(b.items, up.data(), up.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: upload failed");
   ApplyParams prm;
   prm.maxdim = maxdim;
   prm.cutoff = cutoff;
@@ -1864,12 +1873,19 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
   }
   std::vector<int> st(n);
   std::vector<int> kp(n);
-  if (ek == hipSuccess) ek = hipMemcpyAsync(truncerr, b.te, n * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (ek == hipSuccess && svals) ek = hipMemcpyAsync(svals, b.sv, (size_t)n * mk * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (ek == hipSuccess) ek = hipMemcpyAsync(kp.data(), b.kept, n * 4, hipMemcpyDeviceToHost, ctx->stream);
-  if (ek == hipSuccess) ek = hipMemcpyAsync(st.data(), b.st, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  // one read-back of the outputs (truncation errors, singular values, kept
+  // ranks, status: consecutive in b)
+  std::vector<char> dn(sz[3] + sz[4] + sz[5] + (size_t)n * 4);
+  if (ek == hipSuccess && ((char*)b.sv != (char*)b.te + sz[3] || (char*)b.kept != (char*)b.sv + sz[4] ||
+                           (char*)b.st != (char*)b.kept + sz[5]))
+    return itnbp_fail(ctx, ITNBP_ERR_HIP, "apply: workspace layout");
+  if (ek == hipSuccess) ek = hipMemcpyAsync(dn.data(), b.te, dn.size(), hipMemcpyDeviceToHost, ctx->stream);
   if (ek == hipSuccess) ek = hipStreamSynchronize(ctx->stream);
   if (ek != hipSuccess) return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: ") + hipGetErrorString(ek));
+  std::memcpy(truncerr, dn.data(), (size_t)n * 8);
+  if (svals) std::memcpy(svals, dn.data() + sz[3], (size_t)n * mk * 8);
+  std::memcpy(kp.data(), dn.data() + sz[3] + sz[4], (size_t)n * 4);
+  std::memcpy(st.data(), dn.data() + sz[3] + sz[4] + sz[5], (size_t)n * 4);
   // a gate the kernel refused (its bond untouched) reports kept = -1
   for (int i = 0; i < n; ++i)
     if (st[i]) kp[i] = -1;

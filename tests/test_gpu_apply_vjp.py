@@ -193,3 +193,24 @@ def test_gradient_of_a_tebd_step(checkpoint):
         fd = (vals[0] - vals[1]) / (2 * h)
         print(f"TEBD step, direction {trial}: gradient {pred:.10f} finite differences {fd:.10f}", flush=True)
         assert pred == pytest.approx(fd, rel=1e-6, abs=1e-8)
+
+
+def test_apply_vjp_refuses_bonds_past_the_lds_kernel():
+    """The adjoint covers the forward's LDS kernel only: a bond with an external
+    bond of dimension 12 (the forward runs it on the global-memory variant) is
+    refused with ITNBP_ERR_UNSUPPORTED, and the cache is left as it was."""
+    g = itnbp.named_grid((3, 3))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(91), np.float64, g, link_space=12)
+    bpc = itnbp.update(itnbp.initialize_cache(psi), maxiter=30, tol=1e-10, edge_sequence="parallel")
+    before = bpc.tensornetwork()
+    G = O.expm_sym(O.ising_bond_hamiltonian(2.5, 4, 3), -0.1)
+    gates = [itnbp.Gate(G, ((2, 2), (2, 3)))]
+    new = bpc.copy()
+    errs = itnbp.apply_gates_(new, gates, maxdim=12)  # the forward runs
+    assert len(errs) == 1
+    dout = [(np.ones_like(new.tensornetwork()[(2, 2)]), np.ones_like(new.tensornetwork()[(2, 3)]))]
+    with pytest.raises(RuntimeError, match="external bond dimensions <= 8"):
+        itnbp.apply_gates_vjp(bpc, gates, [12], dout, maxdim=12)
+    after = bpc.tensornetwork()
+    for v in g.vertices:
+        np.testing.assert_array_equal(after[v], before[v])
