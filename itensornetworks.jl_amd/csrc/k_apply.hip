@@ -1156,6 +1156,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   __shared__ int sh_keep;
   __shared__ int order[TH];
   __shared__ int gkeep[BIG ? AP_N_BIG : 1];  // BIG: the directions the Gram QR keeps
+  __shared__ int qgemm[2];                    // LDS kernel: side rebuilt as X R^{-1} F
 
   const ApplyItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
@@ -1224,6 +1225,11 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     double* bts = side == 0 ? beta : betav;
     double* R = side == 0 ? Ru : Rv;
     double* scr = side == 0 ? scr_u : scr_v;
+    if (!BIG) {
+      // X itself, for a rebuild by Q = X R^{-1} (kept if the side qualifies)
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) scr[i] = X[i];
+      __syncthreads();
+    }
     if (side == 0) phase_mark(prm, 11);
     if (!BIG && tsqr_ok(g)) {
       // the stacked R factors S (4n x n) in Y, which is free after the copy to X
@@ -1262,7 +1268,31 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       }
     }
     if (side == 0) phase_mark(prm, 12);
-    for (int i = tid; i < g.E * g.n; i += blockDim.x) scr[i] = X[i];
+    if constexpr (BIG) {
+      for (int i = tid; i < g.E * g.n; i += blockDim.x) scr[i] = X[i];
+    } else {
+      // a tall side whose R has no diagonal entry below 1e-5 of its largest
+      // keeps X and R: the rebuild's Q [F; 0] is then X (R^{-1} F), one small
+      // product, instead of applying the n reflectors (latency-bound), with an
+      // orthogonality error ~eps cond(X); other sides keep their reflectors
+      __syncthreads();  // R is complete
+      if (tid == 0) {
+        bool ok = g.E >= g.n;
+        double dmax = 0, dmin = INFINITY;
+        for (int j = 0; j < g.n && ok; ++j) {
+          const double dj = fabs(R[j + g.r * j]);
+          dmax = dj > dmax ? dj : dmax;
+          dmin = dj < dmin ? dj : dmin;
+        }
+        qgemm[side] = ok && dmax > 0 && isfinite(dmax) && dmin >= 1e-5 * dmax;
+      }
+      __syncthreads();
+      if (qgemm[side]) {
+        for (int i = tid; i < g.n * g.n; i += blockDim.x) scr[AP_XY + i] = R[i];  // ld n (r = n)
+      } else {
+        for (int i = tid; i < g.E * g.n; i += blockDim.x) scr[i] = X[i];  // the reflectors
+      }
+    }
     __syncthreads();
     if (side == 0) phase_mark(prm, 13);
   }
@@ -1431,6 +1461,19 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
       }
       __syncthreads();
       rows_times(scr, g.E, g.n, A, cols, Y);
+    } else if (qgemm[side]) {
+      // W = R^{-1} F (n x cols, back substitution per column) in LDS X, Y = X_side W
+      const double* Rg = scr + AP_XY;
+      for (int cc = tid; cc < cols; cc += blockDim.x) {
+        const int b = cc / g.d, s = cc - b * g.d;
+        for (int i = g.n - 1; i >= 0; --i) {
+          double t = F[(i + g.n * s) + (long long)(g.n * g.d) * b];
+          for (int k = i + 1; k < g.n; ++k) t -= Rg[i + g.n * k] * X[k + g.n * cc];
+          X[i + g.n * cc] = t / Rg[i + g.n * i];
+        }
+      }
+      __syncthreads();
+      rows_times(scr, g.E, g.n, X, cols, Y);
     } else {
       // the side's reflectors, copied back into LDS
       for (int i = tid; i < g.E * g.n; i += blockDim.x) X[i] = scr[i];
