@@ -692,7 +692,7 @@ __device__ void wave_apply_q16(const double* V, int ldv, int rows, int n, const 
 }
 
 // One-sided Jacobi SVD of A (M x N, column-major, M >= N) in place:
-// A <- U diag(s), V (N x N column-major) accumulated.  A pair is rotated while
+// A <- U diag(s), V (N x N column-major) accumulated unless V is null.  A pair is rotated while
 // |<a_p, a_q>| > tol |a_p| |a_q| with tol = 8 eps sqrt(M), the rounding error
 // of the inner product itself, and while both columns are above the rounding
 // level of A (|a|^2 > eps^2 ||A||_F^2): a column at that level is zero to
@@ -702,7 +702,8 @@ __device__ void wave_apply_q16(const double* V, int ldv, int rows, int n, const 
 // barrier (before the next step's pairing).  Returns the number of sweeps.
 __device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, double* red) {
   const int tid = threadIdx.x;
-  for (int i = tid; i < N * N; i += blockDim.x) V[i] = (i % N == i / N) ? 1.0 : 0.0;
+  if (V)
+    for (int i = tid; i < N * N; i += blockDim.x) V[i] = (i % N == i / N) ? 1.0 : 0.0;
   double fro = 0;
   for (int i = tid; i < M * N; i += blockDim.x) fro += A[i] * A[i];
   fro = block_sum_d(fro, red);
@@ -754,11 +755,12 @@ __device__ int jacobi_svd(double* A, int M, int N, double* V, double* rot, doubl
           A[i + (long long)M * p] = c * ap - s * aq;
           A[i + (long long)M * q] = s * ap + c * aq;
         }
-        for (int i = gl; i < N; i += G) {
-          const double vp = V[i + N * p], vq = V[i + N * q];
-          V[i + N * p] = c * vp - s * vq;
-          V[i + N * q] = s * vp + c * vq;
-        }
+        if (V)
+          for (int i = gl; i < N; i += G) {
+            const double vp = V[i + N * p], vq = V[i + N * q];
+            V[i + N * p] = c * vp - s * vq;
+            V[i + N * q] = s * vp + c * vq;
+          }
       }
       __syncthreads();
     }
@@ -1111,6 +1113,7 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   constexpr int ENV = BIG ? AP_ENV_BIG : AP_ENV;        // external bond dimension
   constexpr int RS = ENV * ENV;                         // one environment root
   static_assert(4 * ENVB_WORK <= AP_TH_BIG * AP_TH_BIG, "environment work must fit the Theta area");
+  static_assert(2 * AP_F_BIG >= AP_TH_BIG * AP_TH_BIG, "a copy of Theta must fit the R factors' area");
   extern __shared__ __align__(16) double sm[];
   double *X, *Y, *Ru, *Rv, *roots, *small, *A, *VV, *scr_u, *scr_v;
   double *tq_u = nullptr, *tq_v = nullptr;  // BIG: Q = X Tq of each side
@@ -1351,7 +1354,17 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
     if (tid < 64) nsweeps = wave_jacobi_svd32<false>(Xr, N, N, nullptr);
     __syncthreads();
   } else {
-    nsweeps = jacobi_svd(A, M, N, VV, rot, red);
+    if constexpr (BIG) {
+      // Theta > 32 x 32 (global-memory variant): V of the kept columns from a
+      // copy of Theta afterwards, as for the preconditioned path, which spares
+      // the rotation of V (N x N, in global memory at N = 128) in every step;
+      // the copy goes where R_u and R_v were (free once Theta is built)
+      for (int i = tid; i < M * N; i += blockDim.x) Ru[i] = A[i];
+      __syncthreads();
+      nsweeps = jacobi_svd(A, M, N, nullptr, rot, red);
+    } else {
+      nsweeps = jacobi_svd(A, M, N, VV, rot, red);
+    }
   }
   if (prm.ts && tid == 0) {
     prm.ts[blockIdx.x * 16 + 8] = (unsigned long long)nsweeps;
@@ -1405,6 +1418,17 @@ __global__ __launch_bounds__(256) void k_apply_gates(const ApplyItem* __restrict
   // finite, is left untouched (the host reports the error); every write to psi
   // comes after this point
   if (keep < 0) return;
+  if (BIG && !pre) {
+    // the kept right singular vectors: V_o = Theta^T A_o / s_o^2 (zero for s_o = 0)
+    for (int idx = tid; idx < N * keep; idx += blockDim.x) {
+      const int i = idx % N, o = order[idx / N];
+      const double s2 = svec[o] * svec[o];
+      double t = 0;
+      for (int r = 0; r < M; ++r) t += Ru[r + (long long)M * i] * A[r + (long long)M * o];
+      VV[i + N * o] = s2 > 0 ? t / s2 : 0.0;
+    }
+    __syncthreads();
+  }
   if (pre) {
     // the kept singular vectors: V_o = P U_x,o = P X_o / s_o, then
     // A_o = U_o s_o = Theta V_o (a zero singular value gets zero columns, which
