@@ -273,3 +273,34 @@ def test_mixed_small_and_large_bonds_and_a_rank_deficient_side():
             assert np.linalg.norm(new[x] - r) <= 1e-10 * np.linalg.norm(r), (u, v, x)
         np.testing.assert_allclose(svals[i], s_ref, rtol=1e-10, atol=1e-13 * s_ref[0])
         assert errs[i] == pytest.approx(err_ref, rel=1e-8, abs=1e-15)
+
+
+def test_large_bonds_with_physical_dimension_three():
+    """The global-memory variant at d = 3 (qutrits): a 3x3 chi = 12 PEPS, a bulk
+    bond and one next to a corner (E = 12 < n = 36, the wide side) with random
+    symmetric gates exp(-0.2 H) (9 x 9), truncated to maxdim = 12, against the
+    oracle."""
+    g = itnbp.named_grid((3, 3))
+    psi = itnbp.random_tensornetwork(np.random.default_rng(51), np.float64, g, link_space=12, phys_dim=3)
+    bpc = converged_cache(psi)
+    og = oracle_graph(g)
+    opsi = oracle_psi(psi)
+    msgs = to_index(g, bpc.messages())
+    rng = np.random.default_rng(52)
+    # each bond in the edge's own orientation: the deterministic bond gauge
+    # (the largest entry of every new-bond slice of the first site made
+    # positive) then refers to the same site in the kernel and in the oracle
+    bonds = [((2, 2), (3, 2)), ((1, 2), (1, 3))]
+    gates = []
+    for _ in bonds:
+        H = rng.standard_normal((9, 9))
+        gates.append(O.expm_sym(0.5 * (H + H.T), -0.2))
+    errs = itnbp.apply_gates_(bpc, [itnbp.Gate(G, b) for G, b in zip(gates, bonds)], maxdim=12, normalize=True)
+    new = bpc.tensornetwork()
+    for i, ((u, v), G) in enumerate(zip(bonds, gates)):
+        ref, s_ref, err_ref = O.apply_gate(G, og, opsi, msgs, g.index[u], g.index[v], maxdim=12, normalize=True)
+        assert new.bond_dim(u, v) == len(s_ref)
+        for x in (u, v):
+            r = ref[g.index[x]]
+            assert np.linalg.norm(new[x] - r) <= 1e-10 * np.linalg.norm(r), (u, v, x)
+        assert errs[i] == pytest.approx(err_ref, rel=1e-8, abs=1e-15)
