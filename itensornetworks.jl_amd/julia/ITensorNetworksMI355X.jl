@@ -532,7 +532,7 @@ function ITensors.apply(os::Vector{ITensor}, psi::AbstractITensorNetwork, bpc::M
             push!(ids, L.eindex[(a, b)])
             append!(mats, vec(gate_matrix(o, work, a, b)))
         end
-        max_keep = 32
+        max_keep = 64  # the largest kept rank (bonds grow to at most 64)
         te = Vector{Float64}(undef, n)
         sv = zeros(n * max_keep)
         kept = Vector{Cint}(undef, n)
