@@ -1850,16 +1850,14 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
     ITNBP_HIPCHK(ctx, hipMemsetAsync(tsb.p, 0, (size_t)n * 16 * sizeof(unsigned long long), ctx->stream));
     prm.ts = (unsigned long long*)tsb.p;
   }
-  const int pm0 = prof_mark(ctx);
-  if (nl > 0)
-    hipLaunchKernelGGL(k_apply_gates<false>, dim3(nl), dim3(256), lds, ctx->stream, (const ApplyItem*)b.items,
-                       net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)b.gates, prm,
-                       b.work, b.te, svals ? b.sv : nullptr, b.kept, b.st);
-  hipError_t ek = hipGetLastError();
-  prof_add(ctx, "apply_gates", pm0, prof_mark(ctx));
   DevBuf bw;
   std::unique_ptr<DevBuf, void (*)(DevBuf*)> bw_guard(&bw, [](DevBuf* x) { free_buf(*x); });
-  if (ek == hipSuccess && nb > 0) {
+  // the global-memory variant's workspace first, so that a failed allocation
+  // leaves every tensor of the call untouched
+  ApplyParams pb = prm;
+  int chunk = 1;
+  size_t lds_big = 0;
+  if (nb > 0) {
     // side matrices of the largest gate: E n for the QR, E max(n, d chi) for the work
     long long q = 1, xs = 1;
     for (int i = nl; i < n; ++i)
@@ -1873,7 +1871,6 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
         q = std::max(q, E * nn);
         xs = std::max(xs, E * std::max(nn, cols));
       }
-    ApplyParams pb = prm;
     pb.big_q = q;
     pb.big_x = xs;
     pb.big_stride = (big_stride(q, xs) + 31) & ~31LL;
@@ -1881,13 +1878,22 @@ int run_apply(itnbp_net* net, int n, const std::vector<ApplyItem>& items_in, con
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
     const size_t budget = std::min<size_t>(fr / 2, 16ULL << 30);
     const long long per = pb.big_stride * 8;
-    const int chunk = (int)std::max<long long>(1, std::min<long long>(nb, (long long)(budget / (size_t)per)));
+    chunk = (int)std::max<long long>(1, std::min<long long>(nb, (long long)(budget / (size_t)per)));
     ITNBP_RETCHK(ensure_buf(ctx, bw, (size_t)chunk * per));
-    const size_t lds_big = apply_big_lds_bytes();
-    ea = hipFuncSetAttribute((const void*)k_apply_gates<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    lds_big = apply_big_lds_bytes();
+    hipError_t eb = hipFuncSetAttribute((const void*)k_apply_gates<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds_big);
-    if (ea != hipSuccess)
-      return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(ea));
+    if (eb != hipSuccess)
+      return itnbp_fail(ctx, ITNBP_ERR_HIP, std::string("apply: LDS attribute: ") + hipGetErrorString(eb));
+  }
+  const int pm0 = prof_mark(ctx);
+  if (nl > 0)
+    hipLaunchKernelGGL(k_apply_gates<false>, dim3(nl), dim3(256), lds, ctx->stream, (const ApplyItem*)b.items,
+                       net->d_vdesc, (double*)net->d_psi, net->d_msg[net->cur], (const double*)b.gates, prm,
+                       b.work, b.te, svals ? b.sv : nullptr, b.kept, b.st);
+  hipError_t ek = hipGetLastError();
+  prof_add(ctx, "apply_gates", pm0, prof_mark(ctx));
+  if (ek == hipSuccess && nb > 0) {
     const int pm1 = prof_mark(ctx);
     for (int c0 = nl; c0 < n && ek == hipSuccess; c0 += chunk) {
       const int cn = std::min(chunk, n - c0);
