@@ -1031,14 +1031,41 @@ __device__ int gram_qr(const double* X, long long E, int n, double* Xw, double* 
   const int wcap = lds_cap - 3 * n * n;
   __shared__ int s_rank;
   gram_rows(X, E, n, G, T, lds_cap - 2 * n * n);
-  jacobi_svd(G, n, n, V, rot, red);
-  // eigenvalues of the PSD G: the column norms of U diag(l)
-  for (int j = tid; j < n; j += blockDim.x) {
-    double t = 0;
-    for (int i = 0; i < n; ++i) t += G[i + n * j] * G[i + n * j];
-    lam[j] = sqrt(t);
+  if (n <= 32) {
+    // one wave, QR-preconditioned (as for Theta): G P = Q R, Jacobi on R^T ->
+    // U_x diag(l); G's eigenvectors are P U_x, its eigenvalues the column norms
+    __shared__ int gperm[32];
+    double* Xr = T;  // free until T is built
+    if (tid < 64) wave_qrcp32(G, n, n, gperm);
+    __syncthreads();
+    for (int i = tid; i < n * n; i += blockDim.x) {
+      const int row = i % n, col = i / n;
+      Xr[i] = col <= row ? G[col + n * row] : 0.0;
+    }
+    __syncthreads();
+    if (tid < 64) wave_jacobi_svd32<false>(Xr, n, n, nullptr);
+    __syncthreads();
+    for (int j = tid; j < n; j += blockDim.x) {
+      double t = 0;
+      for (int i = 0; i < n; ++i) t += Xr[i + n * j] * Xr[i + n * j];
+      lam[j] = sqrt(t);
+    }
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += blockDim.x) {
+      const int i = idx % n, j = idx / n;
+      V[gperm[i] + n * j] = lam[j] > 0 ? Xr[i + n * j] / lam[j] : 0.0;
+    }
+    __syncthreads();
+  } else {
+    jacobi_svd(G, n, n, V, rot, red);
+    // eigenvalues of the PSD G: the column norms of U diag(l)
+    for (int j = tid; j < n; j += blockDim.x) {
+      double t = 0;
+      for (int i = 0; i < n; ++i) t += G[i + n * j] * G[i + n * j];
+      lam[j] = sqrt(t);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if (tid == 0) {
     double lmax = 0;
     for (int j = 0; j < n; ++j) lmax = lam[j] > lmax ? lam[j] : lmax;
