@@ -29,6 +29,8 @@
 
 namespace {
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 constexpr int AP_XY = 8192;   // max E * n (external size x d chi_b) per side
 // tall-skinny QR (E >= 4 n, n <= 16): the stacked R factors (4n x n) and the
 // heads and betas of their reflectors, per side, after the side's reflectors
@@ -952,18 +954,21 @@ __device__ int wave_jacobi_svd32(double* A, int M, int N, double* V) {
 // column-by-column Householder QR reads it once per column.
 
 // G = X^T X (n x n, ld n, in LDS) of X (E x n column-major, global), the rows
-// staged through `tile` (tile_cap doubles of LDS) in chunks; thread (bi, bj)
-// accumulates the 4 x 4 block of G at (4 bi, 4 bj) (n <= 64)
+// staged through `tile` (tile_cap doubles of LDS) in chunks, the products on
+// the FP64 matrix cores: wave w owns the 16-row block w of G and its four
+// 16 x 16 tiles (n <= 64), v_mfma_f64_16x16x4f64 with A[i][k] = X[e + k][16 w + i]
+// and B[k][j] = X[e + k][16 bj + j] for four rows e.. e + 3 per step (rows past
+// the chunk and columns past n read as zero); D[4 v + l / 16][l % 16] is value
+// v of lane l
 __device__ void gram_rows(const double* X, long long E, int n, double* G, double* tile, int tile_cap) {
-  const int tid = threadIdx.x, bi = tid & 15, bj = tid >> 4;
+  const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6, li = l & 15, lk = l >> 4;
   const int ld = tile_cap / n;  // rows of a chunk + 1 (odd pitch against bank conflicts)
   const int cr = (ld & 1) ? ld - 1 : ld - 2;
   const int pitch = cr + 1;
-  double acc[4][4];
+  f64x4 acc[4];
 #pragma unroll
-  for (int p = 0; p < 4; ++p)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+  for (int t = 0; t < 4; ++t) acc[t] = f64x4{0, 0, 0, 0};
+  const int ri = 16 * wv + li;  // this lane's row of G (A operand column of X)
   for (long long e0 = 0; e0 < E; e0 += cr) {
     const int rows = (int)((E - e0) < cr ? (E - e0) : cr);
     __syncthreads();  // the previous chunk is consumed
@@ -972,26 +977,27 @@ __device__ void gram_rows(const double* X, long long E, int n, double* G, double
       tile[c * pitch + rr] = X[e0 + rr + E * c];
     }
     __syncthreads();
-    if (4 * bi < n && 4 * bj < n)
-      for (int rr = 0; rr < rows; ++rr) {
-        double a[4], b[4];
+    if (16 * wv < n)
+      for (int k0 = 0; k0 < rows; k0 += 4) {
+        const int rr = k0 + lk;
+        const bool rin = rr < rows;
+        const double av = (rin && ri < n) ? tile[ri * pitch + rr] : 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          a[q] = 4 * bi + q < n ? tile[(4 * bi + q) * pitch + rr] : 0.0;
-          b[q] = 4 * bj + q < n ? tile[(4 * bj + q) * pitch + rr] : 0.0;
+        for (int bj = 0; bj < 4; ++bj) {
+          const int cj = 16 * bj + li;
+          const double bv = (rin && cj < n) ? tile[cj * pitch + rr] : 0.0;
+          acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bj], 0, 0, 0);
         }
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
       }
   }
   __syncthreads();
 #pragma unroll
-  for (int p = 0; p < 4; ++p)
+  for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (4 * bi + p < n && 4 * bj + q < n) G[(4 * bi + p) + n * (4 * bj + q)] = acc[p][q];
+    for (int v = 0; v < 4; ++v) {
+      const int row = 16 * wv + 4 * v + lk, col = 16 * bj + li;
+      if (row < n && col < n) G[row + n * col] = acc[bj][v];
+    }
   __syncthreads();
 }
 
