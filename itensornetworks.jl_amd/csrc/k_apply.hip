@@ -1001,25 +1001,41 @@ __device__ void gram_rows(const double* X, long long E, int n, double* G, double
   __syncthreads();
 }
 
-// out[e + E j] = sum_c X[e + E c] W[c + n j] for j < m: one row per thread,
-// 16 outputs at a time, W (n x m, ld n) in LDS (every lane reads the same
-// entry); the row's entries are re-read (from cache) per group of 16 outputs
+// out[e + E j] = sum_c X[e + E c] W[c + n j] for j < m (n <= 64), W (n x m,
+// ld n) in LDS, on the FP64 matrix cores: wave w takes the 16-row blocks
+// w, w + 4, ..; a block's A operands (A[i][k] = X[e0 + i][4 s + k], lane
+// i + 16 k, one register per step s) are loaded once and reused for every
+// 16-column tile of the output (B[k][j] = W[4 s + k][16 t + j]); rows, columns
+// and steps past E, m and n read as zero.  D[4 v + l / 16][l % 16] is value v
+// of lane l.
 __device__ void rows_times(const double* X, long long E, int n, const double* W, int m, double* out) {
-  for (long long e = threadIdx.x; e < E; e += blockDim.x)
-    for (int j0 = 0; j0 < m; j0 += 16) {
-      double acc[16];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, li = l & 15, lk = l >> 4;
+  const int nsteps = (n + 3) >> 2;  // <= 16
+  for (long long e0 = 16LL * wv; e0 < E; e0 += 64) {
+    double av[16];
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) acc[jj] = 0.0;
-#pragma unroll 4
-      for (int c = 0; c < n; ++c) {
-        const double x = X[e + E * c];
+    for (int s = 0; s < 16; ++s) {
+      const int c = 4 * s + lk;
+      av[s] = (s < nsteps && c < n && e0 + li < E) ? X[e0 + li + E * c] : 0.0;
+    }
+    for (int t0 = 0; t0 < m; t0 += 16) {
+      f64x4 acc = {0, 0, 0, 0};
+      const int j = t0 + li;
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) acc[jj] += x * (j0 + jj < m ? W[c + n * (j0 + jj)] : 0.0);
+      for (int s = 0; s < 16; ++s) {
+        if (s < nsteps) {
+          const int c = 4 * s + lk;
+          const double bv = (c < n && j < m) ? W[c + n * j] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv, acc, 0, 0, 0);
+        }
       }
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj)
-        if (j0 + jj < m) out[e + E * (j0 + jj)] = acc[jj];
+      for (int v = 0; v < 4; ++v) {
+        const long long e = e0 + 4 * v + lk;
+        if (e < E && j < m) out[e + E * j] = acc[v];
+      }
     }
+  }
   __syncthreads();
 }
 
